@@ -1,0 +1,217 @@
+#!/usr/bin/env python
+"""Benchmark of the BFE hot path on one MI355X.
+
+Workload (one "step"): 1M exponential-disc particles and 1M Hernquist-halo
+particles resident in HBM as float32 SoA.  A step accumulates the disc EOF
+coefficients (mmax 6, norder 18, 256x128 tables) and the halo SL coefficients
+(lmax 6, nmax 20, numr 2000), freezes the disc+halo field (folds the
+coefficients into the tables) and evaluates Phi, F at all 2M particles
+(Fields.return_forces_cart semantics).  value = particles / s through the
+whole step (accumulate + force evaluation of every particle).
+
+Also reported: per-kernel HIP-event timings, the roofline record of the
+dominant kernel, orbit-steps/s of the plain-BFE leapfrog in the frozen field,
+and the FP64 NumPy oracle timed on a bounded sample on the host (cpu_baseline).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+# Algorithmic work per unit (reference formulation, M = mmax, N = norder,
+# L = lmax, n = nmax).  FLOPs: EOF accumulate 10 (2M+1) N per particle;
+# SL accumulate 2 (L+1)^2 n + 4 (L+1) n; bytes: 16 B in per particle.
+def eof_acc_flops(M, N):
+    return 10 * (2 * M + 1) * N
+
+
+def sl_acc_flops(L, n):
+    return 2 * (L + 1) ** 2 * n + 4 * (L + 1) * n
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n-disc", type=int, default=1_000_000)
+    ap.add_argument("--n-halo", type=int, default=1_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--orbit-steps", type=int, default=200)
+    ap.add_argument("--orbits", type=int, default=1_000_000)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    import torch
+    from exptool_amd.bfe import BFESystem
+    from exptool_amd.utils import workloads as W
+    from exptool_amd.utils import samplers as S
+
+    t_setup = time.time()
+    sl_t, eof_t = W.disc_halo_tables(seed=1)
+    (dxyzm, hxyzm) = W.disc_halo_particles(a.n_disc, a.n_halo, seed=0)
+    sysm = BFESystem(sl=sl_t, eof=eof_t, device=0)
+    dev = sysm.device
+    T = lambda v: torch.as_tensor(np.asarray(v, np.float32), device=dev)
+    dx, dy, dz, dm = (T(v) for v in dxyzm)
+    hx, hy, hz, hm = (T(v) for v in hxyzm)
+    ax, ay, az = (torch.cat([p, q]) for p, q in ((dx, hx), (dy, hy), (dz, hz)))
+    ntot = ax.numel()
+    setup_s = time.time() - t_setup
+
+    def step():
+        c, s = sysm.eof_accumulate(dx, dy, dz, dm)
+        h = sysm.sl_accumulate(hx, hy, hz, hm)
+        sysm.set_field(c, s, h)
+        return sysm.field_forces(ax, ay, az)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / a.steps
+    value = ntot / ((t1 - t0) / a.steps)
+
+    # ---- per-kernel timings (HIP events on the launch stream) ---------------
+    def timed(fn, reps):
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        fn()
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(reps):
+            fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / reps * 1e-3
+
+    c, s = sysm.eof_accumulate(dx, dy, dz, dm)
+    h = sysm.sl_accumulate(hx, hy, hz, hm)
+    reps = max(3, a.steps)
+    kt = {
+        "eof_accumulate": timed(lambda: sysm.eof_accumulate(dx, dy, dz, dm), reps),
+        "sl_accumulate": timed(lambda: sysm.sl_accumulate(hx, hy, hz, hm), reps),
+        "field_set": timed(lambda: sysm.set_field(c, s, h), reps),
+        "field_eval_cart": timed(lambda: sysm.field_forces(ax, ay, az), reps),
+    }
+    M, N = eof_t["mmax"], eof_t["norder"]
+    L, n = sl_t["lmax"], sl_t["nmax"]
+    units = {
+        "eof_accumulate": (a.n_disc * eof_acc_flops(M, N), "flop"),
+        "sl_accumulate": (a.n_halo * sl_acc_flops(L, n), "flop"),
+        "field_eval_cart": (ntot * 28, "byte"),
+    }
+    dom = max((k for k in units), key=lambda k: kt[k])
+    work, kind = units[dom]
+    if kind == "flop":
+        achieved = work / kt[dom] / 1e12
+        roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None}
+    else:
+        achieved = work / kt[dom] / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+
+    # ---- orbit integration in the frozen field -------------------------------
+    orbit = None
+    if a.orbit_steps > 0 and a.orbits > 0:
+        sysm.set_field(c, s, h)
+        no = a.orbits
+        ox, oy, oz, _ = S.exponential_disc(no, seed=5, **W.DISC)
+        Rr = np.sqrt(ox * ox + oy * oy)
+        fx, fy, fz, _ = sysm.field_forces(Rr, np.zeros_like(Rr), np.zeros_like(Rr))
+        vx, vy, vz = S.circular_orbits(ox, oy, oz, fx.cpu().numpy().astype(np.float64), 0.1, seed=6)
+        st = [T(v) for v in (ox, oy, oz, vx, vy, vz)]
+        sysm.leapfrog([t.clone() for t in st], dt=1e-3, nsteps=10, energy_every=0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = sysm.leapfrog(st, dt=1e-3, nsteps=a.orbit_steps, energy_every=100)
+        torch.cuda.synchronize()
+        dt_o = time.perf_counter() - t0
+        dr = out["drift"].cpu().numpy()
+        orbit = {"orbit_steps_per_s": no * a.orbit_steps / dt_o, "orbits": no,
+                 "steps": a.orbit_steps, "seconds": dt_o,
+                 "energy_drift_median": float(np.median(dr)), "energy_drift_max": float(np.max(dr))}
+
+    # ---- CPU baseline: the FP64 oracle on a bounded sample --------------------
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(sl_t, eof_t, dxyzm, hxyzm, a.cpu_seconds)
+
+    line = {
+        "metric": "particle force evals/s (accumulate + Phi,F eval, disc EOF + halo SL)",
+        "value": value, "unit": "particles/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 geometry and cross-block sums)",
+        "data": "synthetic (seeded exponential disc + Hernquist halo)",
+        "config": {"workload": "disc EOF mmax6 norder18 256x128 + halo SL lmax6 nmax20 numr2000, "
+                               "1M+1M particles, accumulate + return_forces_cart at 2M",
+                   "n_disc": a.n_disc, "n_halo": a.n_halo, "parallelism": "single GPU"},
+        "kernel_seconds": kt, "roofline": roof, "orbits": orbit, "cpu_baseline": cpu,
+        "setup_seconds": setup_s,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline(sl_t, eof_t, dxyzm, hxyzm, budget_s):
+    """Time the oracle's accumulate + force evaluation on a sample sized to
+    roughly budget_s seconds of one core; report particles/s."""
+    from oracle import bfe_oracle as O
+    slo = O.SLTable(sl_t["lmax"], sl_t["nmax"], sl_t["numr"], sl_t["cmap"], sl_t["rmin"],
+                    sl_t["rmax"], sl_t["scale"], sl_t["ev"], sl_t["ef"], sl_t["xi"], sl_t["p0"],
+                    sl_t["d0"])
+    eo = O.EOFTable(eof_t["mmax"], eof_t["norder"], eof_t["numx"], eof_t["numy"], eof_t["cmap"],
+                    eof_t["rmin"], eof_t["rmax"], eof_t["ascale"], eof_t["hscale"], eof_t["potC"],
+                    eof_t["rforceC"], eof_t["zforceC"], eof_t["potS"], eof_t["rforceS"],
+                    eof_t["zforceS"])
+
+    def run(k):
+        d = [v[:k] for v in dxyzm]
+        h = [v[:k] for v in hxyzm]
+        t0 = time.perf_counter()
+        c, s = O.eof_accumulate(eo, *d)
+        hc = O.sl_accumulate(slo, *h)
+        F = O.FieldSpec(eo, c, s, slo, hc)
+        for chunk in range(0, k, 4096):
+            O.total_force(F, d[0][chunk:chunk + 4096], d[1][chunk:chunk + 4096], d[2][chunk:chunk + 4096])
+            O.total_force(F, h[0][chunk:chunk + 4096], h[1][chunk:chunk + 4096], h[2][chunk:chunk + 4096])
+        return time.perf_counter() - t0
+
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1):
+        return _cpu_sized(run, budget_s)
+
+
+def _cpu_sized(run, budget_s):
+    k = 2000
+    t = run(k)
+    k2 = int(min(max(k * budget_s / max(t, 1e-3), k), 200000))
+    t2 = run(k2)
+    return {"value": 2 * k2 / t2, "unit": "particles/s", "cores": 1, "kind": "port",
+            "sample": f"{k2} disc + {k2} halo particles of the same workload, FP64 NumPy oracle "
+                      f"(accumulate + return_forces_cart), {t2:.1f} s, single thread"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,265 @@
+"""Cylindrical EOF basis: drop-in for exptool.basis.eof on the GPU.
+
+Table I/O (host):
+  eof_params        eof.py:98-200   (new YAML header 0x0C0A57A1 or old 76-byte header)
+  parse_eof         eof.py:224-313
+  set_table_params  eof.py:316-347
+  write_eof_cache   (writer of the same layout; the reference has none)
+Device work (libexptool_bfe.so through exptool_amd.bfe.BFESystem):
+  accumulate               eof.py:492-640
+  compute_coefficients     eof.py:1158-1260
+  accumulated_eval_particles / compute_forces   eof.py:989-1144, :1263-1317
+  force_eval               eof.py:756-872 (many points per call)
+Return types follow the reference: float64 NumPy arrays and EOF_Object.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import yaml
+
+from .compatibility import r_to_xi, z_to_y
+
+HMAGIC = 0x0C0A57A1
+
+
+# ----------------------------------------------------------------------------
+# cache file I/O
+# ----------------------------------------------------------------------------
+
+def _read_header(f):
+    tmagic = np.fromfile(f, dtype="<i4", count=1)
+    if tmagic.size and tmagic[0] == HMAGIC:
+        ssize = int(np.fromfile(f, dtype="<i4", count=1)[0])
+        data = yaml.safe_load(f.read(ssize).decode())
+        cmap = data.get("cmap", data.get("cmapr"))
+        hdr = dict(mmax=int(data["mmax"]), numx=int(data["numx"]), numy=int(data["numy"]),
+                   nmax=int(data["nmax"]), norder=int(data["norder"]), dens=int(data["dens"]),
+                   cmap=int(cmap), rmin=float(data["rmin"]), rmax=float(data["rmax"]),
+                   ascale=float(data["ascl"]), hscale=float(data["hscl"]),
+                   cylmass=float(data["cmass"]), time=float(data["time"]),
+                   cmapz=int(data.get("cmapz", 0)), offset=8 + ssize, newstyle=True)
+    else:
+        f.seek(0)
+        a = np.fromfile(f, dtype="<u4", count=7)
+        b = np.fromfile(f, dtype="<f8", count=6)
+        hdr = dict(mmax=int(a[0]), numx=int(a[1]), numy=int(a[2]), nmax=int(a[3]),
+                   norder=int(a[4]), dens=int(a[5]), cmap=int(a[6]), rmin=float(b[0]),
+                   rmax=float(b[1]), ascale=float(b[2]), hscale=float(b[3]),
+                   cylmass=float(b[4]), time=float(b[5]), cmapz=0, offset=76, newstyle=False)
+    return hdr
+
+
+def eof_header(file):
+    with open(file, "rb") as f:
+        return _read_header(f)
+
+
+def eof_params(file, verbose=0):
+    """eof.py:98-200: returns rmin, rmax, numx, numy, mmax, norder, ascale,
+    hscale, cmap, dens."""
+    h = eof_header(file)
+    if verbose:
+        print("eof.eof_params:", {k: v for k, v in h.items() if k != "offset"})
+    return (h["rmin"], h["rmax"], h["numx"], h["numy"], h["mmax"], h["norder"], h["ascale"],
+            h["hscale"], h["cmap"], h["dens"])
+
+
+def parse_eof(file, verbose=0):
+    """eof.py:224-313: potC, rforcec, zforcec, densc, potS, rforces, zforces,
+    denss, each (mmax+1, norder, numx+1, numy+1)."""
+    h = eof_header(file)
+    mmax, norder, numx, numy, dens = h["mmax"], h["norder"], h["numx"], h["numy"], h["dens"]
+    shape = (mmax + 1, norder, numx + 1, numy + 1)
+    out = [np.zeros(shape) for _ in range(8)]
+    plane = (numx + 1) * (numy + 1)
+    nq = 4 if dens == 1 else 3
+    raw = np.fromfile(file, dtype="<f8", offset=h["offset"])
+    pos = 0
+    for kind, mlist in ((0, range(0, mmax + 1)), (4, range(1, mmax + 1))):
+        for i in mlist:
+            for j in range(norder):
+                for q in range(nq):
+                    out[kind + q][i, j] = raw[pos:pos + plane].reshape(numx + 1, numy + 1)
+                    pos += plane
+    return tuple(out)
+
+
+def write_eof_cache(file, t, newstyle=True):
+    """Write tables (dict from tablegen.make_eof_tables) in the EmpCylSL cache
+    layout parse_eof reads: header, then for m=0..mmax, n<norder: potC,
+    rforceC, zforceC[, densC]; then m=1..mmax: the S tables."""
+    dens = int(t.get("dens", 1))
+    with open(file, "wb") as f:
+        if newstyle:
+            hdr = dict(mmax=int(t["mmax"]), numx=int(t["numx"]), numy=int(t["numy"]),
+                       nmax=int(t.get("nmax", t["norder"])), norder=int(t["norder"]), dens=dens,
+                       cmap=int(t["cmap"]), rmin=float(t["rmin"]), rmax=float(t["rmax"]),
+                       ascl=float(t["ascale"]), hscl=float(t["hscale"]),
+                       cmass=float(t.get("cylmass", 1.0)), time=float(t.get("time", 0.0)))
+            b = yaml.safe_dump(hdr).encode()
+            np.array([HMAGIC, len(b)], dtype="<i4").tofile(f)
+            f.write(b)
+        else:
+            np.array([t["mmax"], t["numx"], t["numy"], t.get("nmax", t["norder"]), t["norder"],
+                      dens, t["cmap"]], dtype="<u4").tofile(f)
+            np.array([t["rmin"], t["rmax"], t["ascale"], t["hscale"], t.get("cylmass", 1.0),
+                      t.get("time", 0.0)], dtype="<f8").tofile(f)
+        groups = ("pot", "rforce", "zforce", "dens")[:4 if dens == 1 else 3]
+        for sfx, mlist in (("C", range(0, t["mmax"] + 1)), ("S", range(1, t["mmax"] + 1))):
+            for i in mlist:
+                for j in range(t["norder"]):
+                    for g in groups:
+                        np.asarray(t[g + sfx][i, j], dtype="<f8").tofile(f)
+
+
+def load_eof(eof_file):
+    """Read an EOF cache into the table dict used by BFESystem."""
+    h = eof_header(eof_file)
+    potC, rforceC, zforceC, densC, potS, rforceS, zforceS, densS = parse_eof(eof_file)
+    return dict(mmax=h["mmax"], norder=h["norder"], numx=h["numx"], numy=h["numy"],
+                cmap=h["cmap"], rmin=h["rmin"], rmax=h["rmax"], ascale=h["ascale"],
+                hscale=h["hscale"], dens=h["dens"], potC=potC, rforceC=rforceC,
+                zforceC=zforceC, densC=densC if h["dens"] == 1 else None, potS=potS,
+                rforceS=rforceS, zforceS=zforceS, densS=densS if h["dens"] == 1 else None)
+
+
+def set_table_params(RMAX=20.0, RMIN=0.001, ASCALE=0.01, HSCALE=0.001, NUMX=128, NUMY=64, CMAP=0):
+    """eof.py:316-347."""
+    Rtable = np.sqrt(0.5) * RMAX
+    XMIN = r_to_xi(RMIN * ASCALE, CMAP, ASCALE)
+    XMAX = r_to_xi(Rtable * ASCALE, CMAP, ASCALE)
+    dX = (XMAX - XMIN) / NUMX
+    YMIN = z_to_y(-Rtable * ASCALE, HSCALE)
+    YMAX = z_to_y(Rtable * ASCALE, HSCALE)
+    dY = (YMAX - YMIN) / NUMY
+    return XMIN, XMAX, dX, YMIN, YMAX, dY
+
+
+# ----------------------------------------------------------------------------
+# objects
+# ----------------------------------------------------------------------------
+
+class EOF_Object(object):
+    """eof.py:1627-1636."""
+    time = None
+    dump = None
+    comp = None
+    nbodies = None
+    eof_file = None
+    mmax = None
+    nmax = None
+    cos = None
+    sin = None
+
+
+_SYSTEMS = {}
+
+
+def system_for(eof_file):
+    """BFESystem holding the tables of eof_file (cached per file and mtime)."""
+    from ..bfe import BFESystem
+    key = (os.path.abspath(eof_file), os.path.getmtime(eof_file))
+    s = _SYSTEMS.get(key)
+    if s is None:
+        s = BFESystem(eof=load_eof(eof_file))
+        _SYSTEMS.clear()
+        _SYSTEMS[key] = s
+    return s
+
+
+def _particle_arrays(P):
+    """x, y, z, m from a particle.holder (xpos..mass) or an object with .data."""
+    if hasattr(P, "data") and isinstance(getattr(P, "data"), dict):
+        d = P.data
+        return d["x"], d["y"], d["z"], d.get("m")
+    return P.xpos, P.ypos, P.zpos, getattr(P, "mass", None)
+
+
+def _nan_to_origin(x, y, z):
+    """eof.py:1190-1204: particles with NaN coordinates are moved to the origin."""
+    x, y, z = (np.array(a, dtype=np.float64, copy=True) for a in (x, y, z))
+    bad = np.isnan(x) | np.isnan(y) | np.isnan(z)
+    if bad.any():
+        x[bad] = 0.0
+        y[bad] = 0.0
+        z[bad] = 0.0
+    return x, y, z
+
+
+# ----------------------------------------------------------------------------
+# device-backed API
+# ----------------------------------------------------------------------------
+
+def accumulate_tables(system, x, y, z, m):
+    """Coefficients (cos, sin) as float64 numpy arrays from a BFESystem."""
+    c, s = system.eof_accumulate(x, y, z, m)
+    return c.cpu().numpy(), s.cpu().numpy()
+
+
+def compute_coefficients(PSPInput, eof_file, verbose=1, no_odd=False, nprocs_max=-1, VAR=False,
+                         nanblock=False):
+    """eof.py:1158-1260 on the GPU.  NaN particles are moved to the origin
+    (the reference intends this at :1190-1204 but raises under NumPy >= 2);
+    nanblock=True raises instead.  no_odd has no effect, as in the reference
+    accumulate (eof.py:545-551).  VAR (jackknife) is not implemented."""
+    if VAR:
+        raise NotImplementedError("VAR jackknife accumulation is not implemented")
+    x, y, z, m = _particle_arrays(PSPInput)
+    if nanblock and (np.isnan(x) | np.isnan(y) | np.isnan(z)).any():
+        raise ValueError(f"eof.compute_coefficients: NaN values in {getattr(PSPInput, 'filename', '')}")
+    x, y, z = _nan_to_origin(x, y, z)
+    sysm = system_for(eof_file)
+    out = EOF_Object()
+    out.time = getattr(PSPInput, "time", None)
+    out.filename = getattr(PSPInput, "filename", None)
+    out.comp = getattr(PSPInput, "comp", None)
+    out.nbodies = np.asarray(m).size
+    out.eof_file = eof_file
+    out.mmax = sysm.eof["mmax"]
+    out.nmax = sysm.eof["norder"]
+    out.cos, out.sin = accumulate_tables(sysm, x, y, z, m)
+    return out
+
+
+def accumulate(ParticleInstance, potC, potS, MMAX, NMAX, XMIN, dX, YMIN, dY, NUMX, NUMY, ASCALE,
+               HSCALE, CMAP, verbose=0, no_odd=False, VAR=0):
+    """eof.py:492-640 with tables passed as arrays.  The tables are uploaded to
+    a device context for the call (use compute_coefficients or a BFESystem to
+    keep them resident)."""
+    if VAR:
+        raise NotImplementedError("VAR jackknife accumulation is not implemented")
+    from ..bfe import BFESystem
+    potC, potS = np.asarray(potC), np.asarray(potS)
+    z0 = np.zeros(potC.shape)
+    t = dict(mmax=potC.shape[0] - 1, norder=potC.shape[1], numx=NUMX, numy=NUMY, cmap=CMAP,
+             ascale=ASCALE, hscale=HSCALE, XMIN=XMIN, dX=dX, YMIN=YMIN, dY=dY, potC=potC,
+             rforceC=z0, zforceC=z0, potS=potS, rforceS=z0, zforceS=z0, dens=0)
+    x, y, z, m = _particle_arrays(ParticleInstance)
+    c, sn = accumulate_tables(BFESystem(eof=t), x, y, z, m)
+    return c[:MMAX + 1, :NMAX], sn[:MMAX + 1, :NMAX]
+
+
+def compute_forces(PSPInput, EOF_Object, verbose=1, nprocs=-1, m1=0, m2=1000, density=False):
+    """eof.py:1263-1317 on the GPU: returns p0, p, fr, fp, fz, r (float64)
+    or, with density, p0, p, d0, d, fr, fp, fz, r."""
+    x, y, z, _ = _particle_arrays(PSPInput)
+    s = system_for(EOF_Object.eof_file)
+    o = s.eof_eval(x, y, z, EOF_Object.cos, EOF_Object.sin, m1=m1, m2=m2, density=density)
+    g = {k: v.double().cpu().numpy() for k, v in o.items()}
+    x, y = np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64)
+    R = (x * x + y * y + 1.e-10) ** 0.5
+    if density:
+        return g["p0"], g["p"], g["d0"], g["d"], g["fr"], g["fp"], g["fz"], R
+    return g["p0"], g["p"], g["fr"], g["fp"], g["fz"], R
+
+
+def accumulated_eval_particles(Particles, accum_cos, accum_sin, eof_file="", m1=0, m2=1000,
+                               verbose=1, density=False, **kw):
+    """eof.py:989-1144 on the GPU (tables given by eof_file)."""
+    if not eof_file:
+        raise ValueError("exptool_amd accumulated_eval_particles needs eof_file")
+    obj = EOF_Object()
+    obj.eof_file, obj.cos, obj.sin = eof_file, accum_cos, accum_sin
+    return compute_forces(Particles, obj, m1=m1, m2=m2, density=density)

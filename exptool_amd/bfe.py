@@ -1,0 +1,220 @@
+"""Device-side BFE system: tables resident on one GPU, torch tensors in and out.
+
+``BFESystem`` owns one library context with (optionally) one spherical SL
+table set and one cylindrical EOF table set, the pairing the reference's
+``Fields`` uses (exptool/basis/potential.py:49-294).  Particle arguments are
+float32 device tensors (numpy arrays are copied to the device); coefficients
+are float64 device tensors.  Every method enqueues on the current torch
+stream and returns device tensors without synchronising.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import BFEError, c64, f32, f64, host_ptr, ptr, stream_ptr
+
+
+def eof_table_params(t):
+    """set_table_params (exptool/basis/eof.py:316-347) for a table dict."""
+    from .basis.compatibility import r_to_xi, z_to_y
+    Rtable = math.sqrt(0.5) * t["rmax"]
+    a, h = t["ascale"], t["hscale"]
+    XMIN = float(r_to_xi(t["rmin"] * a, t["cmap"], a))
+    XMAX = float(r_to_xi(Rtable * a, t["cmap"], a))
+    dX = (XMAX - XMIN) / t["numx"]
+    YMIN = float(z_to_y(-Rtable * a, h))
+    YMAX = float(z_to_y(Rtable * a, h))
+    dY = (YMAX - YMIN) / t["numy"]
+    return XMIN, XMAX, dX, YMIN, YMAX, dY
+
+
+class BFESystem:
+    """SL + EOF tables on one device."""
+
+    def __init__(self, sl=None, eof=None, device=0):
+        self.ctx = _lib.Context(device)
+        self.device = torch.device("cuda", self.ctx.device)
+        self.sl = None
+        self.eof = None
+        if sl is not None:
+            self.set_sl(sl)
+        if eof is not None:
+            self.set_eof(eof)
+
+    # ------------------------------------------------------------------ tables
+    def set_sl(self, t):
+        """t: dict with lmax, nmax, numr, cmap, scale, xi, ev, ef, p0, d0."""
+        lmax, nmax, numr = int(t["lmax"]), int(t["nmax"]), int(t["numr"])
+        ev = c64(t["ev"]).reshape(lmax + 1, nmax)
+        ef = c64(t["ef"]).reshape(lmax + 1, nmax, numr)
+        xi, p0, d0 = c64(t["xi"]), c64(t["p0"]), c64(t["d0"])
+        rc = self.ctx.lib.bfe_sl_tables_set(self.ctx.h, lmax, nmax, numr, int(t["cmap"]),
+                                            float(t["scale"]), host_ptr(xi), host_ptr(ev),
+                                            host_ptr(ef), host_ptr(p0), host_ptr(d0))
+        self.ctx.check(rc, "bfe_sl_tables_set")
+        self.sl = dict(t, lmax=lmax, nmax=nmax, numr=numr)
+
+    def set_eof(self, t):
+        """t: dict with mmax, norder, numx, numy, cmap, rmin, rmax, ascale,
+        hscale, potC, rforceC, zforceC, potS, rforceS, zforceS[, densC, densS].
+        Grid parameters XMIN, dX, YMIN, dY, when present, are used as given
+        instead of being derived from rmin/rmax (set_table_params)."""
+        mmax, norder, numx, numy = (int(t[k]) for k in ("mmax", "norder", "numx", "numy"))
+        if all(k in t for k in ("XMIN", "dX", "YMIN", "dY")):
+            XMIN, dX, YMIN, dY = (float(t[k]) for k in ("XMIN", "dX", "YMIN", "dY"))
+        else:
+            XMIN, _, dX, YMIN, _, dY = eof_table_params(t)
+        shape = (mmax + 1, norder, numx + 1, numy + 1)
+        arrs = {k: c64(t[k]).reshape(shape) for k in
+                ("potC", "rforceC", "zforceC", "potS", "rforceS", "zforceS")}
+        dens = t.get("densC") is not None and t.get("densS") is not None and int(t.get("dens", 1))
+        dC = c64(t["densC"]).reshape(shape) if dens else None
+        dS = c64(t["densS"]).reshape(shape) if dens else None
+        rc = self.ctx.lib.bfe_eof_tables_set(
+            self.ctx.h, mmax, norder, numx, numy, int(t["cmap"]), float(t["ascale"]),
+            float(t["hscale"]), XMIN, dX, YMIN, dY, host_ptr(arrs["potC"]),
+            host_ptr(arrs["rforceC"]), host_ptr(arrs["zforceC"]), host_ptr(dC),
+            host_ptr(arrs["potS"]), host_ptr(arrs["rforceS"]), host_ptr(arrs["zforceS"]),
+            host_ptr(dS))
+        self.ctx.check(rc, "bfe_eof_tables_set")
+        self.eof = dict(t, mmax=mmax, norder=norder, numx=numx, numy=numy, XMIN=XMIN, dX=dX,
+                        YMIN=YMIN, dY=dY, has_dens=bool(dens))
+
+    # --------------------------------------------------------------- helpers
+    def _xyz(self, x, y, z):
+        x, y, z = (f32(a, self.device) for a in (x, y, z))
+        n = x.numel()
+        if y.numel() != n or z.numel() != n:
+            raise ValueError("x, y, z must have the same length")
+        return x, y, z, n
+
+    def _need(self, what):
+        if getattr(self, what) is None:
+            raise BFEError(f"{what} tables are not set on this BFESystem")
+
+    # ------------------------------------------------------------- accumulate
+    def sl_accumulate(self, x, y, z, m, no_odd=False, stream=None):
+        """Spherical coefficients ((lmax+1)^2, nmax) float64 (device)."""
+        self._need("sl")
+        x, y, z, n = self._xyz(x, y, z)
+        m = f32(m, self.device)
+        if m.numel() != n:
+            raise ValueError("mass must have the same length as x")
+        out = torch.empty(((self.sl["lmax"] + 1) ** 2, self.sl["nmax"]), dtype=torch.float64,
+                          device=self.device)
+        rc = self.ctx.lib.bfe_sl_accumulate(self.ctx.h, ptr(x), ptr(y), ptr(z), ptr(m), n,
+                                            _lib.NO_ODD if no_odd else 0, ptr(out),
+                                            stream_ptr(stream))
+        self.ctx.check(rc, "bfe_sl_accumulate")
+        return out
+
+    def eof_accumulate(self, x, y, z, m, stream=None):
+        """Cylindrical coefficients (cos, sin), each (mmax+1, norder) float64."""
+        self._need("eof")
+        x, y, z, n = self._xyz(x, y, z)
+        m = f32(m, self.device)
+        if m.numel() != n:
+            raise ValueError("mass must have the same length as x")
+        shape = (self.eof["mmax"] + 1, self.eof["norder"])
+        cos = torch.empty(shape, dtype=torch.float64, device=self.device)
+        sin = torch.empty(shape, dtype=torch.float64, device=self.device)
+        rc = self.ctx.lib.bfe_eof_accumulate(self.ctx.h, ptr(x), ptr(y), ptr(z), ptr(m), n, 0,
+                                             ptr(cos), ptr(sin), stream_ptr(stream))
+        self.ctx.check(rc, "bfe_eof_accumulate")
+        return cos, sin
+
+    # --------------------------------------------------------------- evaluate
+    def sl_eval(self, x, y, z, coef, L1=-1000, L2=1000, no_odd=False, compat=False,
+                density=True, stream=None):
+        """all_eval_particles outputs as float32 device tensors:
+        dict(den0, den1, pot0, pot1, potr, pott, potp)."""
+        self._need("sl")
+        x, y, z, n = self._xyz(x, y, z)
+        coef = f64(coef, self.device)
+        names = ("den0", "den1", "pot0", "pot1", "potr", "pott", "potp")
+        out = {k: torch.empty(n, dtype=torch.float32, device=self.device) for k in names
+               if density or not k.startswith("den")}
+        flags = (_lib.NO_ODD if no_odd else 0)
+        if compat:
+            flags |= _lib.COMPAT_SL_DENS | _lib.COMPAT_SL_EDGE
+        rc = self.ctx.lib.bfe_sl_eval(self.ctx.h, ptr(x), ptr(y), ptr(z), n, ptr(coef),
+                                      int(max(min(L1, 2**30), -2**30)),
+                                      int(max(min(L2, 2**30), -2**30)), flags,
+                                      *[ptr(out.get(k)) for k in names], stream_ptr(stream))
+        self.ctx.check(rc, "bfe_sl_eval")
+        return out
+
+    def eof_eval(self, x, y, z, cos, sin, m1=0, m2=1000, density=False, stream=None):
+        """accumulated_eval_particles outputs: dict(p0, p, fr, fp, fz[, d0, d])."""
+        self._need("eof")
+        x, y, z, n = self._xyz(x, y, z)
+        cos, sin = f64(cos, self.device), f64(sin, self.device)
+        names = ("p0", "p", "fr", "fp", "fz", "d0", "d")
+        out = {k: torch.empty(n, dtype=torch.float32, device=self.device) for k in names
+               if density or k not in ("d0", "d")}
+        rc = self.ctx.lib.bfe_eof_eval(self.ctx.h, ptr(x), ptr(y), ptr(z), n, ptr(cos), ptr(sin),
+                                       int(m1), int(min(m2, 2**30)), 0,
+                                       *[ptr(out.get(k)) for k in names], stream_ptr(stream))
+        self.ctx.check(rc, "bfe_eof_eval")
+        return out
+
+    # ------------------------------------------------------------------ field
+    def set_field(self, eof_cos, eof_sin, sl_coef, halofac=1.0, no_odd=False, disk_m=-1,
+                  disk_n=-1, halo_l=-1, halo_n=-1, compat=False, stream=None):
+        """Freeze a disc+halo field (Fields.prep_tables + set_field_parameters)."""
+        self._need("sl")
+        self._need("eof")
+        ec, es, sc = f64(eof_cos, self.device), f64(eof_sin, self.device), f64(sl_coef, self.device)
+        flags = (_lib.NO_ODD if no_odd else 0) | (_lib.COMPAT_ALL if compat else 0)
+        rc = self.ctx.lib.bfe_field_set(self.ctx.h, ptr(ec), ptr(es), ptr(sc), float(halofac),
+                                        int(disk_m), int(disk_n), int(halo_l), int(halo_n),
+                                        flags, stream_ptr(stream))
+        self.ctx.check(rc, "bfe_field_set")
+        self._field_keep = (ec, es, sc)
+
+    def field_forces(self, x, y, z, rotpos=0.0, components=False, stream=None):
+        """Return (fx, fy, fz, pot) float32 device tensors, and with
+        components=True also the [8, n] return_forces_cart outputs."""
+        x, y, z, n = self._xyz(x, y, z)
+        fx, fy, fz, pot = (torch.empty(n, dtype=torch.float32, device=self.device) for _ in range(4))
+        comp = torch.empty((8, n), dtype=torch.float32, device=self.device) if components else None
+        rc = self.ctx.lib.bfe_field_eval_cart(self.ctx.h, ptr(x), ptr(y), ptr(z), n,
+                                              float(rotpos), ptr(fx), ptr(fy), ptr(fz), ptr(pot),
+                                              ptr(comp), stream_ptr(stream))
+        self.ctx.check(rc, "bfe_field_eval_cart")
+        if components:
+            return fx, fy, fz, pot, comp
+        return fx, fy, fz, pot
+
+    def leapfrog(self, state, dt, nsteps, t0=0.0, rotfreq=0.0, energy_every=0,
+                 record_every=0, use_graph=False, stream=None):
+        """Advance state = (x, y, z, vx, vy, vz) float32 device tensors in place.
+        Returns dict(drift=(n,) float64 max relative energy drift per orbit,
+        energy=(checks,) float64 total energy log or None,
+        traj=(nrec, 7, n) float32 or None)."""
+        x, y, z, vx, vy, vz = state
+        n = x.numel()
+        for t in state:
+            if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != n or t.device != self.device:
+                raise ValueError("state tensors must be contiguous float32 on the context device")
+        drift = torch.empty(n, dtype=torch.float64, device=self.device)
+        elog = None
+        if energy_every > 0:
+            elog = torch.empty(nsteps // energy_every + 1, dtype=torch.float64, device=self.device)
+        traj = None
+        if record_every > 0:
+            traj = torch.empty((nsteps // record_every + 1, 7, n), dtype=torch.float32,
+                               device=self.device)
+        s = stream if stream is not None else torch.cuda.current_stream()
+        if use_graph and s.cuda_stream == 0:
+            raise ValueError("use_graph needs a non-default stream (torch.cuda.Stream())")
+        rc = self.ctx.lib.bfe_leapfrog(self.ctx.h, *[ptr(t) for t in state], n, float(dt),
+                                       int(nsteps), float(t0), float(rotfreq), int(energy_every),
+                                       ptr(drift), ptr(elog), int(record_every), ptr(traj),
+                                       1 if use_graph else 0, stream_ptr(s))
+        self.ctx.check(rc, "bfe_leapfrog")
+        return dict(drift=drift, energy=elog, traj=traj)

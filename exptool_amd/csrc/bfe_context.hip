@@ -1,0 +1,201 @@
+// Context management and table upload.
+#include <stdarg.h>
+#include <string.h>
+
+#include <cmath>
+
+#include "bfe_internal.hpp"
+
+namespace bfe {
+
+static thread_local std::string g_create_err;
+
+int set_error(Context* c, int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf; else g_create_err = buf;
+  return code;
+}
+
+int check_hip(Context* c, hipError_t e, const char* what) {
+  return set_error(c, BFE_EHIP, "HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+}
+
+int ensure_scratch(Context* c, size_t bytes) {
+  if (bytes <= c->scratch_bytes) return 0;
+  if (c->scratch) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return check_hip(c, e, "hipDeviceSynchronize (scratch grow)");
+    (void)hipFree(c->scratch);
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+  }
+  size_t want = bytes + bytes / 4;
+  hipError_t e = hipMalloc(&c->scratch, want);
+  if (e != hipSuccess) return check_hip(c, e, "hipMalloc scratch");
+  c->scratch_bytes = want;
+  return 0;
+}
+
+template <typename T>
+static int dev_upload(Context* c, T** dst, const T* src, size_t count) {
+  if (*dst) {
+    (void)hipFree(*dst);
+    *dst = nullptr;
+  }
+  BFE_HIP(c, hipMalloc((void**)dst, count * sizeof(T)));
+  BFE_HIP(c, hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static void free_ptr(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+static void free_sl(Context* c) {
+  free_ptr(c->sl_xi); free_ptr(c->sl_p0); free_ptr(c->sl_d0);
+  free_ptr(c->sl_ef64); free_ptr(c->sl_ev64); free_ptr(c->sl_e32);
+  c->sl_xi = c->sl_p0 = c->sl_d0 = c->sl_ef64 = c->sl_ev64 = nullptr;
+  c->sl_e32 = nullptr;
+  for (auto& f : c->slf) { free_ptr(f.d); f = SLFolded(); }
+  c->sl_ok = false;
+}
+
+static void free_eof(Context* c) {
+  for (auto& p : c->eof64) { free_ptr(p); p = nullptr; }
+  free_ptr(c->eof_pot32);
+  c->eof_pot32 = nullptr;
+  for (auto& f : c->eoff) { free_ptr(f.d); f = EOFFolded(); }
+  c->eof_ok = false;
+}
+
+}  // namespace bfe
+
+using namespace bfe;
+
+extern "C" int bfe_create(int device, bfe_ctx** out) {
+  if (!out) return set_error(nullptr, BFE_EINVAL, "bfe_create: out is NULL");
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0)
+    return set_error(nullptr, BFE_EHIP, "bfe_create: no HIP device (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= ndev)
+    return set_error(nullptr, BFE_EINVAL, "bfe_create: device %d out of range [0,%d)", device, ndev);
+  e = hipSetDevice(device);
+  if (e != hipSuccess)
+    return set_error(nullptr, BFE_EHIP, "bfe_create: hipSetDevice: %s", hipGetErrorString(e));
+  bfe_ctx* h = new bfe_ctx();
+  h->c.device = device;
+  *out = h;
+  return 0;
+}
+
+extern "C" int bfe_destroy(bfe_ctx* h) {
+  if (!h) return 0;
+  Context* c = &h->c;
+  (void)hipSetDevice(c->device);
+  (void)hipDeviceSynchronize();
+  free_sl(c);
+  free_eof(c);
+  if (c->scratch) (void)hipFree(c->scratch);
+  delete h;
+  return 0;
+}
+
+extern "C" const char* bfe_last_error(const bfe_ctx* h) {
+  return h ? h->c.err.c_str() : g_create_err.c_str();
+}
+
+// Builds the node-major float32 table ef/sqrt(ev) used by accumulate.
+extern "C" int bfe_sl_tables_set(bfe_ctx* h, int lmax, int nmax, int numr, int cmap, double scale,
+                                 const double* xi, const double* evtable, const double* eftable,
+                                 const double* p0, const double* d0) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (lmax < 0 || lmax > kMaxL) return set_error(c, BFE_ELIMIT, "sl: lmax=%d outside [0,%d]", lmax, kMaxL);
+  if (nmax < 1 || nmax > 64) return set_error(c, BFE_ELIMIT, "sl: nmax=%d outside [1,64]", nmax);
+  if (numr < 3) return set_error(c, BFE_EINVAL, "sl: numr=%d < 3", numr);
+  if (cmap < 0 || cmap > 2) return set_error(c, BFE_EINVAL, "sl: cmap=%d", cmap);
+  if (!xi || !evtable || !eftable || !p0 || !d0) return set_error(c, BFE_EINVAL, "sl: NULL table");
+  if (!(xi[1] > xi[0])) return set_error(c, BFE_EINVAL, "sl: xi must increase");
+  for (int i = 0; i < (lmax + 1) * nmax; ++i)
+    if (!(evtable[i] > 0.0)) return set_error(c, BFE_EINVAL, "sl: evtable[%d] <= 0", i);
+  BFE_HIP(c, hipSetDevice(c->device));
+  free_sl(c);
+  const size_t LN = (size_t)(lmax + 1) * nmax;
+  std::vector<float> e32(LN * numr);
+  for (size_t ln = 0; ln < LN; ++ln) {
+    double inv = 1.0 / sqrt(evtable[ln]);
+    for (int i = 0; i < numr; ++i) e32[(size_t)i * LN + ln] = (float)(eftable[ln * numr + i] * inv);
+  }
+  int rc;
+  if ((rc = dev_upload(c, &c->sl_xi, xi, numr))) return rc;
+  if ((rc = dev_upload(c, &c->sl_p0, p0, numr))) return rc;
+  if ((rc = dev_upload(c, &c->sl_d0, d0, numr))) return rc;
+  if ((rc = dev_upload(c, &c->sl_ev64, evtable, LN))) return rc;
+  if ((rc = dev_upload(c, &c->sl_ef64, eftable, LN * numr))) return rc;
+  if ((rc = dev_upload(c, &c->sl_e32, e32.data(), e32.size()))) return rc;
+  c->h_xi.assign(xi, xi + numr);
+  c->lmax = lmax; c->nmax = nmax; c->numr = numr; c->cmap = cmap; c->scale = scale;
+  c->sl_ok = true;
+  return 0;
+}
+
+// Reorders (mmax+1, norder, numx+1, numy+1) C and S tables into node-major
+// [node][K] rows with K = (2 mmax + 1) norder: cos m=0..mmax, then sin m=1..mmax.
+extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, int numy, int cmap,
+                                  double ascale, double hscale, double xmin, double dx, double ymin,
+                                  double dy, const double* potC, const double* rforceC,
+                                  const double* zforceC, const double* densC, const double* potS,
+                                  const double* rforceS, const double* zforceS, const double* densS) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (mmax < 0 || mmax > kMaxM) return set_error(c, BFE_ELIMIT, "eof: mmax=%d outside [0,%d]", mmax, kMaxM);
+  if (norder < 1 || norder > 64) return set_error(c, BFE_ELIMIT, "eof: norder=%d outside [1,64]", norder);
+  if (numx < 1 || numy < 1) return set_error(c, BFE_EINVAL, "eof: numx/numy < 1");
+  if (cmap < 0 || cmap > 2) return set_error(c, BFE_EINVAL, "eof: cmap=%d", cmap);
+  if (!potC || !rforceC || !zforceC || !potS || !rforceS || !zforceS)
+    return set_error(c, BFE_EINVAL, "eof: NULL table");
+  if (!(dx > 0) || !(dy > 0)) return set_error(c, BFE_EINVAL, "eof: dx, dy must be > 0");
+  const bool dens = densC && densS;
+  const int K = (2 * mmax + 1) * norder;
+  if (K > 1024) return set_error(c, BFE_ELIMIT, "eof: (2 mmax+1) norder = %d > 1024", K);
+  BFE_HIP(c, hipSetDevice(c->device));
+  free_eof(c);
+  const size_t nodes = (size_t)(numx + 1) * (numy + 1);
+  const double* srcC[4] = {potC, rforceC, zforceC, densC};
+  const double* srcS[4] = {potS, rforceS, zforceS, densS};
+  std::vector<double> buf(nodes * K);
+  std::vector<float> pot32;
+  for (int q = 0; q < (dens ? 4 : 3); ++q) {
+    for (int m = 0; m <= mmax; ++m)
+      for (int n = 0; n < norder; ++n) {
+        const double* s = srcC[q] + ((size_t)m * norder + n) * nodes;
+        const int k = m * norder + n;
+        for (size_t node = 0; node < nodes; ++node) buf[node * K + k] = s[node];
+      }
+    for (int m = 1; m <= mmax; ++m)
+      for (int n = 0; n < norder; ++n) {
+        const double* s = srcS[q] + ((size_t)m * norder + n) * nodes;
+        const int k = (mmax + m) * norder + n;
+        for (size_t node = 0; node < nodes; ++node) buf[node * K + k] = s[node];
+      }
+    int rc = dev_upload(c, &c->eof64[q], buf.data(), buf.size());
+    if (rc) return rc;
+    if (q == 0) {
+      pot32.resize(buf.size());
+      for (size_t i = 0; i < buf.size(); ++i) pot32[i] = (float)buf[i];
+      if ((rc = dev_upload(c, &c->eof_pot32, pot32.data(), pot32.size()))) return rc;
+    }
+  }
+  c->mmax = mmax; c->norder = norder; c->numx = numx; c->numy = numy; c->ecmap = cmap;
+  c->ascale = ascale; c->hscale = hscale; c->xmin = xmin; c->dx = dx; c->ymin = ymin; c->dy = dy;
+  c->eof_K = K;
+  c->eof_dens = dens;
+  c->eof_ok = true;
+  return 0;
+}

@@ -1,0 +1,118 @@
+// Internal state of the BFE library: the context, device table layouts and
+// the error helpers shared by the translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <string>
+#include <vector>
+
+#include "bfe_math.hpp"
+#include "../../include/exptool_bfe.h"
+
+namespace bfe {
+
+// Device limits of the kernels (checked by the host before every launch).
+constexpr int kMaxL = 16;          // SL lmax
+constexpr int kMaxM = 16;          // EOF mmax
+constexpr int kMaxLM = (kMaxL + 1) * (kMaxL + 2) / 2;
+
+// A folded SL table: per radial node, 4 groups of R = (lmax+1)^2 rows:
+//   E  = sum_n c N_lm ef/sqrt(ev)          (potential, times p0-interp)
+//   D1 = (G_{i+1} - G_{i-1})/2, D2 = G_{i-1} - 2G_i + G_{i+1},
+//        G = sum_n c N_lm ef p0/sqrt(ev)    (radial derivative stencil)
+//   Dn = sum_n c N_lm ef sqrt(ev)          (density, times d0-interp)
+// Row stride padded to a multiple of 4 floats.
+struct SLFolded {
+  float* d = nullptr;
+  int rows = 0, stride = 0;   // stride = floats per group
+  int groups = 0;             // 3 (no density) or 4
+  int lmax_use = 0;
+  bool valid = false;
+};
+
+// A folded EOF table: per grid node, groups pot, rforce, zforce[, dens], each
+// of 2*mmax+1 entries (cos m=0..M, then sin m=1..M), padded to 4 floats.
+struct EOFFolded {
+  float* d = nullptr;
+  int q = 0;        // 2*mmax+1
+  int groups = 0;   // 3 or 4
+  int stride = 0;   // floats per node
+  bool valid = false;
+};
+
+struct Context {
+  int device = 0;
+  std::string err;
+
+  // ---- SL tables ---------------------------------------------------------
+  bool sl_ok = false;
+  int lmax = 0, nmax = 0, numr = 0, cmap = 0;
+  double scale = 1.0;
+  double* sl_xi = nullptr;     // numr
+  double* sl_p0 = nullptr;     // numr
+  double* sl_d0 = nullptr;     // numr
+  double* sl_ef64 = nullptr;   // (lmax+1, nmax, numr) float64 master copy
+  double* sl_ev64 = nullptr;   // (lmax+1, nmax)
+  float* sl_e32 = nullptr;     // [numr][(lmax+1)*nmax]  ef/sqrt(ev), node-major
+  std::vector<double> h_xi;    // host copy (min xi for the field guard)
+
+  // ---- EOF tables --------------------------------------------------------
+  bool eof_ok = false;
+  bool eof_dens = false;
+  int mmax = 0, norder = 0, numx = 0, numy = 0, ecmap = 0;
+  double ascale = 1.0, hscale = 1.0, xmin = 0, dx = 1, ymin = 0, dy = 1;
+  int eof_K = 0;               // (2*mmax+1)*norder rows of the node-major tables
+  double* eof64[4] = {nullptr, nullptr, nullptr, nullptr};  // pot, rf, zf, dens: [node][K]
+  float* eof_pot32 = nullptr;  // [node][K] potential for accumulate
+
+  // ---- folded tables: slot 0 = batch evaluators, slot 1 = frozen field ----
+  SLFolded slf[2];
+  EOFFolded eoff[2];
+  double field_halofac = 1.0;
+  int field_flags = 0;
+
+  // ---- scratch (grown on demand, freed by bfe_destroy) -------------------
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+// Table geometry passed to kernels by value.
+struct SLGeom {
+  const double* xi;
+  const double* p0;
+  const double* d0;
+  int numr, cmap, lmax, nmax;
+  double scale;
+};
+
+struct EOFGeom {
+  int cmap, numx, numy, mmax, norder;
+  double ascale, hscale, xmin, dx, ymin, dy;
+};
+
+inline SLGeom sl_geom(const Context* c) {
+  return SLGeom{c->sl_xi, c->sl_p0, c->sl_d0, c->numr, c->cmap, c->lmax, c->nmax, c->scale};
+}
+inline EOFGeom eof_geom(const Context* c) {
+  return EOFGeom{c->ecmap, c->numx, c->numy, c->mmax, c->norder,
+                 c->ascale, c->hscale, c->xmin, c->dx, c->ymin, c->dy};
+}
+
+int set_error(Context* c, int code, const char* fmt, ...);
+int check_hip(Context* c, hipError_t e, const char* what);
+int ensure_scratch(Context* c, size_t bytes);
+
+#define BFE_HIP(ctx, call)                                   \
+  do {                                                       \
+    hipError_t _e = (call);                                  \
+    if (_e != hipSuccess) return check_hip((ctx), _e, #call); \
+  } while (0)
+
+}  // namespace bfe
+
+struct bfe_ctx {
+  bfe::Context c;
+};

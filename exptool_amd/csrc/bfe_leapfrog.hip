@@ -1,0 +1,267 @@
+// Orbit integration in the frozen disc+halo field (integrate.py:53-190).
+//
+// One lane per orbit.  A launch advances every orbit by a chunk of steps with
+// the state (position, velocity, acceleration) in registers; the field is
+// read from the folded tables, which stay resident in L2 / Infinity Cache.
+// Chunks end on energy checkpoints; the float64 energy 0.5 v^2 + Phi of each
+// orbit is compared with its initial value and per-block totals are kept for
+// a deterministic energy log.  Optionally the chunk launches are captured in
+// a hipGraph and replayed as one unit.
+#include <math.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "bfe_field.hpp"
+
+namespace bfe {
+
+int field_tables(Context* c, double rotpos, FieldTables& T);
+
+struct OrbitAux {
+  float ax, ay, az, pad;
+  double e0, drift;
+};
+
+struct LeapArgs {
+  double dt, t0, omega;  // omega = 2 pi rotfreq
+  int64_t n;
+  int64_t k0, k1;        // steps (k0, k1] are taken; step k is at time t0 + k dt
+  int energy_every;      // 0 = no checks
+  int check0;            // index of the checkpoint at step k0
+  int record_every;      // 0 = no trajectory
+  int nblocks;
+};
+
+__device__ __forceinline__ void rot_of(const LeapArgs& a, int64_t k, FieldTables& T) {
+  if (a.omega != 0.0) {
+    const double ang = a.omega * (a.t0 + (double)k * a.dt);
+    T.crot = cos(ang);
+    T.srot = sin(ang);
+  } else {
+    T.crot = 1.0;
+    T.srot = 0.0;
+  }
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (tid < w) sh[tid] += sh[tid + w];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Step 0: initial acceleration, energy and trajectory record.
+__global__ __launch_bounds__(256) void leapfrog_init_kernel(
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+    const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz,
+    FieldTables T, LeapArgs a, OrbitAux* __restrict__ aux, double* __restrict__ partE,
+    float* __restrict__ traj) {
+  __shared__ double sh[256];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (i < a.n) {
+    rot_of(a, a.k0, T);
+    double cv[8];
+    field_eval_point(T, x[i], y[i], z[i], cv);
+    const float fx = (float)(cv[0] + cv[1]), fy = (float)(cv[2] + cv[3]), fz = (float)(cv[4] + cv[5]);
+    const float pot = (float)(cv[6] + cv[7]);
+    const double v2 = (double)vx[i] * vx[i] + (double)vy[i] * vy[i] + (double)vz[i] * vz[i];
+    e = 0.5 * v2 + (double)pot;
+    OrbitAux o;
+    o.ax = fx; o.ay = fy; o.az = fz; o.pad = 0.f;
+    o.e0 = e;
+    o.drift = 0.0;
+    aux[i] = o;
+    if (traj) {
+      const size_t n = (size_t)a.n;
+      float* t = traj;  // record 0
+      t[0 * n + i] = x[i]; t[1 * n + i] = y[i]; t[2 * n + i] = z[i];
+      t[3 * n + i] = vx[i]; t[4 * n + i] = vy[i]; t[5 * n + i] = vz[i];
+      t[6 * n + i] = pot;
+    }
+  }
+  if (partE) {
+    const double s = block_sum(e, sh);
+    if (threadIdx.x == 0) partE[(size_t)a.check0 * a.nblocks + blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void leapfrog_chunk_kernel(
+    float* __restrict__ x, float* __restrict__ y, float* __restrict__ z, float* __restrict__ vx,
+    float* __restrict__ vy, float* __restrict__ vz, FieldTables T, LeapArgs a,
+    OrbitAux* __restrict__ aux, double* __restrict__ partE, float* __restrict__ traj) {
+  __shared__ double sh[256];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < a.n;
+  float px = 0, py = 0, pz = 0, ux = 0, uy = 0, uz = 0, ax = 0, ay = 0, az = 0;
+  OrbitAux o{};
+  if (live) {
+    px = x[i]; py = y[i]; pz = z[i];
+    ux = vx[i]; uy = vy[i]; uz = vz[i];
+    o = aux[i];
+    ax = o.ax; ay = o.ay; az = o.az;
+  }
+  const float dt = (float)a.dt;
+  const float hdt2 = 0.5f * dt * dt;
+  const float hdt = 0.5f * dt;
+  float pot = 0.f;
+  for (int64_t k = a.k0 + 1; k <= a.k1; ++k) {
+    if (live) {
+      // integrate.py:129-143
+      px = px + ux * dt + ax * hdt2;
+      py = py + uy * dt + ay * hdt2;
+      pz = pz + uz * dt + az * hdt2;
+      rot_of(a, k, T);
+      double cv[8];
+      field_eval_point(T, px, py, pz, cv);
+      const float nx = (float)(cv[0] + cv[1]), ny = (float)(cv[2] + cv[3]), nz = (float)(cv[4] + cv[5]);
+      pot = (float)(cv[6] + cv[7]);
+      ux = ux + (ax + nx) * hdt;
+      uy = uy + (ay + ny) * hdt;
+      uz = uz + (az + nz) * hdt;
+      ax = nx; ay = ny; az = nz;
+      if (traj && a.record_every > 0 && (k % a.record_every) == 0) {
+        const size_t n = (size_t)a.n;
+        float* t = traj + (size_t)(k / a.record_every) * 7 * n;
+        t[0 * n + i] = px; t[1 * n + i] = py; t[2 * n + i] = pz;
+        t[3 * n + i] = ux; t[4 * n + i] = uy; t[5 * n + i] = uz;
+        t[6 * n + i] = pot;
+      }
+    }
+  }
+  double e = 0.0;
+  if (live) {
+    x[i] = px; y[i] = py; z[i] = pz;
+    vx[i] = ux; vy[i] = uy; vz[i] = uz;
+    o.ax = ax; o.ay = ay; o.az = az;
+    if (a.energy_every > 0 && a.k1 > a.k0) {
+      e = 0.5 * ((double)ux * ux + (double)uy * uy + (double)uz * uz) + (double)pot;
+      const double rel = fabs(e - o.e0) / fabs(o.e0);
+      if (rel > o.drift || !(rel == rel)) o.drift = rel;
+    }
+    aux[i] = o;
+  }
+  if (partE && a.energy_every > 0 && a.k1 > a.k0) {
+    const double s = block_sum(e, sh);
+    if (threadIdx.x == 0) partE[(size_t)(a.check0 + 1) * a.nblocks + blockIdx.x] = s;
+  }
+}
+
+__global__ void leapfrog_finish_kernel(const OrbitAux* __restrict__ aux, int64_t n,
+                                       double* __restrict__ drift_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) drift_out[i] = aux[i].drift;
+}
+
+__global__ void energy_reduce_kernel(const double* __restrict__ partE, int nchecks, int nblocks,
+                                     double* __restrict__ log_out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchecks) return;
+  double s = 0.0, comp = 0.0;
+  for (int b = 0; b < nblocks; ++b) {
+    const double yv = partE[(size_t)c * nblocks + b] - comp;
+    const double tv = s + yv;
+    comp = (tv - s) - yv;
+    s = tv;
+  }
+  log_out[c] = s;
+}
+
+}  // namespace bfe
+
+using namespace bfe;
+
+extern "C" int bfe_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float* vx, float* vy,
+                            float* vz, int64_t n, double dt, int64_t nsteps, double t0,
+                            double rotfreq, int energy_every, double* drift_out,
+                            double* energy_log, int record_every, float* traj, int use_graph,
+                            void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (n < 0 || nsteps < 0) return set_error(c, BFE_EINVAL, "leapfrog: n or nsteps < 0");
+  if (n > 0 && (!x || !y || !z || !vx || !vy || !vz)) return set_error(c, BFE_EINVAL, "leapfrog: NULL state");
+  if (energy_every < 0 || record_every < 0) return set_error(c, BFE_EINVAL, "leapfrog: negative interval");
+  if (traj && record_every == 0) return set_error(c, BFE_EINVAL, "leapfrog: traj needs record_every > 0");
+  if (energy_log && energy_every == 0) return set_error(c, BFE_EINVAL, "leapfrog: energy_log needs energy_every > 0");
+  FieldTables T;
+  int rc = field_tables(c, 0.0, T);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  BFE_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = (int)((n + 255) / 256);
+  const int chunk = energy_every > 0 ? energy_every : (int)std::min<int64_t>(nsteps > 0 ? nsteps : 1, 100);
+  const int nchecks = energy_every > 0 ? (int)(nsteps / energy_every) + 1 : 1;
+  const size_t aux_bytes = (size_t)n * sizeof(OrbitAux);
+  const size_t part_bytes = (size_t)nchecks * nb * sizeof(double);
+  rc = ensure_scratch(c, aux_bytes + part_bytes + 256);
+  if (rc) return rc;
+  OrbitAux* aux = (OrbitAux*)c->scratch;
+  double* partE = (double*)((char*)c->scratch + ((aux_bytes + 255) & ~(size_t)255));
+  LeapArgs a;
+  a.dt = dt;
+  a.t0 = t0;
+  a.omega = 2.0 * kPi * rotfreq;
+  a.n = n;
+  a.k0 = 0;
+  a.k1 = 0;
+  a.energy_every = energy_every;
+  a.check0 = 0;
+  a.record_every = record_every;
+  a.nblocks = nb;
+  double* pe = energy_log ? partE : nullptr;
+  hipLaunchKernelGGL(leapfrog_init_kernel, dim3(nb), dim3(256), 0, s, x, y, z, vx, vy, vz, T, a,
+                     aux, pe, traj);
+  BFE_HIP(c, hipGetLastError());
+
+  std::vector<LeapArgs> chunks;
+  for (int64_t k0 = 0, ci = 0; k0 < nsteps; k0 += chunk, ++ci) {
+    LeapArgs b = a;
+    b.k0 = k0;
+    b.k1 = std::min<int64_t>(nsteps, k0 + chunk);
+    b.check0 = (int)ci;
+    // a partial final chunk is not a checkpoint (nsteps % energy_every != 0)
+    if (energy_every > 0 && (b.k1 - b.k0) != energy_every) b.energy_every = 0;
+    chunks.push_back(b);
+  }
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (use_graph) {
+    if (s == nullptr) return set_error(c, BFE_EINVAL, "leapfrog: use_graph needs a non-default stream");
+    BFE_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  }
+  for (const LeapArgs& b : chunks)
+    hipLaunchKernelGGL(leapfrog_chunk_kernel, dim3(nb), dim3(256), 0, s, x, y, z, vx, vy, vz, T, b,
+                       aux, pe, traj);
+  if (use_graph) {
+    hipError_t e = hipStreamEndCapture(s, &graph);
+    if (e != hipSuccess) return check_hip(c, e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return check_hip(c, e, "hipGraphInstantiate");
+    }
+    e = hipGraphLaunch(exec, s);
+    if (e != hipSuccess) return check_hip(c, e, "hipGraphLaunch");
+    // the graph is released after the stream has drained it
+    e = hipStreamSynchronize(s);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return check_hip(c, e, "hipStreamSynchronize (graph)");
+  }
+  BFE_HIP(c, hipGetLastError());
+  if (drift_out)
+    hipLaunchKernelGGL(leapfrog_finish_kernel, dim3(nb), dim3(256), 0, s, aux, n, drift_out);
+  if (energy_log)
+    hipLaunchKernelGGL(energy_reduce_kernel, dim3((nchecks + 63) / 64), dim3(64), 0, s, partE,
+                       nchecks, nb, energy_log);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
+}

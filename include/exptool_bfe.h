@@ -1,0 +1,141 @@
+/* exptool_bfe.h -- C ABI of the MI355X basis-function-expansion library.
+ *
+ * The reference (exptool) has no FFI on this path: its interface is plain
+ * Python functions over NumPy arrays.  Each entry point below replaces the
+ * reference function cited beside it; the Python package exptool_amd keeps the
+ * reference signatures and calls these through ctypes.
+ *
+ * Conventions
+ *  - Particle arrays (x, y, z, m, velocities) and outputs are DEVICE pointers
+ *    owned by the caller (float32 SoA, contiguous).  Coefficient arrays are
+ *    DEVICE float64 pointers.  Table arrays passed to *_tables_set are HOST
+ *    float64 pointers; the context copies them to device memory it owns.
+ *  - Every call returns 0 on success and a negative BFE_E* code on error; the
+ *    message is available from bfe_last_error().  No exception crosses the ABI.
+ *  - Work is enqueued on `stream` (a hipStream_t; NULL = default stream).
+ *    Calls are asynchronous unless stated otherwise.
+ *  - One context per host thread; a context is bound to one device.
+ *  - Results are deterministic: reductions use fixed orders, no float atomics.
+ */
+#ifndef EXPTOOL_BFE_H
+#define EXPTOOL_BFE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bfe_ctx bfe_ctx;
+
+/* error codes */
+#define BFE_OK 0
+#define BFE_EINVAL (-1)   /* bad argument / shape */
+#define BFE_ENOTABLE (-2) /* tables or field not set */
+#define BFE_EHIP (-3)     /* HIP runtime error */
+#define BFE_ENOMEM (-4)
+#define BFE_ELIMIT (-5)   /* size beyond a kernel limit */
+
+/* flags (bit mask) */
+#define BFE_COMPAT_SL_COS_L 0x1u   /* spheresl.force_eval cos/sin(l phi), spheresl.py:1222-1225 */
+#define BFE_COMPAT_SL_EDGE 0x2u    /* first-cell derivative stencil, spheresl.py:150-153 */
+#define BFE_COMPAT_SL_DENS 0x4u    /* all_eval_particles den1 P_1^m + 0.25*pi, spheresl.py:1323,1351 */
+#define BFE_COMPAT_EOF_MASK 0x8u   /* eof.force_eval odd-m mask |cos(pi m/2)|, eof.py:833 */
+#define BFE_NO_ODD 0x10u           /* skip odd l (SL) / odd m (EOF eval) */
+#define BFE_COMPAT_ALL 0xFu
+
+/* ---- context ------------------------------------------------------------ */
+/* replaces: the Pool set-up of spheresl.compute_coefficients (spheresl.py:456)
+ * and eof.compute_coefficients (eof.py:1215); there is no per-call process. */
+int bfe_create(int device, bfe_ctx** out);
+int bfe_destroy(bfe_ctx* ctx);
+/* last error message of ctx (or of the last failed bfe_create when ctx=NULL) */
+const char* bfe_last_error(const bfe_ctx* ctx);
+
+/* ---- tables ------------------------------------------------------------- */
+/* replaces: halo_methods.read_cached_table + init_table as consumed by
+ * spheresl.compute_coefficients_solitary (spheresl.py:587-588).
+ * evtable (lmax+1, nmax), eftable (lmax+1, nmax, numr), xi/p0/d0 (numr). */
+int bfe_sl_tables_set(bfe_ctx* ctx, int lmax, int nmax, int numr, int cmap, double scale,
+                      const double* xi, const double* evtable, const double* eftable,
+                      const double* p0, const double* d0);
+
+/* replaces: eof.parse_eof + eof_params + set_table_params as consumed by
+ * eof.accumulate / force_eval (eof.py:1225-1227).  Each table is
+ * (mmax+1, norder, numx+1, numy+1) float64; S tables have an unused m=0 slab.
+ * densC/densS may be NULL. */
+int bfe_eof_tables_set(bfe_ctx* ctx, int mmax, int norder, int numx, int numy, int cmap,
+                       double ascale, double hscale, double xmin, double dx, double ymin,
+                       double dy, const double* potC, const double* rforceC,
+                       const double* zforceC, const double* densC, const double* potS,
+                       const double* rforceS, const double* zforceS, const double* densS);
+
+/* ---- accumulate --------------------------------------------------------- */
+/* replaces: spheresl.compute_coefficients_solitary (spheresl.py:567-656) and
+ * its Pool driver multi_compute_coefficients + np.sum (spheresl.py:411-471).
+ * coef_out: device ((lmax+1)^2, nmax) float64.  flags: BFE_NO_ODD. */
+int bfe_sl_accumulate(bfe_ctx* ctx, const float* x, const float* y, const float* z,
+                      const float* m, int64_t n, uint32_t flags, double* coef_out,
+                      void* stream);
+
+/* replaces: eof.accumulate (eof.py:492-640) and make_coefficients_multi
+ * (eof.py:1415-1455).  cos_out/sin_out: device (mmax+1, norder) float64. */
+int bfe_eof_accumulate(bfe_ctx* ctx, const float* x, const float* y, const float* z,
+                       const float* m, int64_t n, uint32_t flags, double* cos_out,
+                       double* sin_out, void* stream);
+
+/* ---- batch evaluation --------------------------------------------------- */
+/* replaces: spheresl.all_eval_particles (spheresl.py:1240-1362) /
+ * eval_particles (spheresl.py:502-528).  coef: device ((lmax+1)^2, nmax).
+ * Terms with l outside [L1, L2] are skipped (l >= 1).  Any output may be NULL.
+ * flags: BFE_NO_ODD, BFE_COMPAT_SL_DENS, BFE_COMPAT_SL_EDGE. */
+int bfe_sl_eval(bfe_ctx* ctx, const float* x, const float* y, const float* z, int64_t n,
+                const double* coef, int L1, int L2, uint32_t flags, float* den0, float* den1,
+                float* pot0, float* pot1, float* potr, float* pott, float* potp, void* stream);
+
+/* replaces: eof.accumulated_eval_particles (eof.py:989-1144) / compute_forces
+ * (eof.py:1263-1317), with the intended semantics (the reference cannot run).
+ * Orders m outside [m1, m2] are skipped.  d0/d need density tables. */
+int bfe_eof_eval(bfe_ctx* ctx, const float* x, const float* y, const float* z, int64_t n,
+                 const double* cos_coef, const double* sin_coef, int m1, int m2,
+                 uint32_t flags, float* p0, float* p, float* fr, float* fp, float* fz,
+                 float* d0, float* d, void* stream);
+
+/* ---- frozen disc+halo field -------------------------------------------- */
+/* replaces: Fields.prep_tables + set_field_parameters (potential.py:257-294,
+ * :365-378).  Folds the coefficients into the tables (exact linear rewrite).
+ * disk_m/disk_n/halo_l/halo_n < 0 mean "all".  flags: BFE_NO_ODD and the
+ * BFE_COMPAT_* bits of force_eval. */
+int bfe_field_set(bfe_ctx* ctx, const double* eof_cos, const double* eof_sin,
+                  const double* sl_coef, double halofac, int disk_m, int disk_n, int halo_l,
+                  int halo_n, uint32_t flags, void* stream);
+
+/* replaces: Fields.return_forces_cart (potential.py:445-497) over many points.
+ * fx/fy/fz/pot receive the disc+halo sums; comp (may be NULL) receives the 8
+ * reference outputs as [8][n]: fx_d, fx_h, fy_d, fy_h, fz_d, fz_h, pot_d, pot_h. */
+int bfe_field_eval_cart(bfe_ctx* ctx, const float* x, const float* y, const float* z,
+                        int64_t n, double rotpos, float* fx, float* fy, float* fz,
+                        float* pot, float* comp, void* stream);
+
+/* ---- orbit integration -------------------------------------------------- */
+/* replaces: utils/integrate.leapfrog_integrate (integrate.py:53-190) for many
+ * orbits at once.  State (x..vz) is advanced in place by nsteps steps of
+ * x += v dt + a dt^2/2; a' = F(x', 2 pi rotfreq t); v += (a + a') dt/2,
+ * starting at time t0.  Every `energy_every` steps (0 = never) and at the
+ * start, the float64 energy 0.5 v^2 + Phi of every orbit is compared with its
+ * initial value; drift_out (device float64 [n], may be NULL) receives
+ * max |E - E0| / |E0| per orbit and energy_log (device float64
+ * [nsteps/energy_every + 1], may be NULL) the total energy at each check.
+ * traj (device float32 [nrec][7][n], may be NULL) records t-ordered
+ * (x, y, z, vx, vy, vz, Phi) every record_every steps (step 0 included).
+ * use_graph: capture the chunk launches in a hipGraph and replay it. */
+int bfe_leapfrog(bfe_ctx* ctx, float* x, float* y, float* z, float* vx, float* vy, float* vz,
+                 int64_t n, double dt, int64_t nsteps, double t0, double rotfreq,
+                 int energy_every, double* drift_out, double* energy_log, int record_every,
+                 float* traj, int use_graph, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EXPTOOL_BFE_H */

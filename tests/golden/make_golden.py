@@ -1,0 +1,225 @@
+"""Write golden input/output vectors from the reference exptool (this container only).
+
+    PYTHONDONTWRITEBYTECODE=1 python tests/golden/make_golden.py [--ref /root/reference]
+
+The reference's arithmetic quirks (first-cell derivative stencil, cos(l phi)
+in spheresl.force_eval, EOF upper-edge extrapolation, den1/densfac, the
+|cos(pi m/2)| mask, the Fields guard and epsilons, the leapfrog update) are
+fully specified by its source text, but only its own arithmetic confirms that
+the oracle restates them correctly.  This script runs the reference's leaf
+functions on small synthetic tables (written in the EXP formats by
+exptool_amd's writers) and on seeded particles, and stores inputs and outputs
+as .npz fixtures next to this file.  The fixtures are data; no reference code
+is stored.  tests/test_golden.py checks the oracle and the readers against
+them, and the GPU tests check the device path against the oracle.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+
+from exptool_amd.basis import tablegen as tg  # noqa: E402
+from exptool_amd.basis import eof as my_eof  # noqa: E402
+from exptool_amd.utils import halo_methods as my_hm  # noqa: E402
+from exptool_amd.utils import samplers  # noqa: E402
+
+
+def small_tables():
+    sl = tg.make_sl_tables(lmax=3, nmax=5, numr=64, rmin=1e-3, rmax=20.0, cmap=1, scale=1.0)
+    x, y, z, m = samplers.exponential_disc(20000, rd=1.0, zd=0.1, rmax=10.0, seed=11)
+    eof = tg.make_eof_tables((x, y, z, m), mmax=2, norder=4, numx=16, numy=8, rmin=0.001,
+                             rmax=20.0, ascale=1.0, hscale=0.1, cmap=1, lmax_s=10, nmax_s=6)
+    return sl, eof
+
+
+def edge_particles(g, n):
+    """Seeded particles plus the edge cases of the path."""
+    x = g.normal(0, 2.0, n); y = g.normal(0, 2.0, n); z = g.normal(0, 0.5, n)
+    extra = np.array([
+        [0.0, 0.0, 0.0],        # r = 0
+        [0.0, 0.0, 0.7],        # on axis, cos theta = 1
+        [0.0, 0.0, -1.3],       # on axis, cos theta = -1
+        [1e-4, 0.0, 0.0],       # inside the first SL cell
+        [13.9, 0.5, 0.01],      # near the EOF outer edge
+        [30.0, -5.0, 0.2],      # beyond the EOF table (extrapolation)
+        [2.0, 1.0, 25.0],       # beyond the EOF table in z
+        [-3.0, -0.0, 0.0],      # phi = pi
+        [0.3, -0.2, 1e-9],      # just off the plane
+        [150.0, 40.0, -60.0],   # far outside everything
+    ])
+    x = np.concatenate([x, extra[:, 0]]); y = np.concatenate([y, extra[:, 1]])
+    z = np.concatenate([z, extra[:, 2]])
+    m = g.uniform(0.5, 1.5, x.size) / x.size
+    return x, y, z, m
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    a = ap.parse_args()
+    sys.path.insert(0, a.ref)
+    from exptool.basis import spheresl as R_sl  # noqa: E402
+    from exptool.basis import eof as R_eof  # noqa: E402
+    from exptool.utils import halo_methods as R_hm  # noqa: E402
+    from exptool.io import particle as R_particle  # noqa: E402
+
+    g = np.random.Generator(np.random.PCG64(20241202))
+    sl, eof = small_tables()
+    tmp = tempfile.mkdtemp(prefix="golden_")
+    sph_file = os.path.join(tmp, "SLGridSph.cache")
+    mod_file = os.path.join(tmp, "SLGridSph.model")
+    eof_file = os.path.join(tmp, "eof.cache")
+    eof_file_old = os.path.join(tmp, "eof_old.cache")
+    my_hm.save_sl(sl, sph_file, mod_file)
+    my_eof.write_eof_cache(eof_file, eof, newstyle=True)
+    my_eof.write_eof_cache(eof_file_old, eof, newstyle=False)
+
+    out = {}
+    # ---------------- table readers -------------------------------------
+    lmax, nmax, numr, cmap, rmin, rmax, scale, ltable, ev, ef = R_hm.read_cached_table(sph_file)
+    xi, rarr, p0, d0 = R_hm.init_table(mod_file, numr, rmin, rmax, cmap=cmap, scale=scale)
+    out.update(sl_lmax=lmax, sl_nmax=nmax, sl_numr=numr, sl_cmap=cmap, sl_rmin=rmin,
+               sl_rmax=rmax, sl_scale=scale, sl_ev=ev, sl_ef=ef, sl_xi=xi, sl_r=rarr, sl_p0=p0,
+               sl_d0=d0)
+    eofs = R_eof.parse_eof(eof_file)
+    eofs_old = R_eof.parse_eof(eof_file_old)
+    prm = R_eof.eof_params(eof_file)
+    prm_old = R_eof.eof_params(eof_file_old)
+    names = ("potC", "rforceC", "zforceC", "densC", "potS", "rforceS", "zforceS", "densS")
+    for k, v, vo in zip(names, eofs, eofs_old):
+        out["eof_" + k] = v
+        assert np.array_equal(v, vo)
+    out["eof_params"] = np.array([float(v) for v in prm])
+    out["eof_params_old"] = np.array([float(v) for v in prm_old])
+    rmin_e, rmax_e, numx, numy, mmax, norder, ascale, hscale, ecmap, dens = prm
+    XMIN, XMAX, dX, YMIN, YMAX, dY = R_eof.set_table_params(RMAX=rmax_e, RMIN=rmin_e, ASCALE=ascale,
+                                                           HSCALE=hscale, NUMX=numx, NUMY=numy,
+                                                           CMAP=ecmap)
+    out["eof_grid"] = np.array([XMIN, XMAX, dX, YMIN, YMAX, dY])
+
+    # ---------------- Legendre and normalisation -------------------------
+    xs = np.concatenate([np.linspace(-1, 1, 41), [1 - 1e-9, -1 + 1e-9, 0.3333, -0.77]])
+    P = np.stack([R_sl.legendre_R(6, v) for v in xs])
+    PD = [R_sl.dlegendre_R(6, v) for v in xs]
+    out.update(leg_x=xs, leg_P=P, dleg_P=np.stack([p for p, _ in PD]),
+               dleg_dP=np.stack([d for _, d in PD]), factorial6=R_sl.factorial_return(6))
+
+    # ---------------- SL radial table functions ---------------------------
+    rs = np.concatenate([g.uniform(1e-3, 25.0, 60), [0.0, 1e-4, 1e-3, 1.2e-3, 19.99, 20.0, 40.0]])
+    out["sl_rs"] = rs
+    out["sl_potmat"] = np.stack([R_sl.get_halo_pot_matrix(r, lmax, nmax, ev, ef, xi, p0, cmap, scale)
+                                 for r in rs])
+    dpf = [R_sl.get_halo_dens_pot_force(r, lmax, nmax, ev, ef, xi, d0, p0, cmap, scale) for r in rs]
+    out["sl_dens"] = np.stack([d for d, _, _ in dpf])
+    out["sl_dpot"] = np.stack([f for _, f, _ in dpf])
+    out["sl_pot"] = np.stack([p for _, _, p in dpf])
+
+    # ---------------- SL accumulate / evaluate ----------------------------
+    x, y, z, m = edge_particles(g, 400)
+    out.update(px=x, py=y, pz=z, pm=m)
+    H = R_particle.holder()
+    H.xpos, H.ypos, H.zpos, H.mass = x, y, z, m
+    coef = R_sl.compute_coefficients_solitary(H, sph_file, mod_file, verbose=0, no_odd=False)
+    coef_noodd = R_sl.compute_coefficients_solitary(H, sph_file, mod_file, verbose=0, no_odd=True)
+    out.update(sl_coef=np.asarray(coef, dtype=np.float64),
+               sl_coef_noodd=np.asarray(coef_noodd, dtype=np.float64))
+
+    class _P:
+        pass
+    PI = _P()
+    PI.data = dict(x=x, y=y, z=z, m=m)
+    with np.errstate(all="ignore"):
+        ev_all = R_sl.all_eval_particles(PI, coef, sph_file, mod_file, 0)
+        ev_l = R_sl.all_eval_particles(PI, coef, sph_file, mod_file, 0, L1=1, L2=2, NO_ODD=True)
+    out["sl_alleval"] = np.stack(ev_all)
+    out["sl_alleval_L12_noodd"] = np.stack(ev_l)
+    # force_eval over points (r, costh, phi as Fields passes them)
+    fe = []
+    fe_tr = []
+    fe_no = []
+    npt = 120
+    for i in range(npt):
+        r3 = np.sqrt(x[i] ** 2 + y[i] ** 2 + z[i] ** 2) + 1e-15
+        costh = z[i] / r3
+        phi = np.arctan2(y[i], x[i])
+        fe.append(R_sl.force_eval(r3, costh, phi, coef, xi, p0, d0, cmap, scale, lmax, nmax, ev, ef))
+        fe_tr.append(R_sl.force_eval(r3, costh, phi, coef, xi, p0, d0, cmap, scale, 2, 3, ev, ef))
+        fe_no.append(R_sl.force_eval(r3, costh, phi, coef, xi, p0, d0, cmap, scale, lmax, nmax,
+                                     ev, ef, no_odd=True))
+    out["sl_forceeval"] = np.array(fe)
+    out["sl_forceeval_l2n3"] = np.array(fe_tr)
+    out["sl_forceeval_noodd"] = np.array(fe_no)
+
+    # ---------------- EOF bins / accumulate / evaluate --------------------
+    potC, rforceC, zforceC, densC, potS, rforceS, zforceS, densS = eofs
+    R = (x * x + y * y + 1e-10) ** 0.5
+    X, Y, ix, iy = R_eof.return_bins(R, z, rmin=XMIN, dR=dX, zmin=YMIN, dZ=dY, numx=numx,
+                                     numy=numy, ASCALE=ascale, HSCALE=hscale, CMAP=ecmap)
+    out.update(eof_X=X, eof_Y=Y, eof_ix=ix, eof_iy=iy)
+    ac, as_ = R_eof.accumulate(H, potC, potS, mmax, norder, XMIN, dX, YMIN, dY, numx, numy,
+                               ascale, hscale, ecmap)
+    out.update(eof_cos=ac, eof_sin=as_)
+    fev = []
+    fev_no = []
+    fev_tr = []
+    for i in range(npt):
+        r2 = (x[i] ** 2 + y[i] ** 2) ** 0.5 + 1e-15
+        phi = np.arctan2(y[i], x[i])
+        kw = dict(rmin=XMIN, dR=dX, zmin=YMIN, dZ=dY, numx=numx, numy=numy, ASCALE=ascale,
+                  HSCALE=hscale, CMAP=ecmap)
+        fev.append(R_eof.force_eval(r2, z[i], phi, ac, as_, potC, rforceC, zforceC, potS, rforceS,
+                                    zforceS, MMAX=mmax, NMAX=norder, **kw))
+        fev_no.append(R_eof.force_eval(r2, z[i], phi, ac, as_, potC, rforceC, zforceC, potS,
+                                       rforceS, zforceS, MMAX=mmax, NMAX=norder, no_odd=True, **kw))
+        fev_tr.append(R_eof.force_eval(r2, z[i], phi, ac, as_, potC, rforceC, zforceC, potS,
+                                       rforceS, zforceS, MMAX=1, NMAX=2, **kw))
+    out["eof_forceeval"] = np.array([[float(v) for v in t] for t in fev])
+    out["eof_forceeval_noodd"] = np.array([[float(v) for v in t] for t in fev_no])
+    out["eof_forceeval_m1n2"] = np.array([[float(v) for v in t] for t in fev_tr])
+
+    # ---------------- Fields.return_forces_cart + leapfrog ----------------
+    try:
+        from exptool.basis import potential as R_pot
+        from exptool.utils import integrate as R_int
+        F = R_pot.Fields("none", eof_file, sph_file, mod_file, verbose=0)
+
+        class _O:
+            pass
+        F.EOF = _O(); F.EOF.eof_file = eof_file; F.EOF.cos = ac; F.EOF.sin = as_
+        F.SL = _O(); F.SL.sph_file = sph_file; F.SL.model_file = mod_file; F.SL.expcoef = coef
+        F.halofac = 1.0
+        F.prep_tables()
+        F.set_field_parameters()
+        cart = np.array([F.return_forces_cart(x[i], y[i], z[i], rotpos=0.3 * (i % 3))
+                         for i in range(npt)], dtype=np.float64)
+        out["fields_cart"] = cart
+        out["fields_rotpos"] = 0.3 * (np.arange(npt) % 3)
+        orbits = []
+        init = [([2.0, 0.0, 0.05], [0.0, 0.55, 0.02]), ([1.0, 1.0, -0.1], [-0.3, 0.4, 0.0]),
+                ([4.0, -1.0, 0.3], [0.1, 0.45, -0.05])]
+        for pos, vel in init:
+            O = R_int.leapfrog_integrate(F, 200, 0.01, pos, vel, rotfreq=0.05, force=True)
+            orbits.append(np.stack([O[k] for k in ("T", "X", "Y", "Z", "VX", "VY", "VZ", "P",
+                                                  "FX", "FY", "FZ")]))
+        out["leap_init"] = np.array([p + v for p, v in init])
+        out["leap_orbits"] = np.array(orbits)
+        out["leap_dt"] = 0.01
+        out["leap_rotfreq"] = 0.05
+    except Exception as e:  # the glue imports analysis modules; record why if it fails
+        print("Fields/leapfrog vectors skipped:", repr(e))
+
+    path = os.path.join(HERE, "golden_small.npz")
+    np.savez_compressed(path, **{k: np.asarray(v) for k, v in out.items()})
+    print("wrote", path, os.path.getsize(path), "bytes;", len(out), "arrays")
+
+
+if __name__ == "__main__":
+    main()
