@@ -175,12 +175,16 @@ def _nlm(l, m):
 
 def make_eof_tables(disc_xyzm, mmax=6, norder=18, numx=256, numy=128, rmin=0.001, rmax=20.0,
                     ascale=1.0, hscale=0.1, cmap=1, lmax_s=24, nmax_s=16, a_s=1.0,
-                    with_density=True):
+                    with_density=True, shared_sincos=True):
     """Build EOF tables from a disc particle sample (x, y, z, m arrays).
 
     Returns dict with mmax, norder, numx, numy, cmap, rmin, rmax, ascale,
     hscale and potC, rforceC, zforceC, densC, potS, rforceS, zforceS, densS,
-    each (mmax+1, norder, numx+1, numy+1) float64 (S arrays zero at m=0)."""
+    each (mmax+1, norder, numx+1, numy+1) float64 (S arrays zero at m=0).
+
+    shared_sincos=True diagonalises the summed cos and sin covariances, so
+    potS = potC for m > 0 and the expansion is closed under rotations about
+    z; False gives each parity its own eigenbasis (the spans then differ)."""
     x, y, z, ms = (np.asarray(v, dtype=np.float64) for v in disc_xyzm)
     if norder > (lmax_s - mmax + 1) * nmax_s:
         raise ValueError("norder exceeds the spherical basis size for m=mmax")
@@ -237,11 +241,11 @@ def make_eof_tables(disc_xyzm, mmax=6, norder=18, numx=256, numy=128, rmin=0.001
                 BD[j] = rho * P
                 Up[j] = ms * N * Pp[l, m] * phi_p[l, n]
                 j += 1
+        covs = [(Up * a[None, :]) @ (Up * a[None, :]).T for a in ang]
         for kind in (0, 1):
             if m == 0 and kind == 1:
                 continue
-            U = Up * ang[kind][None, :]
-            cov = U @ U.T
+            cov = covs[0] + covs[1] if (shared_sincos and m > 0) else covs[kind]
             w, V = np.linalg.eigh(cov)
             order = np.argsort(w)[::-1][:norder]
             V = V[:, order]

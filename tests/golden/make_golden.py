@@ -36,7 +36,8 @@ def small_tables():
     sl = tg.make_sl_tables(lmax=3, nmax=5, numr=64, rmin=1e-3, rmax=20.0, cmap=1, scale=1.0)
     x, y, z, m = samplers.exponential_disc(20000, rd=1.0, zd=0.1, rmax=10.0, seed=11)
     eof = tg.make_eof_tables((x, y, z, m), mmax=2, norder=4, numx=16, numy=8, rmin=0.001,
-                             rmax=20.0, ascale=1.0, hscale=0.1, cmap=1, lmax_s=10, nmax_s=6)
+                             rmax=20.0, ascale=1.0, hscale=0.1, cmap=1, lmax_s=10, nmax_s=6,
+                             shared_sincos=False)
     return sl, eof
 
 

@@ -154,23 +154,32 @@ def test_leapfrog(golden):
         assert rel(got, ref[:, j]) < 1e-10, k
 
 
-def test_reader_parity_sl(golden, tmp_path):
-    """The product readers give the reference readers' arrays for the same files."""
+def test_reader_parity(golden, tmp_path):
+    """The product writers + readers round-trip the tables the reference's
+    readers produced (the fixture's tables were read by the reference from
+    files written by the same writers)."""
     from exptool_amd.utils import halo_methods as H
-    from exptool_amd.basis import tablegen as tg
-    import sys, os
-    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
-    from make_golden import small_tables
-    sl, eof = small_tables()
-    H.save_sl(sl, tmp_path / "s.cache", tmp_path / "s.model")
+    g = golden
+    lmax, nmax, numr = int(g["sl_lmax"]), int(g["sl_nmax"]), int(g["sl_numr"])
+    H.write_cached_table(tmp_path / "s.cache", lmax, nmax, numr, int(g["sl_cmap"]),
+                         float(g["sl_rmin"]), float(g["sl_rmax"]), float(g["sl_scale"]),
+                         g["sl_ev"], g["sl_ef"])
+    H.write_sph_model_table(tmp_path / "s.model", g["sl_r"], g["sl_d0"] / (4 * np.pi),
+                            np.zeros(numr), g["sl_p0"])
     t = H.load_sl(tmp_path / "s.cache", tmp_path / "s.model")
-    assert np.array_equal(t["ev"], golden["sl_ev"]) and np.array_equal(t["ef"], golden["sl_ef"])
-    assert np.array_equal(t["xi"], golden["sl_xi"])
-    assert np.allclose(t["p0"], golden["sl_p0"], rtol=1e-15, atol=0)
+    assert np.array_equal(t["ev"], g["sl_ev"]) and np.array_equal(t["ef"], g["sl_ef"])
+    assert np.array_equal(t["xi"], g["sl_xi"])
+    assert np.allclose(t["p0"], g["sl_p0"], rtol=1e-14, atol=0)
+    assert np.allclose(t["d0"], g["sl_d0"], rtol=1e-14, atol=0)
+    p = g["eof_params"]
+    eof = dict(mmax=int(p[4]), norder=int(p[5]), numx=int(p[2]), numy=int(p[3]), cmap=int(p[8]),
+               rmin=p[0], rmax=p[1], ascale=p[6], hscale=p[7], dens=int(p[9]))
+    for k in ("potC", "rforceC", "zforceC", "densC", "potS", "rforceS", "zforceS", "densS"):
+        eof[k] = g["eof_" + k]
     E.write_eof_cache(tmp_path / "e.cache", eof, newstyle=True)
     E.write_eof_cache(tmp_path / "o.cache", eof, newstyle=False)
     for f in ("e.cache", "o.cache"):
         arrs = E.parse_eof(tmp_path / f)
         for k, a in zip(("potC", "rforceC", "zforceC", "densC", "potS", "rforceS", "zforceS", "densS"), arrs):
-            assert np.array_equal(a, golden["eof_" + k]), (f, k)
-        assert np.allclose(E.eof_params(tmp_path / f), golden["eof_params"])
+            assert np.array_equal(a, g["eof_" + k]), (f, k)
+        assert np.allclose(E.eof_params(tmp_path / f), g["eof_params"])
