@@ -51,8 +51,10 @@ def main():
     ap.add_argument("--n-halo", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--orbit-steps", type=int, default=200)
+    ap.add_argument("--orbit-steps", type=int, default=10_000)
     ap.add_argument("--orbits", type=int, default=1_000_000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (reference for PMC byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
@@ -63,6 +65,15 @@ def main():
     from exptool_amd.bfe import BFESystem
     from exptool_amd.utils import workloads as W
     from exptool_amd.utils import samplers as S
+
+    if a.copy_calibration > 0:
+        nel = a.copy_calibration * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     t_setup = time.time()
     sl_t, eof_t = W.disc_halo_tables(seed=1)

@@ -58,11 +58,16 @@ __global__ __launch_bounds__(256) void eof_accumulate_kernel(
       s_c[1][tid] = (float)cell.c10;
       s_c[2][tid] = (float)cell.c01;
       s_c[3][tid] = (float)cell.c11;
-      double cm[kMaxM + 1], sm[kMaxM + 1];
-      chebyshev<kMaxM>(c1, s1, g.mmax, cm, sm);
       const double w = -kFourPi * mass;
-      for (int m = 0; m <= g.mmax; ++m) s_w[m][tid] = (float)(w * cm[m]);
-      for (int m = 1; m <= g.mmax; ++m) s_w[g.mmax + m][tid] = (float)(w * sm[m]);
+      double cm = 1.0, sm = 0.0, cm1 = c1, sm1 = -s1;
+      s_w[0][tid] = (float)w;
+      for (int m = 1; m <= g.mmax; ++m) {
+        const double cn = (m == 1) ? c1 : 2.0 * c1 * cm - cm1;
+        const double sn = (m == 1) ? s1 : 2.0 * c1 * sm - sm1;
+        cm1 = cm; sm1 = sm; cm = cn; sm = sn;
+        s_w[m][tid] = (float)(w * cm);
+        s_w[g.mmax + m][tid] = (float)(w * sm);
+      }
     } else {
       s_base[tid] = 0;
       s_c[0][tid] = s_c[1][tid] = s_c[2][tid] = s_c[3][tid] = 0.f;
@@ -101,19 +106,30 @@ __global__ __launch_bounds__(256) void eof_accumulate_kernel(
   }
 }
 
-// Fixed-order Kahan reduction of partials[nb][K] -> out[K].
-__global__ void reduce_partials_kernel(const double* __restrict__ partials, int nb, int K,
-                                       double* __restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= K) return;
+// Fixed-order reduction of partials[nb][K] -> out[K]: one block per k; lane t
+// sums b = t, t + 256, ... with Kahan compensation, then a fixed pairwise tree
+// in LDS combines the 256 lane sums.  The order depends only on nb, so results
+// are bitwise reproducible.
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __restrict__ partials,
+                                                              int nb, int K,
+                                                              double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int k = blockIdx.x;
+  const int t = threadIdx.x;
   double s = 0.0, comp = 0.0;
-  for (int b = 0; b < nb; ++b) {
+  for (int b = t; b < nb; b += 256) {
     const double yv = partials[(size_t)b * K + k] - comp;
     const double tv = s + yv;
     comp = (tv - s) - yv;
     s = tv;
   }
-  out[k] = s;
+  sh[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) sh[t] += sh[t + w];
+    __syncthreads();
+  }
+  if (t == 0) out[k] = sh[0];
 }
 
 // Scatter the K-ordered EOF sums into (mmax+1, norder) cos and sin arrays.
@@ -178,25 +194,40 @@ __global__ __launch_bounds__(256) void sl_accumulate_kernel(
         s_i[tid] = cell.i;
         s_w1[tid] = (float)(cell.x1 * P0);
         s_w2[tid] = (float)(cell.x2 * P0);
-        double P[kMaxLM];
-        legendre(g.lmax, costh, P);
-        double cm[kMaxL + 1], sm[kMaxL + 1];
-        chebyshev<kMaxL>(c1, s1, g.lmax, cm, sm);
+        // Y rows by the legendre_R recurrences, column by column (m outer);
+        // N_lm^2 = (2l+1)/(4 pi) (l-m)!/(l+m)! is carried along the same path.
         const double w = -kFourPi * mass;
-        for (int l = 0; l <= g.lmax; ++l) {
-          const bool skip = no_odd && (l & 1);
-          // N_lm = sqrt((2l+1)/(4 pi) (l-m)!/(l+m)!) [* sqrt 2 for m>0] (spheresl.py:858-861)
-          double ratio = 1.0;
-          for (int m = 0; m <= l; ++m) {
-            if (m > 0) ratio /= (double)(l - m + 1) * (double)(l + m);
-            double Nlm = sqrt((0.5 * l + 0.25) / kPi * ratio);
+        const bool bad = !(costh == costh);
+        const double somx2 = sqrt((1.0 - costh) * (1.0 + costh));
+        double pmm = 1.0, fact = 1.0, nmm2 = 0.25 / kPi;
+        double cmm = 1.0, smm = 0.0, cmm1 = c1, smm1 = -s1;
+        for (int m = 0; m <= g.lmax; ++m) {
+          if (m > 0) {
+            pmm *= -fact * somx2;
+            fact += 2.0;
+            nmm2 *= (2.0 * m + 1.0) / (2.0 * m - 1.0) / ((2.0 * m) * (2.0 * m - 1.0));
+            const double cn = (m == 1) ? c1 : 2.0 * c1 * cmm - cmm1;
+            const double sn = (m == 1) ? s1 : 2.0 * c1 * smm - smm1;
+            cmm1 = cmm; smm1 = smm; cmm = cn; smm = sn;
+          }
+          double plm1 = 0.0, pl = pmm, n2 = nmm2;
+          for (int l = m; l <= g.lmax; ++l) {
+            if (l > m) {
+              const double pn = (l == m + 1) ? costh * (2.0 * m + 1.0) * pmm
+                                             : (costh * (2 * l - 1) * pl - (l + m - 1) * plm1) / (l - m);
+              plm1 = pl;
+              pl = pn;
+              n2 *= (2.0 * l + 1.0) / (2.0 * l - 1.0) * (double)(l - m) / (double)(l + m);
+            }
+            const bool skip = (no_odd && (l & 1)) || (bad && l > 0);
+            double Nlm = sqrt(n2);
             if (m > 0) Nlm *= 1.4142135623730951;
-            const double base = skip ? 0.0 : w * Nlm * P[lm_index(l, m)];
+            const double base = w * Nlm * (l == 0 ? 1.0 : pl);
             if (m == 0) {
-              s_Y[(l * l) * YS + tid] = (float)base;
+              s_Y[(l * l) * YS + tid] = skip ? 0.f : (float)base;
             } else {
-              s_Y[(l * l + 2 * m - 1) * YS + tid] = (float)(base * cm[m]);
-              s_Y[(l * l + 2 * m) * YS + tid] = (float)(base * sm[m]);
+              s_Y[(l * l + 2 * m - 1) * YS + tid] = skip ? 0.f : (float)(base * cmm);
+              s_Y[(l * l + 2 * m) * YS + tid] = skip ? 0.f : (float)(base * smm);
             }
           }
         }
@@ -276,7 +307,7 @@ extern "C" int bfe_eof_accumulate(bfe_ctx* h, const float* x, const float* y, co
     hipLaunchKernelGGL(eof_accumulate_kernel<4>, dim3(nb), dim3(256), 0, s, x, y, z, m, n, g, c->eof_pot32, K, partials);
   }
   BFE_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3((K + 255) / 256), dim3(256), 0, s, partials, nb, K, sums);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), dim3(256), 0, s, partials, nb, K, sums);
   const int nc = (c->mmax + 1) * c->norder;
   hipLaunchKernelGGL(eof_split_kernel, dim3((nc + 255) / 256), dim3(256), 0, s, sums, c->mmax, c->norder, cos_out, sin_out);
   BFE_HIP(c, hipGetLastError());
@@ -319,7 +350,7 @@ extern "C" int bfe_sl_accumulate(bfe_ctx* h, const float* x, const float* y, con
     hipLaunchKernelGGL(sl_accumulate_kernel<4>, dim3(nb), dim3(256), lds, s, x, y, z, m, n, g, c->sl_e32, no_odd, partials);
   }
   BFE_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3((K + 255) / 256), dim3(256), 0, s, partials, nb, K, coef_out);
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), dim3(256), 0, s, partials, nb, K, coef_out);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }

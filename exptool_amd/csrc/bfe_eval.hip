@@ -10,103 +10,113 @@ namespace bfe {
 
 struct SLFoldArgs {
   double lmul[kMaxL + 1];  // per-l multiplier (halofac, 0 for skipped l)
-  int lmax_use, nmax_use, rows, stride, groups;
+  int lmax, nmax_use;
 };
 
-// One thread per (node i, row).  E, G (at i-1, i, i+1) and Dn are summed over
-// n in float64 from the float64 master tables, then D1/D2 are formed and the
-// results rounded once to float32.
+// One thread per (radial node i, (l, m) pair p in m-major order).  E, Dn and
+// G at i-1, i, i+1 are summed over n in float64 from the float64 master
+// tables, D1/D2 formed, and the 8-float block rounded once to float32.
 __global__ void sl_fold_kernel(const double* __restrict__ ef, const double* __restrict__ ev,
                                const double* __restrict__ p0, const double* __restrict__ coef,
                                int nmax, int numr, SLFoldArgs a, float* __restrict__ out) {
+  const int np = sl_npairs(a.lmax);
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)numr * a.rows) return;
-  const int i = (int)(idx / a.rows);
-  const int row = (int)(idx - (int64_t)i * a.rows);
-  int l = 0;
-  while ((l + 1) * (l + 1) <= row) ++l;
-  int m = (row - l * l + 1) / 2;
+  if (idx >= (int64_t)numr * np) return;
+  const int i = (int)(idx / np);
+  const int p = (int)(idx - (int64_t)i * np);
+  int m = 0, rem = p;
+  while (rem > a.lmax - m) { rem -= a.lmax - m + 1; ++m; }
+  const int l = m + rem;
   // N_lm (spheresl.py:858-861) folded into the table
   double ratio = 1.0;
   for (int k = 1; k <= m; ++k) ratio /= (double)(l - k + 1) * (double)(l + k);
   double N = sqrt((0.5 * l + 0.25) / kPi * ratio);
   if (m > 0) N *= 1.4142135623730951;
-  const double mul = (l <= a.lmax_use) ? a.lmul[l] * N : 0.0;
-  double E = 0, Dn = 0, Gm = 0, G0 = 0, Gp = 0;
+  const double mul = a.lmul[l] * N;
+  const int rc = l * l + (m > 0 ? 2 * m - 1 : 0);
+  const int rs = l * l + 2 * m;
   const bool inner = (i >= 1 && i <= numr - 2);
-  for (int n = 0; n < a.nmax_use; ++n) {
-    const double c = coef[(size_t)row * nmax + n] * mul;
-    const size_t base = ((size_t)l * nmax + n) * numr;
-    const double sev = sqrt(ev[l * nmax + n]);
-    const double e = ef[base + i];
-    E += c * e / sev;
-    Dn += c * e * sev;
-    G0 += c * e * p0[i] / sev;
-    if (inner) {
-      Gm += c * ef[base + i - 1] * p0[i - 1] / sev;
-      Gp += c * ef[base + i + 1] * p0[i + 1] / sev;
+  double v[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};  // E, Dn, Gm, G0, Gp
+  for (int cs = 0; cs < (m > 0 ? 2 : 1); ++cs) {
+    const int row = cs ? rs : rc;
+    for (int n = 0; n < a.nmax_use; ++n) {
+      const double c = coef[(size_t)row * nmax + n] * mul;
+      const size_t base = ((size_t)l * nmax + n) * numr;
+      const double sev = sqrt(ev[l * nmax + n]);
+      const double e = ef[base + i];
+      v[cs][0] += c * e / sev;
+      v[cs][1] += c * e * sev;
+      v[cs][3] += c * e * p0[i] / sev;
+      if (inner) {
+        v[cs][2] += c * ef[base + i - 1] * p0[i - 1] / sev;
+        v[cs][4] += c * ef[base + i + 1] * p0[i + 1] / sev;
+      }
     }
   }
-  float* o = out + (size_t)i * a.groups * a.stride;
-  o[row] = (float)E;
-  o[a.stride + row] = inner ? (float)(0.5 * (Gp - Gm)) : 0.f;
-  o[2 * a.stride + row] = inner ? (float)(Gm - 2.0 * G0 + Gp) : 0.f;
-  if (a.groups > 3) o[3 * a.stride + row] = (float)Dn;
+  float* o = out + ((size_t)i * np + p) * 8;
+  o[0] = (float)v[0][0];
+  o[1] = (float)v[1][0];
+  o[2] = (float)v[0][1];
+  o[3] = (float)v[1][1];
+  o[4] = inner ? (float)(0.5 * (v[0][4] - v[0][2])) : 0.f;
+  o[5] = inner ? (float)(0.5 * (v[1][4] - v[1][2])) : 0.f;
+  o[6] = inner ? (float)(v[0][2] - 2.0 * v[0][3] + v[0][4]) : 0.f;
+  o[7] = inner ? (float)(v[1][2] - 2.0 * v[1][3] + v[1][4]) : 0.f;
 }
 
 struct EOFFoldArgs {
   double mmul[kMaxM + 1];  // per-m multiplier (range / no_odd mask)
-  int nuse, groups, stride;
+  int nuse, dens;
 };
 
-// One thread per (node, j) with j = cos m (0..M) or sin m (M+1..2M); sums the
-// n < nuse terms of every group in float64.
+// One thread per (node, m); writes [potC, rfC, zfC, dC, potS, rfS, zfS, dS].
 __global__ void eof_fold_kernel(const double* __restrict__ T0, const double* __restrict__ T1,
                                 const double* __restrict__ T2, const double* __restrict__ T3,
                                 const double* __restrict__ ccos, const double* __restrict__ csin,
                                 int mmax, int norder, int64_t nodes, EOFFoldArgs a,
                                 float* __restrict__ out) {
-  const int q = 2 * mmax + 1;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nodes * q) return;
-  const int64_t node = idx / q;
-  const int j = (int)(idx - node * q);
-  const int m = (j <= mmax) ? j : j - mmax;
-  const double* cf = (j <= mmax) ? ccos + (size_t)m * norder : csin + (size_t)m * norder;
-  const double mul = a.mmul[m];
-  const int K = q * norder;
+  if (idx >= nodes * (mmax + 1)) return;
+  const int64_t node = idx / (mmax + 1);
+  const int m = (int)(idx - node * (mmax + 1));
+  const int K = (2 * mmax + 1) * norder;
   const double* tabs[4] = {T0, T1, T2, T3};
-  float* o = out + node * a.stride;
-  for (int g = 0; g < a.groups; ++g) {
-    const double* t = tabs[g] + node * K + (size_t)j * norder;
-    double s = 0.0;
-    for (int n = 0; n < a.nuse; ++n) s += cf[n] * t[n];
-    o[g * q + j] = (float)(s * mul);
+  const double mul = a.mmul[m];
+  float* o = out + (size_t)idx * 8;
+  for (int cs = 0; cs < 2; ++cs) {
+    const bool have = (cs == 0) || (m > 0);
+    const int j = cs ? mmax + m : m;
+    const double* cf = (cs ? csin : ccos) + (size_t)m * norder;
+    for (int g = 0; g < 4; ++g) {
+      double s = 0.0;
+      if (have && (g < 3 || a.dens)) {
+        const double* t = tabs[g] + node * K + (size_t)j * norder;
+        for (int n = 0; n < a.nuse; ++n) s += cf[n] * t[n];
+      }
+      o[cs * 4 + g] = (float)(s * mul);
+    }
   }
 }
 
-static int round4(int v) { return (v + 3) & ~3; }
+static int ensure_folded(Context* c, float** d, size_t* have, size_t floats, hipStream_t s) {
+  if (*d && *have >= floats) return 0;
+  if (*d) {
+    BFE_HIP(c, hipStreamSynchronize(s));
+    (void)hipFree(*d);
+    *d = nullptr;
+    *have = 0;
+  }
+  BFE_HIP(c, hipMalloc(d, floats * sizeof(float)));
+  *have = floats;
+  return 0;
+}
 
 static int sl_fold(Context* c, int slot, const double* coef, double halofac, int lmax_use,
-                   int nmax_use, int L1, int L2, bool no_odd, bool dens, hipStream_t s) {
+                   int nmax_use, int L1, int L2, bool no_odd, hipStream_t s) {
   SLFolded& f = c->slf[slot];
-  const int rows = (c->lmax + 1) * (c->lmax + 1);
-  const int stride = round4(rows);
-  const int groups = dens ? 4 : 3;
-  const size_t need = (size_t)c->numr * groups * stride;
-  if (!f.d || f.rows != rows || f.groups < groups) {
-    if (f.d) {
-      BFE_HIP(c, hipStreamSynchronize(s));
-      (void)hipFree(f.d);
-      f.d = nullptr;
-    }
-    BFE_HIP(c, hipMalloc(&f.d, need * sizeof(float)));
-    BFE_HIP(c, hipMemsetAsync(f.d, 0, need * sizeof(float), s));
-  }
-  f.rows = rows;
-  f.stride = stride;
-  f.groups = groups;
-  f.lmax_use = lmax_use;
+  const int np = sl_npairs(c->lmax);
+  int rc = ensure_folded(c, &f.d, &f.floats, (size_t)c->numr * np * 8, s);
+  if (rc) return rc;
   SLFoldArgs a;
   for (int l = 0; l <= kMaxL; ++l) {
     bool keep = l <= lmax_use;
@@ -114,12 +124,9 @@ static int sl_fold(Context* c, int slot, const double* coef, double halofac, int
     if (l >= 1 && no_odd && (l & 1)) keep = false;
     a.lmul[l] = keep ? halofac : 0.0;
   }
-  a.lmax_use = lmax_use;
+  a.lmax = c->lmax;
   a.nmax_use = nmax_use;
-  a.rows = rows;
-  a.stride = stride;
-  a.groups = groups;
-  const int64_t total = (int64_t)c->numr * rows;
+  const int64_t total = (int64_t)c->numr * np;
   hipLaunchKernelGGL(sl_fold_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
                      c->sl_ef64, c->sl_ev64, c->sl_p0, coef, c->nmax, c->numr, a, f.d);
   BFE_HIP(c, hipGetLastError());
@@ -131,22 +138,9 @@ static int eof_fold(Context* c, int slot, const double* ccos, const double* csin
                     int nuse, int m1, int m2, bool no_odd, bool compat_mask, bool dens,
                     hipStream_t s) {
   EOFFolded& f = c->eoff[slot];
-  const int q = 2 * c->mmax + 1;
-  const int groups = dens ? 4 : 3;
-  const int stride = round4(groups * q);
   const int64_t nodes = (int64_t)(c->numx + 1) * (c->numy + 1);
-  if (!f.d || f.q != q || f.stride != stride) {
-    if (f.d) {
-      BFE_HIP(c, hipStreamSynchronize(s));
-      (void)hipFree(f.d);
-      f.d = nullptr;
-    }
-    BFE_HIP(c, hipMalloc(&f.d, (size_t)nodes * stride * sizeof(float)));
-    BFE_HIP(c, hipMemsetAsync(f.d, 0, (size_t)nodes * stride * sizeof(float), s));
-  }
-  f.q = q;
-  f.groups = groups;
-  f.stride = stride;
+  int rc = ensure_folded(c, &f.d, &f.floats, (size_t)nodes * (c->mmax + 1) * 8, s);
+  if (rc) return rc;
   EOFFoldArgs a;
   for (int m = 0; m <= kMaxM; ++m) {
     double mul = 1.0;
@@ -158,9 +152,8 @@ static int eof_fold(Context* c, int slot, const double* ccos, const double* csin
     a.mmul[m] = mul;
   }
   a.nuse = nuse;
-  a.groups = groups;
-  a.stride = stride;
-  const int64_t total = nodes * q;
+  a.dens = dens ? 1 : 0;
+  const int64_t total = nodes * (c->mmax + 1);
   hipLaunchKernelGGL(eof_fold_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
                      c->eof64[0], c->eof64[1], c->eof64[2], dens ? c->eof64[3] : c->eof64[0],
                      ccos, csin, c->mmax, c->norder, nodes, a, f.d);
@@ -175,8 +168,7 @@ static int eof_fold(Context* c, int slot, const double* ccos, const double* csin
 template <bool DENS>
 __global__ __launch_bounds__(256) void sl_eval_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-    int64_t n, SLGeom g, const float* __restrict__ F, int stride, int groups, int lmax_use,
-    uint32_t flags, double densfac, float* den0, float* den1, float* pot0, float* pot1,
+    int64_t n, SLGeom g, const float* __restrict__ F, int L, uint32_t flags, double densfac, float* den0, float* den1, float* pot0, float* pot1,
     float* potr, float* pott, float* potp) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -186,7 +178,7 @@ __global__ __launch_bounds__(256) void sl_eval_kernel(
   double c1, s1;
   planar_angle(x, y, c1, s1);
   SLOut o;
-  sl_eval_folded<DENS>(g, F, stride, groups, lmax_use, r, costh, c1, s1, flags, o);
+  sl_eval_folded<DENS>(g, F, L, r, costh, c1, s1, flags, o);
   if (den0) den0[p] = (float)(o.den0 * densfac);
   if (den1) den1[p] = (float)(o.den1 * densfac);
   if (pot0) pot0[p] = (float)o.pot0;
@@ -199,7 +191,7 @@ __global__ __launch_bounds__(256) void sl_eval_kernel(
 template <bool DENS>
 __global__ __launch_bounds__(256) void eof_eval_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-    int64_t n, EOFGeom g, const float* __restrict__ F, int stride, float* p0, float* p, float* fr,
+    int64_t n, EOFGeom g, const float* __restrict__ F, float* p0, float* p, float* fr,
     float* fp, float* fz, float* d0, float* d) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -208,7 +200,7 @@ __global__ __launch_bounds__(256) void eof_eval_kernel(
   double c1, s1;
   planar_angle(x, y, c1, s1);
   EOFOut o;
-  eof_eval_folded<DENS>(g, F, stride, R, z, c1, s1, o);
+  eof_eval_folded<DENS>(g, F, R, z, c1, s1, o);
   if (p0) p0[i] = (float)o.p0;
   if (p) p[i] = (float)o.p;
   if (fr) fr[i] = (float)o.fr;
@@ -244,10 +236,7 @@ int field_tables(Context* c, double rotpos, FieldTables& T) {
   T.eg = eof_geom(c);
   T.slF = c->slf[1].d;
   T.eofF = c->eoff[1].d;
-  T.sl_stride = c->slf[1].stride;
-  T.sl_groups = c->slf[1].groups;
-  T.lmax_use = c->slf[1].lmax_use;
-  T.eof_stride = c->eoff[1].stride;
+  T.lmax = c->lmax;
   T.flags = (uint32_t)c->field_flags;
   T.ximin = *std::min_element(c->h_xi.begin(), c->h_xi.end());
   T.crot = cos(rotpos);
@@ -273,7 +262,7 @@ extern "C" int bfe_sl_eval(bfe_ctx* h, const float* x, const float* y, const flo
   BFE_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
   const bool dens = den0 || den1;
-  int rc = sl_fold(c, 0, coef, 1.0, c->lmax, c->nmax, L1, L2, (flags & BFE_NO_ODD) != 0, dens, s);
+  int rc = sl_fold(c, 0, coef, 1.0, c->lmax, c->nmax, L1, L2, (flags & BFE_NO_ODD) != 0, s);
   if (rc) return rc;
   if (n == 0) return 0;
   const SLFolded& f = c->slf[0];
@@ -281,12 +270,12 @@ extern "C" int bfe_sl_eval(bfe_ctx* h, const float* x, const float* y, const flo
   const uint32_t ef = flags & (BFE_COMPAT_SL_DENS | BFE_COMPAT_SL_EDGE);
   if (dens)
     hipLaunchKernelGGL(sl_eval_kernel<true>, dim3(nblocks256(n)), dim3(256), 0, s, x, y, z, n,
-                       sl_geom(c), f.d, f.stride, f.groups, c->lmax, ef, densfac, den0, den1,
-                       pot0, pot1, potr, pott, potp);
+                       sl_geom(c), f.d, c->lmax, ef, densfac, den0, den1, pot0, pot1, potr, pott,
+                       potp);
   else
     hipLaunchKernelGGL(sl_eval_kernel<false>, dim3(nblocks256(n)), dim3(256), 0, s, x, y, z, n,
-                       sl_geom(c), f.d, f.stride, f.groups, c->lmax, ef, densfac, den0, den1,
-                       pot0, pot1, potr, pott, potp);
+                       sl_geom(c), f.d, c->lmax, ef, densfac, den0, den1, pot0, pot1, potr, pott,
+                       potp);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }
@@ -311,10 +300,10 @@ extern "C" int bfe_eof_eval(bfe_ctx* h, const float* x, const float* y, const fl
   const EOFFolded& f = c->eoff[0];
   if (dens)
     hipLaunchKernelGGL(eof_eval_kernel<true>, dim3(nblocks256(n)), dim3(256), 0, s, x, y, z, n,
-                       eof_geom(c), f.d, f.stride, p0, p, fr, fp, fz, d0, d);
+                       eof_geom(c), f.d, p0, p, fr, fp, fz, d0, d);
   else
     hipLaunchKernelGGL(eof_eval_kernel<false>, dim3(nblocks256(n)), dim3(256), 0, s, x, y, z, n,
-                       eof_geom(c), f.d, f.stride, p0, p, fr, fp, fz, d0, d);
+                       eof_geom(c), f.d, p0, p, fr, fp, fz, d0, d);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }
@@ -336,7 +325,7 @@ extern "C" int bfe_field_set(bfe_ctx* h, const double* eof_cos, const double* eo
   int rc = eof_fold(c, 1, eof_cos, eof_sin, mm, nn, 0, 1 << 30, no_odd,
                     (flags & BFE_COMPAT_EOF_MASK) != 0, false, s);
   if (rc) return rc;
-  rc = sl_fold(c, 1, sl_coef, halofac, ll, hn, -(1 << 30), 1 << 30, no_odd, false, s);
+  rc = sl_fold(c, 1, sl_coef, halofac, ll, hn, -(1 << 30), 1 << 30, no_odd, s);
   if (rc) return rc;
   c->field_halofac = halofac;
   c->field_flags = (int)(flags & (BFE_COMPAT_SL_COS_L | BFE_COMPAT_SL_EDGE));

@@ -18,18 +18,29 @@ struct EOFOut {
   double p0, p, fr, fp, fz, d0, d;
 };
 
-// SL evaluation at (r, cos theta, phi) from a folded table F (groups E, D1,
-// D2[, Dn] of `stride` floats per node).  Follows get_halo_dens_pot_force
+// Folded table layouts (float32, streamed as float4):
+//  SL : per radial node, one 8-float block per (l, m) pair in m-major order
+//       (m = 0..L, l = m..L):  [E_c, E_s, Dn_c, Dn_s, D1_c, D1_s, D2_c, D2_s]
+//       E = sum_n c N_lm ef/sqrt(ev), Dn = sum_n c N_lm ef sqrt(ev),
+//       D1 = (G_{i+1} - G_{i-1})/2, D2 = G_{i-1} - 2 G_i + G_{i+1},
+//       G = sum_n c N_lm ef p0/sqrt(ev)   (S entries are 0 for m = 0).
+//  EOF: per grid node, one 8-float block per m = 0..M:
+//       [potC, rforceC, zforceC, densC, potS, rforceS, zforceS, densS].
+__host__ __device__ inline int sl_npairs(int L) { return (L + 1) * (L + 2) / 2; }
+
+// SL evaluation at (r, cos theta, phi): get_halo_dens_pot_force
 // (spheresl.py:106-160) and the l,m sums of all_eval_particles / force_eval
-// (spheresl.py:1276-1341, :1199-1228); N_lm and the coefficients are folded in.
+// (spheresl.py:1276-1341, :1199-1228) with N_lm and the coefficients folded in.
+// The associated Legendre functions are generated column by column (m outer,
+// l inner) with the recurrences of legendre_R / dlegendre_R
+// (spheresl.py:664-770), in the order the table is stored.
 //   flags: BFE_COMPAT_SL_EDGE  first-cell stencil one cell outward (:150-153)
 //          BFE_COMPAT_SL_COS_L cos/sin(l phi) for 0 < m (:1222-1225)
 //          BFE_COMPAT_SL_DENS  den1 uses P_1^0 for m = 0 (:1323)
 template <bool DENS>
 __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __restrict__ F,
-                                               int stride, int groups, int lmax_use, double r,
-                                               double costh, double c1, double s1,
-                                               uint32_t flags, SLOut& o) {
+                                               int L, double r, double costh, double c1,
+                                               double s1, uint32_t flags, SLOut& o) {
   const SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
   const int i = cell.i;
   const double x1 = cell.x1, x2 = cell.x2;
@@ -39,93 +50,149 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
   const double fac = d_xi_to_r(cell.x, g.cmap, g.scale) / dxi;
   const int ctr = (i == 0) ? 1 : i;
   const double ps = (i == 0) ? ((flags & BFE_COMPAT_SL_EDGE) ? x2 : x2 - 1.0) : x2;
-  const int gs = groups * stride;
-  const float* __restrict__ Fi = F + (size_t)i * gs;
-  const float* __restrict__ Fi1 = Fi + gs;
-  const float* __restrict__ Fc = F + (size_t)ctr * gs;
+  const int ns4 = 2 * sl_npairs(L);  // float4 per node
+  const float4* __restrict__ Ai = reinterpret_cast<const float4*>(F) + (size_t)i * ns4;
+  const float4* __restrict__ Ai1 = Ai + ns4;
+  const float4* __restrict__ Bc = reinterpret_cast<const float4*>(F) + (size_t)ctr * ns4 + 1;
 
-  double P[kMaxLM], dP[kMaxLM];
-  legendre(lmax_use, costh, P);
-  dlegendre(lmax_use, costh, P, dP);
-  double cm[kMaxL + 1], sm[kMaxL + 1];
-  chebyshev<kMaxL>(c1, s1, lmax_use, cm, sm);
-
-  auto pot = [&](int row) { return (x1 * (double)Fi[row] + x2 * (double)Fi1[row]) * P0; };
-  auto dpot = [&](int row) { return fac * ((double)Fc[stride + row] + ps * (double)Fc[2 * stride + row]); };
-  auto den = [&](int row) {
-    return (x1 * (double)Fi[3 * stride + row] + x2 * (double)Fi1[3 * stride + row]) * D0;
-  };
-
-  o.pot0 = pot(0);
-  o.potr = dpot(0);
-  o.den0 = DENS ? den(0) : 0.0;
-  o.pot1 = o.pott = o.potp = o.den1 = 0.0;
+  const double x = costh;
+  const bool good = (x == x);
   const bool cosl = (flags & BFE_COMPAT_SL_COS_L) != 0;
   const bool dq = (flags & BFE_COMPAT_SL_DENS) != 0;
-  for (int l = 1; l <= lmax_use; ++l) {
-    const int r0 = l * l;
-    {
-      const double pv = pot(r0), dv = dpot(r0);
-      const double Pl = P[lm_index(l, 0)];
-      o.pot1 += Pl * pv;
-      o.potr += Pl * dv;
-      o.pott += dP[lm_index(l, 0)] * pv;
-      if (DENS) o.den1 += (dq ? P[lm_index(1, 0)] : Pl) * den(r0);
+  double xc = x;  // dlegendre_R clamps x for the derivative only
+  if (good && 1.0 - fabs(xc) < 1.e-8) xc = (xc > 0) ? 1.0 - 1.e-8 : -(1.0 - 1.e-8);
+  const double somx2d = 1.0 / (xc * xc - 1.0);
+  const double somx2 = good ? sqrt((1.0 - x) * (1.0 + x)) : 0.0;
+  double pot0 = 0, potr0 = 0, den0 = 0, pot1 = 0, potr = 0, pott = 0, potp = 0, den1 = 0;
+  double pmm = 1.0, fact = 1.0;
+  double cmm = 1.0, smm = 0.0, cmm1 = c1, smm1 = -s1;
+  int p = 0;
+#pragma unroll 1
+  for (int m = 0; m <= L; ++m) {
+    if (m > 0) {
+      pmm *= -fact * somx2;
+      fact += 2.0;
+      const double cn = (m == 1) ? c1 : 2.0 * c1 * cmm - cmm1;
+      const double sn = (m == 1) ? s1 : 2.0 * c1 * smm - smm1;
+      cmm1 = cmm; smm1 = smm; cmm = cn; smm = sn;
     }
-    for (int m = 1; m <= l; ++m) {
-      const int rc = r0 + 2 * m - 1, rs = rc + 1;
-      const int ang = cosl ? l : m;
-      const double cmv = cm[ang], smv = sm[ang];
-      const double pc = pot(rc), pss = pot(rs);
-      const double dc = dpot(rc), ds = dpot(rs);
-      const double Plm = P[lm_index(l, m)], dPlm = dP[lm_index(l, m)];
-      o.pot1 += Plm * (pc * cmv + pss * smv);
-      o.potr += Plm * (dc * cmv + ds * smv);
-      o.pott += dPlm * (pc * cmv + pss * smv);
-      o.potp += Plm * m * (-pc * smv + pss * cmv);
-      if (DENS) o.den1 += Plm * (den(rc) * cmv + den(rs) * smv);
+    double plm1 = 0.0, pl = pmm;
+    double cl = cmm, sl = smm, clm1 = cmm1, slm1 = smm1;
+#pragma unroll 1
+    for (int l = m; l <= L; ++l, ++p) {
+      if (l > m) {
+        const double pn = (l == m + 1) ? x * (2.0 * m + 1.0) * pmm
+                                       : (x * (2 * l - 1) * pl - (l + m - 1) * plm1) / (l - m);
+        plm1 = pl;
+        pl = pn;
+        if (cosl) {
+          const double cn = 2.0 * c1 * cl - clm1, sn = 2.0 * c1 * sl - slm1;
+          clm1 = cl; slm1 = sl; cl = cn; sl = sn;
+        }
+      }
+      const float4 a0 = Ai[2 * p], a1 = Ai1[2 * p], b = Bc[2 * p];
+      const double ec = x1 * (double)a0.x + x2 * (double)a1.x;
+      const double dc = (double)b.x + ps * (double)b.z;
+      if (l == 0) {
+        pot0 = ec;
+        potr0 = dc;
+        if (DENS) den0 = x1 * (double)a0.z + x2 * (double)a1.z;
+        continue;
+      }
+      if (!good) continue;  // P_l^m = 0 for l > 0 when cos(theta) is NaN (spheresl.py:698)
+      double dpl = somx2d * (xc * l * pl - (l + m) * plm1);
+      if (!isfinite(dpl)) dpl = 0.0;
+      if (m == 0) {
+        pot1 += pl * ec;
+        potr += pl * dc;
+        pott += dpl * ec;
+        if (DENS) den1 += (dq ? x : pl) * (x1 * (double)a0.z + x2 * (double)a1.z);
+      } else {
+        const double es = x1 * (double)a0.y + x2 * (double)a1.y;
+        const double ds = (double)b.y + ps * (double)b.w;
+        const double ca = cosl ? cl : cmm, sa = cosl ? sl : smm;
+        const double pa = ec * ca + es * sa;
+        pot1 += pl * pa;
+        potr += pl * (dc * ca + ds * sa);
+        pott += dpl * pa;
+        potp += pl * m * (-ec * sa + es * ca);
+        if (DENS) {
+          const double nc = x1 * (double)a0.z + x2 * (double)a1.z;
+          const double ns = x1 * (double)a0.w + x2 * (double)a1.w;
+          den1 += pl * (nc * ca + ns * sa);
+        }
+      }
     }
   }
+  o.pot0 = pot0 * P0;
+  o.potr = (potr0 + potr) * fac;
+  o.den0 = den0 * D0;
+  o.pot1 = pot1 * P0;
+  o.pott = pott * P0;
+  o.potp = potp * P0;
+  o.den1 = den1 * D0;
 }
 
-// EOF evaluation at planar radius R, height z, angle phi from a folded table
-// F (per node: pot, rforce, zforce[, dens] groups of q = 2 mmax + 1 entries,
-// cos m=0..M then sin m=1..M).  Bilinear weights as eof.py:443-451; sums as
+// EOF evaluation at planar radius R, height z, angle phi from the folded
+// table (8-float block per m).  Bilinear weights as eof.py:443-451; sums as
 // force_eval (eof.py:842-870) and accumulated_eval_particles (:1092-1138):
 // fp = sum m (C sin m phi potC - S cos m phi potS) = -dPhi/dphi.
 template <bool DENS>
 __device__ __forceinline__ void eof_eval_folded(const EOFGeom& g, const float* __restrict__ F,
-                                                int stride, double R, double z, double c1,
-                                                double s1, EOFOut& o) {
+                                                double R, double z, double c1, double s1,
+                                                EOFOut& o) {
   const EOFCell cell = eof_locate(R, z, g.cmap, g.ascale, g.hscale, g.xmin, g.dx, g.ymin, g.dy,
                                   g.numx, g.numy);
   const int ny1 = g.numy + 1;
-  const size_t n00 = (size_t)cell.ix * ny1 + cell.iy;
-  const float* __restrict__ A = F + n00 * stride;           // (ix, iy)
-  const float* __restrict__ B = A + stride;                  // (ix, iy+1)
-  const float* __restrict__ C = A + (size_t)ny1 * stride;    // (ix+1, iy)
-  const float* __restrict__ D = C + stride;                  // (ix+1, iy+1)
   const int M = g.mmax;
-  const int q = 2 * M + 1;
-  auto v = [&](int k) {
-    return cell.c00 * (double)A[k] + cell.c10 * (double)C[k] + cell.c01 * (double)B[k] +
-           cell.c11 * (double)D[k];
-  };
-  double cm[kMaxM + 1], sm[kMaxM + 1];
-  chebyshev<kMaxM>(c1, s1, M, cm, sm);
-  o.p0 = v(0);
-  o.fr = v(q);
-  o.fz = v(2 * q);
-  o.d0 = DENS ? v(3 * q) : 0.0;
-  o.p = o.fp = o.d = 0.0;
-  for (int m = 1; m <= M; ++m) {
-    const double pc = v(m), pss = v(M + m);
-    o.p += cm[m] * pc + sm[m] * pss;
-    o.fr += cm[m] * v(q + m) + sm[m] * v(q + M + m);
-    o.fz += cm[m] * v(2 * q + m) + sm[m] * v(2 * q + M + m);
-    o.fp += m * (sm[m] * pc - cm[m] * pss);
-    if (DENS) o.d += cm[m] * v(3 * q + m) + sm[m] * v(3 * q + M + m);
+  const int ns4 = 2 * (M + 1);
+  const size_t n00 = (size_t)cell.ix * ny1 + cell.iy;
+  const float4* __restrict__ A = reinterpret_cast<const float4*>(F) + n00 * ns4;  // (ix, iy)
+  const float4* __restrict__ B = A + ns4;                                          // (ix, iy+1)
+  const float4* __restrict__ C = A + (size_t)ny1 * ns4;                            // (ix+1, iy)
+  const float4* __restrict__ D = C + ns4;                                          // (ix+1, iy+1)
+  const double w00 = cell.c00, w10 = cell.c10, w01 = cell.c01, w11 = cell.c11;
+  double p0 = 0, fr = 0, fz = 0, d0 = 0, p = 0, fp = 0, d = 0;
+  double cm = 1.0, sm = 0.0, cm1 = c1, sm1 = -s1;
+#pragma unroll 1
+  for (int m = 0; m <= M; ++m) {
+    if (m > 0) {
+      const double cn = (m == 1) ? c1 : 2.0 * c1 * cm - cm1;
+      const double sn = (m == 1) ? s1 : 2.0 * c1 * sm - sm1;
+      cm1 = cm; sm1 = sm; cm = cn; sm = sn;
+    }
+    const float4 a = A[2 * m], b = B[2 * m], c = C[2 * m], e = D[2 * m];
+    const double pc = w00 * a.x + w10 * c.x + w01 * b.x + w11 * e.x;
+    const double rc = w00 * a.y + w10 * c.y + w01 * b.y + w11 * e.y;
+    const double zc = w00 * a.z + w10 * c.z + w01 * b.z + w11 * e.z;
+    if (m == 0) {
+      p0 = pc;
+      fr = rc;
+      fz = zc;
+      if (DENS) d0 = w00 * a.w + w10 * c.w + w01 * b.w + w11 * e.w;
+      continue;
+    }
+    const float4 a2 = A[2 * m + 1], b2 = B[2 * m + 1], c2 = C[2 * m + 1], e2 = D[2 * m + 1];
+    const double ps = w00 * a2.x + w10 * c2.x + w01 * b2.x + w11 * e2.x;
+    const double rs = w00 * a2.y + w10 * c2.y + w01 * b2.y + w11 * e2.y;
+    const double zs = w00 * a2.z + w10 * c2.z + w01 * b2.z + w11 * e2.z;
+    p += cm * pc + sm * ps;
+    fr += cm * rc + sm * rs;
+    fz += cm * zc + sm * zs;
+    fp += m * (sm * pc - cm * ps);
+    if (DENS) {
+      const double dc = w00 * a.w + w10 * c.w + w01 * b.w + w11 * e.w;
+      const double ds = w00 * a2.w + w10 * c2.w + w01 * b2.w + w11 * e2.w;
+      d += cm * dc + sm * ds;
+    }
   }
+  o.p0 = p0;
+  o.p = p;
+  o.fr = fr;
+  o.fz = fz;
+  o.fp = fp;
+  o.d0 = d0;
+  o.d = d;
 }
 
 // Frozen disc + halo field at a Cartesian point, Fields.return_forces_cart
@@ -135,7 +202,7 @@ struct FieldTables {
   EOFGeom eg;
   const float* slF;
   const float* eofF;
-  int sl_stride, sl_groups, lmax_use, eof_stride;
+  int lmax;          // folded SL layout size (table lmax)
   uint32_t flags;
   double ximin;      // min(xi_halo) for the azimuthal-force guard (:483-485)
   double crot, srot; // cos/sin of rotpos
@@ -151,10 +218,9 @@ __device__ __forceinline__ void field_eval_point(const FieldTables& T, double x,
   const double c1 = c0 * T.crot - s0 * T.srot;   // cos(phi + rotpos)
   const double s1 = s0 * T.crot + c0 * T.srot;   // sin(phi + rotpos)
   EOFOut d;
-  eof_eval_folded<false>(T.eg, T.eofF, T.eof_stride, r2, z, c1, s1, d);
+  eof_eval_folded<false>(T.eg, T.eofF, r2, z, c1, s1, d);
   SLOut h;
-  sl_eval_folded<false>(T.slg, T.slF, T.sl_stride, T.sl_groups, T.lmax_use, r3, costh, c1, s1,
-                        T.flags, h);
+  sl_eval_folded<false>(T.slg, T.slF, T.lmax, r3, costh, c1, s1, T.flags, h);
   double dfp = d.fp, hfp = h.potp;
   if (r3 < T.ximin) {
     hfp = 0.0;

@@ -19,27 +19,17 @@ constexpr int kMaxL = 16;          // SL lmax
 constexpr int kMaxM = 16;          // EOF mmax
 constexpr int kMaxLM = (kMaxL + 1) * (kMaxL + 2) / 2;
 
-// A folded SL table: per radial node, 4 groups of R = (lmax+1)^2 rows:
-//   E  = sum_n c N_lm ef/sqrt(ev)          (potential, times p0-interp)
-//   D1 = (G_{i+1} - G_{i-1})/2, D2 = G_{i-1} - 2G_i + G_{i+1},
-//        G = sum_n c N_lm ef p0/sqrt(ev)    (radial derivative stencil)
-//   Dn = sum_n c N_lm ef sqrt(ev)          (density, times d0-interp)
-// Row stride padded to a multiple of 4 floats.
+// Folded tables (layouts documented in bfe_field.hpp): coefficients summed
+// into the basis tables once per coefficient set.
 struct SLFolded {
   float* d = nullptr;
-  int rows = 0, stride = 0;   // stride = floats per group
-  int groups = 0;             // 3 (no density) or 4
-  int lmax_use = 0;
+  size_t floats = 0;
   bool valid = false;
 };
 
-// A folded EOF table: per grid node, groups pot, rforce, zforce[, dens], each
-// of 2*mmax+1 entries (cos m=0..M, then sin m=1..M), padded to 4 floats.
 struct EOFFolded {
   float* d = nullptr;
-  int q = 0;        // 2*mmax+1
-  int groups = 0;   // 3 or 4
-  int stride = 0;   // floats per node
+  size_t floats = 0;
   bool valid = false;
 };
 
