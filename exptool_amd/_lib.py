@@ -28,8 +28,10 @@ COMPAT_ALL = 0xF
 EXPORTS = (
     "bfe_create", "bfe_destroy", "bfe_last_error", "bfe_sl_tables_set", "bfe_eof_tables_set",
     "bfe_sl_accumulate", "bfe_eof_accumulate", "bfe_sl_eval", "bfe_eof_eval", "bfe_field_set",
-    "bfe_field_eval_cart", "bfe_leapfrog",
+    "bfe_field_eval_cart", "bfe_leapfrog", "bfe_mlp_set_params", "bfe_mlp_get_params",
+    "bfe_mlp_forward", "bfe_mlp_gradient", "bfe_mlp_train_step",
 )
+MLP_NPARAMS = 50564
 
 _lib = None
 _lock = threading.Lock()
@@ -53,6 +55,12 @@ def _declare(lib):
         "bfe_field_eval_cart": (_I32, [_VP, _VP, _VP, _VP, _I64, _F64, _VP, _VP, _VP, _VP, _VP, _VP]),
         "bfe_leapfrog": (_I32, [_VP] + [_VP] * 6 + [_I64, _F64, _I64, _F64, _F64, _I32, _VP, _VP, _I32, _VP,
                                                   _I32, _VP]),
+        "bfe_mlp_set_params": (_I32, [_VP, _VP, _I64, _VP]),
+        "bfe_mlp_get_params": (_I32, [_VP, _VP, _I64]),
+        "bfe_mlp_forward": (_I32, [_VP, _VP, _VP, _VP, _I64, _VP, _VP]),
+        "bfe_mlp_gradient": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _I64, _VP, _VP, _VP]),
+        "bfe_mlp_train_step": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _I64, C.c_float, C.c_float,
+                                      C.c_float, C.c_float, _VP, _VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

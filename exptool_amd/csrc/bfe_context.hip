@@ -101,6 +101,7 @@ extern "C" int bfe_destroy(bfe_ctx* h) {
   (void)hipDeviceSynchronize();
   free_sl(c);
   free_eof(c);
+  mlp_free(c);
   if (c->scratch) (void)hipFree(c->scratch);
   delete h;
   return 0;
@@ -169,6 +170,8 @@ extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, in
   const size_t nodes = (size_t)(numx + 1) * (numy + 1);
   const double* srcC[4] = {potC, rforceC, zforceC, densC};
   const double* srcS[4] = {potS, rforceS, zforceS, densS};
+  // float64 masters are function-major [K][node] (coalesced folding);
+  // the float32 accumulate table is node-major [node][K] (coalesced gathers).
   std::vector<double> buf(nodes * K);
   std::vector<float> pot32;
   for (int q = 0; q < (dens ? 4 : 3); ++q) {
@@ -176,19 +179,21 @@ extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, in
       for (int n = 0; n < norder; ++n) {
         const double* s = srcC[q] + ((size_t)m * norder + n) * nodes;
         const int k = m * norder + n;
-        for (size_t node = 0; node < nodes; ++node) buf[node * K + k] = s[node];
+        memcpy(&buf[(size_t)k * nodes], s, nodes * sizeof(double));
       }
     for (int m = 1; m <= mmax; ++m)
       for (int n = 0; n < norder; ++n) {
         const double* s = srcS[q] + ((size_t)m * norder + n) * nodes;
         const int k = (mmax + m) * norder + n;
-        for (size_t node = 0; node < nodes; ++node) buf[node * K + k] = s[node];
+        memcpy(&buf[(size_t)k * nodes], s, nodes * sizeof(double));
       }
     int rc = dev_upload(c, &c->eof64[q], buf.data(), buf.size());
     if (rc) return rc;
     if (q == 0) {
       pot32.resize(buf.size());
-      for (size_t i = 0; i < buf.size(); ++i) pot32[i] = (float)buf[i];
+      for (int k = 0; k < K; ++k)
+        for (size_t node = 0; node < nodes; ++node)
+          pot32[node * K + k] = (float)buf[(size_t)k * nodes + node];
       if ((rc = dev_upload(c, &c->eof_pot32, pot32.data(), pot32.size()))) return rc;
     }
   }

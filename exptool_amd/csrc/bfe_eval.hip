@@ -69,7 +69,8 @@ struct EOFFoldArgs {
   int nuse, dens;
 };
 
-// One thread per (node, m); writes [potC, rfC, zfC, dC, potS, rfS, zfS, dS].
+// One thread per (node, m), nodes fastest so the function-major float64
+// tables are read coalesced; writes [potC, rfC, zfC, dC, potS, rfS, zfS, dS].
 __global__ void eof_fold_kernel(const double* __restrict__ T0, const double* __restrict__ T1,
                                 const double* __restrict__ T2, const double* __restrict__ T3,
                                 const double* __restrict__ ccos, const double* __restrict__ csin,
@@ -77,12 +78,11 @@ __global__ void eof_fold_kernel(const double* __restrict__ T0, const double* __r
                                 float* __restrict__ out) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nodes * (mmax + 1)) return;
-  const int64_t node = idx / (mmax + 1);
-  const int m = (int)(idx - node * (mmax + 1));
-  const int K = (2 * mmax + 1) * norder;
+  const int m = (int)(idx / nodes);
+  const int64_t node = idx - (int64_t)m * nodes;
   const double* tabs[4] = {T0, T1, T2, T3};
   const double mul = a.mmul[m];
-  float* o = out + (size_t)idx * 8;
+  float* o = out + ((size_t)node * (mmax + 1) + m) * 8;
   for (int cs = 0; cs < 2; ++cs) {
     const bool have = (cs == 0) || (m > 0);
     const int j = cs ? mmax + m : m;
@@ -90,8 +90,8 @@ __global__ void eof_fold_kernel(const double* __restrict__ T0, const double* __r
     for (int g = 0; g < 4; ++g) {
       double s = 0.0;
       if (have && (g < 3 || a.dens)) {
-        const double* t = tabs[g] + node * K + (size_t)j * norder;
-        for (int n = 0; n < a.nuse; ++n) s += cf[n] * t[n];
+        const double* t = tabs[g] + (size_t)j * norder * nodes + node;
+        for (int n = 0; n < a.nuse; ++n) s += cf[n] * t[(size_t)n * nodes];
       }
       o[cs * 4 + g] = (float)(s * mul);
     }

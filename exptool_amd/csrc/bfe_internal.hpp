@@ -33,6 +33,26 @@ struct EOFFolded {
   bool valid = false;
 };
 
+// MLP surrogate state (float32 masters, bf16 MFMA operand copies, Adam moments
+// and the per-batch training buffers).
+struct MLPState {
+  float* params = nullptr;
+  float* grads = nullptr;
+  float* m = nullptr;
+  float* v = nullptr;
+  __bf16* Wp = nullptr;
+  __bf16* Wt = nullptr;
+  float* feat = nullptr;
+  float* Z = nullptr;
+  float* gZ = nullptr;
+  float* gY = nullptr;
+  float* dwpart = nullptr;
+  double* loss_part = nullptr;
+  int64_t bcap = 0;
+  int64_t step = 0;
+  double cfg[11] = {1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0};
+};
+
 struct Context {
   int device = 0;
   std::string err;
@@ -55,7 +75,7 @@ struct Context {
   int mmax = 0, norder = 0, numx = 0, numy = 0, ecmap = 0;
   double ascale = 1.0, hscale = 1.0, xmin = 0, dx = 1, ymin = 0, dy = 1;
   int eof_K = 0;               // (2*mmax+1)*norder rows of the node-major tables
-  double* eof64[4] = {nullptr, nullptr, nullptr, nullptr};  // pot, rf, zf, dens: [node][K]
+  double* eof64[4] = {nullptr, nullptr, nullptr, nullptr};  // pot, rf, zf, dens: [K][node]
   float* eof_pot32 = nullptr;  // [node][K] potential for accumulate
 
   // ---- folded tables: slot 0 = batch evaluators, slot 1 = frozen field ----
@@ -63,6 +83,9 @@ struct Context {
   EOFFolded eoff[2];
   double field_halofac = 1.0;
   int field_flags = 0;
+
+  // ---- MLP surrogate -----------------------------------------------------
+  MLPState mlp;
 
   // ---- scratch (grown on demand, freed by bfe_destroy) -------------------
   void* scratch = nullptr;
@@ -94,6 +117,7 @@ inline EOFGeom eof_geom(const Context* c) {
 int set_error(Context* c, int code, const char* fmt, ...);
 int check_hip(Context* c, hipError_t e, const char* what);
 int ensure_scratch(Context* c, size_t bytes);
+void mlp_free(Context* c);
 
 #define BFE_HIP(ctx, call)                                   \
   do {                                                       \

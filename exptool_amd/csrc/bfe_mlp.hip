@@ -1,0 +1,689 @@
+// MLP neural field x, y, z -> (Phi, F_R, F_phi, F_z): 3 -> 128 -> 128 -> 128 -> 128 -> 4,
+// SiLU activations.  Training and inference kernels for gfx950.
+//
+// Precision: the 3 -> 128 input layer and the 128 -> 4 output layer run in
+// float32 on the vector ALUs (positions are never rounded to bf16); the three
+// 128 x 128 layers run on v_mfma_f32_16x16x32_bf16 with bf16 weights and
+// activations and float32 accumulation.  Master weights and Adam moments are
+// float32.
+//
+// MFMA operand layout (16x16x32, wave64): lane l, g = l / 16, c = l % 16
+//   A (16 x 32):  A[c][8g + j], j < 8        B (32 x 16): B[8g + j][c]
+//   D (16 x 16):  D[4g + i][c], i < 4
+// Activations are kept "neuron x point": M = output neurons, N = points,
+// K = input neurons.  Lane l of an output tile (mt, nt) holds neurons
+// 16 mt + 4 g + i of point 16 nt + c; the next layer's B operand for K-step
+// kk takes the 8 values of tiles 2kk and 2kk+1, i.e. logical k = 32kk + 8g + j
+// stands for neuron pi(k) = 32kk + 4g + j (j < 4) or 32kk + 16 + 4g + j - 4.
+// The weights are stored permuted along K by the same pi, so no shuffle is
+// needed between layers.  The same holds for the backward chain with the
+// transposed weights.
+#include <math.h>
+
+#include <vector>
+
+#include "bfe_field.hpp"
+
+namespace bfe {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kH = 128;
+constexpr int kNIn = 3;
+constexpr int kNOut = 4;
+// parameter offsets (float32 master vector)
+constexpr int kOffW0 = 0;                       // [128][3]
+constexpr int kOffB0 = kOffW0 + kH * kNIn;       // [128]
+constexpr int kOffW1 = kOffB0 + kH;              // [128][128]
+constexpr int kOffB1 = kOffW1 + kH * kH;
+constexpr int kOffW2 = kOffB1 + kH;
+constexpr int kOffB2 = kOffW2 + kH * kH;
+constexpr int kOffW3 = kOffB2 + kH;
+constexpr int kOffB3 = kOffW3 + kH * kH;
+constexpr int kOffW4 = kOffB3 + kH;              // [4][128]
+constexpr int kOffB4 = kOffW4 + kNOut * kH;      // [4]
+constexpr int kNParams = kOffB4 + kNOut;         // 50564
+static_assert(kNParams == BFE_MLP_NPARAMS, "parameter count");
+constexpr int kHidOff[3] = {kOffW1, kOffW2, kOffW3};
+constexpr int kSmallFloats = kH * kNIn + 4 * kH + kNOut * kH + kNOut;  // W0, b0..b3, W4, b4
+constexpr int kNT = 2;                           // 16-point N-tiles per wave
+constexpr int kWaveTile = 16 * kNT;              // points per wave
+constexpr int kBlock = 256;                      // threads (4 waves)
+constexpr int kBlockPts = kWaveTile * kBlock / 64;
+constexpr size_t kLdsBytes = 3 * kH * kH * sizeof(__bf16) + kSmallFloats * sizeof(float);
+
+__device__ __forceinline__ floatx4 mfma16(bf16x8 a, bf16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__host__ __device__ __forceinline__ int pi_neuron(int kk, int g, int j) {
+  return 32 * kk + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
+// inverse of pi within a 32-block: neuron n -> logical k
+__host__ __device__ __forceinline__ int pi_inverse(int n) {
+  const int kk = n >> 5, r = n & 31;
+  const int hi = r >> 4, q = r & 15;
+  return 32 * kk + 8 * (q >> 2) + 4 * hi + (q & 3);
+}
+
+// SiLU and its derivative with the hardware reciprocal (about 1 ulp).
+__device__ __forceinline__ float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = sigmoid(z);
+  return s * (1.f + z * (1.f - s));
+}
+
+struct MLPCfg {
+  float sxy;        // feature 0,1 = x/sxy, y/sxy
+  float hz, wz;     // feature 2 = asinh(z/hz)/wz
+  float os[4];      // physical output k = os[k] * y_k + ob[k]
+  float ob[4];
+};
+
+__device__ __forceinline__ void mlp_features(const MLPCfg& c, float x, float y, float z, float f[3]) {
+  f[0] = x / c.sxy;
+  f[1] = y / c.sxy;
+  f[2] = asinhf(z / c.hz) / c.wz;
+}
+
+struct MLPKArgs {
+  const float* x;
+  const float* y;
+  const float* z;
+  int64_t n;
+  const float* target;   // [4][n] normalised targets (training)
+  const float* weight;   // [n] per-point loss weight or NULL
+  float* out;            // [4][n] physical outputs (inference)
+  float* feat;           // [3][bcap]
+  float* Z;              // [4][128][bcap] pre-activations of layers 1..4
+  float* gZ;             // [4][128][bcap]
+  float* gY;             // [4][bcap]
+  int64_t bcap;
+  const float* params;
+  const __bf16* Wp;      // [3][128][128] forward A operands (K-permuted)
+  const __bf16* Wt;      // [3][128][128] backward A operands (transposed, K-permuted)
+  double* loss_part;     // [gridDim.x]
+  float inv_n;
+  MLPCfg cfg;
+};
+
+__device__ __forceinline__ void load_small(const float* __restrict__ params, float* s_small) {
+  // W0, b0, b1, b2, b3, W4, b4 into LDS
+  for (int t = threadIdx.x; t < kSmallFloats; t += blockDim.x) {
+    int src;
+    if (t < kH * kNIn + kH) src = t;                                       // W0, b0
+    else if (t < kH * kNIn + 2 * kH) src = kOffB1 + (t - kH * kNIn - kH);
+    else if (t < kH * kNIn + 3 * kH) src = kOffB2 + (t - kH * kNIn - 2 * kH);
+    else if (t < kH * kNIn + 4 * kH) src = kOffB3 + (t - kH * kNIn - 3 * kH);
+    else src = kOffW4 + (t - kH * kNIn - 4 * kH);                          // W4, b4
+    s_small[t] = params[src];
+  }
+}
+
+__device__ __forceinline__ void load_weights(const __bf16* __restrict__ W, __bf16* s_w) {
+  const uint4* src = reinterpret_cast<const uint4*>(W);
+  uint4* dst = reinterpret_cast<uint4*>(s_w);
+  for (int t = threadIdx.x; t < 3 * kH * kH / 8; t += blockDim.x) dst[t] = src[t];
+}
+
+// Hidden layer: acc[mt][nt] = bias + W (K-permuted, LDS) x B[kk][nt].
+__device__ __forceinline__ void hidden_mfma(const __bf16* __restrict__ sW, const float* bias,
+                                            const bf16x8 (&B)[4][kNT], floatx4 (&acc)[8][kNT],
+                                            int g, int c) {
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < kNT; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc[mt][nt][i] = bias ? bias[16 * mt + 4 * g + i] : 0.f;
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    // keep the operand loads of one K-step in flight at a time (register budget)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(sW + (16 * mt + c) * kH + 32 * kk + 8 * g);
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt) acc[mt][nt] = mfma16(a, B[kk][nt], acc[mt][nt]);
+    }
+  }
+}
+
+// Forward pass over 32-point tiles per wave.  TRAIN also stores the
+// features and pre-activations, the loss and dL/dy.
+template <bool TRAIN>
+__global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MLPKArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __bf16* sW = reinterpret_cast<__bf16*>(smem);
+  float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
+  const float* sW0 = sS;
+  const float* sB0 = sS + kH * kNIn;
+  const float* sBh = sS + kH * kNIn + kH;           // b1, b2, b3
+  const float* sW4 = sS + kH * kNIn + 4 * kH;
+  const float* sB4 = sW4 + kNOut * kH;
+  __shared__ double s_loss[kBlock / 64];
+
+  load_small(a.params, sS);
+  load_weights(a.Wp, sW);
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  double loss = 0.0;
+  const int64_t ntiles = (a.n + kBlockPts - 1) / kBlockPts;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t base = tile * kBlockPts + wave * kWaveTile;
+    int64_t pid[kNT];
+    bool valid[kNT];
+    float f[kNT][3];
+#pragma unroll
+    for (int nt = 0; nt < kNT; ++nt) {
+      pid[nt] = base + 16 * nt + c;
+      valid[nt] = pid[nt] < a.n;
+      const int64_t q = valid[nt] ? pid[nt] : 0;
+      mlp_features(a.cfg, a.x[q], a.y[q], a.z[q], f[nt]);
+      if (TRAIN && valid[nt] && g == 0)
+        for (int d = 0; d < 3; ++d) a.feat[d * a.bcap + pid[nt]] = f[nt][d];
+    }
+    // layer 0 (float32): each lane evaluates the 32 neurons it feeds to layer 1
+    bf16x8 B[4][kNT];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int nr = pi_neuron(kk, g, j);
+        const float w0 = sW0[nr * 3], w1 = sW0[nr * 3 + 1], w2 = sW0[nr * 3 + 2], b = sB0[nr];
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt) {
+          const float zz = fmaf(w0, f[nt][0], fmaf(w1, f[nt][1], fmaf(w2, f[nt][2], b)));
+          if (TRAIN && valid[nt]) a.Z[(size_t)nr * a.bcap + pid[nt]] = zz;
+          B[kk][nt][j] = (__bf16)silu(zz);
+        }
+      }
+    floatx4 acc[8][kNT];
+#pragma unroll 1
+    for (int layer = 0; layer < 3; ++layer) {
+      hidden_mfma(sW + layer * kH * kH, sBh + layer * kH, B, acc, g, c);
+#pragma unroll
+      for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float zz = acc[mt][nt][i];
+            if (TRAIN && valid[nt])
+              a.Z[((size_t)(layer + 1) * kH + 16 * mt + 4 * g + i) * a.bcap + pid[nt]] = zz;
+            const float hh = silu(zz);
+            if (layer < 2) B[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)hh;
+            else acc[mt][nt][i] = hh;   // H4 in place (Z4 is reloaded in backward)
+          }
+    }
+    // output layer (float32): partial sums over this lane's neurons, then
+    // reduce over the four lane groups that hold the same point column
+    float yk[kNT][4];
+#pragma unroll
+    for (int nt = 0; nt < kNT; ++nt)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float s = 0.f;
+#pragma unroll
+        for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) s = fmaf(sW4[k * kH + 16 * mt + 4 * g + i], acc[mt][nt][i], s);
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        yk[nt][k] = s + sB4[k];
+      }
+    if (!TRAIN) {
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt)
+        if (valid[nt] && g == 0)
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            a.out[(size_t)k * a.n + pid[nt]] = a.cfg.os[k] * yk[nt][k] + a.cfg.ob[k];
+      continue;
+    }
+    // ---- loss and backward ------------------------------------------------
+    float gy[kNT][4];
+#pragma unroll
+    for (int nt = 0; nt < kNT; ++nt) {
+      const float w = valid[nt] ? (a.weight ? a.weight[pid[nt]] : 1.f) : 0.f;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float t = valid[nt] ? a.target[(size_t)k * a.n + pid[nt]] : 0.f;
+        const float d = valid[nt] ? yk[nt][k] - t : 0.f;
+        gy[nt][k] = 2.f * w * d * a.inv_n;
+        if (g == 0) loss += (double)(w * d * d);
+        if (valid[nt] && g == 0) a.gY[(size_t)k * a.bcap + pid[nt]] = gy[nt][k];
+      }
+    }
+  }
+  if (TRAIN) {
+    // fixed-order block sum of the loss
+    for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+    if (lane == 0) s_loss[wave] = loss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) s += s_loss[w];
+      a.loss_part[blockIdx.x] = s;
+    }
+  }
+}
+
+// Backward chain: gZ4 = (W4^T dL/dy) silu'(Z4), then gZ_l = (W_l^T gZ_{l+1})
+// silu'(Z_l) for l = 3, 2, 1, with the transposed K-permuted weights in LDS.
+__global__ __launch_bounds__(kBlock) void mlp_bwd_kernel(MLPKArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __bf16* sW = reinterpret_cast<__bf16*>(smem);
+  float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
+  const float* sW4 = sS + kH * kNIn + 4 * kH;
+  load_small(a.params, sS);
+  load_weights(a.Wt, sW);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const int64_t ntiles = (a.n + kBlockPts - 1) / kBlockPts;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t base = tile * kBlockPts + wave * kWaveTile;
+    int64_t pid[kNT];
+    bool valid[kNT];
+    float gy[kNT][4];
+#pragma unroll
+    for (int nt = 0; nt < kNT; ++nt) {
+      pid[nt] = base + 16 * nt + c;
+      valid[nt] = pid[nt] < a.n;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) gy[nt][k] = valid[nt] ? a.gY[(size_t)k * a.bcap + pid[nt]] : 0.f;
+    }
+    bf16x8 Bg[4][kNT];
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int nr = 16 * mt + 4 * g + i;
+          float gh = 0.f;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) gh = fmaf(sW4[k * kH + nr], gy[nt][k], gh);
+          float gz = 0.f;
+          if (valid[nt]) {
+            const float z4 = a.Z[((size_t)3 * kH + nr) * a.bcap + pid[nt]];
+            gz = gh * dsilu(z4);
+            a.gZ[((size_t)3 * kH + nr) * a.bcap + pid[nt]] = gz;
+          }
+          Bg[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)gz;
+        }
+    floatx4 acc[8][kNT];
+#pragma unroll 1
+    for (int layer = 2; layer >= 0; --layer) {
+      hidden_mfma(sW + layer * kH * kH, nullptr, Bg, acc, g, c);
+#pragma unroll
+      for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int nr = 16 * mt + 4 * g + i;
+            float gz = 0.f;
+            if (valid[nt]) {
+              const float zl = a.Z[((size_t)layer * kH + nr) * a.bcap + pid[nt]];
+              gz = acc[mt][nt][i] * dsilu(zl);
+              a.gZ[((size_t)layer * kH + nr) * a.bcap + pid[nt]] = gz;
+            }
+            Bg[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)gz;
+          }
+    }
+  }
+}
+
+// dW_l[o][i] = sum_p gZ_{l+1}[o][p] * silu(Z_l)[i][p] for hidden layers l = 1..3,
+// one block per (chunk of kChunk points, layer); 4 waves x (2 M-tiles x 8 N-tiles).
+constexpr int kChunk = 256;
+__global__ __launch_bounds__(256) void mlp_dw_kernel(const float* __restrict__ Z,
+                                                      const float* __restrict__ gZ, int64_t n,
+                                                      int64_t bcap, float* __restrict__ part) {
+  const int chunk = blockIdx.x, layer = blockIdx.y;   // layer 0..2 -> W1..W3
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const float* G = gZ + (size_t)(layer + 1) * kH * bcap;   // gZ_{l+1}
+  const float* H = Z + (size_t)layer * kH * bcap;          // Z_l (silu applied on load)
+  floatx4 acc[2][8];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int64_t p0 = (int64_t)chunk * kChunk;
+  for (int ks = 0; ks < kChunk / 32; ++ks) {
+    __builtin_amdgcn_sched_barrier(0);
+    const int64_t pk = p0 + 32 * ks + 8 * g;
+    bf16x8 A[2], Bv[8];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const float* src = G + (size_t)(16 * (2 * wave + a) + c) * bcap + pk;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) A[a][j] = (__bf16)((pk + j < n) ? src[j] : 0.f);
+    }
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const float* src = H + (size_t)(16 * b + c) * bcap + pk;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Bv[b][j] = (__bf16)((pk + j < n) ? silu(src[j]) : 0.f);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) acc[a][b] = mfma16(A[a], Bv[b], acc[a][b]);
+  }
+  float* out = part + ((size_t)layer * gridDim.x + chunk) * kH * kH;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        out[(size_t)(16 * (2 * wave + a) + 4 * g + i) * kH + 16 * b + c] = acc[a][b][i];
+}
+
+// grads[W_l] = sum over chunks (fixed order) of the partials.
+__global__ void mlp_dw_reduce_kernel(const float* __restrict__ part, int nchunk,
+                                     float* __restrict__ grads) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int layer = blockIdx.y;
+  if (e >= kH * kH) return;
+  const float* p = part + (size_t)layer * nchunk * kH * kH + e;
+  double s = 0.0;
+  for (int k = 0; k < nchunk; ++k) s += p[(size_t)k * kH * kH];
+  grads[kHidOff[layer] + e] = (float)s;
+}
+
+// Gradients of W0, b0..b3, W4, b4: one block per parameter, fixed-order sums.
+__global__ __launch_bounds__(256) void mlp_small_grads_kernel(const float* __restrict__ feat,
+                                                              const float* __restrict__ Z,
+                                                              const float* __restrict__ gZ,
+                                                              const float* __restrict__ gY,
+                                                              int64_t n, int64_t bcap,
+                                                              float* __restrict__ grads) {
+  __shared__ double sh[256];
+  int q = blockIdx.x;  // 0..(384 + 512 + 512 + 4)
+  const float* u = nullptr;
+  const float* v = nullptr;
+  bool silu_v = false;
+  int dst;
+  if (q < kH * kNIn) {                      // dW0[nr][d]
+    const int nr = q / 3, d = q % 3;
+    u = gZ + (size_t)nr * bcap;
+    v = feat + (size_t)d * bcap;
+    dst = kOffW0 + q;
+  } else if ((q -= kH * kNIn) < 4 * kH) {   // db0..db3
+    const int l = q / kH, nr = q % kH;
+    u = gZ + ((size_t)l * kH + nr) * bcap;
+    dst = (l == 0 ? kOffB0 : l == 1 ? kOffB1 : l == 2 ? kOffB2 : kOffB3) + nr;
+  } else if ((q -= 4 * kH) < kNOut * kH) {  // dW4[k][nr]
+    const int k = q / kH, nr = q % kH;
+    u = gY + (size_t)k * bcap;
+    v = Z + ((size_t)3 * kH + nr) * bcap;
+    silu_v = true;
+    dst = kOffW4 + q;
+  } else {                                  // db4[k]
+    q -= kNOut * kH;
+    u = gY + (size_t)q * bcap;
+    dst = kOffB4 + q;
+  }
+  double s = 0.0;
+  for (int64_t p = threadIdx.x; p < n; p += 256) {
+    double t = u[p];
+    if (v) t *= silu_v ? (double)silu(v[p]) : (double)v[p];
+    s += t;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) grads[dst] = (float)sh[0];
+}
+
+__global__ void loss_reduce_kernel(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += part[b];
+    out[0] = s;
+  }
+}
+
+// bf16 operand copies of W1..W3: Wp[l][o][k] = W[o][pi(k)], Wt[l][i][k] = W[pi(k)][i].
+__global__ void mlp_sync_kernel(const float* __restrict__ params, __bf16* __restrict__ Wp,
+                                __bf16* __restrict__ Wt) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int layer = blockIdx.y;
+  if (e >= kH * kH) return;
+  const int o = e / kH, i = e % kH;
+  const __bf16 w = (__bf16)params[kHidOff[layer] + e];
+  Wp[(size_t)layer * kH * kH + o * kH + pi_inverse(i)] = w;
+  Wt[(size_t)layer * kH * kH + i * kH + pi_inverse(o)] = w;
+}
+
+struct AdamArgs {
+  float lr, b1, b2, eps, c1, c2;  // c1 = 1 - b1^t, c2 = 1 - b2^t
+};
+
+__global__ void mlp_adam_kernel(float* __restrict__ params, const float* __restrict__ grads,
+                                float* __restrict__ m, float* __restrict__ v, AdamArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kNParams) return;
+  const float gr = grads[e];
+  const float mm = a.b1 * m[e] + (1.f - a.b1) * gr;
+  const float vv = a.b2 * v[e] + (1.f - a.b2) * gr * gr;
+  m[e] = mm;
+  v[e] = vv;
+  params[e] -= a.lr * (mm / a.c1) / (sqrtf(vv / a.c2) + a.eps);
+}
+
+}  // namespace bfe
+
+using namespace bfe;
+
+namespace {
+
+MLPCfg make_cfg(const double* cfg) {
+  MLPCfg c;
+  c.sxy = (float)cfg[0];
+  c.hz = (float)cfg[1];
+  c.wz = (float)cfg[2];
+  for (int k = 0; k < 4; ++k) {
+    c.os[k] = (float)cfg[3 + k];
+    c.ob[k] = (float)cfg[7 + k];
+  }
+  return c;
+}
+
+int mlp_ready(Context* c) {
+  if (!c->mlp.params) return set_error(c, BFE_ENOTABLE, "mlp: parameters not set (bfe_mlp_set_params)");
+  return 0;
+}
+
+int mlp_batch_buffers(Context* c, int64_t n, hipStream_t s) {
+  MLPState& M = c->mlp;
+  if (n <= M.bcap) return 0;
+  BFE_HIP(c, hipStreamSynchronize(s));
+  for (float** p : {&M.feat, &M.Z, &M.gZ, &M.gY, &M.dwpart}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (M.loss_part) (void)hipFree(M.loss_part);
+  M.loss_part = nullptr;
+  const int64_t cap = ((n + kChunk - 1) / kChunk) * kChunk;
+  const int64_t nchunk = cap / kChunk;
+  BFE_HIP(c, hipMalloc(&M.feat, sizeof(float) * 3 * cap));
+  BFE_HIP(c, hipMalloc(&M.Z, sizeof(float) * 4 * kH * cap));
+  BFE_HIP(c, hipMalloc(&M.gZ, sizeof(float) * 4 * kH * cap));
+  BFE_HIP(c, hipMalloc(&M.gY, sizeof(float) * 4 * cap));
+  BFE_HIP(c, hipMalloc(&M.dwpart, sizeof(float) * 3 * nchunk * kH * kH));
+  BFE_HIP(c, hipMalloc(&M.loss_part, sizeof(double) * ((cap + kBlockPts - 1) / kBlockPts)));
+  M.bcap = cap;
+  return 0;
+}
+
+MLPKArgs base_args(Context* c) {
+  MLPKArgs a{};
+  a.params = c->mlp.params;
+  a.Wp = c->mlp.Wp;
+  a.Wt = c->mlp.Wt;
+  a.cfg = make_cfg(c->mlp.cfg);
+  return a;
+}
+
+// Forward + backward + gradient assembly for n points (n <= bcap).
+int mlp_gradient(Context* c, const float* x, const float* y, const float* z, const float* target,
+                 const float* weight, int64_t n, float* grads, double* loss_out, hipStream_t s) {
+  int rc = mlp_batch_buffers(c, n, s);
+  if (rc) return rc;
+  MLPState& M = c->mlp;
+  MLPKArgs a = base_args(c);
+  a.x = x; a.y = y; a.z = z; a.n = n;
+  a.target = target;
+  a.weight = weight;
+  a.feat = M.feat; a.Z = M.Z; a.gZ = M.gZ; a.gY = M.gY; a.bcap = M.bcap;
+  a.loss_part = M.loss_part;
+  a.inv_n = 1.f / (float)n;
+  const int nb = (int)((n + kBlockPts - 1) / kBlockPts);
+  hipLaunchKernelGGL(mlp_fwd_kernel<true>, dim3(nb), dim3(kBlock), kLdsBytes, s, a);
+  hipLaunchKernelGGL(mlp_bwd_kernel, dim3(nb), dim3(kBlock), kLdsBytes, s, a);
+  BFE_HIP(c, hipGetLastError());
+  const int nchunk = (int)((n + kChunk - 1) / kChunk);
+  hipLaunchKernelGGL(mlp_dw_kernel, dim3(nchunk, 3), dim3(256), 0, s, M.Z, M.gZ, n, M.bcap, M.dwpart);
+  hipLaunchKernelGGL(mlp_dw_reduce_kernel, dim3((kH * kH + 255) / 256, 3), dim3(256), 0, s, M.dwpart,
+                     nchunk, grads);
+  hipLaunchKernelGGL(mlp_small_grads_kernel, dim3(kH * kNIn + 4 * kH + kNOut * kH + kNOut), dim3(256),
+                     0, s, M.feat, M.Z, M.gZ, M.gY, n, M.bcap, grads);
+  if (loss_out) hipLaunchKernelGGL(loss_reduce_kernel, dim3(1), dim3(64), 0, s, M.loss_part, nb, loss_out);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
+int mlp_sync(Context* c, hipStream_t s) {
+  hipLaunchKernelGGL(mlp_sync_kernel, dim3((kH * kH + 255) / 256, 3), dim3(256), 0, s,
+                     c->mlp.params, c->mlp.Wp, c->mlp.Wt);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+namespace bfe {
+void mlp_free(Context* c) {
+  MLPState& M = c->mlp;
+  for (float** p : {&M.params, &M.grads, &M.m, &M.v, &M.feat, &M.Z, &M.gZ, &M.gY, &M.dwpart}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (M.Wp) (void)hipFree(M.Wp);
+  if (M.Wt) (void)hipFree(M.Wt);
+  if (M.loss_part) (void)hipFree(M.loss_part);
+  M = MLPState();
+}
+}  // namespace bfe
+
+extern "C" int bfe_mlp_set_params(bfe_ctx* h, const float* params, int64_t count,
+                                  const double* cfg) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (count != kNParams) return set_error(c, BFE_EINVAL, "mlp: %lld params given, %d expected", (long long)count, kNParams);
+  if (!params || !cfg) return set_error(c, BFE_EINVAL, "mlp: NULL params or cfg");
+  if (!(cfg[0] > 0) || !(cfg[1] > 0) || !(cfg[2] > 0)) return set_error(c, BFE_EINVAL, "mlp: feature scales must be > 0");
+  BFE_HIP(c, hipSetDevice(c->device));
+  MLPState& M = c->mlp;
+  if (!M.params) {
+    BFE_HIP(c, hipMalloc(&M.params, sizeof(float) * kNParams));
+    BFE_HIP(c, hipMalloc(&M.grads, sizeof(float) * kNParams));
+    BFE_HIP(c, hipMalloc(&M.m, sizeof(float) * kNParams));
+    BFE_HIP(c, hipMalloc(&M.v, sizeof(float) * kNParams));
+    BFE_HIP(c, hipMalloc(&M.Wp, sizeof(__bf16) * 3 * kH * kH));
+    BFE_HIP(c, hipMalloc(&M.Wt, sizeof(__bf16) * 3 * kH * kH));
+    BFE_HIP(c, hipFuncSetAttribute((const void*)mlp_fwd_kernel<true>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+    BFE_HIP(c, hipFuncSetAttribute((const void*)mlp_fwd_kernel<false>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+    BFE_HIP(c, hipFuncSetAttribute((const void*)mlp_bwd_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+  }
+  BFE_HIP(c, hipMemcpy(M.params, params, sizeof(float) * kNParams, hipMemcpyHostToDevice));
+  BFE_HIP(c, hipMemset(M.m, 0, sizeof(float) * kNParams));
+  BFE_HIP(c, hipMemset(M.v, 0, sizeof(float) * kNParams));
+  for (int k = 0; k < 11; ++k) M.cfg[k] = cfg[k];
+  M.step = 0;
+  int rc = mlp_sync(c, nullptr);
+  if (rc) return rc;
+  BFE_HIP(c, hipDeviceSynchronize());
+  return 0;
+}
+
+extern "C" int bfe_mlp_get_params(bfe_ctx* h, float* params, int64_t count) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (int rc = mlp_ready(c)) return rc;
+  if (count != kNParams || !params) return set_error(c, BFE_EINVAL, "mlp_get_params: bad buffer");
+  BFE_HIP(c, hipSetDevice(c->device));
+  BFE_HIP(c, hipDeviceSynchronize());
+  BFE_HIP(c, hipMemcpy(params, c->mlp.params, sizeof(float) * kNParams, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int bfe_mlp_forward(bfe_ctx* h, const float* x, const float* y, const float* z,
+                               int64_t n, float* out, void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (int rc = mlp_ready(c)) return rc;
+  if (n < 0) return set_error(c, BFE_EINVAL, "mlp_forward: n < 0");
+  if (n == 0) return 0;
+  if (!x || !y || !z || !out) return set_error(c, BFE_EINVAL, "mlp_forward: NULL pointer");
+  BFE_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  MLPKArgs a = base_args(c);
+  a.x = x; a.y = y; a.z = z; a.n = n; a.out = out;
+  const int64_t ntiles = (n + kBlockPts - 1) / kBlockPts;
+  const int nb = (int)(ntiles < 1024 ? ntiles : 1024);
+  hipLaunchKernelGGL(mlp_fwd_kernel<false>, dim3(nb), dim3(kBlock), kLdsBytes, s, a);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
+extern "C" int bfe_mlp_gradient(bfe_ctx* h, const float* x, const float* y, const float* z,
+                                const float* target, const float* weight, int64_t n,
+                                float* grad_out, double* loss_out, void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (int rc = mlp_ready(c)) return rc;
+  if (n <= 0 || !x || !y || !z || !target || !grad_out)
+    return set_error(c, BFE_EINVAL, "mlp_gradient: bad arguments");
+  BFE_HIP(c, hipSetDevice(c->device));
+  return mlp_gradient(c, x, y, z, target, weight, n, grad_out, loss_out, (hipStream_t)stream);
+}
+
+extern "C" int bfe_mlp_train_step(bfe_ctx* h, const float* x, const float* y, const float* z,
+                                  const float* target, const float* weight, int64_t n, float lr,
+                                  float beta1, float beta2, float eps, double* loss_out,
+                                  void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (int rc = mlp_ready(c)) return rc;
+  if (n <= 0 || !x || !y || !z || !target) return set_error(c, BFE_EINVAL, "mlp_train_step: bad arguments");
+  BFE_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  MLPState& M = c->mlp;
+  int rc = mlp_gradient(c, x, y, z, target, weight, n, M.grads, loss_out, s);
+  if (rc) return rc;
+  M.step += 1;
+  AdamArgs a;
+  a.lr = lr; a.b1 = beta1; a.b2 = beta2; a.eps = eps;
+  a.c1 = (float)(1.0 - pow((double)beta1, (double)M.step));
+  a.c2 = (float)(1.0 - pow((double)beta2, (double)M.step));
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kNParams + 255) / 256), dim3(256), 0, s, M.params, M.grads, M.m, M.v, a);
+  BFE_HIP(c, hipGetLastError());
+  return mlp_sync(c, s);
+}

@@ -1,0 +1,190 @@
+"""MLP neural field of a frozen BFE field: (x, y, z) -> (Phi, F_R, F_phi, F_z).
+
+The network (3 -> 128 -> 128 -> 128 -> 128 -> 4, SiLU) is trained only on
+evaluations of the plain BFE field of a BFESystem, by the library's own
+kernels (bfe_mlp_train_step: bf16 MFMA forward/backward, float32 Adam), and
+evaluated by bfe_mlp_forward.  F_phi is the physical azimuthal component
+-(1/R) dPhi/dphi.  Inputs are mapped to features (x/sxy, y/sxy,
+asinh(z/hz)/asinh(zmax/hz)) so that the thin disc is resolved; outputs are
+standardised per component.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .._lib import ptr, stream_ptr
+
+H = 128
+NPARAMS = _lib.MLP_NPARAMS
+
+
+def init_params(seed=0):
+    """He-uniform weights (limit sqrt(6 / fan_in)), zero biases, in the
+    parameter order W0, b0, W1, b1, W2, b2, W3, b3, W4, b4."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    parts = []
+    for fo, fi in ((H, 3), (H, H), (H, H), (H, H), (4, H)):
+        lim = math.sqrt(6.0 / fi)
+        parts.append(g.uniform(-lim, lim, fo * fi))
+        parts.append(np.zeros(fo))
+    p = np.concatenate(parts).astype(np.float32)
+    assert p.size == NPARAMS
+    return p
+
+
+def bfe_labels(system, x, y, z):
+    """(4, n) float32 device tensor Phi, F_R, F_phi, F_z from the frozen BFE
+    field of `system` (Fields.return_forces_cart semantics)."""
+    fx, fy, fz, pot = system.field_forces(x, y, z)
+    x = x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x, np.float32), device=system.device)
+    y = y if isinstance(y, torch.Tensor) else torch.as_tensor(np.asarray(y, np.float32), device=system.device)
+    R = torch.sqrt(x * x + y * y).clamp_min(1e-20)
+    fR = (fx * x + fy * y) / R
+    fphi = (fy * x - fx * y) / R
+    return torch.stack([pot, fR, fphi, fz])
+
+
+def sample_domain(n, rmax=12.0, zmax=3.0, seed=0, device="cuda"):
+    """n points uniform in the cylinder R < rmax, |z| < zmax (seeded)."""
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed)
+    u = torch.rand(3, n, generator=gen, device=device, dtype=torch.float32)
+    R = rmax * torch.sqrt(u[0])
+    ph = 2.0 * math.pi * u[1]
+    return R * torch.cos(ph), R * torch.sin(ph), zmax * (2.0 * u[2] - 1.0)
+
+
+class MLPField:
+    """Network parameters resident in the context of a BFESystem."""
+
+    def __init__(self, system, sxy=12.0, hz=0.1, zmax=3.0, rmax=12.0, seed=0):
+        self.sys = system
+        self.ctx = system.ctx
+        self.rmax, self.zmax = float(rmax), float(zmax)
+        self.cfg = np.array([sxy, hz, math.asinh(zmax / hz), 1, 1, 1, 1, 0, 0, 0, 0],
+                            dtype=np.float64)
+        self.params = init_params(seed)
+        self.upload()
+
+    # ---------------------------------------------------------------- state
+    def set_scaling(self, os_, ob):
+        self.cfg[3:7] = np.asarray(os_, dtype=np.float64)
+        self.cfg[7:11] = np.asarray(ob, dtype=np.float64)
+
+    def upload(self, params=None):
+        if params is not None:
+            self.params = np.ascontiguousarray(params, dtype=np.float32)
+        rc = self.ctx.lib.bfe_mlp_set_params(self.ctx.h, _lib.host_ptr(self.params), NPARAMS,
+                                             _lib.host_ptr(self.cfg))
+        self.ctx.check(rc, "bfe_mlp_set_params")
+
+    def download(self):
+        out = np.empty(NPARAMS, dtype=np.float32)
+        rc = self.ctx.lib.bfe_mlp_get_params(self.ctx.h, _lib.host_ptr(out), NPARAMS)
+        self.ctx.check(rc, "bfe_mlp_get_params")
+        self.params = out
+        return out
+
+    # ------------------------------------------------------------- compute
+    def forward(self, x, y, z, stream=None):
+        """(4, n) float32 device tensor: Phi, F_R, F_phi, F_z."""
+        x, y, z, n = self.sys._xyz(x, y, z)
+        out = torch.empty((4, n), dtype=torch.float32, device=self.sys.device)
+        rc = self.ctx.lib.bfe_mlp_forward(self.ctx.h, ptr(x), ptr(y), ptr(z), n, ptr(out),
+                                          stream_ptr(stream))
+        self.ctx.check(rc, "bfe_mlp_forward")
+        return out
+
+    def normalise(self, labels):
+        os_ = torch.as_tensor(self.cfg[3:7], dtype=torch.float32, device=labels.device)[:, None]
+        ob = torch.as_tensor(self.cfg[7:11], dtype=torch.float32, device=labels.device)[:, None]
+        return ((labels - ob) / os_).contiguous()
+
+    def gradient(self, x, y, z, target, weight=None, stream=None):
+        """(grad (NPARAMS,) float32 tensor, n*loss) for normalised targets (4, n)."""
+        x, y, z, n = self.sys._xyz(x, y, z)
+        target = target.to(torch.float32).contiguous()
+        g = torch.empty(NPARAMS, dtype=torch.float32, device=self.sys.device)
+        loss = torch.empty(1, dtype=torch.float64, device=self.sys.device)
+        w = None if weight is None else weight.to(torch.float32).contiguous()
+        rc = self.ctx.lib.bfe_mlp_gradient(self.ctx.h, ptr(x), ptr(y), ptr(z), ptr(target),
+                                           ptr(w), n, ptr(g), ptr(loss), stream_ptr(stream))
+        self.ctx.check(rc, "bfe_mlp_gradient")
+        return g, loss
+
+    def train_step(self, x, y, z, target, weight, lr, betas=(0.9, 0.999), eps=1e-8, loss=None,
+                   stream=None):
+        n = x.numel()
+        rc = self.ctx.lib.bfe_mlp_train_step(self.ctx.h, ptr(x), ptr(y), ptr(z), ptr(target),
+                                             ptr(weight), n, C.c_float(lr), C.c_float(betas[0]),
+                                             C.c_float(betas[1]), C.c_float(eps), ptr(loss),
+                                             stream_ptr(stream))
+        self.ctx.check(rc, "bfe_mlp_train_step")
+
+    # --------------------------------------------------------------- training
+    def fit(self, npoints=16_000_000, steps=20000, batch=65536, lr=2e-3, lr_min=1e-5,
+            seed=0, log_every=0, time_budget=None):
+        """Train on `npoints` BFE evaluations drawn uniformly in R < rmax,
+        |z| < zmax.  Cosine learning-rate decay.  Returns a training log."""
+        dev = self.sys.device
+        t0 = time.perf_counter()
+        X = []
+        T = []
+        chunk = 4_000_000
+        for k in range(0, npoints, chunk):
+            m = min(chunk, npoints - k)
+            x, y, z = sample_domain(m, self.rmax, self.zmax, seed=seed + k, device=dev)
+            X.append(torch.stack([x, y, z]))
+            T.append(bfe_labels(self.sys, x, y, z))
+        X = torch.cat(X, dim=1)
+        T = torch.cat(T, dim=1)
+        ob = T.mean(dim=1).double().cpu().numpy()
+        os_ = T.std(dim=1).double().cpu().numpy()
+        ob[1:] = 0.0          # forces: scale only
+        self.set_scaling(os_, ob)
+        self.upload()
+        Tn = self.normalise(T)
+        # relative-force weighting: w = <|F|^2> / |F|^2 (clipped)
+        F2 = (T[1] ** 2 + T[2] ** 2 + T[3] ** 2)
+        w = (F2.mean() / F2.clamp_min(1e-30)).clamp(0.05, 20.0)
+        w = (w / w.mean()).contiguous()
+        t_data = time.perf_counter() - t0
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed + 7)
+        loss = torch.zeros(1, dtype=torch.float64, device=dev)
+        log = []
+        t1 = time.perf_counter()
+        for s in range(steps):
+            idx = torch.randint(0, npoints, (batch,), generator=gen, device=dev)
+            xb = X[0, idx].contiguous()
+            yb = X[1, idx].contiguous()
+            zb = X[2, idx].contiguous()
+            tb = Tn[:, idx].contiguous()
+            wb = w[idx].contiguous()
+            lr_s = lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * s / max(steps - 1, 1)))
+            self.train_step(xb, yb, zb, tb, wb, lr_s, loss=loss)
+            if log_every and (s % log_every == 0 or s == steps - 1):
+                log.append((s, lr_s, float(loss.item()) / batch))
+            if time_budget is not None and (s % 200 == 0) and time.perf_counter() - t1 > time_budget:
+                torch.cuda.synchronize()
+                log.append((s, lr_s, float(loss.item()) / batch))
+                break
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t1
+        self.download()
+        return dict(log=log, data_seconds=t_data, train_seconds=t_train, steps=s + 1,
+                    batch=batch, samples=(s + 1) * batch, npoints=npoints)
+
+    def force_error(self, x, y, z):
+        """Per-point |F_mlp - F_bfe| / |F_bfe| and |Phi_mlp - Phi|/|Phi| (float64 numpy)."""
+        ref = bfe_labels(self.sys, x, y, z).double()
+        got = self.forward(x, y, z).double()
+        dF = torch.sqrt(((got[1:] - ref[1:]) ** 2).sum(0))
+        nF = torch.sqrt((ref[1:] ** 2).sum(0))
+        return (dF / nF).cpu().numpy(), (torch.abs(got[0] - ref[0]) / torch.abs(ref[0])).cpu().numpy()

@@ -134,6 +134,30 @@ int bfe_leapfrog(bfe_ctx* ctx, float* x, float* y, float* z, float* vx, float* v
                  int energy_every, double* drift_out, double* energy_log, int record_every,
                  float* traj, int use_graph, void* stream);
 
+/* ---- MLP neural field (no counterpart in the reference) ----------------- */
+/* A 3 -> 128 -> 128 -> 128 -> 128 -> 4 SiLU network mapping (x, y, z) to
+ * (Phi, F_R, F_phi, F_z) of a frozen field, trained only on evaluations of the
+ * plain BFE.  Inputs are features (x/sxy, y/sxy, asinh(z/hz)/wz); outputs are
+ * physical = os[k] * y_k + ob[k].  cfg = {sxy, hz, wz, os[4], ob[4]}. */
+#define BFE_MLP_NPARAMS 50564
+/* Parameter vector (float32): W0[128][3], b0[128], W1[128][128], b1[128],
+ * W2, b2, W3, b3, W4[4][128], b4[4].  Resets the Adam moments. */
+int bfe_mlp_set_params(bfe_ctx* ctx, const float* params_host, int64_t count, const double* cfg);
+int bfe_mlp_get_params(bfe_ctx* ctx, float* params_host, int64_t count);
+/* out: device float32 [4][n] = Phi, F_R, F_phi, F_z (F_phi = -dPhi/dphi / R). */
+int bfe_mlp_forward(bfe_ctx* ctx, const float* x, const float* y, const float* z, int64_t n,
+                    float* out, void* stream);
+/* Gradient of L = sum_p w_p sum_k (y_k - t_k)^2 / n (t: device [4][n] normalised
+ * targets, w: device [n] or NULL) into grad_out (device float32 [BFE_MLP_NPARAMS]);
+ * loss_out (device float64 [1], may be NULL) receives n L. */
+int bfe_mlp_gradient(bfe_ctx* ctx, const float* x, const float* y, const float* z,
+                     const float* target, const float* weight, int64_t n, float* grad_out,
+                     double* loss_out, void* stream);
+/* One Adam step on the batch (gradient as bfe_mlp_gradient). */
+int bfe_mlp_train_step(bfe_ctx* ctx, const float* x, const float* y, const float* z,
+                       const float* target, const float* weight, int64_t n, float lr,
+                       float beta1, float beta2, float eps, double* loss_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

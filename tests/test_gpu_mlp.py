@@ -1,0 +1,81 @@
+"""GPU MLP kernels (bfe_mlp_forward / bfe_mlp_gradient / bfe_mlp_train_step)
+against the FP64 MLP oracle with the same bfloat16 rounding points.
+
+Tolerances: outputs max|d| / max|ref| <= 5e-3; gradient blocks
+|g - g_ref| / |g_ref| <= 2e-2 (bf16 operands: a value near a rounding
+boundary may round the other way when its float32 and float64 values differ).
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def field():
+    import torch
+    from exptool_amd.bfe import BFESystem
+    from exptool_amd.utils import workloads as W
+    sl, eof = W.disc_halo_tables(seed=1, n_eof_sample=20000, lmax_s=12, nmax_s=10,
+                                 eof=dict(mmax=4, norder=8, numx=128, numy=64),
+                                 sl=dict(lmax=4, nmax=10, numr=1000))
+    s = BFESystem(sl=sl, eof=eof)
+    (d, h) = W.disc_halo_particles(200000, 200000, seed=3)
+    c, sn = s.eof_accumulate(*d)
+    hc = s.sl_accumulate(*h)
+    s.set_field(c, sn, hc)
+    torch.cuda.synchronize()
+    return s
+
+
+def _pts(n, seed):
+    g = np.random.default_rng(seed)
+    R = 12 * np.sqrt(g.random(n))
+    ph = 2 * np.pi * g.random(n)
+    z = g.uniform(-3, 3, n)
+    return (R * np.cos(ph)).astype(np.float32), (R * np.sin(ph)).astype(np.float32), z.astype(np.float32)
+
+
+def test_forward_matches_oracle(field):
+    from exptool_amd.surrogate.mlp import MLPField
+    m = MLPField(field, seed=4)
+    m.set_scaling([2.0, 0.5, 0.1, 0.3], [-1.0, 0.0, 0.0, 0.0])
+    m.upload()
+    x, y, z = _pts(1000, 1)     # not a multiple of the 128-point tile
+    got = m.forward(x, y, z).cpu().numpy().astype(np.float64)
+    Y, _ = MO.forward(m.params, m.cfg, x, y, z, bf=True)
+    ref = MO.physical(m.cfg, Y)
+    for k in range(4):
+        assert np.max(np.abs(got[k] - ref[k])) / np.max(np.abs(ref[k])) <= 5e-3, k
+
+
+def test_gradient_matches_oracle(field):
+    import torch
+    from exptool_amd.surrogate.mlp import MLPField
+    m = MLPField(field, seed=5)
+    x, y, z = _pts(777, 2)
+    g = np.random.default_rng(3)
+    t = g.normal(size=(4, 777)).astype(np.float32)
+    w = g.uniform(0.5, 2, 777).astype(np.float32)
+    gg, loss = m.gradient(x, y, z, torch.tensor(t, device="cuda"), torch.tensor(w, device="cuda"))
+    got = gg.cpu().numpy().astype(np.float64)
+    ref, rloss = MO.gradient(m.params, m.cfg, x, y, z, t, w, bf=True)
+    assert float(loss.item()) == pytest.approx(rloss, rel=2e-3)
+    for name, (o, s) in MO.OFF.items():
+        a, b = got[o:o + s], ref[o:o + s]
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 2e-2, name
+
+
+def test_training_reduces_loss_and_force_error(field):
+    from exptool_amd.surrogate.mlp import MLPField
+    m = MLPField(field, seed=6)
+    x, y, z = _pts(4000, 9)
+    e0, _ = m.force_error(x, y, z)       # untrained (scaling set by fit below)
+    rep = m.fit(npoints=400_000, steps=600, batch=16384, lr=3e-3, seed=1, log_every=100)
+    losses = [l for _, _, l in rep["log"]]
+    assert losses[-1] < 0.2 * losses[0], losses
+    e1, p1 = m.force_error(x, y, z)
+    assert np.median(e1) < 0.1, np.median(e1)
+    assert np.median(e1) < 0.5 * np.median(e0)
