@@ -53,6 +53,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--orbit-steps", type=int, default=10_000)
     ap.add_argument("--orbits", type=int, default=1_000_000)
+    ap.add_argument("--mlp", type=int, default=1, help="train and time the MLP surrogate (0/1)")
+    ap.add_argument("--mlp-points", type=int, default=16_000_000)
+    ap.add_argument("--mlp-train-seconds", type=float, default=60.0)
+    ap.add_argument("--mlp-steps", type=int, default=40000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (reference for PMC byte counters)")
     a = ap.parse_args()
@@ -144,8 +148,9 @@ def main():
         roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
 
-    # ---- orbit integration in the frozen field -------------------------------
+    # ---- orbit integration in the frozen field (plain BFE and MLP surrogate) --
     orbit = None
+    surrogate = None
     if a.orbit_steps > 0 and a.orbits > 0:
         sysm.set_field(c, s, h)
         no = a.orbits
@@ -153,17 +158,22 @@ def main():
         Rr = np.sqrt(ox * ox + oy * oy)
         fx, fy, fz, _ = sysm.field_forces(Rr, np.zeros_like(Rr), np.zeros_like(Rr))
         vx, vy, vz = S.circular_orbits(ox, oy, oz, fx.cpu().numpy().astype(np.float64), 0.1, seed=6)
-        st = [T(v) for v in (ox, oy, oz, vx, vy, vz)]
-        sysm.leapfrog([t.clone() for t in st], dt=1e-3, nsteps=10, energy_every=0)
+        st0 = [T(v) for v in (ox, oy, oz, vx, vy, vz)]
+        sysm.leapfrog([t.clone() for t in st0], dt=1e-3, nsteps=10, energy_every=0)
         torch.cuda.synchronize()
+        st = [t.clone() for t in st0]
         t0 = time.perf_counter()
         out = sysm.leapfrog(st, dt=1e-3, nsteps=a.orbit_steps, energy_every=100)
         torch.cuda.synchronize()
         dt_o = time.perf_counter() - t0
         dr = out["drift"].cpu().numpy()
         orbit = {"orbit_steps_per_s": no * a.orbit_steps / dt_o, "orbits": no,
-                 "steps": a.orbit_steps, "seconds": dt_o,
-                 "energy_drift_median": float(np.median(dr)), "energy_drift_max": float(np.max(dr))}
+                 "steps": a.orbit_steps, "dt": 1e-3, "seconds": dt_o,
+                 "energy_drift_median": float(np.median(dr)),
+                 "energy_drift_p99": float(np.quantile(dr, 0.99)),
+                 "energy_drift_max": float(np.max(dr))}
+        if a.mlp:
+            surrogate = mlp_stage(a, sysm, st0, orbit)
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample --------------------
     cpu = None
@@ -179,10 +189,50 @@ def main():
         "config": {"workload": "disc EOF mmax6 norder18 256x128 + halo SL lmax6 nmax20 numr2000, "
                                "1M+1M particles, accumulate + return_forces_cart at 2M",
                    "n_disc": a.n_disc, "n_halo": a.n_halo, "parallelism": "single GPU"},
-        "kernel_seconds": kt, "roofline": roof, "orbits": orbit, "cpu_baseline": cpu,
+        "kernel_seconds": kt, "roofline": roof, "orbits": orbit, "surrogate": surrogate,
+        "cpu_baseline": cpu,
         "setup_seconds": setup_s,
     }
     print(json.dumps(line))
+
+
+def mlp_stage(a, sysm, st0, orbit):
+    """Train the MLP surrogate on BFE evaluations (time-bounded), measure its
+    force error on held-out points and integrate the same orbits with it."""
+    import torch
+    from exptool_amd.surrogate.mlp import MLPField, sample_domain
+    m = MLPField(sysm, seed=0)
+    rep = m.fit(npoints=a.mlp_points, steps=a.mlp_steps, batch=65536, lr=2e-3,
+                time_budget=a.mlp_train_seconds, seed=11)
+    hx, hy, hz = sample_domain(1_000_000, m.rmax, m.zmax, seed=99, device=sysm.device)
+    ferr, perr = m.force_error(hx, hy, hz)
+    no = st0[0].numel()
+    m.leapfrog([t.clone() for t in st0], dt=1e-3, nsteps=10, energy_every=0)
+    torch.cuda.synchronize()
+    st = [t.clone() for t in st0]
+    t0 = time.perf_counter()
+    out = m.leapfrog(st, dt=1e-3, nsteps=a.orbit_steps, energy_every=100, check_period=100)
+    torch.cuda.synchronize()
+    dt_o = time.perf_counter() - t0
+    dr = out["drift"].cpu().numpy()
+    me = out["maxerr"].cpu().numpy()
+    ck = out["checked"].cpu().numpy()
+    fb = out["fallback"].cpu().numpy()
+    rate = no * a.orbit_steps / dt_o
+    return {"orbit_steps_per_s": rate, "speedup_vs_bfe": rate / orbit["orbit_steps_per_s"],
+            "seconds": dt_o, "train_seconds": rep["train_seconds"], "train_steps": rep["steps"],
+            "train_samples": rep["samples"], "train_points": rep["npoints"],
+            "force_rel_err_heldout": {"median": float(np.median(ferr)),
+                                      "p99": float(np.quantile(ferr, 0.99)),
+                                      "max": float(np.max(ferr))},
+            "pot_rel_err_heldout_median": float(np.median(perr)),
+            "checked_force_err": {"orbits_checked": int((ck > 0).sum()),
+                                  "median_of_max": float(np.median(me[ck > 0])) if (ck > 0).any() else None,
+                                  "max": float(np.max(me[ck > 0])) if (ck > 0).any() else None},
+            "fallback_evals_fraction": float(fb.sum()) / (no * (a.orbit_steps + 1)),
+            "energy_drift_median": float(np.median(dr)),
+            "energy_drift_p99": float(np.quantile(dr, 0.99)),
+            "energy_drift_max": float(np.max(dr))}
 
 
 def cpu_baseline(sl_t, eof_t, dxyzm, hxyzm, budget_s):

@@ -29,7 +29,7 @@ EXPORTS = (
     "bfe_create", "bfe_destroy", "bfe_last_error", "bfe_sl_tables_set", "bfe_eof_tables_set",
     "bfe_sl_accumulate", "bfe_eof_accumulate", "bfe_sl_eval", "bfe_eof_eval", "bfe_field_set",
     "bfe_field_eval_cart", "bfe_leapfrog", "bfe_mlp_set_params", "bfe_mlp_get_params",
-    "bfe_mlp_forward", "bfe_mlp_gradient", "bfe_mlp_train_step",
+    "bfe_mlp_forward", "bfe_mlp_gradient", "bfe_mlp_train_step", "bfe_mlp_leapfrog",
 )
 MLP_NPARAMS = 50564
 
@@ -61,6 +61,9 @@ def _declare(lib):
         "bfe_mlp_gradient": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _I64, _VP, _VP, _VP]),
         "bfe_mlp_train_step": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _I64, C.c_float, C.c_float,
                                       C.c_float, C.c_float, _VP, _VP]),
+        "bfe_mlp_leapfrog": (_I32, [_VP] + [_VP] * 6 + [_I64, _F64, _I64, _F64, _F64, _I32,
+                                                      C.c_float, C.c_float, _I32, _VP, _VP, _VP,
+                                                      _VP, _VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -151,6 +151,71 @@ __device__ __forceinline__ void hidden_mfma(const __bf16* __restrict__ sW, const
   }
 }
 
+// Forward pass of one wave tile (kNT x 16 points, column c of lane group g):
+// features f -> normalised outputs yk (all four lane groups of a column end
+// with the same yk).  With TRAIN the pre-activations are stored to Z
+// ([4][128][bcap]) for points with valid[nt].
+template <bool TRAIN>
+__device__ __forceinline__ void mlp_tile_forward(const __bf16* __restrict__ sW,
+                                                 const float* __restrict__ sS,
+                                                 const float (&f)[kNT][3], float (&yk)[kNT][4],
+                                                 int g, int c, float* Z, int64_t bcap,
+                                                 const int64_t (&pid)[kNT], const bool (&valid)[kNT]) {
+  const float* sW0 = sS;
+  const float* sB0 = sS + kH * kNIn;
+  const float* sBh = sS + kH * kNIn + kH;           // b1, b2, b3
+  const float* sW4 = sS + kH * kNIn + 4 * kH;
+  const float* sB4 = sW4 + kNOut * kH;
+  // layer 0 (float32): each lane evaluates the 32 neurons it feeds to layer 1
+  bf16x8 B[4][kNT];
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int nr = pi_neuron(kk, g, j);
+      const float w0 = sW0[nr * 3], w1 = sW0[nr * 3 + 1], w2 = sW0[nr * 3 + 2], b = sB0[nr];
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt) {
+        const float zz = fmaf(w0, f[nt][0], fmaf(w1, f[nt][1], fmaf(w2, f[nt][2], b)));
+        if (TRAIN && valid[nt]) Z[(size_t)nr * bcap + pid[nt]] = zz;
+        B[kk][nt][j] = (__bf16)silu(zz);
+      }
+    }
+  floatx4 acc[8][kNT];
+#pragma unroll 1
+  for (int layer = 0; layer < 3; ++layer) {
+    hidden_mfma(sW + layer * kH * kH, sBh + layer * kH, B, acc, g, c);
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float zz = acc[mt][nt][i];
+          if (TRAIN && valid[nt])
+            Z[((size_t)(layer + 1) * kH + 16 * mt + 4 * g + i) * bcap + pid[nt]] = zz;
+          const float hh = silu(zz);
+          if (layer < 2) B[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)hh;
+          else acc[mt][nt][i] = hh;   // H4 in place (Z4 is reloaded in backward)
+        }
+  }
+  // output layer (float32): partial sums over this lane's neurons, then
+  // reduce over the four lane groups that hold the same point column
+#pragma unroll
+  for (int nt = 0; nt < kNT; ++nt)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float s = 0.f;
+#pragma unroll
+      for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) s = fmaf(sW4[k * kH + 16 * mt + 4 * g + i], acc[mt][nt][i], s);
+      s += __shfl_xor(s, 16);
+      s += __shfl_xor(s, 32);
+      yk[nt][k] = s + sB4[k];
+    }
+}
+
 // Forward pass over 32-point tiles per wave.  TRAIN also stores the
 // features and pre-activations, the loss and dL/dy.
 template <bool TRAIN>
@@ -158,11 +223,6 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MLPKArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
   float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
-  const float* sW0 = sS;
-  const float* sB0 = sS + kH * kNIn;
-  const float* sBh = sS + kH * kNIn + kH;           // b1, b2, b3
-  const float* sW4 = sS + kH * kNIn + 4 * kH;
-  const float* sB4 = sW4 + kNOut * kH;
   __shared__ double s_loss[kBlock / 64];
 
   load_small(a.params, sS);
@@ -187,55 +247,8 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MLPKArgs a) {
       if (TRAIN && valid[nt] && g == 0)
         for (int d = 0; d < 3; ++d) a.feat[d * a.bcap + pid[nt]] = f[nt][d];
     }
-    // layer 0 (float32): each lane evaluates the 32 neurons it feeds to layer 1
-    bf16x8 B[4][kNT];
-#pragma unroll
-    for (int kk = 0; kk < 4; ++kk)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int nr = pi_neuron(kk, g, j);
-        const float w0 = sW0[nr * 3], w1 = sW0[nr * 3 + 1], w2 = sW0[nr * 3 + 2], b = sB0[nr];
-#pragma unroll
-        for (int nt = 0; nt < kNT; ++nt) {
-          const float zz = fmaf(w0, f[nt][0], fmaf(w1, f[nt][1], fmaf(w2, f[nt][2], b)));
-          if (TRAIN && valid[nt]) a.Z[(size_t)nr * a.bcap + pid[nt]] = zz;
-          B[kk][nt][j] = (__bf16)silu(zz);
-        }
-      }
-    floatx4 acc[8][kNT];
-#pragma unroll 1
-    for (int layer = 0; layer < 3; ++layer) {
-      hidden_mfma(sW + layer * kH * kH, sBh + layer * kH, B, acc, g, c);
-#pragma unroll
-      for (int mt = 0; mt < 8; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < kNT; ++nt)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const float zz = acc[mt][nt][i];
-            if (TRAIN && valid[nt])
-              a.Z[((size_t)(layer + 1) * kH + 16 * mt + 4 * g + i) * a.bcap + pid[nt]] = zz;
-            const float hh = silu(zz);
-            if (layer < 2) B[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)hh;
-            else acc[mt][nt][i] = hh;   // H4 in place (Z4 is reloaded in backward)
-          }
-    }
-    // output layer (float32): partial sums over this lane's neurons, then
-    // reduce over the four lane groups that hold the same point column
     float yk[kNT][4];
-#pragma unroll
-    for (int nt = 0; nt < kNT; ++nt)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        float s = 0.f;
-#pragma unroll
-        for (int mt = 0; mt < 8; ++mt)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) s = fmaf(sW4[k * kH + 16 * mt + 4 * g + i], acc[mt][nt][i], s);
-        s += __shfl_xor(s, 16);
-        s += __shfl_xor(s, 32);
-        yk[nt][k] = s + sB4[k];
-      }
+    mlp_tile_forward<TRAIN>(sW, sS, f, yk, g, c, a.Z, a.bcap, pid, valid);
     if (!TRAIN) {
 #pragma unroll
       for (int nt = 0; nt < kNT; ++nt)
@@ -484,6 +497,200 @@ __global__ void mlp_adam_kernel(float* __restrict__ params, const float* __restr
   params[e] -= a.lr * (mm / a.c1) / (sqrtf(vv / a.c2) + a.eps);
 }
 
+
+// ---------------------------------------------------------------------------
+// Orbit integration with the surrogate (integrate.py:129-143 update rule).
+// ---------------------------------------------------------------------------
+struct MLPOrbitAux {
+  float ax, ay, az, maxerr;   // current acceleration, largest checked force error
+  double e0, drift;           // initial energy (BFE potential), max |E - E0| / |E0|
+  int nfallback, nchecked;    // BFE fallback evaluations, checked evaluations
+};
+
+struct MLPLeapArgs {
+  float *x, *y, *z, *vx, *vy, *vz;
+  int64_t n;
+  double dt, t0, omega;
+  int64_t k0, k1;             // steps (k0, k1] are taken
+  int init;                   // 1: compute a0 and E0 at step k0 only
+  int energy_check;           // 1: compare the energy at k1 with E0
+  int check_period;           // every step 1 wave in check_period is checked against the BFE
+  float rmax2, zmax;          // training domain R^2 < rmax2, |z| < zmax
+  MLPOrbitAux* aux;
+  double* partE;              // [checks][nblocks] block energy sums, or NULL
+  int check_index, nblocks;
+  const float* params;
+  const __bf16* Wp;
+  MLPCfg cfg;
+  FieldTables T;
+};
+
+// Cartesian acceleration and potential of orbit (x, y, z) from the
+// surrogate outputs, or from the BFE when the point is outside the domain.
+__device__ __forceinline__ void surrogate_accel(const MLPLeapArgs& a, FieldTables& T, float x,
+                                                float y, float z, const float (&yv)[4], float& fx,
+                                                float& fy, float& fz, float& pot, bool& fb) {
+  const float R2 = x * x + y * y;
+  fb = !(R2 < a.rmax2 && fabsf(z) < a.zmax);
+  if (fb) {
+    double cv[8];
+    field_eval_point(T, x, y, z, cv);
+    fx = (float)(cv[0] + cv[1]);
+    fy = (float)(cv[2] + cv[3]);
+    fz = (float)(cv[4] + cv[5]);
+    pot = (float)(cv[6] + cv[7]);
+    return;
+  }
+  const float FR = a.cfg.os[1] * yv[1] + a.cfg.ob[1];
+  const float Fp = a.cfg.os[2] * yv[2] + a.cfg.ob[2];
+  const float R = sqrtf(R2);
+  const float cp = R > 0.f ? x / R : 1.f, sp = R > 0.f ? y / R : 0.f;
+  fx = FR * cp - Fp * sp;
+  fy = FR * sp + Fp * cp;
+  fz = a.cfg.os[3] * yv[3] + a.cfg.ob[3];
+  pot = a.cfg.os[0] * yv[0] + a.cfg.ob[0];
+}
+
+__global__ __launch_bounds__(kBlock) void mlp_leapfrog_kernel(MLPLeapArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __bf16* sW = reinterpret_cast<__bf16*>(smem);
+  float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
+  __shared__ double s_e[kBlock / 64];
+  load_small(a.params, sS);
+  load_weights(a.Wp, sW);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const bool owner = g < kNT;                      // lane (g, c) owns orbit 16 g + c of the wave
+  const int64_t ntiles = (a.n + kBlockPts - 1) / kBlockPts;
+  double esum = 0.0;
+  FieldTables T = a.T;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t base = tile * kBlockPts + wave * kWaveTile;
+    const int64_t me = base + 16 * (owner ? g : 0) + c;
+    const bool live = owner && me < a.n;
+    float px = 0, py = 0, pz = 0, ux = 0, uy = 0, uz = 0;
+    MLPOrbitAux o{};
+    if (live) {
+      px = a.x[me]; py = a.y[me]; pz = a.z[me];
+      ux = a.vx[me]; uy = a.vy[me]; uz = a.vz[me];
+      if (!a.init) o = a.aux[me];
+    }
+    const float dt = (float)a.dt, hdt = 0.5f * dt, hdt2 = 0.5f * dt * dt;
+    const int64_t gwave = tile * (kBlock / 64) + wave;
+    const int64_t kstart = a.init ? a.k0 : a.k0 + 1;
+    for (int64_t k = kstart; k <= a.k1; ++k) {
+      if (!a.init && live) {
+        px = px + ux * dt + o.ax * hdt2;
+        py = py + uy * dt + o.ay * hdt2;
+        pz = pz + uz * dt + o.az * hdt2;
+      }
+      // surrogate forward for the wave's points (column c, tiles nt)
+      float f[kNT][3];
+      int64_t pid[kNT];
+      bool valid[kNT];
+#pragma unroll
+      for (int nt = 0; nt < kNT; ++nt) {
+        const float qx = __shfl(px, 16 * nt + c), qy = __shfl(py, 16 * nt + c), qz = __shfl(pz, 16 * nt + c);
+        mlp_features(a.cfg, qx, qy, qz, f[nt]);
+        pid[nt] = 0;
+        valid[nt] = false;
+      }
+      float yk[kNT][4];
+      mlp_tile_forward<false>(sW, sS, f, yk, g, c, nullptr, 0, pid, valid);
+      if (live) {
+        float yv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) yv[q] = (g == 0) ? yk[0][q] : yk[kNT - 1][q];
+        if (a.omega != 0.0) {
+          const double ang = a.omega * (a.t0 + (double)k * a.dt);
+          T.crot = cos(ang);
+          T.srot = sin(ang);
+        }
+        float nx, ny, nz, pot;
+        bool fb;
+        surrogate_accel(a, T, px, py, pz, yv, nx, ny, nz, pot, fb);
+        if (fb) o.nfallback += 1;
+        if (!fb && a.check_period > 0 && ((gwave + k) % a.check_period) == 0) {
+          double cv[8];
+          field_eval_point(T, px, py, pz, cv);
+          const double bx = cv[0] + cv[1], by = cv[2] + cv[3], bz = cv[4] + cv[5];
+          const double err = sqrt((nx - bx) * (nx - bx) + (ny - by) * (ny - by) + (nz - bz) * (nz - bz)) /
+                             sqrt(bx * bx + by * by + bz * bz);
+          if (err > o.maxerr || !(err == err)) o.maxerr = (float)err;
+          o.nchecked += 1;
+        }
+        if (a.init) {
+          o.ax = nx; o.ay = ny; o.az = nz;
+        } else {
+          ux = ux + (o.ax + nx) * hdt;
+          uy = uy + (o.ay + ny) * hdt;
+          uz = uz + (o.az + nz) * hdt;
+          o.ax = nx; o.ay = ny; o.az = nz;
+        }
+      }
+    }
+    if (live) {
+      // energy with the BFE potential (the judge of drift)
+      if (a.init || a.energy_check) {
+        if (a.omega != 0.0) {
+          const double ang = a.omega * (a.t0 + (double)a.k1 * a.dt);
+          T.crot = cos(ang);
+          T.srot = sin(ang);
+        }
+        double cv[8];
+        field_eval_point(T, px, py, pz, cv);
+        const double e = 0.5 * ((double)ux * ux + (double)uy * uy + (double)uz * uz) + (cv[6] + cv[7]);
+        if (a.init) {
+          o.e0 = e;
+          o.drift = 0.0;
+        } else {
+          const double rel = fabs(e - o.e0) / fabs(o.e0);
+          if (rel > o.drift || !(rel == rel)) o.drift = rel;
+        }
+        esum += e;
+      }
+      a.x[me] = px; a.y[me] = py; a.z[me] = pz;
+      a.vx[me] = ux; a.vy[me] = uy; a.vz[me] = uz;
+      a.aux[me] = o;
+    }
+  }
+  if (a.partE) {
+    for (int off = 32; off > 0; off >>= 1) esum += __shfl_xor(esum, off);
+    if (lane == 0) s_e[wave] = esum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < kBlock / 64; ++w) sum += s_e[w];
+      a.partE[(size_t)a.check_index * a.nblocks + blockIdx.x] = sum;
+    }
+  }
+}
+
+__global__ void mlp_orbit_stats_kernel(const MLPOrbitAux* __restrict__ aux, int64_t n,
+                                       double* __restrict__ drift, float* __restrict__ maxerr,
+                                       int* __restrict__ counts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (drift) drift[i] = aux[i].drift;
+  if (maxerr) maxerr[i] = aux[i].maxerr;
+  if (counts) {
+    counts[2 * i] = aux[i].nfallback;
+    counts[2 * i + 1] = aux[i].nchecked;
+  }
+}
+
+__global__ void mlp_energy_reduce_kernel(const double* __restrict__ partE, int nchecks, int nblocks,
+                                         double* __restrict__ log_out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchecks) return;
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += partE[(size_t)c * nblocks + b];
+  log_out[c] = s;
+}
+
+int field_tables(Context* c, double rotpos, FieldTables& T);
+
 }  // namespace bfe
 
 using namespace bfe;
@@ -611,6 +818,8 @@ extern "C" int bfe_mlp_set_params(bfe_ctx* h, const float* params, int64_t count
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
     BFE_HIP(c, hipFuncSetAttribute((const void*)mlp_bwd_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+    BFE_HIP(c, hipFuncSetAttribute((const void*)mlp_leapfrog_kernel,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
   }
   BFE_HIP(c, hipMemcpy(M.params, params, sizeof(float) * kNParams, hipMemcpyHostToDevice));
   BFE_HIP(c, hipMemset(M.m, 0, sizeof(float) * kNParams));
@@ -686,4 +895,74 @@ extern "C" int bfe_mlp_train_step(bfe_ctx* h, const float* x, const float* y, co
   hipLaunchKernelGGL(mlp_adam_kernel, dim3((kNParams + 255) / 256), dim3(256), 0, s, M.params, M.grads, M.m, M.v, a);
   BFE_HIP(c, hipGetLastError());
   return mlp_sync(c, s);
+}
+
+extern "C" int bfe_mlp_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float* vx, float* vy,
+                                float* vz, int64_t n, double dt, int64_t nsteps, double t0,
+                                double rotfreq, int energy_every, float rmax, float zmax,
+                                int check_period, double* drift_out, float* maxerr_out,
+                                int* counts_out, double* energy_log, void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (int rc = mlp_ready(c)) return rc;
+  if (n < 0 || nsteps < 0 || energy_every < 0 || check_period < 0)
+    return set_error(c, BFE_EINVAL, "mlp_leapfrog: negative size");
+  if (n > 0 && (!x || !y || !z || !vx || !vy || !vz)) return set_error(c, BFE_EINVAL, "mlp_leapfrog: NULL state");
+  if (energy_log && energy_every == 0) return set_error(c, BFE_EINVAL, "mlp_leapfrog: energy_log needs energy_every > 0");
+  FieldTables T;
+  int rc = field_tables(c, 0.0, T);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  BFE_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t ntiles = (n + kBlockPts - 1) / kBlockPts;
+  const int nb = (int)(ntiles < 2048 ? ntiles : 2048);
+  const int chunk = energy_every > 0 ? energy_every : (int)(nsteps < 100 ? (nsteps > 0 ? nsteps : 1) : 100);
+  const int nchecks = energy_every > 0 ? (int)(nsteps / energy_every) + 1 : 1;
+  const size_t aux_bytes = (size_t)n * sizeof(MLPOrbitAux);
+  const size_t part_bytes = (size_t)nchecks * nb * sizeof(double);
+  rc = ensure_scratch(c, ((aux_bytes + 255) & ~(size_t)255) + part_bytes);
+  if (rc) return rc;
+  MLPOrbitAux* aux = (MLPOrbitAux*)c->scratch;
+  double* partE = (double*)((char*)c->scratch + ((aux_bytes + 255) & ~(size_t)255));
+  MLPLeapArgs a{};
+  a.x = x; a.y = y; a.z = z; a.vx = vx; a.vy = vy; a.vz = vz;
+  a.n = n;
+  a.dt = dt; a.t0 = t0; a.omega = 2.0 * kPi * rotfreq;
+  a.check_period = check_period;
+  a.rmax2 = rmax * rmax;
+  a.zmax = zmax;
+  a.aux = aux;
+  a.nblocks = nb;
+  a.params = c->mlp.params;
+  a.Wp = c->mlp.Wp;
+  a.cfg = make_cfg(c->mlp.cfg);
+  a.T = T;
+  // step 0: accelerations and initial energies
+  a.init = 1;
+  a.k0 = 0;
+  a.k1 = 0;
+  a.energy_check = 1;
+  a.partE = energy_log ? partE : nullptr;
+  a.check_index = 0;
+  hipLaunchKernelGGL(mlp_leapfrog_kernel, dim3(nb), dim3(kBlock), kLdsBytes, s, a);
+  BFE_HIP(c, hipGetLastError());
+  a.init = 0;
+  for (int64_t k0 = 0, ci = 0; k0 < nsteps; k0 += chunk, ++ci) {
+    a.k0 = k0;
+    a.k1 = k0 + chunk < nsteps ? k0 + chunk : nsteps;
+    a.energy_check = (energy_every > 0 && a.k1 - a.k0 == energy_every) ? 1 : 0;
+    a.partE = (energy_log && a.energy_check) ? partE : nullptr;
+    a.check_index = (int)ci + 1;
+    hipLaunchKernelGGL(mlp_leapfrog_kernel, dim3(nb), dim3(kBlock), kLdsBytes, s, a);
+    BFE_HIP(c, hipGetLastError());
+  }
+  if (drift_out || maxerr_out || counts_out)
+    hipLaunchKernelGGL(mlp_orbit_stats_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       aux, n, drift_out, maxerr_out, counts_out);
+  if (energy_log)
+    hipLaunchKernelGGL(mlp_energy_reduce_kernel, dim3((nchecks + 63) / 64), dim3(64), 0, s, partE,
+                       nchecks, nb, energy_log);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
 }

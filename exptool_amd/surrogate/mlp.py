@@ -188,3 +188,32 @@ class MLPField:
         dF = torch.sqrt(((got[1:] - ref[1:]) ** 2).sum(0))
         nF = torch.sqrt((ref[1:] ** 2).sum(0))
         return (dF / nF).cpu().numpy(), (torch.abs(got[0] - ref[0]) / torch.abs(ref[0])).cpu().numpy()
+
+    # ------------------------------------------------------------ integration
+    def leapfrog(self, state, dt, nsteps, t0=0.0, rotfreq=0.0, energy_every=100,
+                 check_period=100, stream=None):
+        """Advance state = (x, y, z, vx, vy, vz) (float32 device tensors) in
+        place with the surrogate field; points outside R < rmax, |z| < zmax
+        use the BFE.  Returns dict(drift (n,) f64, maxerr (n,) f32 largest
+        checked force error, fallback (n,) int, checked (n,) int,
+        energy (checks,) f64 or None)."""
+        x = state[0]
+        n = x.numel()
+        for t in state:
+            if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != n:
+                raise ValueError("state tensors must be contiguous float32 of equal length")
+        dev = self.sys.device
+        drift = torch.empty(n, dtype=torch.float64, device=dev)
+        maxerr = torch.empty(n, dtype=torch.float32, device=dev)
+        counts = torch.empty((n, 2), dtype=torch.int32, device=dev)
+        elog = (torch.empty(nsteps // energy_every + 1, dtype=torch.float64, device=dev)
+                if energy_every > 0 else None)
+        rc = self.ctx.lib.bfe_mlp_leapfrog(self.ctx.h, *[ptr(t) for t in state], n, float(dt),
+                                           int(nsteps), float(t0), float(rotfreq),
+                                           int(energy_every), C.c_float(self.rmax),
+                                           C.c_float(self.zmax), int(check_period), ptr(drift),
+                                           ptr(maxerr), ptr(counts), ptr(elog),
+                                           stream_ptr(stream))
+        self.ctx.check(rc, "bfe_mlp_leapfrog")
+        return dict(drift=drift, maxerr=maxerr, fallback=counts[:, 0], checked=counts[:, 1],
+                    energy=elog)

@@ -158,6 +158,21 @@ int bfe_mlp_train_step(bfe_ctx* ctx, const float* x, const float* y, const float
                        const float* target, const float* weight, int64_t n, float lr,
                        float beta1, float beta2, float eps, double* loss_out, void* stream);
 
+/* Orbit integration with the surrogate field (same update rule and state
+ * conventions as bfe_leapfrog; needs bfe_field_set and bfe_mlp_set_params).
+ * Points with R >= rmax or |z| >= zmax use the plain BFE field.  Every step,
+ * one wave in check_period (rotating) also evaluates the BFE and records the
+ * force error |F_mlp - F_bfe| / |F_bfe|.  Energies use the BFE potential.
+ * Outputs (device, each may be NULL): drift_out float64 [n] max relative
+ * energy drift, maxerr_out float32 [n] largest checked force error,
+ * counts_out int32 [n][2] (BFE fallback evaluations, checked evaluations),
+ * energy_log float64 [nsteps/energy_every + 1] total energy. */
+int bfe_mlp_leapfrog(bfe_ctx* ctx, float* x, float* y, float* z, float* vx, float* vy, float* vz,
+                     int64_t n, double dt, int64_t nsteps, double t0, double rotfreq,
+                     int energy_every, float rmax, float zmax, int check_period,
+                     double* drift_out, float* maxerr_out, int* counts_out, double* energy_log,
+                     void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,3 +79,41 @@ def test_training_reduces_loss_and_force_error(field):
     e1, p1 = m.force_error(x, y, z)
     assert np.median(e1) < 0.1, np.median(e1)
     assert np.median(e1) < 0.5 * np.median(e0)
+
+
+def test_surrogate_leapfrog(field):
+    """Orbits that stay outside the training domain follow the BFE exactly;
+    orbits inside are integrated by the surrogate, checked against the BFE
+    and keep their energy close to the BFE integrator's."""
+    import torch
+    from exptool_amd.surrogate.mlp import MLPField
+    m = MLPField(field, seed=8)
+    m.fit(npoints=400_000, steps=800, batch=16384, lr=3e-3, seed=2)
+    g = np.random.default_rng(4)
+    n_in, n_out = 3000, 200
+    R = np.concatenate([g.uniform(1.0, 8.0, n_in), g.uniform(40.0, 60.0, n_out)])
+    ph = g.uniform(0, 2 * np.pi, R.size)
+    x, y = R * np.cos(ph), R * np.sin(ph)
+    z = np.concatenate([g.normal(0, 0.05, n_in), g.uniform(10.0, 20.0, n_out)])
+    fx, fy, fz, _ = field.field_forces(R, 0 * R, 0 * R)
+    from exptool_amd.utils.samplers import circular_orbits
+    vx, vy, vz = circular_orbits(x, y, z, fx.cpu().numpy().astype(np.float64), 0.05, seed=5)
+    st0 = [np.asarray(v, np.float32) for v in (x, y, z, vx, vy, vz)]
+    mk = lambda: [torch.tensor(v, device="cuda") for v in st0]
+    a = mk()
+    ra = m.leapfrog(a, dt=2e-3, nsteps=300, energy_every=100, check_period=10)
+    b = mk()
+    rb = field.leapfrog(b, dt=2e-3, nsteps=300, energy_every=100)
+    torch.cuda.synchronize()
+    A = torch.stack(a).cpu().numpy()
+    Bv = torch.stack(b).cpu().numpy()
+    fb = ra["fallback"].cpu().numpy()
+    ck = ra["checked"].cpu().numpy()
+    assert np.all(fb[n_in:] == 301)                  # every evaluation of the outer orbits fell back
+    assert np.allclose(A[:, n_in:], Bv[:, n_in:], rtol=1e-5, atol=1e-6)  # and matches the BFE integrator
+    assert ck[:n_in].sum() > 0
+    err = ra["maxerr"].cpu().numpy()[:n_in][ck[:n_in] > 0]
+    assert np.all(np.isfinite(err)) and np.median(err) < 0.1
+    dr = ra["drift"].cpu().numpy()[:n_in]
+    assert np.median(dr) < 1e-2
+    assert np.all(np.isfinite(A))
