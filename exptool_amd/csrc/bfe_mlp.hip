@@ -47,10 +47,24 @@ constexpr int kNParams = kOffB4 + kNOut;         // 50564
 static_assert(kNParams == BFE_MLP_NPARAMS, "parameter count");
 constexpr int kHidOff[3] = {kOffW1, kOffW2, kOffW3};
 constexpr int kSmallFloats = kH * kNIn + 4 * kH + kNOut * kH + kNOut;  // W0, b0..b3, W4, b4
-constexpr int kNT = 2;                           // 16-point N-tiles per wave
+#ifndef BFE_MLP_NT
+#define BFE_MLP_NT 2
+#endif
+#ifndef BFE_MLP_BLOCK
+#define BFE_MLP_BLOCK 512
+#endif
+constexpr int kNT = BFE_MLP_NT;                  // 16-point N-tiles per wave
 constexpr int kWaveTile = 16 * kNT;              // points per wave
-constexpr int kBlock = 256;                      // threads (4 waves)
-constexpr int kBlockPts = kWaveTile * kBlock / 64;
+// Threads per block.  The inference and integrator kernels run 8 waves on
+// one copy of the LDS weights (two waves per SIMD hide the SiLU latency);
+// the training kernels keep 4 waves, whose stores of the intermediates need
+// the registers.
+constexpr int kBlockInf = BFE_MLP_BLOCK;
+constexpr int kBlockTrain = 256;
+template <bool TRAIN>
+constexpr int block_threads() { return TRAIN ? kBlockTrain : kBlockInf; }
+template <bool TRAIN>
+constexpr int block_points() { return kWaveTile * block_threads<TRAIN>() / 64; }
 constexpr size_t kLdsBytes = 3 * kH * kH * sizeof(__bf16) + kSmallFloats * sizeof(float);
 
 __device__ __forceinline__ floatx4 mfma16(bf16x8 a, bf16x8 b, floatx4 c) {
@@ -219,7 +233,8 @@ __device__ __forceinline__ void mlp_tile_forward(const __bf16* __restrict__ sW,
 // Forward pass over 32-point tiles per wave.  TRAIN also stores the
 // features and pre-activations, the loss and dL/dy.
 template <bool TRAIN>
-__global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MLPKArgs a) {
+__global__ __launch_bounds__(block_threads<TRAIN>()) void mlp_fwd_kernel(MLPKArgs a) {
+  constexpr int kBlock = block_threads<TRAIN>(), kBlockPts = block_points<TRAIN>();
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
   float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
@@ -288,7 +303,8 @@ __global__ __launch_bounds__(kBlock) void mlp_fwd_kernel(MLPKArgs a) {
 
 // Backward chain: gZ4 = (W4^T dL/dy) silu'(Z4), then gZ_l = (W_l^T gZ_{l+1})
 // silu'(Z_l) for l = 3, 2, 1, with the transposed K-permuted weights in LDS.
-__global__ __launch_bounds__(kBlock) void mlp_bwd_kernel(MLPKArgs a) {
+__global__ __launch_bounds__(kBlockTrain) void mlp_bwd_kernel(MLPKArgs a) {
+  constexpr int kBlockPts = block_points<true>();
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
   float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
@@ -525,6 +541,21 @@ struct MLPLeapArgs {
   FieldTables T;
 };
 
+// The plain-BFE evaluation on the surrogate's rare paths (fallback, checks,
+// energies) is an out-of-line call so that its float64 register demand does
+// not add to the MLP loop's.
+#ifndef BFE_MLP_INLINE_BFE
+#define BFE_MLP_INLINE_BFE 0
+#endif
+#if BFE_MLP_INLINE_BFE
+__device__ __forceinline__
+#else
+__device__ __noinline__
+#endif
+void bfe_point_call(const FieldTables& T, double x, double y, double z, double* cv) {
+  field_eval_point(T, x, y, z, cv);
+}
+
 // Cartesian acceleration and potential of orbit (x, y, z) from the
 // surrogate outputs, or from the BFE when the point is outside the domain.
 __device__ __forceinline__ void surrogate_accel(const MLPLeapArgs& a, FieldTables& T, float x,
@@ -534,7 +565,7 @@ __device__ __forceinline__ void surrogate_accel(const MLPLeapArgs& a, FieldTable
   fb = !(R2 < a.rmax2 && fabsf(z) < a.zmax);
   if (fb) {
     double cv[8];
-    field_eval_point(T, x, y, z, cv);
+    bfe_point_call(T, x, y, z, cv);
     fx = (float)(cv[0] + cv[1]);
     fy = (float)(cv[2] + cv[3]);
     fz = (float)(cv[4] + cv[5]);
@@ -551,7 +582,8 @@ __device__ __forceinline__ void surrogate_accel(const MLPLeapArgs& a, FieldTable
   pot = a.cfg.os[0] * yv[0] + a.cfg.ob[0];
 }
 
-__global__ __launch_bounds__(kBlock) void mlp_leapfrog_kernel(MLPLeapArgs a) {
+__global__ __launch_bounds__(kBlockInf) void mlp_leapfrog_kernel(MLPLeapArgs a) {
+  constexpr int kBlock = kBlockInf, kBlockPts = block_points<false>();
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
   float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
@@ -601,7 +633,12 @@ __global__ __launch_bounds__(kBlock) void mlp_leapfrog_kernel(MLPLeapArgs a) {
       if (live) {
         float yv[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) yv[q] = (g == 0) ? yk[0][q] : yk[kNT - 1][q];
+        for (int q = 0; q < 4; ++q) {
+          yv[q] = yk[0][q];
+#pragma unroll
+          for (int nt = 1; nt < kNT; ++nt)
+            if (g == nt) yv[q] = yk[nt][q];
+        }
         if (a.omega != 0.0) {
           const double ang = a.omega * (a.t0 + (double)k * a.dt);
           T.crot = cos(ang);
@@ -613,7 +650,7 @@ __global__ __launch_bounds__(kBlock) void mlp_leapfrog_kernel(MLPLeapArgs a) {
         if (fb) o.nfallback += 1;
         if (!fb && a.check_period > 0 && ((gwave + k) % a.check_period) == 0) {
           double cv[8];
-          field_eval_point(T, px, py, pz, cv);
+          bfe_point_call(T, px, py, pz, cv);
           const double bx = cv[0] + cv[1], by = cv[2] + cv[3], bz = cv[4] + cv[5];
           const double err = sqrt((nx - bx) * (nx - bx) + (ny - by) * (ny - by) + (nz - bz) * (nz - bz)) /
                              sqrt(bx * bx + by * by + bz * bz);
@@ -639,7 +676,7 @@ __global__ __launch_bounds__(kBlock) void mlp_leapfrog_kernel(MLPLeapArgs a) {
           T.srot = sin(ang);
         }
         double cv[8];
-        field_eval_point(T, px, py, pz, cv);
+        bfe_point_call(T, px, py, pz, cv);
         const double e = 0.5 * ((double)ux * ux + (double)uy * uy + (double)uz * uz) + (cv[6] + cv[7]);
         if (a.init) {
           o.e0 = e;
@@ -731,7 +768,7 @@ int mlp_batch_buffers(Context* c, int64_t n, hipStream_t s) {
   BFE_HIP(c, hipMalloc(&M.gZ, sizeof(float) * 4 * kH * cap));
   BFE_HIP(c, hipMalloc(&M.gY, sizeof(float) * 4 * cap));
   BFE_HIP(c, hipMalloc(&M.dwpart, sizeof(float) * 3 * nchunk * kH * kH));
-  BFE_HIP(c, hipMalloc(&M.loss_part, sizeof(double) * ((cap + kBlockPts - 1) / kBlockPts)));
+  BFE_HIP(c, hipMalloc(&M.loss_part, sizeof(double) * ((cap + block_points<true>() - 1) / block_points<true>())));
   M.bcap = cap;
   return 0;
 }
@@ -758,9 +795,9 @@ int mlp_gradient(Context* c, const float* x, const float* y, const float* z, con
   a.feat = M.feat; a.Z = M.Z; a.gZ = M.gZ; a.gY = M.gY; a.bcap = M.bcap;
   a.loss_part = M.loss_part;
   a.inv_n = 1.f / (float)n;
-  const int nb = (int)((n + kBlockPts - 1) / kBlockPts);
-  hipLaunchKernelGGL(mlp_fwd_kernel<true>, dim3(nb), dim3(kBlock), kLdsBytes, s, a);
-  hipLaunchKernelGGL(mlp_bwd_kernel, dim3(nb), dim3(kBlock), kLdsBytes, s, a);
+  const int nb = (int)((n + block_points<true>() - 1) / block_points<true>());
+  hipLaunchKernelGGL(mlp_fwd_kernel<true>, dim3(nb), dim3(kBlockTrain), kLdsBytes, s, a);
+  hipLaunchKernelGGL(mlp_bwd_kernel, dim3(nb), dim3(kBlockTrain), kLdsBytes, s, a);
   BFE_HIP(c, hipGetLastError());
   const int nchunk = (int)((n + kChunk - 1) / kChunk);
   hipLaunchKernelGGL(mlp_dw_kernel, dim3(nchunk, 3), dim3(256), 0, s, M.Z, M.gZ, n, M.bcap, M.dwpart);
@@ -855,9 +892,9 @@ extern "C" int bfe_mlp_forward(bfe_ctx* h, const float* x, const float* y, const
   hipStream_t s = (hipStream_t)stream;
   MLPKArgs a = base_args(c);
   a.x = x; a.y = y; a.z = z; a.n = n; a.out = out;
-  const int64_t ntiles = (n + kBlockPts - 1) / kBlockPts;
+  const int64_t ntiles = (n + block_points<false>() - 1) / block_points<false>();
   const int nb = (int)(ntiles < 1024 ? ntiles : 1024);
-  hipLaunchKernelGGL(mlp_fwd_kernel<false>, dim3(nb), dim3(kBlock), kLdsBytes, s, a);
+  hipLaunchKernelGGL(mlp_fwd_kernel<false>, dim3(nb), dim3(kBlockInf), kLdsBytes, s, a);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }
@@ -915,7 +952,7 @@ extern "C" int bfe_mlp_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float*
   if (n == 0) return 0;
   BFE_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
-  const int64_t ntiles = (n + kBlockPts - 1) / kBlockPts;
+  const int64_t ntiles = (n + block_points<false>() - 1) / block_points<false>();
   const int nb = (int)(ntiles < 2048 ? ntiles : 2048);
   const int chunk = energy_every > 0 ? energy_every : (int)(nsteps < 100 ? (nsteps > 0 ? nsteps : 1) : 100);
   const int nchecks = energy_every > 0 ? (int)(nsteps / energy_every) + 1 : 1;
@@ -945,7 +982,7 @@ extern "C" int bfe_mlp_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float*
   a.energy_check = 1;
   a.partE = energy_log ? partE : nullptr;
   a.check_index = 0;
-  hipLaunchKernelGGL(mlp_leapfrog_kernel, dim3(nb), dim3(kBlock), kLdsBytes, s, a);
+  hipLaunchKernelGGL(mlp_leapfrog_kernel, dim3(nb), dim3(kBlockInf), kLdsBytes, s, a);
   BFE_HIP(c, hipGetLastError());
   a.init = 0;
   for (int64_t k0 = 0, ci = 0; k0 < nsteps; k0 += chunk, ++ci) {
@@ -954,7 +991,7 @@ extern "C" int bfe_mlp_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float*
     a.energy_check = (energy_every > 0 && a.k1 - a.k0 == energy_every) ? 1 : 0;
     a.partE = (energy_log && a.energy_check) ? partE : nullptr;
     a.check_index = (int)ci + 1;
-    hipLaunchKernelGGL(mlp_leapfrog_kernel, dim3(nb), dim3(kBlock), kLdsBytes, s, a);
+    hipLaunchKernelGGL(mlp_leapfrog_kernel, dim3(nb), dim3(kBlockInf), kLdsBytes, s, a);
     BFE_HIP(c, hipGetLastError());
   }
   if (drift_out || maxerr_out || counts_out)
