@@ -149,8 +149,8 @@ __global__ void eof_split_kernel(const double* __restrict__ sums, int mmax, int 
 template <int OPT>
 __global__ __launch_bounds__(256) void sl_accumulate_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-    const float* __restrict__ pm, int64_t n, SLGeom g, const float* __restrict__ E, int no_odd,
-    double* __restrict__ partials) {
+    const float* __restrict__ pm, int64_t n, SLGeom g, const float* __restrict__ E,
+    const double* __restrict__ lmc, int no_odd, double* __restrict__ partials) {
   extern __shared__ float smem[];
   const int R = (g.lmax + 1) * (g.lmax + 1);
   const int LN = (g.lmax + 1) * g.nmax;
@@ -163,6 +163,8 @@ __global__ __launch_bounds__(256) void sl_accumulate_kernel(
   int* s_i = (int*)(s_w2 + kSLTile);           // [kSLTile]
 
   const int tid = threadIdx.x;
+  const int qp = tid & (kSLTile - 1);  // particle of the tile
+  const int h = tid >> 6;              // this wave builds the Y columns m = h, h + 4, ...
   float acc[OPT];
   double accd[OPT];
   int yrow[OPT], prow[OPT];
@@ -180,62 +182,61 @@ __global__ __launch_bounds__(256) void sl_accumulate_kernel(
   }
   const int64_t ntiles = (n + kSLTile - 1) / kSLTile;
   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const int64_t p = t * kSLTile + tid;
-    if (tid < kSLTile) {
-      if (p < n) {
-        const double x = px[p], y = py[p], z = pz[p], mass = pm[p];
-        double r = sqrt(x * x + y * y + z * z);
-        r = (r == r) ? (r > 1.0e-10 ? r : 1.0e-10) : 1.0e-10;   // np.nanmax([r, 1e-10])
-        const double costh = z / r;
-        double c1, s1;
-        planar_angle(x, y, c1, s1);
+    const int64_t p = t * kSLTile + qp;
+    {
+      // padding particles: unit position, zero mass, so every Y row is 0
+      const bool live = p < n;
+      const double x = live ? (double)px[p] : 1.0, y = live ? (double)py[p] : 0.0;
+      const double z = live ? (double)pz[p] : 0.0, mass = live ? (double)pm[p] : 0.0;
+      double r = sqrt(x * x + y * y + z * z);
+      r = (r == r) ? (r > 1.0e-10 ? r : 1.0e-10) : 1.0e-10;   // np.nanmax([r, 1e-10])
+      const double costh = z / r;
+      double c1, s1;
+      planar_angle(x, y, c1, s1);
+      if (h == 0) {
         SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
         const double P0 = cell.x1 * g.p0[cell.i] + cell.x2 * g.p0[cell.i + 1];
-        s_i[tid] = cell.i;
-        s_w1[tid] = (float)(cell.x1 * P0);
-        s_w2[tid] = (float)(cell.x2 * P0);
-        // Y rows by the legendre_R recurrences, column by column (m outer);
-        // N_lm^2 = (2l+1)/(4 pi) (l-m)!/(l+m)! is carried along the same path.
-        const double w = -kFourPi * mass;
-        const bool bad = !(costh == costh);
-        const double somx2 = sqrt((1.0 - costh) * (1.0 + costh));
-        double pmm = 1.0, fact = 1.0, nmm2 = 0.25 / kPi;
-        double cmm = 1.0, smm = 0.0, cmm1 = c1, smm1 = -s1;
-        for (int m = 0; m <= g.lmax; ++m) {
-          if (m > 0) {
-            pmm *= -fact * somx2;
-            fact += 2.0;
-            nmm2 *= (2.0 * m + 1.0) / (2.0 * m - 1.0) / ((2.0 * m) * (2.0 * m - 1.0));
-            const double cn = (m == 1) ? c1 : 2.0 * c1 * cmm - cmm1;
-            const double sn = (m == 1) ? s1 : 2.0 * c1 * smm - smm1;
-            cmm1 = cmm; smm1 = smm; cmm = cn; smm = sn;
+        s_i[qp] = cell.i;
+        s_w1[qp] = live ? (float)(cell.x1 * P0) : 0.f;
+        s_w2[qp] = live ? (float)(cell.x2 * P0) : 0.f;
+      }
+      // Y rows -4 pi m N_lm P_l^m(cos th) {1, cos m phi, sin m phi}: P_m^m in
+      // closed form, then the legendre_R recurrence in l (spheresl.py:678-696)
+      // with tabulated coefficients; e^{i m phi} by complex powers.
+      const bool bad = !(costh == costh);
+      const double somx2 = sqrt((1.0 - costh) * (1.0 + costh));
+      double ch = 1.0, sh = 0.0, pw = 1.0;
+      for (int j = 0; j < h; ++j) {
+        const double cn = ch * c1 - sh * s1;
+        sh = sh * c1 + ch * s1;
+        ch = cn;
+        pw *= somx2;
+      }
+      const double c2 = c1 * c1 - s1 * s1, s2 = 2.0 * c1 * s1;
+      const double c4 = c2 * c2 - s2 * s2, s4 = 2.0 * c2 * s2;
+      const double sq2 = somx2 * somx2, p4 = sq2 * sq2;
+      for (int m = h; m <= g.lmax; m += 4) {
+        double plm1 = 0.0, pl = lmc[kLmcD + m] * pw;
+        for (int l = m; l <= g.lmax; ++l) {
+          const int lm = lm_index(l, m);
+          if (l > m) {
+            const double pn = costh * lmc[kLmcA + lm] * pl - lmc[kLmcB + lm] * plm1;
+            plm1 = pl;
+            pl = pn;
           }
-          double plm1 = 0.0, pl = pmm, n2 = nmm2;
-          for (int l = m; l <= g.lmax; ++l) {
-            if (l > m) {
-              const double pn = (l == m + 1) ? costh * (2.0 * m + 1.0) * pmm
-                                             : (costh * (2 * l - 1) * pl - (l + m - 1) * plm1) / (l - m);
-              plm1 = pl;
-              pl = pn;
-              n2 *= (2.0 * l + 1.0) / (2.0 * l - 1.0) * (double)(l - m) / (double)(l + m);
-            }
-            const bool skip = (no_odd && (l & 1)) || (bad && l > 0);
-            double Nlm = sqrt(n2);
-            if (m > 0) Nlm *= 1.4142135623730951;
-            const double base = w * Nlm * (l == 0 ? 1.0 : pl);
-            if (m == 0) {
-              s_Y[(l * l) * YS + tid] = skip ? 0.f : (float)base;
-            } else {
-              s_Y[(l * l + 2 * m - 1) * YS + tid] = skip ? 0.f : (float)(base * cmm);
-              s_Y[(l * l + 2 * m) * YS + tid] = skip ? 0.f : (float)(base * smm);
-            }
+          const bool skip = (no_odd && (l & 1)) || (bad && l > 0);
+          const double base = mass * lmc[kLmcY + lm] * pl;
+          if (m == 0) {
+            s_Y[(l * l) * YS + qp] = skip ? 0.f : (float)base;
+          } else {
+            s_Y[(l * l + 2 * m - 1) * YS + qp] = skip ? 0.f : (float)(base * ch);
+            s_Y[(l * l + 2 * m) * YS + qp] = skip ? 0.f : (float)(base * sh);
           }
         }
-      } else {
-        s_i[tid] = 0;
-        s_w1[tid] = 0.f;
-        s_w2[tid] = 0.f;
-        for (int row = 0; row < R; ++row) s_Y[row * YS + tid] = 0.f;
+        const double cn = ch * c4 - sh * s4;
+        sh = sh * c4 + ch * s4;
+        ch = cn;
+        pw *= p4;
       }
     }
     __syncthreads();
@@ -343,11 +344,11 @@ extern "C" int bfe_sl_accumulate(bfe_ctx* h, const float* x, const float* y, con
   if (n == 0) {
     BFE_HIP(c, hipMemsetAsync(partials, 0, pbytes, s));
   } else if (opt == 1) {
-    hipLaunchKernelGGL(sl_accumulate_kernel<1>, dim3(nb), dim3(256), lds, s, x, y, z, m, n, g, c->sl_e32, no_odd, partials);
+    hipLaunchKernelGGL(sl_accumulate_kernel<1>, dim3(nb), dim3(256), lds, s, x, y, z, m, n, g, c->sl_e32, c->sl_lmc, no_odd, partials);
   } else if (opt == 2) {
-    hipLaunchKernelGGL(sl_accumulate_kernel<2>, dim3(nb), dim3(256), lds, s, x, y, z, m, n, g, c->sl_e32, no_odd, partials);
+    hipLaunchKernelGGL(sl_accumulate_kernel<2>, dim3(nb), dim3(256), lds, s, x, y, z, m, n, g, c->sl_e32, c->sl_lmc, no_odd, partials);
   } else {
-    hipLaunchKernelGGL(sl_accumulate_kernel<4>, dim3(nb), dim3(256), lds, s, x, y, z, m, n, g, c->sl_e32, no_odd, partials);
+    hipLaunchKernelGGL(sl_accumulate_kernel<4>, dim3(nb), dim3(256), lds, s, x, y, z, m, n, g, c->sl_e32, c->sl_lmc, no_odd, partials);
   }
   BFE_HIP(c, hipGetLastError());
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), dim3(256), 0, s, partials, nb, K, coef_out);

@@ -57,9 +57,10 @@ static void free_ptr(void* p) {
 
 static void free_sl(Context* c) {
   free_ptr(c->sl_xi); free_ptr(c->sl_p0); free_ptr(c->sl_d0);
-  free_ptr(c->sl_ef64); free_ptr(c->sl_ev64); free_ptr(c->sl_e32);
+  free_ptr(c->sl_ef64); free_ptr(c->sl_ev64); free_ptr(c->sl_e32); free_ptr(c->sl_lmc);
   c->sl_xi = c->sl_p0 = c->sl_d0 = c->sl_ef64 = c->sl_ev64 = nullptr;
   c->sl_e32 = nullptr;
+  c->sl_lmc = nullptr;
   for (auto& f : c->slf) { free_ptr(f.d); f = SLFolded(); }
   c->sl_ok = false;
 }
@@ -133,7 +134,31 @@ extern "C" int bfe_sl_tables_set(bfe_ctx* h, int lmax, int nmax, int numr, int c
     double inv = 1.0 / sqrt(evtable[ln]);
     for (int i = 0; i < numr; ++i) e32[(size_t)i * LN + ln] = (float)(eftable[ln * numr + i] * inv);
   }
+  // recurrence and normalisation constants (spheresl.py:664-700, :823-863);
+  // N_lm^2 = (2l+1)/(4 pi) (l-m)!/(l+m)! from an exact product of integers
+  std::vector<double> lmc(kLmcSize, 0.0);
+  for (int l = 0; l <= kMaxL; ++l)
+    for (int m = 0; m <= l; ++m) {
+      long double ratio = 1.0L;
+      for (int k = l - m + 1; k <= l + m; ++k) ratio /= (long double)k;
+      long double n2 = (2.0L * l + 1.0L) / (4.0L * 3.14159265358979323846264338327950288L) * ratio;
+      long double nlm = sqrtl(n2) * (m > 0 ? sqrtl(2.0L) : 1.0L);
+      const int lm = lm_index(l, m);
+      lmc[kLmcY + lm] = (double)(-4.0L * 3.14159265358979323846264338327950288L * nlm);
+      if (l > m) {
+        lmc[kLmcA + lm] = (double)(2 * l - 1) / (double)(l - m);
+        lmc[kLmcB + lm] = (double)(l + m - 1) / (double)(l - m);
+      }
+    }
+  {
+    double d = 1.0;
+    for (int m = 0; m <= kMaxL; ++m) {
+      if (m > 0) d *= -(2.0 * m - 1.0);
+      lmc[kLmcD + m] = d;
+    }
+  }
   int rc;
+  if ((rc = dev_upload(c, &c->sl_lmc, lmc.data(), lmc.size()))) return rc;
   if ((rc = dev_upload(c, &c->sl_xi, xi, numr))) return rc;
   if ((rc = dev_upload(c, &c->sl_p0, p0, numr))) return rc;
   if ((rc = dev_upload(c, &c->sl_d0, d0, numr))) return rc;

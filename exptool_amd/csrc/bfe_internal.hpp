@@ -18,6 +18,13 @@ namespace bfe {
 constexpr int kMaxL = 16;          // SL lmax
 constexpr int kMaxM = 16;          // EOF mmax
 constexpr int kMaxLM = (kMaxL + 1) * (kMaxL + 2) / 2;
+// Layout of Context::sl_lmc (index lm_index(l, m)):
+//   [kLmcY + lm]  -4 pi N_lm (x sqrt 2 for m > 0), N_lm of factorial_return
+//   [kLmcA + lm]  (2l - 1)/(l - m)      P_l^m = x A P_{l-1}^m - B P_{l-2}^m
+//   [kLmcB + lm]  (l + m - 1)/(l - m)
+//   [kLmcD + m]   (-1)^m (2m - 1)!!     P_m^m = D_m (1 - x^2)^{m/2}
+constexpr int kLmcY = 0, kLmcA = kMaxLM, kLmcB = 2 * kMaxLM, kLmcD = 3 * kMaxLM;
+constexpr int kLmcSize = 3 * kMaxLM + kMaxL + 1;
 
 // Folded tables (layouts documented in bfe_field.hpp): coefficients summed
 // into the basis tables once per coefficient set.
@@ -67,6 +74,7 @@ struct Context {
   double* sl_ef64 = nullptr;   // (lmax+1, nmax, numr) float64 master copy
   double* sl_ev64 = nullptr;   // (lmax+1, nmax)
   float* sl_e32 = nullptr;     // [numr][(lmax+1)*nmax]  ef/sqrt(ev), node-major
+  double* sl_lmc = nullptr;    // per-(l, m) constants of the accumulate recurrences
   std::vector<double> h_xi;    // host copy (min xi for the field guard)
 
   // ---- EOF tables --------------------------------------------------------
