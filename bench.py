@@ -30,16 +30,33 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+PMC_SUMMARY = os.path.join(REPO, "profiles", "pmc_summary.json")
 
-# Algorithmic work per unit (reference formulation, M = mmax, N = norder,
-# L = lmax, n = nmax).  FLOPs: EOF accumulate 10 (2M+1) N per particle;
-# SL accumulate 2 (L+1)^2 n + 4 (L+1) n; bytes: 16 B in per particle.
+
+# Reference-formulation work per unit (M = mmax, N = norder, L = lmax, n = nmax):
+# FLOPs of the unfolded sums the reference evaluates, and the compulsory HBM
+# bytes (float32 SoA in, float32 out).
 def eof_acc_flops(M, N):
     return 10 * (2 * M + 1) * N
 
 
 def sl_acc_flops(L, n):
     return 2 * (L + 1) ** 2 * n + 4 * (L + 1) * n
+
+
+def field_ref_flops(M, N, L, n):
+    return 32 * (2 * M + 1) * N + 8 * (L + 1) ** 2 * n + 12 * (L + 1) * n
+
+
+def pmc_summary():
+    """Executed-flop counts and HBM traffic per dispatch measured with
+    rocprofv3 PMC counters for this workload (profiles/pmc_summary.json)."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def main():
@@ -132,21 +149,14 @@ def main():
     }
     M, N = eof_t["mmax"], eof_t["norder"]
     L, n = sl_t["lmax"], sl_t["nmax"]
+    # units per launch, reference-formulation flops and compulsory HBM bytes
     units = {
-        "eof_accumulate": (a.n_disc * eof_acc_flops(M, N), "flop"),
-        "sl_accumulate": (a.n_halo * sl_acc_flops(L, n), "flop"),
-        "field_eval_cart": (ntot * 28, "byte"),
+        "eof_accumulate": (a.n_disc, eof_acc_flops(M, N), 16),
+        "sl_accumulate": (a.n_halo, sl_acc_flops(L, n), 16),
+        "field_eval_cart": (ntot, field_ref_flops(M, N, L, n), 28),
     }
-    dom = max((k for k in units), key=lambda k: kt[k])
-    work, kind = units[dom]
-    if kind == "flop":
-        achieved = work / kt[dom] / 1e12
-        roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None}
-    else:
-        achieved = work / kt[dom] / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    dom = max(units, key=lambda k: kt[k])
+    roof = roofline(dom, units[dom], kt[dom], pmc_summary())
 
     # ---- orbit integration in the frozen field (plain BFE and MLP surrogate) --
     orbit = None
@@ -194,6 +204,39 @@ def main():
         "setup_seconds": setup_s,
     }
     print(json.dumps(line))
+
+
+def roofline(kernel, unit, seconds, pmc):
+    """Roofline record of the dominant kernel.  achieved = executed flops per
+    launch (the kernels evaluate exact rewrites of the reference sums, so their
+    own work, counted by PMC, is what the hardware did) / HIP-event time; the
+    bound is the vector ALU of the precision that dominates the executed flops.
+    traffic = HBM bytes per launch from the FETCH_SIZE/WRITE_SIZE passes."""
+    nunits, ref_flops, hbm_bytes = unit
+    rec = {"kernel": kernel, "units_per_launch": nunits,
+           "reference_flops_per_unit": ref_flops, "algorithmic_hbm_bytes_per_unit": hbm_bytes,
+           "algorithmic_hbm_gbs": nunits * hbm_bytes / seconds / 1e9}
+    ex = (pmc or {}).get("executed_flops", {}).get(kernel)
+    tr = (pmc or {}).get("traffic", {}).get(kernel)
+    if ex:
+        f64, f32 = ex["f64_flops_per_unit"], ex["f32_flops_per_unit"]
+        use64 = f64 >= f32
+        per_unit = f64 if use64 else f32
+        peak = FP64_PEAK_TFLOPS if use64 else FP32_PEAK_TFLOPS
+        achieved = nunits * per_unit / seconds / 1e12
+        rec.update({"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                    "frac": achieved / peak, "flops_per_unit": per_unit,
+                    "flops_precision": "f64" if use64 else "f32",
+                    "flops_source": "executed, rocprofv3 SQ_INSTS_VALU_* (profiles/bench_step_pmc_valu.md)"})
+    else:
+        achieved = rec["algorithmic_hbm_gbs"]
+        rec.update({"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS})
+    rec["traffic"] = tr["bytes"] if tr else None
+    if tr:
+        rec["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/bench_step_pmc_traffic.md)"
+        rec["traffic_gbs"] = tr["bytes"] / seconds / 1e9
+    return rec
 
 
 def mlp_stage(a, sysm, st0, orbit):
