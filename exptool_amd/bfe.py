@@ -129,9 +129,11 @@ class BFESystem:
 
     # --------------------------------------------------------------- evaluate
     def sl_eval(self, x, y, z, coef, L1=-1000, L2=1000, no_odd=False, compat=False,
-                density=True, stream=None):
+                density=True, cos_l=False, stream=None):
         """all_eval_particles outputs as float32 device tensors:
-        dict(den0, den1, pot0, pot1, potr, pott, potp)."""
+        dict(den0, den1, pot0, pot1, potr, pott, potp).  compat reproduces the
+        density and first-cell quirks; cos_l the cos/sin(l phi) factor of
+        spheresl.force_eval."""
         self._need("sl")
         x, y, z, n = self._xyz(x, y, z)
         coef = f64(coef, self.device)
@@ -141,6 +143,8 @@ class BFESystem:
         flags = (_lib.NO_ODD if no_odd else 0)
         if compat:
             flags |= _lib.COMPAT_SL_DENS | _lib.COMPAT_SL_EDGE
+        if cos_l:
+            flags |= _lib.COMPAT_SL_COS_L
         rc = self.ctx.lib.bfe_sl_eval(self.ctx.h, ptr(x), ptr(y), ptr(z), n, ptr(coef),
                                       int(max(min(L1, 2**30), -2**30)),
                                       int(max(min(L2, 2**30), -2**30)), flags,

@@ -267,7 +267,7 @@ extern "C" int bfe_sl_eval(bfe_ctx* h, const float* x, const float* y, const flo
   if (n == 0) return 0;
   const SLFolded& f = c->slf[0];
   const double densfac = (flags & BFE_COMPAT_SL_DENS) ? 0.25 * kPi : 0.25 / kPi;
-  const uint32_t ef = flags & (BFE_COMPAT_SL_DENS | BFE_COMPAT_SL_EDGE);
+  const uint32_t ef = flags & (BFE_COMPAT_SL_DENS | BFE_COMPAT_SL_EDGE | BFE_COMPAT_SL_COS_L);
   if (dens)
     hipLaunchKernelGGL(sl_eval_kernel<true>, dim3(nblocks256(n)), dim3(256), 0, s, x, y, z, n,
                        sl_geom(c), f.d, c->lmax, ef, densfac, den0, den1, pot0, pot1, potr, pott,

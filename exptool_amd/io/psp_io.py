@@ -9,7 +9,7 @@ File layout (little-endian):
                  (f4 or f8) + niatr i4 + ndatr floats
 The magic number selects float (f4) records, its absence double records
 (io/psp_io.py:176-189).  The info string is YAML in current EXP files; older
-files carry "name : force : force params : component params"
+files carry "name : force id : component params : force params"
 (tests/initial001dump.dat), for which indexing is off.
 
 Differences from the reference reader: old-style info strings are parsed
@@ -53,10 +53,12 @@ def parse_info_string(raw):
         if "indexing" not in d["parameters"]:
             d["parameters"]["indexing"] = str(d.get("indexing", "false")).lower() == "true"
         return d
+    # "name : force id : component parameters : force parameters"
     parts = [p.strip() for p in text.split(":")]
     d = {"name": parts[0], "id": parts[1] if len(parts) > 1 else "",
-         "force": {"parameters": _parse_params(parts[2]) if len(parts) > 2 else {}},
-         "parameters": _parse_params(parts[3]) if len(parts) > 3 else {}}
+         "parameters": _parse_params(parts[2]) if len(parts) > 2 else {},
+         "force": {"id": parts[1] if len(parts) > 1 else "",
+                   "parameters": _parse_params(parts[3]) if len(parts) > 3 else {}}}
     d["parameters"]["indexing"] = False
     d["style"] = "old"
     return d

@@ -88,7 +88,8 @@ int bfe_eof_accumulate(bfe_ctx* ctx, const float* x, const float* y, const float
 /* replaces: spheresl.all_eval_particles (spheresl.py:1240-1362) /
  * eval_particles (spheresl.py:502-528).  coef: device ((lmax+1)^2, nmax).
  * Terms with l outside [L1, L2] are skipped (l >= 1).  Any output may be NULL.
- * flags: BFE_NO_ODD, BFE_COMPAT_SL_DENS, BFE_COMPAT_SL_EDGE. */
+ * flags: BFE_NO_ODD, BFE_COMPAT_SL_DENS, BFE_COMPAT_SL_EDGE, and
+ * BFE_COMPAT_SL_COS_L for the force_eval azimuthal factor (spheresl.py:1107-1234). */
 int bfe_sl_eval(bfe_ctx* ctx, const float* x, const float* y, const float* z, int64_t n,
                 const double* coef, int L1, int L2, uint32_t flags, float* den0, float* den1,
                 float* pot0, float* pot1, float* potr, float* pott, float* potp, void* stream);

@@ -1,0 +1,96 @@
+"""Host logic of the drop-in modules: the PSP reader on a data file the
+reference's own tests hold (tests/initial001dump.dat, old-style info strings,
+float records), PSP and SL-coefficient writers and readers, the apse stop and
+the force_eval truncation rule.  No GPU calls."""
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PSP = os.path.join(HERE, "golden", "psp_initial001dump.dat")
+
+
+def test_psp_reference_fixture():
+    from exptool_amd.io import particle, psp_io
+    I = psp_io.Input(PSP)
+    assert I.time == 0.0 and list(I.header) == ["dark", "star"]
+    assert I._float_len == 4
+    d = I.header["dark"]
+    assert d["nbodies"] == 30000 and d["id"] == "sphereSL"
+    assert d["force"]["parameters"]["Lmax"] == 6 and d["force"]["parameters"]["nmax"] == 20
+    assert d["parameters"]["indexing"] is False
+    s = I.header["star"]
+    assert s["force"]["parameters"]["mmax"] == 6 and s["force"]["parameters"]["ncylorder"] == 12
+    # the two components tile the file exactly
+    assert d["data_start"] == 16 + 24 + 1024 and s["data_end"] == os.path.getsize(PSP)
+    star = psp_io.Input(PSP, comp="star").data
+    assert set(star) == {"m", "x", "y", "z", "vx", "vy", "vz", "potE"}
+    assert star["x"].dtype == np.float32 and star["x"].size == 30000
+    assert np.all(star["m"] > 0) and np.all(np.isfinite(star["potE"]))
+    L = particle.Input(PSP, comp="dark", legacy=True)
+    assert L.xpos.size == 30000 and L.mass.dtype == np.float32
+    with pytest.raises(IOError):
+        psp_io.Input(PSP, comp="gas")
+
+
+@pytest.mark.parametrize("float_len,index", [(4, False), (8, True)])
+def test_psp_roundtrip(tmp_path, float_len, index):
+    from exptool_amd.io import psp_io
+    g = np.random.default_rng(1)
+    comps = []
+    for name, n in (("disc", 100), ("halo", 57)):
+        d = {k: g.normal(size=n) for k in ("m", "x", "y", "z", "vx", "vy", "vz", "potE")}
+        if index:
+            d["id"] = np.arange(n, dtype=np.int64) + 7
+        comps.append(({"name": name, "id": "x", "parameters": {"a": 1}}, d))
+    f = tmp_path / "OUT.t.00001"
+    psp_io.write_psp(f, 2.25, comps, float_len=float_len)
+    for (h, d) in comps:
+        I = psp_io.Input(str(f), comp=h["name"])
+        assert I.time == 2.25 and I._float_len == float_len
+        dt = np.float32 if float_len == 4 else np.float64
+        for k, v in d.items():
+            assert np.array_equal(I.data[k], v.astype(np.int64 if k == "id" else dt)), k
+
+
+def test_sl_coefficient_file_roundtrip(tmp_path):
+    from exptool_amd.basis import spheresl
+    f = str(tmp_path / "coefs.sl")
+    objs = []
+    for t in (0.5, 1.0):
+        o = spheresl.SL_Object()
+        o.time, o.filename, o.comp, o.nbodies = t, "OUT.run.001", "dark", 1234
+        o.sph_file, o.model_file, o.lmax, o.nmax = "sl.cache", "sl.model", 3, 5
+        o.expcoef = np.random.default_rng(int(t * 10)).normal(size=(16, 5))
+        spheresl.save_sl_coefficients(f, o)
+        objs.append(o)
+    assert os.path.getsize(f) == 4 + 2 * (324 + 8 * 16 * 5)
+    last, d = spheresl.restore_sl_coefficients(f)
+    assert sorted(d) == [0.5, 1.0]
+    for o in objs:
+        r = d[o.time]
+        assert np.array_equal(r.expcoef, o.expcoef) and r.lmax == 3 and r.nmax == 5
+        assert r.comp == b"dark" and r.nbodies == 1234
+    assert last.time == 1.0
+
+
+def test_force_eval_truncation_rule():
+    from exptool_amd.basis.spheresl import _truncated
+    c = np.ones((16, 6))
+    assert np.array_equal(_truncated(c, 3, 5, 3, 6), c)          # nmax_table - 1: no cut
+    assert np.array_equal(_truncated(c, 3, 6, 3, 6), c)
+    t = _truncated(c, 1, 2, 3, 6)
+    assert t[:4, :2].sum() == 8 and t.sum() == 8
+
+
+def test_apse_stop_counts_apocentres():
+    from exptool_amd.utils.integrate import _apse_stop
+    t = np.linspace(0, 10 * np.pi, 2001)
+    R = 1.0 + 0.2 * np.cos(t)          # apocentres at t = 2 pi k
+    X, Y = R * np.cos(0.3 * t), R * np.sin(0.3 * t)
+    n1 = _apse_stop(X, Y, 1)
+    n2 = _apse_stop(X, Y, 2)
+    # first interior maximum of R at t = 2 pi (step 400): counted at step 401
+    assert n1 == 402 and n2 == 802
+    assert _apse_stop(X, Y, 100) == t.size

@@ -1,0 +1,206 @@
+"""The drop-in modules (exptool_amd.basis.spheresl / eof / potential.Fields,
+exptool_amd.utils.integrate) against the FP64 oracle and the golden vectors
+the reference wrote, going through files in the EXP formats exactly as a
+reference user would.
+
+Tolerances as tests/test_gpu_parity.py: coefficients <= 1e-6 per block,
+evaluator outputs and forces <= 1e-5 of the largest value (<= 1e-4 against
+the reference's own vectors, which use float64 particles), trajectories
+<= 1e-4 over 200 steps.
+"""
+import numpy as np
+import pytest
+
+from oracle import bfe_oracle as O
+from tests.test_golden import eof_table, sl_table
+from tests.test_gpu_parity import block_rel_eof, block_rel_sl, relmax
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def files(golden, tmp_path_factory):
+    from exptool_amd.basis import eof as E
+    from exptool_amd.io import psp_io
+    from exptool_amd.utils import halo_methods as H
+    g = golden
+    d = tmp_path_factory.mktemp("dropin")
+    lmax, nmax, numr = int(g["sl_lmax"]), int(g["sl_nmax"]), int(g["sl_numr"])
+    H.write_cached_table(d / "sl.cache", lmax, nmax, numr, int(g["sl_cmap"]), float(g["sl_rmin"]),
+                         float(g["sl_rmax"]), float(g["sl_scale"]), g["sl_ev"], g["sl_ef"])
+    H.write_sph_model_table(d / "sl.model", g["sl_r"], g["sl_d0"] / (4 * np.pi), np.zeros(numr),
+                            g["sl_p0"])
+    p = g["eof_params"]
+    eof = dict(mmax=int(p[4]), norder=int(p[5]), numx=int(p[2]), numy=int(p[3]), cmap=int(p[8]),
+               rmin=p[0], rmax=p[1], ascale=p[6], hscale=p[7], dens=int(p[9]))
+    for k in ("potC", "rforceC", "zforceC", "densC", "potS", "rforceS", "zforceS", "densS"):
+        eof[k] = g["eof_" + k]
+    E.write_eof_cache(d / "eof.cache", eof)
+    part = {k: g[q].astype(np.float32) for k, q in (("x", "px"), ("y", "py"), ("z", "pz"), ("m", "pm"))}
+    for k in ("vx", "vy", "vz", "potE"):
+        part[k] = np.zeros_like(part["x"])
+    psp_io.write_psp(d / "OUT.run.00000", 1.5,
+                     [({"name": "star", "id": "cylinder"}, part), ({"name": "dark", "id": "sphereSL"}, part)])
+    return d
+
+
+def _f32(*a):
+    return [np.asarray(v, np.float32).astype(np.float64) for v in a]
+
+
+def test_spheresl_coefficients(files, golden):
+    from exptool_amd.basis import spheresl
+    from exptool_amd.io import particle
+    P = particle.Input(str(files / "OUT.run.00000"), comp="dark")
+    slt = sl_table(golden)
+    x, y, z, m = _f32(*(golden[k] for k in ("px", "py", "pz", "pm")))
+    for no_odd in (False, True):
+        obj = spheresl.compute_coefficients(P, str(files / "sl.cache"), str(files / "sl.model"),
+                                            no_odd=no_odd)
+        assert obj.lmax == slt.lmax and obj.nmax == slt.nmax and obj.nbodies == x.size
+        ref = O.sl_accumulate(slt, x, y, z, m, no_odd=no_odd)
+        assert block_rel_sl(obj.expcoef, ref, slt.lmax) <= 1e-6
+    h = particle.make_holder(x, y, z, m)
+    c = spheresl.compute_coefficients_solitary(h, str(files / "sl.cache"), str(files / "sl.model"))
+    assert block_rel_sl(c, golden["sl_coef"], slt.lmax) <= 1e-5    # reference, float64 particles
+
+
+def test_spheresl_eval_and_force_eval(files, golden):
+    from exptool_amd.basis import spheresl
+    from exptool_amd.io import particle
+    P = particle.Input(str(files / "OUT.run.00000"), comp="dark")
+    c = golden["sl_coef"]
+    got = spheresl.all_eval_particles(P, c, str(files / "sl.cache"), str(files / "sl.model"),
+                                      compat=True)
+    ref = golden["sl_alleval"]
+    for j in range(7):
+        assert relmax(got[j], ref[j]) <= 1e-4, j
+    x, y, z = (golden[k][:120] for k in ("px", "py", "pz"))
+    r3 = np.sqrt(x * x + y * y + z * z) + 1e-15
+    s = sl_table(golden)
+    for key, kw in (("sl_forceeval", dict(lmax=s.lmax, nmax=s.nmax)),
+                    ("sl_forceeval_l2n3", dict(lmax=2, nmax=3)),
+                    ("sl_forceeval_noodd", dict(lmax=s.lmax, nmax=s.nmax, no_odd=True))):
+        out = spheresl.force_eval(r3, z / r3, np.arctan2(y, x), c, s.xi, s.p0, s.d0, s.cmap, s.scale,
+                                  evtable=s.evtable, eftable=s.eftable, compat=True, **kw)
+        for j in range(5):
+            assert relmax(out[j], golden[key][:, j]) <= 1e-4, (key, j)
+
+
+def test_spheresl_truncation_quirk(files, golden):
+    """nmax = nmax_table - 1 keeps every radial order (spheresl.py:1143)."""
+    from exptool_amd.basis import spheresl
+    s = sl_table(golden)
+    x, y, z = (golden[k][:50] for k in ("px", "py", "pz"))
+    r3 = np.sqrt(x * x + y * y + z * z)
+    args = (r3, z / r3, np.arctan2(y, x), golden["sl_coef"], s.xi, s.p0, s.d0, s.cmap, s.scale)
+    full = spheresl.force_eval(*args, lmax=s.lmax, nmax=s.nmax, evtable=s.evtable, eftable=s.eftable)
+    quirk = spheresl.force_eval(*args, lmax=s.lmax, nmax=s.nmax - 1, evtable=s.evtable, eftable=s.eftable)
+    cut = spheresl.force_eval(*args, lmax=s.lmax, nmax=s.nmax - 2, evtable=s.evtable, eftable=s.eftable)
+    assert all(np.array_equal(a, b) for a, b in zip(full, quirk))
+    assert not np.allclose(full[3], cut[3])
+
+
+def test_eof_dropin(files, golden):
+    from exptool_amd.basis import eof as E
+    from exptool_amd.io import particle
+    P = particle.Input(str(files / "OUT.run.00000"), comp="star")
+    obj = E.compute_coefficients(P, str(files / "eof.cache"))
+    t = eof_table(golden)
+    x, y, z, m = _f32(*(golden[k] for k in ("px", "py", "pz", "pm")))
+    rc, rs = O.eof_accumulate(t, x, y, z, m)
+    assert block_rel_eof(obj.cos, rc) <= 1e-6 and block_rel_eof(obj.sin, rs) <= 1e-6
+    out = E.compute_forces(P, obj)
+    ref = O.eof_eval_particles(t, x, y, z, obj.cos, obj.sin)
+    for j in range(5):
+        assert relmax(out[j], ref[j]) <= 1e-5, j
+
+
+def _fields(files, golden, compat=True):
+    from exptool_amd.basis.potential import Fields
+    F = Fields(str(files / "OUT.run.00000"), str(files / "eof.cache"), str(files / "sl.cache"),
+               str(files / "sl.model"), compat=compat)
+    F.prep_tables()
+    F.total_coefficients()
+    # use the reference's coefficients so outputs compare with its vectors
+    F.EOF.cos, F.EOF.sin, F.SL.expcoef = golden["eof_cos"], golden["eof_sin"], golden["sl_coef"]
+    return F
+
+
+def test_fields_cart_scalar_and_array(files, golden):
+    F = _fields(files, golden)
+    x, y, z = (golden[k][:120] for k in ("px", "py", "pz"))
+    rot = golden["fields_rotpos"]
+    ref = golden["fields_cart"]
+    one = np.array(F.return_forces_cart(float(x[0]), float(y[0]), float(z[0]), rotpos=float(rot[0])))
+    scale = np.max(np.abs(ref), axis=0)
+    assert np.all(np.abs(one - ref[0]) <= 1e-4 * scale)
+    same = np.flatnonzero(rot == rot[0])
+    if same.size > 1:
+        arr = np.array(F.return_forces_cart(x[same], y[same], z[same], rotpos=float(rot[0]))).T
+        assert np.all(np.abs(arr - ref[same]) <= 1e-4 * scale)
+    got = np.array([F.return_forces_cart(float(x[i]), float(y[i]), float(z[i]), rotpos=float(rot[i]))
+                    for i in range(0, 120, 7)])
+    assert np.all(np.abs(got - ref[::7]) <= 1e-4 * scale)
+
+
+def test_fields_total_coefficients_match_oracle(files, golden):
+    from exptool_amd.basis.potential import Fields
+    F = Fields(str(files / "OUT.run.00000"), str(files / "eof.cache"), str(files / "sl.cache"),
+               str(files / "sl.model"))
+    F.total_coefficients()
+    x, y, z, m = _f32(*(golden[k] for k in ("px", "py", "pz", "pm")))
+    rc, _ = O.eof_accumulate(eof_table(golden), x, y, z, m)
+    assert block_rel_eof(F.EOF.cos, rc) <= 1e-6
+    assert block_rel_sl(F.SL.expcoef, O.sl_accumulate(sl_table(golden), x, y, z, m), 3) <= 1e-6
+    assert F.halofac == 1.0 and F.time == pytest.approx(1.5)
+
+
+def test_fields_cyl_composition(files, golden):
+    """return_forces_cyl is the reference's composition (potential.py:389-440)
+    of the two force_eval calls, restated here from the oracle's pieces."""
+    F = _fields(files, golden, compat=True)
+    x, y, z = (golden[k][:60] for k in ("px", "py", "pz"))
+    rot = 0.3
+    got = np.array(F.return_forces_cyl(x, y, z, rotpos=rot))
+    r2 = np.sqrt(x * x + y * y) + 1e-10
+    r3 = np.sqrt(r2 * r2 + z * z) + 1e-10
+    ph = np.arctan2(y, x) + rot
+    dfr, dfp, dfz, dp, _ = O.eof_force_eval(eof_table(golden), r2, z, ph, golden["eof_cos"],
+                                            golden["eof_sin"], compat=True)
+    hfr, hft, hfp, hp1, hp0 = O.sl_force_eval(sl_table(golden), r3, z / r3, ph, golden["sl_coef"],
+                                              compat=True)
+    g = r3 < np.min(golden["sl_xi"])
+    dfp, hfp = np.where(g, 0, dfp), np.where(g, 0, hfp)
+    ref = np.array([dfr, -(r2 * hfr + z * hft) / r3, dfp, -hfp, dfz, -(z * hfr - r2 * hft) / r3,
+                    -dp, hp1 + hp0])
+    for j in range(8):
+        assert relmax(got[j], ref[j]) <= 1e-4, j
+
+
+def test_leapfrog_integrate_single_and_batch(files, golden):
+    from exptool_amd.utils.integrate import leapfrog_integrate
+    F = _fields(files, golden)
+    init = golden["leap_init"]
+    ref = golden["leap_orbits"]
+    keys = ("T", "X", "Y", "Z", "VX", "VY", "VZ", "P", "FX", "FY", "FZ")
+    rf, dt = float(golden["leap_rotfreq"]), float(golden["leap_dt"])
+    for i in range(init.shape[0]):
+        O1 = leapfrog_integrate(F, 200, dt, init[i, :3], init[i, 3:], rotfreq=rf, force=True)
+        for j, k in enumerate(keys):
+            assert relmax(O1[k], ref[i, j]) <= 1e-4, (i, k)
+        assert set(("TX", "TY", "VTX", "VTY")) <= set(O1)
+    OB = leapfrog_integrate(F, 200, dt, init[:, :3].T, init[:, 3:].T, rotfreq=rf)
+    for j, k in enumerate(keys[:8]):
+        assert relmax(OB[k], ref[:, j]) <= 1e-4, k
+
+
+def test_leapfrog_integrate_apse(files, golden):
+    from exptool_amd.utils.integrate import leapfrog_integrate, _apse_stop
+    F = _fields(files, golden)
+    init = golden["leap_init"][0]
+    full = leapfrog_integrate(F, 200, float(golden["leap_dt"]), init[:3], init[3:])
+    n = _apse_stop(full["X"], full["Y"], 1)
+    cut = leapfrog_integrate(F, 200, float(golden["leap_dt"]), init[:3], init[3:], apse=True, ap_max=1)
+    assert cut["X"].size == n
+    assert np.array_equal(cut["X"], full["X"][:n])
