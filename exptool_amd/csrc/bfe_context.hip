@@ -103,6 +103,7 @@ extern "C" int bfe_destroy(bfe_ctx* h) {
   free_sl(c);
   free_eof(c);
   mlp_free(c);
+  order_free(c);
   if (c->scratch) (void)hipFree(c->scratch);
   delete h;
   return 0;

@@ -212,11 +212,16 @@ __global__ __launch_bounds__(256) void eof_eval_kernel(
   }
 }
 
+// perm (may be NULL): cell order of the points; block b then takes logical
+// block xcd_block(nb) so that each XCD evaluates a contiguous range of cells.
 __global__ __launch_bounds__(256) void field_eval_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-    int64_t n, FieldTables T, float* fx, float* fy, float* fz, float* pot, float* comp) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+    int64_t n, const int* __restrict__ perm, FieldTables T, float* fx, float* fy, float* fz,
+    float* pot, float* comp) {
+  const int64_t nb = (n + 255) / 256;
+  const int64_t j = (perm ? xcd_block(nb) : (int64_t)blockIdx.x) * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int64_t i = perm ? (int64_t)perm[j] : j;
   double cv[8];
   field_eval_point(T, px[i], py[i], pz[i], cv);
   if (fx) fx[i] = (float)(cv[0] + cv[1]);
@@ -345,7 +350,13 @@ extern "C" int bfe_field_eval_cart(bfe_ctx* h, const float* x, const float* y, c
   if (n == 0) return 0;
   BFE_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(field_eval_kernel, dim3(nblocks256(n)), dim3(256), 0, s, x, y, z, n, T, fx, fy,
+  const int* perm = nullptr;
+  if (n >= kOrderMinPoints) {
+    rc = order_points(c, T, x, y, z, n, s, &perm);
+    if (rc) return rc;
+  }
+  const unsigned grid = perm ? xcd_grid(nblocks256(n)) : nblocks256(n);
+  hipLaunchKernelGGL(field_eval_kernel, dim3(grid), dim3(256), 0, s, x, y, z, n, perm, T, fx, fy,
                      fz, pot, comp);
   BFE_HIP(c, hipGetLastError());
   return 0;

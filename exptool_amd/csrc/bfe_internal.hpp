@@ -98,6 +98,10 @@ struct Context {
   // ---- scratch (grown on demand, freed by bfe_destroy) -------------------
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+
+  // ---- spatial ordering of evaluation points (bfe_order.hip) -------------
+  void* order = nullptr;       // keys[n] | perm[n] | per-orbit energies[n] | bins[nbins + 1]
+  size_t order_bytes = 0;
 };
 
 // Table geometry passed to kernels by value.
@@ -120,6 +124,32 @@ inline SLGeom sl_geom(const Context* c) {
 inline EOFGeom eof_geom(const Context* c) {
   return EOFGeom{c->ecmap, c->numx, c->numy, c->mmax, c->norder,
                  c->ascale, c->hscale, c->xmin, c->dx, c->ymin, c->dy};
+}
+
+// Points with at least this many entries are evaluated in cell order.
+constexpr int64_t kOrderMinPoints = 65536;
+
+struct FieldTables;
+// Permutation perm[j] = i listing the points in order of their table cell
+// (EOF (R, z) cell inside the disc grid, else the SL radial cell), so that
+// the lanes of a wave read the same table rows.  Rebuilt by each call; the
+// returned pointer stays valid until the next call on this context.
+int order_points(Context* c, const FieldTables& T, const float* x, const float* y,
+                 const float* z, int64_t n, hipStream_t s, const int** perm_out);
+// float64 per-point scratch in the ordering buffer (n entries), valid after
+// order_points for the same n.
+double* order_energy_buffer(Context* c, int64_t n);
+// Release the ordering buffer (bfe_destroy).
+void order_free(Context* c);
+
+// Workgroups are dispatched to the 8 XCDs round-robin; this maps them so that
+// each XCD runs one contiguous eighth of the logical blocks (and so of the
+// cell-ordered points), keeping each XCD's L2 on its own part of the tables.
+// Launch with xcd_grid(nb) blocks; logical blocks >= nb do nothing.
+inline unsigned xcd_grid(int64_t nb) { return (unsigned)(8 * ((nb + 7) / 8)); }
+__device__ __forceinline__ int64_t xcd_block(int64_t nb) {
+  const int64_t per = (nb + 7) / 8;
+  return (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
 int set_error(Context* c, int code, const char* fmt, ...);

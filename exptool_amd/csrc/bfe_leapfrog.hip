@@ -57,16 +57,21 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
+// Orbits are processed in cell order: lane j of logical block xcd_block(nb)
+// advances orbit perm[j].  Energies at a checkpoint go to ebuf[orbit] and are
+// summed per block of 256 consecutive orbits by energy_block_kernel, so the
+// energy log does not depend on the order.
+
 // Step 0: initial acceleration, energy and trajectory record.
 __global__ __launch_bounds__(256) void leapfrog_init_kernel(
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
     const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz,
-    FieldTables T, LeapArgs a, OrbitAux* __restrict__ aux, double* __restrict__ partE,
-    float* __restrict__ traj) {
-  __shared__ double sh[256];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double e = 0.0;
-  if (i < a.n) {
+    const int* __restrict__ perm, FieldTables T, LeapArgs a, OrbitAux* __restrict__ aux,
+    double* __restrict__ ebuf, float* __restrict__ traj) {
+  const int64_t j = xcd_block(a.nblocks) * 256 + threadIdx.x;
+  if (j < a.n) {
+    const int64_t i = perm[j];
+    double e;
     rot_of(a, a.k0, T);
     double cv[8];
     field_eval_point(T, x[i], y[i], z[i], cv);
@@ -74,6 +79,7 @@ __global__ __launch_bounds__(256) void leapfrog_init_kernel(
     const float pot = (float)(cv[6] + cv[7]);
     const double v2 = (double)vx[i] * vx[i] + (double)vy[i] * vy[i] + (double)vz[i] * vz[i];
     e = 0.5 * v2 + (double)pot;
+    if (ebuf) ebuf[i] = e;
     OrbitAux o;
     o.ax = fx; o.ay = fy; o.az = fz; o.pad = 0.f;
     o.e0 = e;
@@ -87,19 +93,15 @@ __global__ __launch_bounds__(256) void leapfrog_init_kernel(
       t[6 * n + i] = pot;
     }
   }
-  if (partE) {
-    const double s = block_sum(e, sh);
-    if (threadIdx.x == 0) partE[(size_t)a.check0 * a.nblocks + blockIdx.x] = s;
-  }
 }
 
 __global__ __launch_bounds__(256) void leapfrog_chunk_kernel(
     float* __restrict__ x, float* __restrict__ y, float* __restrict__ z, float* __restrict__ vx,
-    float* __restrict__ vy, float* __restrict__ vz, FieldTables T, LeapArgs a,
-    OrbitAux* __restrict__ aux, double* __restrict__ partE, float* __restrict__ traj) {
-  __shared__ double sh[256];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = i < a.n;
+    float* __restrict__ vy, float* __restrict__ vz, const int* __restrict__ perm, FieldTables T,
+    LeapArgs a, OrbitAux* __restrict__ aux, double* __restrict__ ebuf, float* __restrict__ traj) {
+  const int64_t j = xcd_block(a.nblocks) * 256 + threadIdx.x;
+  const bool live = j < a.n;
+  const int64_t i = live ? (int64_t)perm[j] : 0;
   float px = 0, py = 0, pz = 0, ux = 0, uy = 0, uz = 0, ax = 0, ay = 0, az = 0;
   OrbitAux o{};
   if (live) {
@@ -136,22 +138,28 @@ __global__ __launch_bounds__(256) void leapfrog_chunk_kernel(
       }
     }
   }
-  double e = 0.0;
   if (live) {
     x[i] = px; y[i] = py; z[i] = pz;
     vx[i] = ux; vy[i] = uy; vz[i] = uz;
     o.ax = ax; o.ay = ay; o.az = az;
     if (a.energy_every > 0 && a.k1 > a.k0) {
-      e = 0.5 * ((double)ux * ux + (double)uy * uy + (double)uz * uz) + (double)pot;
+      const double e = 0.5 * ((double)ux * ux + (double)uy * uy + (double)uz * uz) + (double)pot;
       const double rel = fabs(e - o.e0) / fabs(o.e0);
       if (rel > o.drift || !(rel == rel)) o.drift = rel;
+      if (ebuf) ebuf[i] = e;
     }
     aux[i] = o;
   }
-  if (partE && a.energy_every > 0 && a.k1 > a.k0) {
-    const double s = block_sum(e, sh);
-    if (threadIdx.x == 0) partE[(size_t)(a.check0 + 1) * a.nblocks + blockIdx.x] = s;
-  }
+}
+
+// Block sums of the checkpoint energies over 256 consecutive orbits (input
+// order), fixed tree: partE[check * nblocks + b].
+__global__ __launch_bounds__(256) void energy_block_kernel(const double* __restrict__ ebuf,
+                                                           int64_t n, double* __restrict__ partE) {
+  __shared__ double sh[256];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double s = block_sum(i < n ? ebuf[i] : 0.0, sh);
+  if (threadIdx.x == 0) partE[blockIdx.x] = s;
 }
 
 __global__ void leapfrog_finish_kernel(const OrbitAux* __restrict__ aux, int64_t n,
@@ -205,6 +213,13 @@ extern "C" int bfe_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float* vx,
   if (rc) return rc;
   OrbitAux* aux = (OrbitAux*)c->scratch;
   double* partE = (double*)((char*)c->scratch + ((aux_bytes + 255) & ~(size_t)255));
+  // cell order of the initial positions; without a graph it is rebuilt from
+  // the current positions before every chunk
+  const int* perm = nullptr;
+  rc = order_points(c, T, x, y, z, n, s, &perm);
+  if (rc) return rc;
+  double* ebuf = energy_log ? order_energy_buffer(c, n) : nullptr;
+  const unsigned grid = xcd_grid(nb);
   LeapArgs a;
   a.dt = dt;
   a.t0 = t0;
@@ -216,9 +231,10 @@ extern "C" int bfe_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float* vx,
   a.check0 = 0;
   a.record_every = record_every;
   a.nblocks = nb;
-  double* pe = energy_log ? partE : nullptr;
-  hipLaunchKernelGGL(leapfrog_init_kernel, dim3(nb), dim3(256), 0, s, x, y, z, vx, vy, vz, T, a,
-                     aux, pe, traj);
+  hipLaunchKernelGGL(leapfrog_init_kernel, dim3(grid), dim3(256), 0, s, x, y, z, vx, vy, vz, perm,
+                     T, a, aux, ebuf, traj);
+  if (ebuf)
+    hipLaunchKernelGGL(energy_block_kernel, dim3(nb), dim3(256), 0, s, ebuf, n, partE);
   BFE_HIP(c, hipGetLastError());
 
   std::vector<LeapArgs> chunks;
@@ -237,9 +253,20 @@ extern "C" int bfe_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float* vx,
     if (s == nullptr) return set_error(c, BFE_EINVAL, "leapfrog: use_graph needs a non-default stream");
     BFE_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   }
-  for (const LeapArgs& b : chunks)
-    hipLaunchKernelGGL(leapfrog_chunk_kernel, dim3(nb), dim3(256), 0, s, x, y, z, vx, vy, vz, T, b,
-                       aux, pe, traj);
+  for (const LeapArgs& b : chunks) {
+    // re-sort by the current cells between chunks (not inside a captured
+    // graph: the sort issues hipMemsetAsync)
+    if (b.k0 > 0 && !use_graph) {
+      rc = order_points(c, T, x, y, z, n, s, &perm);
+      if (rc) return rc;
+    }
+    const bool check = ebuf && b.energy_every > 0;
+    hipLaunchKernelGGL(leapfrog_chunk_kernel, dim3(grid), dim3(256), 0, s, x, y, z, vx, vy, vz,
+                       perm, T, b, aux, check ? ebuf : nullptr, traj);
+    if (check)
+      hipLaunchKernelGGL(energy_block_kernel, dim3(nb), dim3(256), 0, s, ebuf, n,
+                         partE + (size_t)(b.check0 + 1) * nb);
+  }
   if (use_graph) {
     hipError_t e = hipStreamEndCapture(s, &graph);
     if (e != hipSuccess) return check_hip(c, e, "hipStreamEndCapture");

@@ -1,0 +1,121 @@
+// Cell ordering of evaluation points.
+//
+// The folded field tables (SL 1.8 MB, EOF 7.4 MB at the workload size) do
+// not fit in one XCD's 4 MB L2, and points in input order send every lane of
+// a wave to a different table cell.  Sorting the points by table cell lets a
+// wave's lanes share cells and, with the XCD-aware block mapping, each XCD
+// work on one contiguous range of cells.
+//
+// Key: the EOF (ix, iy) cell for points inside the disc grid, else
+// numx * numy + the SL radial cell.  The keys are sorted with rocPRIM's radix
+// sort (values: the point indices); a counting sort with global atomics was
+// slower here because thousands of points share the busiest cells.  No
+// result depends on the order (evaluation and orbit integration are per
+// point, and energy sums are taken in input order).  The sort issues
+// hipMemsetAsync, so it is never placed inside a captured graph.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
+
+#include "bfe_field.hpp"
+
+namespace bfe {
+
+__device__ __forceinline__ uint32_t cell_key(const FieldTables& T, float xf, float yf, float zf,
+                                             uint32_t neof) {
+  const double x = xf, y = yf, z = zf;
+  const double R = sqrt(x * x + y * y);
+  const EOFGeom& g = T.eg;
+  const double X = (r_to_xi(R, g.cmap, g.ascale) - g.xmin) / g.dx;
+  const double Y = (z_to_y(z, g.hscale) - g.ymin) / g.dy;
+  if (X >= 0.0 && X < (double)g.numx && Y >= 0.0 && Y < (double)g.numy)
+    return (uint32_t)X * (uint32_t)g.numy + (uint32_t)Y;
+  const SLGeom& s = T.slg;
+  const double xi = r_to_xi(sqrt(R * R + z * z), s.cmap, s.scale);
+  const double f = floor((xi - s.xi[0]) / (s.xi[1] - s.xi[0]));
+  int i = 0;
+  if (f == f) i = f < 0.0 ? 0 : (f > (double)(s.numr - 2) ? s.numr - 2 : (int)f);
+  return neof + (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void order_key_kernel(
+    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+    int64_t n, FieldTables T, uint32_t neof, uint32_t* __restrict__ keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = cell_key(T, x[i], y[i], z[i], neof);
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct OrderLayout {
+  size_t keys, sorted, perm, energy, temp, total;
+};
+
+static OrderLayout order_layout(int64_t n, size_t temp_bytes) {
+  OrderLayout L;
+  L.keys = 0;
+  L.sorted = L.keys + align256((size_t)n * 4);
+  L.perm = L.sorted + align256((size_t)n * 4);
+  L.energy = L.perm + align256((size_t)n * 4);
+  L.temp = L.energy + align256((size_t)n * 8);
+  L.total = L.temp + align256(temp_bytes);
+  return L;
+}
+
+static int ensure_order(Context* c, size_t bytes) {
+  if (bytes <= c->order_bytes) return 0;
+  if (c->order) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return check_hip(c, e, "hipDeviceSynchronize (order grow)");
+    (void)hipFree(c->order);
+    c->order = nullptr;
+    c->order_bytes = 0;
+  }
+  const size_t want = bytes + bytes / 4;
+  hipError_t e = hipMalloc(&c->order, want);
+  if (e != hipSuccess) return check_hip(c, e, "hipMalloc order buffer");
+  c->order_bytes = want;
+  return 0;
+}
+
+int order_points(Context* c, const FieldTables& T, const float* x, const float* y, const float* z,
+                 int64_t n, hipStream_t s, const int** perm_out) {
+  *perm_out = nullptr;
+  if (n > (int64_t)0x7fffffff) return set_error(c, BFE_ELIMIT, "order: n = %lld > 2^31 - 1", (long long)n);
+  const uint64_t nkeys = (uint64_t)T.eg.numx * T.eg.numy + (uint64_t)(T.slg.numr - 1);
+  if (nkeys > 0xffffffffull) return set_error(c, BFE_ELIMIT, "order: too many table cells");
+  unsigned bits = 1;
+  while (bits < 32 && (1ull << bits) < nkeys) ++bits;
+  const int ni = (int)n;
+  rocprim::counting_iterator<int> iota(0);
+  size_t temp_bytes = 0;
+  BFE_HIP(c, rocprim::radix_sort_pairs(nullptr, temp_bytes, (const uint32_t*)nullptr,
+                                       (uint32_t*)nullptr, iota, (int*)nullptr, ni, 0u, bits, s));
+  const OrderLayout L = order_layout(n, temp_bytes);
+  int rc = ensure_order(c, L.total);
+  if (rc) return rc;
+  char* base = (char*)c->order;
+  uint32_t* keys = (uint32_t*)(base + L.keys);
+  uint32_t* sorted = (uint32_t*)(base + L.sorted);
+  int* perm = (int*)(base + L.perm);
+  const uint32_t neof = (uint32_t)T.eg.numx * (uint32_t)T.eg.numy;
+  hipLaunchKernelGGL(order_key_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z,
+                     n, T, neof, keys);
+  BFE_HIP(c, hipGetLastError());
+  BFE_HIP(c, rocprim::radix_sort_pairs((void*)(base + L.temp), temp_bytes, keys, sorted, iota, perm,
+                                       ni, 0u, bits, s));
+  *perm_out = perm;
+  return 0;
+}
+
+double* order_energy_buffer(Context* c, int64_t n) {
+  return (double*)((char*)c->order + order_layout(n, 0).energy);
+}
+
+void order_free(Context* c) {
+  if (c->order) (void)hipFree(c->order);
+  c->order = nullptr;
+  c->order_bytes = 0;
+}
+
+}  // namespace bfe
