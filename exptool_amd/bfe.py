@@ -127,6 +127,87 @@ class BFESystem:
         self.ctx.check(rc, "bfe_eof_accumulate")
         return cos, sin
 
+    # ------------------------------------------------------------- batches
+    @staticmethod
+    def _offsets(counts_or_offsets, n=None, offsets=False):
+        """int64 host offsets [nsnap + 1] from per-snapshot counts (or given offsets)."""
+        a = np.asarray(counts_or_offsets, dtype=np.int64)
+        off = a.copy() if offsets else np.concatenate([[0], np.cumsum(a)])
+        if off[0] != 0 or np.any(np.diff(off) < 0):
+            raise ValueError("snapshot offsets must start at 0 and not decrease")
+        if n is not None and off[-1] != n:
+            raise ValueError(f"snapshot sizes sum to {off[-1]}, not {n} particles")
+        return np.ascontiguousarray(off, dtype=np.int64)
+
+    def eof_accumulate_batch(self, x, y, z, m, counts, stream=None):
+        """Coefficients of many snapshots in one launch: particles concatenated,
+        counts[s] of snapshot s.  Returns (cos, sin), each (nsnap, mmax+1, norder)."""
+        self._need("eof")
+        x, y, z, n = self._xyz(x, y, z)
+        m = f32(m, self.device)
+        off = self._offsets(counts, n)
+        ns = off.size - 1
+        shape = (ns, self.eof["mmax"] + 1, self.eof["norder"])
+        cos = torch.empty(shape, dtype=torch.float64, device=self.device)
+        sin = torch.empty(shape, dtype=torch.float64, device=self.device)
+        rc = self.ctx.lib.bfe_eof_accumulate_batch(self.ctx.h, ptr(x), ptr(y), ptr(z), ptr(m),
+                                                   host_ptr(off), ns, 0, ptr(cos), ptr(sin),
+                                                   stream_ptr(stream))
+        self.ctx.check(rc, "bfe_eof_accumulate_batch")
+        return cos, sin
+
+    def sl_accumulate_batch(self, x, y, z, m, counts, no_odd=False, stream=None):
+        """SL coefficients of many snapshots: (nsnap, (lmax+1)^2, nmax)."""
+        self._need("sl")
+        x, y, z, n = self._xyz(x, y, z)
+        m = f32(m, self.device)
+        off = self._offsets(counts, n)
+        ns = off.size - 1
+        out = torch.empty((ns, (self.sl["lmax"] + 1) ** 2, self.sl["nmax"]), dtype=torch.float64,
+                          device=self.device)
+        rc = self.ctx.lib.bfe_sl_accumulate_batch(self.ctx.h, ptr(x), ptr(y), ptr(z), ptr(m),
+                                                  host_ptr(off), ns, _lib.NO_ODD if no_odd else 0,
+                                                  ptr(out), stream_ptr(stream))
+        self.ctx.check(rc, "bfe_sl_accumulate_batch")
+        return out
+
+    def field_forces_batch(self, x, y, z, counts, eof_cos, eof_sin, sl_coef, halofac=1.0,
+                           no_odd=False, disk_m=-1, disk_n=-1, halo_l=-1, halo_n=-1,
+                           compat=False, rotpos=0.0, stream=None):
+        """Each snapshot's own frozen field at its own points (one launch):
+        (fx, fy, fz, pot) float32 device tensors over the concatenated points."""
+        self._need("sl")
+        self._need("eof")
+        x, y, z, n = self._xyz(x, y, z)
+        off = self._offsets(counts, n)
+        ns = off.size - 1
+        ec, es, sc = (f64(v, self.device) for v in (eof_cos, eof_sin, sl_coef))
+        if ec.numel() != ns * (self.eof["mmax"] + 1) * self.eof["norder"] or es.numel() != ec.numel():
+            raise ValueError("eof_cos/eof_sin must be (nsnap, mmax+1, norder)")
+        if sc.numel() != ns * (self.sl["lmax"] + 1) ** 2 * self.sl["nmax"]:
+            raise ValueError("sl_coef must be (nsnap, (lmax+1)^2, nmax)")
+        outs = [torch.empty(n, dtype=torch.float32, device=self.device) for _ in range(4)]
+        flags = (_lib.NO_ODD if no_odd else 0) | (_lib.COMPAT_ALL if compat else 0)
+        rc = self.ctx.lib.bfe_field_eval_batch(self.ctx.h, ptr(x), ptr(y), ptr(z), host_ptr(off), ns,
+                                               ptr(ec), ptr(es), ptr(sc), float(halofac), int(disk_m),
+                                               int(disk_n), int(halo_l), int(halo_n), flags,
+                                               float(rotpos), *[ptr(o) for o in outs],
+                                               stream_ptr(stream))
+        self.ctx.check(rc, "bfe_field_eval_batch")
+        return tuple(outs)
+
+    # ---------------------------------------------------------------- tuning
+    def set_tuning(self, name, value):
+        self.ctx.check(self.ctx.lib.bfe_set_tuning(self.ctx.h, name.encode(), int(value)),
+                       "bfe_set_tuning")
+
+    def get_tuning(self, name):
+        import ctypes as C
+        v = C.c_int64()
+        self.ctx.check(self.ctx.lib.bfe_get_tuning(self.ctx.h, name.encode(), C.byref(v)),
+                       "bfe_get_tuning")
+        return int(v.value)
+
     # --------------------------------------------------------------- evaluate
     def sl_eval(self, x, y, z, coef, L1=-1000, L2=1000, no_odd=False, compat=False,
                 density=True, cos_l=False, stream=None):

@@ -11,20 +11,45 @@
 // and every block writes float64 partials; a second kernel reduces the
 // partials over blocks in a fixed order with Kahan compensation, so results
 // are bitwise reproducible for a given n.
+#include <vector>
+
 #include "bfe_internal.hpp"
 
 namespace bfe {
 
 constexpr int kTile = 256;   // EOF: particles per tile (one per lane for geometry)
 constexpr int kSLTile = 64;  // SL: particles per tile
+constexpr int kFlush = 16;   // float32 partial sums are moved to float64 every kFlush particles
 
 // ---------------------------------------------------------------------------
 // EOF accumulate
 // ---------------------------------------------------------------------------
+// Work is split into segments of consecutive particles (segment sg covers
+// [segb[sg], segb[sg + 1]), or [sg * seglen, min((sg + 1) * seglen, n)) when
+// segb is NULL); each segment writes one float64 partial row, so a batch of
+// snapshots whose particles are concatenated can share one launch.
+struct Segments {
+  const int64_t* segb;  // nseg + 1 boundaries, or NULL
+  int64_t seglen;
+  int64_t n;
+  int64_t nseg;
+};
+
+__device__ __forceinline__ void segment_range(const Segments& S, int64_t sg, int64_t& p0,
+                                              int64_t& p1) {
+  if (S.segb) {
+    p0 = S.segb[sg];
+    p1 = S.segb[sg + 1];
+  } else {
+    p0 = sg * S.seglen;
+    p1 = p0 + S.seglen < S.n ? p0 + S.seglen : S.n;
+  }
+}
+
 template <int OPT>
 __global__ __launch_bounds__(256) void eof_accumulate_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-    const float* __restrict__ pm, int64_t n, EOFGeom g, const float* __restrict__ T, int K,
+    const float* __restrict__ pm, Segments S, EOFGeom g, const float* __restrict__ T, int K,
     double* __restrict__ partials) {
   __shared__ int s_base[kTile];
   __shared__ float s_c[4][kTile];
@@ -39,14 +64,17 @@ __global__ __launch_bounds__(256) void eof_accumulate_kernel(
 #pragma unroll
   for (int o = 0; o < OPT; ++o) {
     acc[o] = 0.f;
-    accd[o] = 0.0;
     const int k = tid + o * 256;
     rowg[o] = (k < K) ? k / g.norder : 0;
   }
-  const int64_t ntiles = (n + kTile - 1) / kTile;
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const int64_t p = t * kTile + tid;
-    if (p < n) {
+  for (int64_t sg = blockIdx.x; sg < S.nseg; sg += gridDim.x) {
+  int64_t s0, s1;
+  segment_range(S, sg, s0, s1);
+#pragma unroll
+  for (int o = 0; o < OPT; ++o) accd[o] = 0.0;
+  for (int64_t t0 = s0; t0 < s1; t0 += kTile) {
+    const int64_t p = t0 + tid;
+    if (p < s1) {
       const double x = px[p], y = py[p], z = pz[p], mass = pm[p];
       const double R = sqrt(x * x + y * y + 1.e-10);
       double c1, s1;
@@ -74,7 +102,7 @@ __global__ __launch_bounds__(256) void eof_accumulate_kernel(
       for (int j = 0; j < J; ++j) s_w[j][tid] = 0.f;
     }
     __syncthreads();
-    const int64_t left = n - t * kTile;
+    const int64_t left = s1 - t0;
     const int np = left < kTile ? (int)left : kTile;
     for (int q = 0; q < np; ++q) {
       const size_t b = (size_t)s_base[q];
@@ -91,6 +119,16 @@ __global__ __launch_bounds__(256) void eof_accumulate_kernel(
           acc[o] = fmaf(s_w[rowg[o]][q], v, acc[o]);
         }
       }
+      // float32 sums over at most kFlush particles: for noise-dominated
+      // coefficients (|a| ~ sqrt(N) |term|) the rounding of long float32
+      // sums, not of the terms, sets the error
+      if ((q & (kFlush - 1)) == kFlush - 1) {
+#pragma unroll
+        for (int o = 0; o < OPT; ++o) {
+          accd[o] += (double)acc[o];
+          acc[o] = 0.f;
+        }
+      }
     }
 #pragma unroll
     for (int o = 0; o < OPT; ++o) {
@@ -102,7 +140,8 @@ __global__ __launch_bounds__(256) void eof_accumulate_kernel(
 #pragma unroll
   for (int o = 0; o < OPT; ++o) {
     const int k = tid + o * 256;
-    if (k < K) partials[(size_t)blockIdx.x * K + k] = accd[o];
+    if (k < K) partials[(size_t)sg * K + k] = accd[o];
+  }
   }
 }
 
@@ -110,14 +149,23 @@ __global__ __launch_bounds__(256) void eof_accumulate_kernel(
 // sums b = t, t + 256, ... with Kahan compensation, then a fixed pairwise tree
 // in LDS combines the 256 lane sums.  The order depends only on nb, so results
 // are bitwise reproducible.
+// With rowb (nsnap + 1 row boundaries), block (k, snapshot) sums the rows of
+// its snapshot into out[snapshot * K + k].
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __restrict__ partials,
                                                               int nb, int K,
-                                                              double* __restrict__ out) {
+                                                              double* __restrict__ out,
+                                                              const int64_t* __restrict__ rowb) {
   __shared__ double sh[256];
   const int k = blockIdx.x;
   const int t = threadIdx.x;
+  int64_t r0 = 0, r1 = nb;
+  if (rowb) {
+    r0 = rowb[blockIdx.y];
+    r1 = rowb[blockIdx.y + 1];
+    out += (size_t)blockIdx.y * K;
+  }
   double s = 0.0, comp = 0.0;
-  for (int b = t; b < nb; b += 256) {
+  for (int64_t b = r0 + t; b < r1; b += 256) {
     const double yv = partials[(size_t)b * K + k] - comp;
     const double tv = s + yv;
     comp = (tv - s) - yv;
@@ -132,11 +180,20 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __re
   if (t == 0) out[k] = sh[0];
 }
 
+__global__ void fill_f64_kernel(double* __restrict__ a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+
 // Scatter the K-ordered EOF sums into (mmax+1, norder) cos and sin arrays.
 __global__ void eof_split_kernel(const double* __restrict__ sums, int mmax, int norder,
                                  double* __restrict__ cos_out, double* __restrict__ sin_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int nc = (mmax + 1) * norder;
+  const int K = (2 * mmax + 1) * norder;
+  sums += (size_t)blockIdx.y * K;
+  cos_out += (size_t)blockIdx.y * nc;
+  sin_out += (size_t)blockIdx.y * nc;
   if (i < nc) {
     cos_out[i] = sums[i];
     sin_out[i] = (i < norder) ? 0.0 : sums[nc + i - norder];
@@ -149,7 +206,7 @@ __global__ void eof_split_kernel(const double* __restrict__ sums, int mmax, int 
 template <int OPT>
 __global__ __launch_bounds__(256) void sl_accumulate_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-    const float* __restrict__ pm, int64_t n, SLGeom g, const float* __restrict__ E,
+    const float* __restrict__ pm, Segments S, SLGeom g, const float* __restrict__ E,
     const double* __restrict__ lmc, int no_odd, double* __restrict__ partials) {
   extern __shared__ float smem[];
   const int R = (g.lmax + 1) * (g.lmax + 1);
@@ -171,7 +228,6 @@ __global__ __launch_bounds__(256) void sl_accumulate_kernel(
 #pragma unroll
   for (int o = 0; o < OPT; ++o) {
     acc[o] = 0.f;
-    accd[o] = 0.0;
     const int k = tid + o * 256;
     const int row = (k < K) ? k / g.nmax : 0;
     const int nn = (k < K) ? k % g.nmax : 0;
@@ -180,12 +236,16 @@ __global__ __launch_bounds__(256) void sl_accumulate_kernel(
     yrow[o] = row * YS;
     prow[o] = l * g.nmax + nn;
   }
-  const int64_t ntiles = (n + kSLTile - 1) / kSLTile;
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const int64_t p = t * kSLTile + qp;
+  for (int64_t sg = blockIdx.x; sg < S.nseg; sg += gridDim.x) {
+  int64_t s0, s1;
+  segment_range(S, sg, s0, s1);
+#pragma unroll
+  for (int o = 0; o < OPT; ++o) accd[o] = 0.0;
+  for (int64_t t0 = s0; t0 < s1; t0 += kSLTile) {
+    const int64_t p = t0 + qp;
     {
       // padding particles: unit position, zero mass, so every Y row is 0
-      const bool live = p < n;
+      const bool live = p < s1;
       const double x = live ? (double)px[p] : 1.0, y = live ? (double)py[p] : 0.0;
       const double z = live ? (double)pz[p] : 0.0, mass = live ? (double)pm[p] : 0.0;
       double r = sqrt(x * x + y * y + z * z);
@@ -247,11 +307,18 @@ __global__ __launch_bounds__(256) void sl_accumulate_kernel(
       s_phi[idx] = s_w1[q] * E[i * LN + ln] + s_w2[q] * E[(i + 1) * LN + ln];
     }
     __syncthreads();
-    const int64_t left = n - t * kSLTile;
+    const int64_t left = s1 - t0;
     const int np = left < kSLTile ? (int)left : kSLTile;
     for (int q = 0; q < np; ++q) {
 #pragma unroll
       for (int o = 0; o < OPT; ++o) acc[o] = fmaf(s_Y[yrow[o] + q], s_phi[q * LN + prow[o]], acc[o]);
+      if ((q & (kFlush - 1)) == kFlush - 1) {
+#pragma unroll
+        for (int o = 0; o < OPT; ++o) {
+          accd[o] += (double)acc[o];
+          acc[o] = 0.f;
+        }
+      }
     }
 #pragma unroll
     for (int o = 0; o < OPT; ++o) {
@@ -263,18 +330,174 @@ __global__ __launch_bounds__(256) void sl_accumulate_kernel(
 #pragma unroll
   for (int o = 0; o < OPT; ++o) {
     const int k = tid + o * 256;
-    if (k < K) partials[(size_t)blockIdx.x * K + k] = accd[o];
+    if (k < K) partials[(size_t)sg * K + k] = accd[o];
+  }
   }
 }
 
-static int accumulate_blocks(int64_t ntiles) {
-  const int64_t cap = 2048;
-  return (int)(ntiles < cap ? (ntiles > 0 ? ntiles : 1) : cap);
+// Segment plan of one accumulate launch.  Single snapshot: at most
+// acc_max_blocks segments of whole tiles, one per block.  Batch: every
+// snapshot is cut into segments of seg_tiles tiles (never spanning two
+// snapshots); the blocks loop over all segments and rowb[s] is the first
+// segment of snapshot s, so each snapshot's partial rows are reduced on
+// their own, in a fixed order.
+struct AccPlan {
+  Segments S;
+  int grid;
+  std::vector<int64_t> segb, rowb;  // batch only (host copies)
+};
+
+static AccPlan plan_single(const Context* c, int64_t n, int tile) {
+  AccPlan P;
+  const int64_t ntiles = (n + tile - 1) / tile;
+  int64_t nseg = ntiles < c->acc_max_blocks ? ntiles : c->acc_max_blocks;
+  if (nseg < 1) nseg = 1;
+  const int64_t per = (ntiles + nseg - 1) / nseg;
+  P.S.segb = nullptr;
+  P.S.seglen = (per > 0 ? per : 1) * tile;
+  P.S.n = n;
+  P.S.nseg = (n + P.S.seglen - 1) / P.S.seglen;
+  if (P.S.nseg < 1) P.S.nseg = 1;
+  P.grid = (int)P.S.nseg;
+  return P;
+}
+
+static AccPlan plan_batch(const Context* c, const int64_t* off, int nsnap, int tile) {
+  AccPlan P;
+  int64_t total_tiles = 0;
+  for (int q = 0; q < nsnap; ++q) total_tiles += (off[q + 1] - off[q] + tile - 1) / tile;
+  int64_t seg_tiles = c->batch_seg_tiles;
+  if (seg_tiles <= 0) {
+    seg_tiles = (total_tiles + c->acc_max_blocks - 1) / c->acc_max_blocks;
+    if (seg_tiles < 1) seg_tiles = 1;
+  }
+  P.rowb.push_back(0);
+  P.segb.push_back(off[0]);
+  for (int q = 0; q < nsnap; ++q) {
+    const int64_t b = off[q], e = off[q + 1];
+    for (int64_t p = b; p < e; p += seg_tiles * tile) {
+      const int64_t pe = p + seg_tiles * tile < e ? p + seg_tiles * tile : e;
+      P.segb.push_back(pe);
+    }
+    if (e == b) P.segb.push_back(e);   // empty snapshot: one empty segment
+    P.rowb.push_back((int64_t)P.segb.size() - 1);
+  }
+  P.S.segb = nullptr;  // device copy set by upload_plan
+  P.S.seglen = 0;
+  P.S.n = off[nsnap];
+  P.S.nseg = (int64_t)P.segb.size() - 1;
+  P.grid = (int)(P.S.nseg < c->acc_max_blocks ? P.S.nseg : c->acc_max_blocks);
+  return P;
+}
+
+// Copies the batch plan to device memory behind the partials in scratch.
+static int upload_plan(Context* c, AccPlan& P, size_t pbytes, const int64_t** rowb_dev) {
+  const size_t segb_off = (pbytes + 255) & ~(size_t)255;
+  const size_t rowb_off = segb_off + ((P.segb.size() * 8 + 255) & ~(size_t)255);
+  int rc = ensure_scratch(c, rowb_off + P.rowb.size() * 8 + 256);
+  if (rc) return rc;
+  int64_t* segb = (int64_t*)((char*)c->scratch + segb_off);
+  int64_t* rowb = (int64_t*)((char*)c->scratch + rowb_off);
+  BFE_HIP(c, hipMemcpy(segb, P.segb.data(), P.segb.size() * 8, hipMemcpyHostToDevice));
+  BFE_HIP(c, hipMemcpy(rowb, P.rowb.data(), P.rowb.size() * 8, hipMemcpyHostToDevice));
+  P.S.segb = segb;
+  *rowb_dev = rowb;
+  return 0;
+}
+
+static int check_offsets(Context* c, const int64_t* off, int nsnap, const char* who) {
+  if (!off || nsnap < 1) return set_error(c, BFE_EINVAL, "%s: need offsets and nsnap >= 1", who);
+  if (off[0] != 0) return set_error(c, BFE_EINVAL, "%s: offsets[0] must be 0", who);
+  for (int q = 0; q < nsnap; ++q)
+    if (off[q + 1] < off[q]) return set_error(c, BFE_EINVAL, "%s: offsets must not decrease", who);
+  return 0;
+}
+
+static int eof_accumulate_run(Context* c, const float* x, const float* y, const float* z,
+                              const float* m, int64_t n, const int64_t* off, int nsnap,
+                              double* cos_out, double* sin_out, hipStream_t s) {
+  const int K = c->eof_K;
+  AccPlan P = off ? plan_batch(c, off, nsnap, kTile) : plan_single(c, n, kTile);
+  const size_t pbytes = (size_t)P.S.nseg * K * sizeof(double);
+  const size_t sums_off = (pbytes + 255) & ~(size_t)255;
+  const size_t need = sums_off + (size_t)nsnap * K * sizeof(double);
+  const int64_t* rowb = nullptr;
+  int rc;
+  if (off) {
+    rc = upload_plan(c, P, need, &rowb);   // partials | sums | segb | rowb
+  } else {
+    rc = ensure_scratch(c, need);
+  }
+  if (rc) return rc;
+  double* partials = (double*)c->scratch;
+  double* sums = (double*)((char*)c->scratch + sums_off);
+  EOFGeom g = eof_geom(c);
+  const int opt = (K + 255) / 256;
+  if (n == 0) {
+    hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((P.S.nseg * K + 255) / 256)), dim3(256), 0, s,
+                       partials, P.S.nseg * (int64_t)K);
+  } else if (opt == 1) {
+    hipLaunchKernelGGL(eof_accumulate_kernel<1>, dim3(P.grid), dim3(256), 0, s, x, y, z, m, P.S, g, c->eof_pot32, K, partials);
+  } else if (opt == 2) {
+    hipLaunchKernelGGL(eof_accumulate_kernel<2>, dim3(P.grid), dim3(256), 0, s, x, y, z, m, P.S, g, c->eof_pot32, K, partials);
+  } else {
+    hipLaunchKernelGGL(eof_accumulate_kernel<4>, dim3(P.grid), dim3(256), 0, s, x, y, z, m, P.S, g, c->eof_pot32, K, partials);
+  }
+  BFE_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(K, nsnap), dim3(256), 0, s, partials,
+                     (int)P.S.nseg, K, sums, rowb);
+  const int nc = (c->mmax + 1) * c->norder;
+  hipLaunchKernelGGL(eof_split_kernel, dim3((nc + 255) / 256, nsnap), dim3(256), 0, s, sums, c->mmax,
+                     c->norder, cos_out, sin_out);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
+static int sl_accumulate_run(Context* c, const float* x, const float* y, const float* z,
+                             const float* m, int64_t n, const int64_t* off, int nsnap, uint32_t flags,
+                             double* coef_out, hipStream_t s) {
+  const int R = (c->lmax + 1) * (c->lmax + 1);
+  const int LN = (c->lmax + 1) * c->nmax;
+  const int K = R * c->nmax;
+  if (K > 1024) return set_error(c, BFE_ELIMIT, "sl_accumulate: (lmax+1)^2 nmax = %d > 1024", K);
+  const size_t lds = ((size_t)R * (kSLTile + 1) + (size_t)kSLTile * LN + 3 * kSLTile) * 4;
+  if (lds > 64 * 1024) return set_error(c, BFE_ELIMIT, "sl_accumulate: LDS %zu B > 64 KiB", lds);
+  AccPlan P = off ? plan_batch(c, off, nsnap, kSLTile) : plan_single(c, n, kSLTile);
+  const size_t pbytes = (size_t)P.S.nseg * K * sizeof(double);
+  const int64_t* rowb = nullptr;
+  int rc = off ? upload_plan(c, P, pbytes, &rowb) : ensure_scratch(c, pbytes);
+  if (rc) return rc;
+  double* partials = (double*)c->scratch;
+  SLGeom g = sl_geom(c);
+  const int no_odd = (flags & BFE_NO_ODD) ? 1 : 0;
+  const int opt = (K + 255) / 256;
+  if (n == 0) {
+    hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((P.S.nseg * K + 255) / 256)), dim3(256), 0, s,
+                       partials, P.S.nseg * (int64_t)K);
+  } else if (opt == 1) {
+    hipLaunchKernelGGL(sl_accumulate_kernel<1>, dim3(P.grid), dim3(256), lds, s, x, y, z, m, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
+  } else if (opt == 2) {
+    hipLaunchKernelGGL(sl_accumulate_kernel<2>, dim3(P.grid), dim3(256), lds, s, x, y, z, m, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
+  } else {
+    hipLaunchKernelGGL(sl_accumulate_kernel<4>, dim3(P.grid), dim3(256), lds, s, x, y, z, m, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
+  }
+  BFE_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(K, nsnap), dim3(256), 0, s, partials,
+                     (int)P.S.nseg, K, coef_out, rowb);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
 }
 
 }  // namespace bfe
 
 using namespace bfe;
+
+static int acc_common_checks(Context* c, const float* x, const float* y, const float* z,
+                             const float* m, int64_t n, const char* who) {
+  if (n < 0) return set_error(c, BFE_EINVAL, "%s: n < 0", who);
+  if (n > 0 && (!x || !y || !z || !m)) return set_error(c, BFE_EINVAL, "%s: NULL input", who);
+  return 0;
+}
 
 extern "C" int bfe_eof_accumulate(bfe_ctx* h, const float* x, const float* y, const float* z,
                                   const float* m, int64_t n, uint32_t flags, double* cos_out,
@@ -283,36 +506,30 @@ extern "C" int bfe_eof_accumulate(bfe_ctx* h, const float* x, const float* y, co
   if (!h) return BFE_EINVAL;
   Context* c = &h->c;
   if (!c->eof_ok) return set_error(c, BFE_ENOTABLE, "eof_accumulate: EOF tables not set");
-  if (n < 0) return set_error(c, BFE_EINVAL, "eof_accumulate: n < 0");
   if (!cos_out || !sin_out) return set_error(c, BFE_EINVAL, "eof_accumulate: NULL output");
-  if (n > 0 && (!x || !y || !z || !m)) return set_error(c, BFE_EINVAL, "eof_accumulate: NULL input");
-  BFE_HIP(c, hipSetDevice(c->device));
-  hipStream_t s = (hipStream_t)stream;
-  const int K = c->eof_K;
-  const int64_t ntiles = (n + kTile - 1) / kTile;
-  const int nb = accumulate_blocks(ntiles);
-  const size_t pbytes = (size_t)nb * K * sizeof(double);
-  int rc = ensure_scratch(c, pbytes + (size_t)K * sizeof(double));
+  int rc = acc_common_checks(c, x, y, z, m, n, "eof_accumulate");
   if (rc) return rc;
-  double* partials = (double*)c->scratch;
-  double* sums = partials + (size_t)nb * K;
-  EOFGeom g = eof_geom(c);
-  const int opt = (K + 255) / 256;
-  if (n == 0) {
-    BFE_HIP(c, hipMemsetAsync(partials, 0, pbytes, s));
-  } else if (opt == 1) {
-    hipLaunchKernelGGL(eof_accumulate_kernel<1>, dim3(nb), dim3(256), 0, s, x, y, z, m, n, g, c->eof_pot32, K, partials);
-  } else if (opt == 2) {
-    hipLaunchKernelGGL(eof_accumulate_kernel<2>, dim3(nb), dim3(256), 0, s, x, y, z, m, n, g, c->eof_pot32, K, partials);
-  } else {
-    hipLaunchKernelGGL(eof_accumulate_kernel<4>, dim3(nb), dim3(256), 0, s, x, y, z, m, n, g, c->eof_pot32, K, partials);
-  }
-  BFE_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), dim3(256), 0, s, partials, nb, K, sums);
-  const int nc = (c->mmax + 1) * c->norder;
-  hipLaunchKernelGGL(eof_split_kernel, dim3((nc + 255) / 256), dim3(256), 0, s, sums, c->mmax, c->norder, cos_out, sin_out);
-  BFE_HIP(c, hipGetLastError());
-  return 0;
+  BFE_HIP(c, hipSetDevice(c->device));
+  return eof_accumulate_run(c, x, y, z, m, n, nullptr, 1, cos_out, sin_out, (hipStream_t)stream);
+}
+
+extern "C" int bfe_eof_accumulate_batch(bfe_ctx* h, const float* x, const float* y, const float* z,
+                                        const float* m, const int64_t* offsets, int nsnap,
+                                        uint32_t flags, double* cos_out, double* sin_out,
+                                        void* stream) {
+  (void)flags;
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (!c->eof_ok) return set_error(c, BFE_ENOTABLE, "eof_accumulate_batch: EOF tables not set");
+  if (!cos_out || !sin_out) return set_error(c, BFE_EINVAL, "eof_accumulate_batch: NULL output");
+  int rc = check_offsets(c, offsets, nsnap, "eof_accumulate_batch");
+  if (rc) return rc;
+  rc = acc_common_checks(c, x, y, z, m, offsets[nsnap], "eof_accumulate_batch");
+  if (rc) return rc;
+  if (nsnap > 65535) return set_error(c, BFE_ELIMIT, "eof_accumulate_batch: nsnap > 65535");
+  BFE_HIP(c, hipSetDevice(c->device));
+  return eof_accumulate_run(c, x, y, z, m, offsets[nsnap], offsets, nsnap, cos_out, sin_out,
+                            (hipStream_t)stream);
 }
 
 extern "C" int bfe_sl_accumulate(bfe_ctx* h, const float* x, const float* y, const float* z,
@@ -321,37 +538,26 @@ extern "C" int bfe_sl_accumulate(bfe_ctx* h, const float* x, const float* y, con
   if (!h) return BFE_EINVAL;
   Context* c = &h->c;
   if (!c->sl_ok) return set_error(c, BFE_ENOTABLE, "sl_accumulate: SL tables not set");
-  if (n < 0) return set_error(c, BFE_EINVAL, "sl_accumulate: n < 0");
   if (!coef_out) return set_error(c, BFE_EINVAL, "sl_accumulate: NULL output");
-  if (n > 0 && (!x || !y || !z || !m)) return set_error(c, BFE_EINVAL, "sl_accumulate: NULL input");
-  const int R = (c->lmax + 1) * (c->lmax + 1);
-  const int LN = (c->lmax + 1) * c->nmax;
-  const int K = R * c->nmax;
-  if (K > 1024) return set_error(c, BFE_ELIMIT, "sl_accumulate: (lmax+1)^2 nmax = %d > 1024", K);
-  const size_t lds = ((size_t)R * (kSLTile + 1) + (size_t)kSLTile * LN + 3 * kSLTile) * 4;
-  if (lds > 64 * 1024) return set_error(c, BFE_ELIMIT, "sl_accumulate: LDS %zu B > 64 KiB", lds);
-  BFE_HIP(c, hipSetDevice(c->device));
-  hipStream_t s = (hipStream_t)stream;
-  const int64_t ntiles = (n + kSLTile - 1) / kSLTile;
-  const int nb = accumulate_blocks(ntiles);
-  const size_t pbytes = (size_t)nb * K * sizeof(double);
-  int rc = ensure_scratch(c, pbytes);
+  int rc = acc_common_checks(c, x, y, z, m, n, "sl_accumulate");
   if (rc) return rc;
-  double* partials = (double*)c->scratch;
-  SLGeom g = sl_geom(c);
-  const int no_odd = (flags & BFE_NO_ODD) ? 1 : 0;
-  const int opt = (K + 255) / 256;
-  if (n == 0) {
-    BFE_HIP(c, hipMemsetAsync(partials, 0, pbytes, s));
-  } else if (opt == 1) {
-    hipLaunchKernelGGL(sl_accumulate_kernel<1>, dim3(nb), dim3(256), lds, s, x, y, z, m, n, g, c->sl_e32, c->sl_lmc, no_odd, partials);
-  } else if (opt == 2) {
-    hipLaunchKernelGGL(sl_accumulate_kernel<2>, dim3(nb), dim3(256), lds, s, x, y, z, m, n, g, c->sl_e32, c->sl_lmc, no_odd, partials);
-  } else {
-    hipLaunchKernelGGL(sl_accumulate_kernel<4>, dim3(nb), dim3(256), lds, s, x, y, z, m, n, g, c->sl_e32, c->sl_lmc, no_odd, partials);
-  }
-  BFE_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(K), dim3(256), 0, s, partials, nb, K, coef_out);
-  BFE_HIP(c, hipGetLastError());
-  return 0;
+  BFE_HIP(c, hipSetDevice(c->device));
+  return sl_accumulate_run(c, x, y, z, m, n, nullptr, 1, flags, coef_out, (hipStream_t)stream);
+}
+
+extern "C" int bfe_sl_accumulate_batch(bfe_ctx* h, const float* x, const float* y, const float* z,
+                                       const float* m, const int64_t* offsets, int nsnap,
+                                       uint32_t flags, double* coef_out, void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (!c->sl_ok) return set_error(c, BFE_ENOTABLE, "sl_accumulate_batch: SL tables not set");
+  if (!coef_out) return set_error(c, BFE_EINVAL, "sl_accumulate_batch: NULL output");
+  int rc = check_offsets(c, offsets, nsnap, "sl_accumulate_batch");
+  if (rc) return rc;
+  rc = acc_common_checks(c, x, y, z, m, offsets[nsnap], "sl_accumulate_batch");
+  if (rc) return rc;
+  if (nsnap > 65535) return set_error(c, BFE_ELIMIT, "sl_accumulate_batch: nsnap > 65535");
+  BFE_HIP(c, hipSetDevice(c->device));
+  return sl_accumulate_run(c, x, y, z, m, offsets[nsnap], offsets, nsnap, flags, coef_out,
+                           (hipStream_t)stream);
 }

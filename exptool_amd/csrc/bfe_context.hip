@@ -104,6 +104,7 @@ extern "C" int bfe_destroy(bfe_ctx* h) {
   free_eof(c);
   mlp_free(c);
   order_free(c);
+  batch_free(c);
   if (c->scratch) (void)hipFree(c->scratch);
   delete h;
   return 0;
@@ -228,5 +229,34 @@ extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, in
   c->eof_K = K;
   c->eof_dens = dens;
   c->eof_ok = true;
+  return 0;
+}
+
+// Launch-shape knobs, set by hand or by the autotuner (exptool_amd/autotune.py).
+extern "C" int bfe_set_tuning(bfe_ctx* h, const char* name, int64_t value) {
+  if (!h || !name) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (!strcmp(name, "acc_max_blocks")) {
+    if (value < 1 || value > (1 << 20)) return set_error(c, BFE_EINVAL, "acc_max_blocks outside [1, 2^20]");
+    c->acc_max_blocks = value;
+  } else if (!strcmp(name, "batch_seg_tiles")) {
+    if (value < 0) return set_error(c, BFE_EINVAL, "batch_seg_tiles < 0");
+    c->batch_seg_tiles = value;
+  } else if (!strcmp(name, "order_min")) {
+    if (value < 0) return set_error(c, BFE_EINVAL, "order_min < 0");
+    c->order_min = value;
+  } else {
+    return set_error(c, BFE_EINVAL, "unknown tuning parameter '%s'", name);
+  }
+  return 0;
+}
+
+extern "C" int bfe_get_tuning(const bfe_ctx* h, const char* name, int64_t* value) {
+  if (!h || !name || !value) return BFE_EINVAL;
+  const Context* c = &h->c;
+  if (!strcmp(name, "acc_max_blocks")) *value = c->acc_max_blocks;
+  else if (!strcmp(name, "batch_seg_tiles")) *value = c->batch_seg_tiles;
+  else if (!strcmp(name, "order_min")) *value = c->order_min;
+  else return BFE_EINVAL;
   return 0;
 }

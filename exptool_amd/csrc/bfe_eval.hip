@@ -16,9 +16,14 @@ struct SLFoldArgs {
 // One thread per (radial node i, (l, m) pair p in m-major order).  E, Dn and
 // G at i-1, i, i+1 are summed over n in float64 from the float64 master
 // tables, D1/D2 formed, and the 8-float block rounded once to float32.
+// grid.y runs over coefficient sets: set q reads coef + q * coef_stride and
+// writes out + q * out_stride.
 __global__ void sl_fold_kernel(const double* __restrict__ ef, const double* __restrict__ ev,
                                const double* __restrict__ p0, const double* __restrict__ coef,
-                               int nmax, int numr, SLFoldArgs a, float* __restrict__ out) {
+                               int nmax, int numr, SLFoldArgs a, float* __restrict__ out,
+                               int64_t coef_stride, int64_t out_stride) {
+  coef += (size_t)blockIdx.y * coef_stride;
+  out += (size_t)blockIdx.y * out_stride;
   const int np = sl_npairs(a.lmax);
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)numr * np) return;
@@ -75,7 +80,10 @@ __global__ void eof_fold_kernel(const double* __restrict__ T0, const double* __r
                                 const double* __restrict__ T2, const double* __restrict__ T3,
                                 const double* __restrict__ ccos, const double* __restrict__ csin,
                                 int mmax, int norder, int64_t nodes, EOFFoldArgs a,
-                                float* __restrict__ out) {
+                                float* __restrict__ out, int64_t coef_stride, int64_t out_stride) {
+  ccos += (size_t)blockIdx.y * coef_stride;
+  csin += (size_t)blockIdx.y * coef_stride;
+  out += (size_t)blockIdx.y * out_stride;
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= nodes * (mmax + 1)) return;
   const int m = (int)(idx / nodes);
@@ -111,12 +119,15 @@ static int ensure_folded(Context* c, float** d, size_t* have, size_t floats, hip
   return 0;
 }
 
-static int sl_fold(Context* c, int slot, const double* coef, double halofac, int lmax_use,
-                   int nmax_use, int L1, int L2, bool no_odd, hipStream_t s) {
-  SLFolded& f = c->slf[slot];
+static size_t sl_fold_floats(const Context* c) { return (size_t)c->numr * sl_npairs(c->lmax) * 8; }
+static size_t eof_fold_floats(const Context* c) {
+  return (size_t)(c->numx + 1) * (c->numy + 1) * (c->mmax + 1) * 8;
+}
+
+// Folds nset coefficient sets (stride (lmax+1)^2 nmax) into out (stride sl_fold_floats).
+static int sl_fold_into(Context* c, const double* coef, int nset, double halofac, int lmax_use,
+                        int nmax_use, int L1, int L2, bool no_odd, float* out, hipStream_t s) {
   const int np = sl_npairs(c->lmax);
-  int rc = ensure_folded(c, &f.d, &f.floats, (size_t)c->numr * np * 8, s);
-  if (rc) return rc;
   SLFoldArgs a;
   for (int l = 0; l <= kMaxL; ++l) {
     bool keep = l <= lmax_use;
@@ -127,20 +138,28 @@ static int sl_fold(Context* c, int slot, const double* coef, double halofac, int
   a.lmax = c->lmax;
   a.nmax_use = nmax_use;
   const int64_t total = (int64_t)c->numr * np;
-  hipLaunchKernelGGL(sl_fold_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                     c->sl_ef64, c->sl_ev64, c->sl_p0, coef, c->nmax, c->numr, a, f.d);
+  hipLaunchKernelGGL(sl_fold_kernel, dim3((unsigned)((total + 255) / 256), nset), dim3(256), 0, s,
+                     c->sl_ef64, c->sl_ev64, c->sl_p0, coef, c->nmax, c->numr, a, out,
+                     (int64_t)(c->lmax + 1) * (c->lmax + 1) * c->nmax, (int64_t)sl_fold_floats(c));
   BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
+static int sl_fold(Context* c, int slot, const double* coef, double halofac, int lmax_use,
+                   int nmax_use, int L1, int L2, bool no_odd, hipStream_t s) {
+  SLFolded& f = c->slf[slot];
+  int rc = ensure_folded(c, &f.d, &f.floats, sl_fold_floats(c), s);
+  if (rc) return rc;
+  rc = sl_fold_into(c, coef, 1, halofac, lmax_use, nmax_use, L1, L2, no_odd, f.d, s);
+  if (rc) return rc;
   f.valid = true;
   return 0;
 }
 
-static int eof_fold(Context* c, int slot, const double* ccos, const double* csin, int mmax_use,
-                    int nuse, int m1, int m2, bool no_odd, bool compat_mask, bool dens,
-                    hipStream_t s) {
-  EOFFolded& f = c->eoff[slot];
+static int eof_fold_into(Context* c, const double* ccos, const double* csin, int nset,
+                         int mmax_use, int nuse, int m1, int m2, bool no_odd, bool compat_mask,
+                         bool dens, float* out, hipStream_t s) {
   const int64_t nodes = (int64_t)(c->numx + 1) * (c->numy + 1);
-  int rc = ensure_folded(c, &f.d, &f.floats, (size_t)nodes * (c->mmax + 1) * 8, s);
-  if (rc) return rc;
   EOFFoldArgs a;
   for (int m = 0; m <= kMaxM; ++m) {
     double mul = 1.0;
@@ -154,10 +173,22 @@ static int eof_fold(Context* c, int slot, const double* ccos, const double* csin
   a.nuse = nuse;
   a.dens = dens ? 1 : 0;
   const int64_t total = nodes * (c->mmax + 1);
-  hipLaunchKernelGGL(eof_fold_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+  hipLaunchKernelGGL(eof_fold_kernel, dim3((unsigned)((total + 255) / 256), nset), dim3(256), 0, s,
                      c->eof64[0], c->eof64[1], c->eof64[2], dens ? c->eof64[3] : c->eof64[0],
-                     ccos, csin, c->mmax, c->norder, nodes, a, f.d);
+                     ccos, csin, c->mmax, c->norder, nodes, a, out,
+                     (int64_t)(c->mmax + 1) * c->norder, (int64_t)eof_fold_floats(c));
   BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
+static int eof_fold(Context* c, int slot, const double* ccos, const double* csin, int mmax_use,
+                    int nuse, int m1, int m2, bool no_odd, bool compat_mask, bool dens,
+                    hipStream_t s) {
+  EOFFolded& f = c->eoff[slot];
+  int rc = ensure_folded(c, &f.d, &f.floats, eof_fold_floats(c), s);
+  if (rc) return rc;
+  rc = eof_fold_into(c, ccos, csin, 1, mmax_use, nuse, m1, m2, no_odd, compat_mask, dens, f.d, s);
+  if (rc) return rc;
   f.valid = true;
   return 0;
 }
@@ -232,6 +263,54 @@ __global__ __launch_bounds__(256) void field_eval_kernel(
 #pragma unroll
     for (int k = 0; k < 8; ++k) comp[(size_t)k * n + i] = (float)cv[k];
   }
+}
+
+// Batched frozen fields (one coefficient set per snapshot): point j of the
+// (snapshot, cell) order belongs to snapshot sorted[j] / ncell and uses that
+// snapshot's folded tables.
+__global__ __launch_bounds__(256) void field_eval_batch_kernel(
+    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    int64_t n, const int* __restrict__ perm, const uint32_t* __restrict__ sorted, uint32_t ncell,
+    FieldTables T, const float* __restrict__ slB, int64_t sl_stride,
+    const float* __restrict__ eofB, int64_t eof_stride, float* fx, float* fy, float* fz,
+    float* pot) {
+  const int64_t nb = (n + 255) / 256;
+  const int64_t j = xcd_block(nb) * 256 + threadIdx.x;
+  if (j >= n) return;
+  const int64_t i = perm[j];
+  const size_t snap = sorted[j] / ncell;
+  T.slF = slB + snap * sl_stride;
+  T.eofF = eofB + snap * eof_stride;
+  double cv[8];
+  field_eval_point(T, px[i], py[i], pz[i], cv);
+  if (fx) fx[i] = (float)(cv[0] + cv[1]);
+  if (fy) fy[i] = (float)(cv[2] + cv[3]);
+  if (fz) fz[i] = (float)(cv[4] + cv[5]);
+  if (pot) pot[i] = (float)(cv[6] + cv[7]);
+}
+
+static int ensure_batch(Context* c, float** d, size_t* have, size_t floats) {
+  if (*d && *have >= floats) return 0;
+  if (*d) {
+    BFE_HIP(c, hipDeviceSynchronize());
+    (void)hipFree(*d);
+    *d = nullptr;
+    *have = 0;
+  }
+  BFE_HIP(c, hipMalloc(d, floats * sizeof(float)));
+  *have = floats;
+  return 0;
+}
+
+void batch_free(Context* c) {
+  if (c->batch_sl) (void)hipFree(c->batch_sl);
+  if (c->batch_eof) (void)hipFree(c->batch_eof);
+  if (c->batch_off) (void)hipFree(c->batch_off);
+  c->batch_sl = nullptr;
+  c->batch_eof = nullptr;
+  c->batch_off = nullptr;
+  c->batch_sl_floats = c->batch_eof_floats = 0;
+  c->batch_off_cap = 0;
 }
 
 int field_tables(Context* c, double rotpos, FieldTables& T) {
@@ -351,13 +430,80 @@ extern "C" int bfe_field_eval_cart(bfe_ctx* h, const float* x, const float* y, c
   BFE_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
   const int* perm = nullptr;
-  if (n >= kOrderMinPoints) {
+  if (n >= c->order_min) {
     rc = order_points(c, T, x, y, z, n, s, &perm);
     if (rc) return rc;
   }
   const unsigned grid = perm ? xcd_grid(nblocks256(n)) : nblocks256(n);
   hipLaunchKernelGGL(field_eval_kernel, dim3(grid), dim3(256), 0, s, x, y, z, n, perm, T, fx, fy,
                      fz, pot, comp);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
+extern "C" int bfe_field_eval_batch(bfe_ctx* h, const float* x, const float* y, const float* z,
+                                    const int64_t* offsets, int nsnap, const double* eof_cos,
+                                    const double* eof_sin, const double* sl_coef, double halofac,
+                                    int disk_m, int disk_n, int halo_l, int halo_n,
+                                    uint32_t flags, double rotpos, float* fx, float* fy, float* fz,
+                                    float* pot, void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (!c->sl_ok || !c->eof_ok) return set_error(c, BFE_ENOTABLE, "field_eval_batch: tables not set");
+  if (!eof_cos || !eof_sin || !sl_coef) return set_error(c, BFE_EINVAL, "field_eval_batch: NULL coef");
+  if (!offsets || nsnap < 1 || nsnap > 65535)
+    return set_error(c, BFE_EINVAL, "field_eval_batch: need offsets and 1 <= nsnap <= 65535");
+  if (offsets[0] != 0) return set_error(c, BFE_EINVAL, "field_eval_batch: offsets[0] must be 0");
+  for (int q = 0; q < nsnap; ++q)
+    if (offsets[q + 1] < offsets[q]) return set_error(c, BFE_EINVAL, "field_eval_batch: offsets must not decrease");
+  const int64_t n = offsets[nsnap];
+  if (n > 0 && (!x || !y || !z)) return set_error(c, BFE_EINVAL, "field_eval_batch: NULL input");
+  const int mm = disk_m < 0 ? c->mmax : std::min(disk_m, c->mmax);
+  const int nn = disk_n < 0 ? c->norder : std::min(disk_n, c->norder);
+  const int ll = halo_l < 0 ? c->lmax : std::min(halo_l, c->lmax);
+  const int hn = halo_n < 0 ? c->nmax : std::min(halo_n, c->nmax);
+  BFE_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  const size_t slf = sl_fold_floats(c), eoff = eof_fold_floats(c);
+  int rc = ensure_batch(c, &c->batch_sl, &c->batch_sl_floats, slf * nsnap);
+  if (rc) return rc;
+  rc = ensure_batch(c, &c->batch_eof, &c->batch_eof_floats, eoff * nsnap);
+  if (rc) return rc;
+  if (c->batch_off_cap < (size_t)nsnap + 1) {
+    if (c->batch_off) {
+      BFE_HIP(c, hipDeviceSynchronize());
+      (void)hipFree(c->batch_off);
+      c->batch_off = nullptr;
+    }
+    BFE_HIP(c, hipMalloc(&c->batch_off, ((size_t)nsnap + 1) * sizeof(int64_t)));
+    c->batch_off_cap = (size_t)nsnap + 1;
+  }
+  BFE_HIP(c, hipMemcpy(c->batch_off, offsets, ((size_t)nsnap + 1) * sizeof(int64_t),
+                       hipMemcpyHostToDevice));
+  const bool no_odd = (flags & BFE_NO_ODD) != 0;
+  rc = eof_fold_into(c, eof_cos, eof_sin, nsnap, mm, nn, 0, 1 << 30, no_odd,
+                     (flags & BFE_COMPAT_EOF_MASK) != 0, false, c->batch_eof, s);
+  if (rc) return rc;
+  rc = sl_fold_into(c, sl_coef, nsnap, halofac, ll, hn, -(1 << 30), 1 << 30, no_odd, c->batch_sl, s);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  FieldTables T;
+  T.slg = sl_geom(c);
+  T.eg = eof_geom(c);
+  T.slF = c->batch_sl;
+  T.eofF = c->batch_eof;
+  T.lmax = c->lmax;
+  T.flags = flags & (BFE_COMPAT_SL_COS_L | BFE_COMPAT_SL_EDGE);
+  T.ximin = *std::min_element(c->h_xi.begin(), c->h_xi.end());
+  T.crot = cos(rotpos);
+  T.srot = sin(rotpos);
+  const int* perm = nullptr;
+  const uint32_t* sorted = nullptr;
+  rc = order_points(c, T, x, y, z, n, s, &perm, c->batch_off, nsnap, &sorted);
+  if (rc) return rc;
+  hipLaunchKernelGGL(field_eval_batch_kernel, dim3(xcd_grid(nblocks256(n))), dim3(256), 0, s, x, y,
+                     z, n, perm, sorted, order_cell_count(T), T, c->batch_sl, (int64_t)slf,
+                     c->batch_eof, (int64_t)eoff, fx, fy, fz, pot);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }

@@ -99,9 +99,21 @@ struct Context {
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
 
+  // ---- tuning (bfe_set_tuning) --------------------------------------------
+  int64_t acc_max_blocks = 2048;   // accumulate: blocks (single) / segment budget (batch)
+  int64_t batch_seg_tiles = 0;     // batch accumulate: tiles per segment, 0 = from acc_max_blocks
+  int64_t order_min = 65536;       // evaluate in cell order from this many points
+
   // ---- spatial ordering of evaluation points (bfe_order.hip) -------------
-  void* order = nullptr;       // keys[n] | perm[n] | per-orbit energies[n] | bins[nbins + 1]
+  void* order = nullptr;       // keys[n] | sorted keys[n] | perm[n] | per-point f64[n] | sort temp
   size_t order_bytes = 0;
+
+  // ---- batched field evaluation: per-snapshot folded tables ---------------
+  float* batch_sl = nullptr;   // [nsnap][SL folded floats]
+  float* batch_eof = nullptr;  // [nsnap][EOF folded floats]
+  size_t batch_sl_floats = 0, batch_eof_floats = 0;
+  int64_t* batch_off = nullptr;  // device copy of the snapshot offsets
+  size_t batch_off_cap = 0;
 };
 
 // Table geometry passed to kernels by value.
@@ -126,21 +138,26 @@ inline EOFGeom eof_geom(const Context* c) {
                  c->ascale, c->hscale, c->xmin, c->dx, c->ymin, c->dy};
 }
 
-// Points with at least this many entries are evaluated in cell order.
-constexpr int64_t kOrderMinPoints = 65536;
-
 struct FieldTables;
 // Permutation perm[j] = i listing the points in order of their table cell
 // (EOF (R, z) cell inside the disc grid, else the SL radial cell), so that
 // the lanes of a wave read the same table rows.  Rebuilt by each call; the
 // returned pointer stays valid until the next call on this context.
+// With off_dev (device int64[nsnap + 1], concatenated snapshots) the key is
+// (snapshot, cell), and sorted_out (may be NULL) receives the sorted keys:
+// snapshot = key / order_cell_count(T).
 int order_points(Context* c, const FieldTables& T, const float* x, const float* y,
-                 const float* z, int64_t n, hipStream_t s, const int** perm_out);
+                 const float* z, int64_t n, hipStream_t s, const int** perm_out,
+                 const int64_t* off_dev = nullptr, int nsnap = 1,
+                 const uint32_t** sorted_out = nullptr);
+uint32_t order_cell_count(const FieldTables& T);
 // float64 per-point scratch in the ordering buffer (n entries), valid after
 // order_points for the same n.
 double* order_energy_buffer(Context* c, int64_t n);
 // Release the ordering buffer (bfe_destroy).
 void order_free(Context* c);
+// Release the batched-field buffers (bfe_destroy).
+void batch_free(Context* c);
 
 // Workgroups are dispatched to the 8 XCDs round-robin; this maps them so that
 // each XCD runs one contiguous eighth of the logical blocks (and so of the

@@ -37,12 +37,25 @@ __device__ __forceinline__ uint32_t cell_key(const FieldTables& T, float xf, flo
   return neof + (uint32_t)i;
 }
 
+// Snapshot of point i for concatenated snapshots: the last s with off[s] <= i.
+__device__ __forceinline__ int snapshot_of(const int64_t* __restrict__ off, int nsnap, int64_t i) {
+  int lo = 0, hi = nsnap - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// key = snapshot * nkeys + cell (snapshot 0 without offsets)
 __global__ __launch_bounds__(256) void order_key_kernel(
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-    int64_t n, FieldTables T, uint32_t neof, uint32_t* __restrict__ keys) {
+    int64_t n, FieldTables T, uint32_t neof, uint32_t nkeys, const int64_t* __restrict__ off,
+    int nsnap, uint32_t* __restrict__ keys) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  keys[i] = cell_key(T, x[i], y[i], z[i], neof);
+  const uint32_t snap = off ? (uint32_t)snapshot_of(off, nsnap, i) : 0u;
+  keys[i] = snap * nkeys + cell_key(T, x[i], y[i], z[i], neof);
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -78,11 +91,18 @@ static int ensure_order(Context* c, size_t bytes) {
   return 0;
 }
 
+uint32_t order_cell_count(const FieldTables& T) {
+  return (uint32_t)T.eg.numx * (uint32_t)T.eg.numy + (uint32_t)(T.slg.numr - 1);
+}
+
 int order_points(Context* c, const FieldTables& T, const float* x, const float* y, const float* z,
-                 int64_t n, hipStream_t s, const int** perm_out) {
+                 int64_t n, hipStream_t s, const int** perm_out, const int64_t* off_dev,
+                 int nsnap, const uint32_t** sorted_out) {
   *perm_out = nullptr;
+  if (sorted_out) *sorted_out = nullptr;
   if (n > (int64_t)0x7fffffff) return set_error(c, BFE_ELIMIT, "order: n = %lld > 2^31 - 1", (long long)n);
-  const uint64_t nkeys = (uint64_t)T.eg.numx * T.eg.numy + (uint64_t)(T.slg.numr - 1);
+  const uint64_t ncell = (uint64_t)T.eg.numx * T.eg.numy + (uint64_t)(T.slg.numr - 1);
+  const uint64_t nkeys = ncell * (uint64_t)(off_dev ? nsnap : 1);
   if (nkeys > 0xffffffffull) return set_error(c, BFE_ELIMIT, "order: too many table cells");
   unsigned bits = 1;
   while (bits < 32 && (1ull << bits) < nkeys) ++bits;
@@ -100,11 +120,12 @@ int order_points(Context* c, const FieldTables& T, const float* x, const float* 
   int* perm = (int*)(base + L.perm);
   const uint32_t neof = (uint32_t)T.eg.numx * (uint32_t)T.eg.numy;
   hipLaunchKernelGGL(order_key_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z,
-                     n, T, neof, keys);
+                     n, T, neof, (uint32_t)ncell, off_dev, nsnap, keys);
   BFE_HIP(c, hipGetLastError());
   BFE_HIP(c, rocprim::radix_sort_pairs((void*)(base + L.temp), temp_bytes, keys, sorted, iota, perm,
                                        ni, 0u, bits, s));
   *perm_out = perm;
+  if (sorted_out) *sorted_out = sorted;
   return 0;
 }
 

@@ -135,6 +135,40 @@ int bfe_leapfrog(bfe_ctx* ctx, float* x, float* y, float* z, float* vx, float* v
                  int energy_every, double* drift_out, double* energy_log, int record_every,
                  float* traj, int use_graph, void* stream);
 
+/* ---- batched snapshots ---------------------------------------------------- */
+/* Many snapshots in one launch: the particles of nsnap snapshots are
+ * concatenated, snapshot s holding [offsets[s], offsets[s+1]) (offsets: HOST
+ * int64[nsnap + 1], offsets[0] = 0).  Replaces a loop of the single-snapshot
+ * calls over snapshots (e.g. eof.compute_coefficients / spheresl
+ * .compute_coefficients per dump, eof.py:1158, spheresl.py:439); each
+ * snapshot's coefficients are reduced on their own in a fixed order, equal to
+ * what the single-snapshot call returns for the same particles up to the
+ * float32 tile sums.  Outputs: [nsnap][...] in the single-call layouts. */
+int bfe_eof_accumulate_batch(bfe_ctx* ctx, const float* x, const float* y, const float* z,
+                             const float* m, const int64_t* offsets, int nsnap, uint32_t flags,
+                             double* cos_out, double* sin_out, void* stream);
+int bfe_sl_accumulate_batch(bfe_ctx* ctx, const float* x, const float* y, const float* z,
+                            const float* m, const int64_t* offsets, int nsnap, uint32_t flags,
+                            double* coef_out, void* stream);
+/* Fields.return_forces_cart (potential.py:445-497) for nsnap frozen fields at
+ * once: snapshot s is evaluated with eof_cos/eof_sin[s] ((mmax+1) norder each)
+ * and sl_coef[s] ((lmax+1)^2 nmax) at its own points; truncation and flags as
+ * bfe_field_set.  Does not change the field set by bfe_field_set. */
+int bfe_field_eval_batch(bfe_ctx* ctx, const float* x, const float* y, const float* z,
+                         const int64_t* offsets, int nsnap, const double* eof_cos,
+                         const double* eof_sin, const double* sl_coef, double halofac, int disk_m,
+                         int disk_n, int halo_l, int halo_n, uint32_t flags, double rotpos,
+                         float* fx, float* fy, float* fz, float* pot, void* stream);
+
+/* ---- launch-shape tuning (no counterpart in the reference) ---------------- */
+/* Knobs: "acc_max_blocks" (accumulate blocks / batch segment budget, default
+ * 2048), "batch_seg_tiles" (tiles per batch segment, 0 = derived),
+ * "order_min" (points from which evaluation runs in table-cell order, 65536).
+ * Results do not depend on "order_min"; accumulate sums depend on the segment
+ * plan only through the float32 tile sums. */
+int bfe_set_tuning(bfe_ctx* ctx, const char* name, int64_t value);
+int bfe_get_tuning(const bfe_ctx* ctx, const char* name, int64_t* value);
+
 /* ---- MLP neural field (no counterpart in the reference) ----------------- */
 /* A 3 -> 128 -> 128 -> 128 -> 128 -> 4 SiLU network mapping (x, y, z) to
  * (Phi, F_R, F_phi, F_z) of a frozen field, trained only on evaluations of the

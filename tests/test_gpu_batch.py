@@ -1,0 +1,112 @@
+"""Batched snapshots (bfe_*_batch): many snapshots' coefficients in one
+accumulate launch and many frozen fields evaluated in one launch.
+
+Each snapshot's coefficients are checked against the FP64 oracle (<= 1e-6
+per block, as the single-snapshot path) and against the single-snapshot call;
+the batched field evaluation must equal bfe_field_set + bfe_field_eval_cart
+per snapshot bit for bit (same folded tables, same per-point code).
+"""
+import numpy as np
+import pytest
+
+from oracle import bfe_oracle as O
+from tests.test_gpu_parity import block_rel_eof, block_rel_sl, disc_halo, f32
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def sysm():
+    from exptool_amd.basis import tablegen as tg
+    from exptool_amd.bfe import BFESystem
+    from exptool_amd.utils import samplers as S
+    slt = tg.make_sl_tables(6, 20, 2000, 1e-3, 200.0, 1, 5.0, model_mass=10.0)
+    d = S.exponential_disc(30000, seed=3)
+    et = tg.make_eof_tables(d, mmax=6, norder=18, numx=256, numy=128, lmax_s=16, nmax_s=12)
+    slo = O.SLTable(slt["lmax"], slt["nmax"], slt["numr"], slt["cmap"], slt["rmin"], slt["rmax"],
+                    slt["scale"], slt["ev"], slt["ef"], slt["xi"], slt["p0"], slt["d0"])
+    eo = O.EOFTable(et["mmax"], et["norder"], et["numx"], et["numy"], et["cmap"], et["rmin"],
+                    et["rmax"], et["ascale"], et["hscale"], et["potC"], et["rforceC"],
+                    et["zforceC"], et["potS"], et["rforceS"], et["zforceS"])
+    return BFESystem(sl=slt, eof=et), slo, eo
+
+
+COUNTS = [3000, 0, 1500, 40000, 257, 9000]
+
+
+def _snapshots():
+    parts_d, parts_h = [], []
+    for k, n in enumerate(COUNTS):
+        d, h = disc_halo(max(n, 1), 100 + k)
+        parts_d.append([v[:n] for v in d])
+        parts_h.append([v[:n] for v in h])
+    cat = lambda P: [np.concatenate([p[j] for p in P]) for j in range(4)]
+    return parts_d, parts_h, f32(*cat(parts_d)), f32(*cat(parts_h))
+
+
+def test_batch_accumulate_matches_oracle_and_single(sysm):
+    s, slo, eo = sysm
+    pd, ph, D, H = _snapshots()
+    for tiles in (0, 1, 7):
+        s.set_tuning("batch_seg_tiles", tiles)
+        c, sn = (v.cpu().numpy() for v in s.eof_accumulate_batch(*D, COUNTS))
+        h = s.sl_accumulate_batch(*H, COUNTS).cpu().numpy()
+        assert c.shape == (len(COUNTS), 7, 18) and h.shape == (len(COUNTS), 49, 20)
+        for k, n in enumerate(COUNTS):
+            if n == 0:
+                assert np.all(c[k] == 0) and np.all(sn[k] == 0) and np.all(h[k] == 0)
+                continue
+            d = f32(*pd[k])
+            rc, rs = O.eof_accumulate(eo, *d)
+            assert block_rel_eof(c[k], rc) <= 1e-6, (tiles, k)
+            assert block_rel_eof(sn[k][1:], rs[1:]) <= 1e-6, (tiles, k)
+            hh = f32(*ph[k])
+            assert block_rel_sl(h[k], O.sl_accumulate(slo, *hh), 6) <= 1e-6, (tiles, k)
+            c1, s1 = (v.cpu().numpy() for v in s.eof_accumulate(*d))
+            assert block_rel_eof(c[k], c1) <= 1e-6
+    s.set_tuning("batch_seg_tiles", 0)
+
+
+def test_batch_field_equals_per_snapshot_field(sysm):
+    s, _, _ = sysm
+    pd, ph, D, H = _snapshots()
+    c, sn = s.eof_accumulate_batch(*D, COUNTS)
+    h = s.sl_accumulate_batch(*H, COUNTS)
+    # evaluate each snapshot's field at its disc particles
+    x, y, z = D[0], D[1], D[2]
+    for no_odd in (False, True):
+        got = [v.cpu().numpy() for v in s.field_forces_batch(x, y, z, COUNTS, c, sn, h,
+                                                             no_odd=no_odd, rotpos=0.3)]
+        off = np.concatenate([[0], np.cumsum(COUNTS)])
+        for k, n in enumerate(COUNTS):
+            if n == 0:
+                continue
+            s.set_field(c[k], sn[k], h[k], no_odd=no_odd)
+            ref = [v.cpu().numpy() for v in s.field_forces(x[off[k]:off[k + 1]], y[off[k]:off[k + 1]],
+                                                            z[off[k]:off[k + 1]], rotpos=0.3)]
+            for j in range(4):
+                assert np.array_equal(got[j][off[k]:off[k + 1]], ref[j]), (no_odd, k, j)
+
+
+def test_batch_sampler_and_errors(sysm):
+    s, _, _ = sysm
+    from exptool_amd._lib import BFEError
+    from exptool_amd.utils import workloads as W
+    b = W.batch_snapshots(nsnap=4, lo=2000, hi=20000, seed=5)
+    assert b["disc"][0].numel() == b["disc_counts"].sum()
+    assert b["halo"][0].numel() == b["halo_counts"].sum()
+    off = np.concatenate([[0], np.cumsum(b["disc_counts"])])
+    m = b["disc"][3].double().cpu().numpy()
+    for k in range(4):
+        assert m[off[k]:off[k + 1]].sum() == pytest.approx(1.0, rel=1e-5)
+    R = np.hypot(b["disc"][0].cpu().numpy(), b["disc"][1].cpu().numpy())
+    assert R.max() <= 10.0 + 1e-4
+    b2 = W.batch_snapshots(nsnap=4, lo=2000, hi=20000, seed=5)
+    assert np.array_equal(b["disc"][0].cpu().numpy(), b2["disc"][0].cpu().numpy())
+    D = [v[:100] for v in b["disc"]]
+    with pytest.raises(ValueError):
+        s.eof_accumulate_batch(*D, [50, 40])          # does not sum to n
+    with pytest.raises(BFEError):
+        s.set_tuning("acc_max_blocks", 0)
+    with pytest.raises(BFEError):
+        s.set_tuning("no_such_knob", 1)
