@@ -74,6 +74,9 @@ def main():
     ap.add_argument("--mlp-points", type=int, default=16_000_000)
     ap.add_argument("--mlp-train-seconds", type=float, default=60.0)
     ap.add_argument("--mlp-steps", type=int, default=40000)
+    ap.add_argument("--batch", type=int, default=1, help="time the batched-snapshot workload (0/1)")
+    ap.add_argument("--batch-nsnap", type=int, default=64)
+    ap.add_argument("--tune", type=int, default=1, help="autotune accumulate launch shapes (0/1)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (reference for PMC byte counters)")
     a = ap.parse_args()
@@ -107,6 +110,11 @@ def main():
     ax, ay, az = (torch.cat([p, q]) for p, q in ((dx, hx), (dy, hy), (dz, hz)))
     ntot = ax.numel()
     setup_s = time.time() - t_setup
+
+    tuned = None
+    if a.tune:
+        from exptool_amd.autotune import autotune
+        tuned = autotune(sysm, disc=(dx, dy, dz, dm), halo=(hx, hy, hz, hm), cache=False)
 
     def step():
         c, s = sysm.eof_accumulate(dx, dy, dz, dm)
@@ -185,6 +193,9 @@ def main():
         if a.mlp:
             surrogate = mlp_stage(a, sysm, st0, orbit)
 
+    # ---- batched snapshots ------------------------------------------------------
+    batch = batch_stage(a, sysm) if a.batch else None
+
     # ---- CPU baseline: the FP64 oracle on a bounded sample --------------------
     cpu = None
     if not a.no_cpu:
@@ -199,7 +210,8 @@ def main():
         "config": {"workload": "disc EOF mmax6 norder18 256x128 + halo SL lmax6 nmax20 numr2000, "
                                "1M+1M particles, accumulate + return_forces_cart at 2M",
                    "n_disc": a.n_disc, "n_halo": a.n_halo, "parallelism": "single GPU"},
-        "kernel_seconds": kt, "roofline": roof, "orbits": orbit, "surrogate": surrogate,
+        "kernel_seconds": kt, "roofline": roof, "tuning": tuned, "orbits": orbit,
+        "surrogate": surrogate, "batch": batch,
         "cpu_baseline": cpu,
         "setup_seconds": setup_s,
     }
@@ -237,6 +249,65 @@ def roofline(kernel, unit, seconds, pmc):
         rec["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated (profiles/bench_step_pmc_traffic.md)"
         rec["traffic_gbs"] = tr["bytes"] / seconds / 1e9
     return rec
+
+
+def batch_stage(a, sysm):
+    """Batched snapshots: nsnap disc + halo snapshots (disc counts log-uniform
+    in [200k, 4M], barred discs), every snapshot's EOF and SL coefficients
+    accumulated in one launch each, every snapshot's field frozen in one call
+    and every particle evaluated in its own snapshot's field."""
+    import torch
+    from exptool_amd.autotune import tune_accumulate
+    from exptool_amd.utils import workloads as W
+    t0 = time.perf_counter()
+    b = W.batch_snapshots(nsnap=a.batch_nsnap, seed=0, device=sysm.device)
+    torch.cuda.synchronize()
+    t_sample = time.perf_counter() - t0
+    D, H = b["disc"], b["halo"]
+    dc, hc = b["disc_counts"], b["halo_counts"]
+    tuned = {}
+    if a.tune:
+        for kind, P, cnt in (("eof", D, dc), ("sl", H, hc)):
+            r = tune_accumulate(sysm, *P, kind=kind, counts=cnt, reps=1, candidates=(2048, 4096))
+            tuned.update({k: v for k, v in r.items() if k != "timings"})
+    stages = {}
+
+    def run(rec):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        ev[0].record()
+        c, s = sysm.eof_accumulate_batch(*D, dc)
+        ev[1].record()
+        h = sysm.sl_accumulate_batch(*H, hc)
+        ev[2].record()
+        sysm.set_field_batch(c, s, h)
+        ev[3].record()
+        sysm.field_forces_batch(D[0], D[1], D[2], dc)
+        ev[4].record()
+        sysm.field_forces_batch(H[0], H[1], H[2], hc)
+        ev[5].record()
+        if rec:
+            torch.cuda.synchronize()
+            for k, (i, j) in {"eof_accumulate_batch": (0, 1), "sl_accumulate_batch": (1, 2),
+                              "field_set_batch": (2, 3), "field_eval_batch_disc": (3, 4),
+                              "field_eval_batch_halo": (4, 5)}.items():
+                stages.setdefault(k, []).append(ev[i].elapsed_time(ev[j]) * 1e-3)
+
+    run(False)
+    torch.cuda.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run(True)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    ntot = int(dc.sum() + hc.sum())
+    return {"workload": f"{a.batch_nsnap} disc+halo snapshots, disc counts log-uniform in [2e5, 4e6] "
+                        "(m=2 bar, A in [0, 0.3]), halo counts equal; same bases as the step; every "
+                        "particle accumulated once and evaluated in its own snapshot's field",
+            "particles": ntot, "snapshots": a.batch_nsnap, "seconds_per_pass": sec,
+            "particles_per_s": ntot / sec,
+            "stage_seconds": {k: float(np.mean(v)) for k, v in stages.items()},
+            "sample_seconds": t_sample, "tuning": tuned}
 
 
 def mlp_stage(a, sysm, st0, orbit):

@@ -171,28 +171,35 @@ class BFESystem:
         self.ctx.check(rc, "bfe_sl_accumulate_batch")
         return out
 
-    def field_forces_batch(self, x, y, z, counts, eof_cos, eof_sin, sl_coef, halofac=1.0,
-                           no_odd=False, disk_m=-1, disk_n=-1, halo_l=-1, halo_n=-1,
-                           compat=False, rotpos=0.0, stream=None):
-        """Each snapshot's own frozen field at its own points (one launch):
-        (fx, fy, fz, pot) float32 device tensors over the concatenated points."""
+    def set_field_batch(self, eof_cos, eof_sin, sl_coef, halofac=1.0, no_odd=False, disk_m=-1,
+                        disk_n=-1, halo_l=-1, halo_n=-1, compat=False, stream=None):
+        """Freeze nsnap fields at once (eof_cos/eof_sin (nsnap, mmax+1, norder),
+        sl_coef (nsnap, (lmax+1)^2, nmax))."""
         self._need("sl")
         self._need("eof")
+        ec, es, sc = (f64(v, self.device) for v in (eof_cos, eof_sin, sl_coef))
+        per_e = (self.eof["mmax"] + 1) * self.eof["norder"]
+        per_s = (self.sl["lmax"] + 1) ** 2 * self.sl["nmax"]
+        ns = ec.numel() // per_e
+        if ec.numel() != ns * per_e or es.numel() != ec.numel() or sc.numel() != ns * per_s:
+            raise ValueError("coefficients must be (nsnap, mmax+1, norder) x2 and (nsnap, (lmax+1)^2, nmax)")
+        flags = (_lib.NO_ODD if no_odd else 0) | (_lib.COMPAT_ALL if compat else 0)
+        rc = self.ctx.lib.bfe_field_set_batch(self.ctx.h, ptr(ec), ptr(es), ptr(sc), ns,
+                                              float(halofac), int(disk_m), int(disk_n), int(halo_l),
+                                              int(halo_n), flags, stream_ptr(stream))
+        self.ctx.check(rc, "bfe_field_set_batch")
+        self._batch_keep = (ec, es, sc)
+        self._batch_nsnap = ns
+
+    def field_forces_batch(self, x, y, z, counts, rotpos=0.0, stream=None):
+        """Forces of the batch fields: the counts[s] points of snapshot s (points
+        concatenated) in field s.  Returns (fx, fy, fz, pot) float32 device tensors."""
         x, y, z, n = self._xyz(x, y, z)
         off = self._offsets(counts, n)
-        ns = off.size - 1
-        ec, es, sc = (f64(v, self.device) for v in (eof_cos, eof_sin, sl_coef))
-        if ec.numel() != ns * (self.eof["mmax"] + 1) * self.eof["norder"] or es.numel() != ec.numel():
-            raise ValueError("eof_cos/eof_sin must be (nsnap, mmax+1, norder)")
-        if sc.numel() != ns * (self.sl["lmax"] + 1) ** 2 * self.sl["nmax"]:
-            raise ValueError("sl_coef must be (nsnap, (lmax+1)^2, nmax)")
         outs = [torch.empty(n, dtype=torch.float32, device=self.device) for _ in range(4)]
-        flags = (_lib.NO_ODD if no_odd else 0) | (_lib.COMPAT_ALL if compat else 0)
-        rc = self.ctx.lib.bfe_field_eval_batch(self.ctx.h, ptr(x), ptr(y), ptr(z), host_ptr(off), ns,
-                                               ptr(ec), ptr(es), ptr(sc), float(halofac), int(disk_m),
-                                               int(disk_n), int(halo_l), int(halo_n), flags,
-                                               float(rotpos), *[ptr(o) for o in outs],
-                                               stream_ptr(stream))
+        rc = self.ctx.lib.bfe_field_eval_batch(self.ctx.h, ptr(x), ptr(y), ptr(z), host_ptr(off),
+                                               off.size - 1, float(rotpos),
+                                               *[ptr(o) for o in outs], stream_ptr(stream))
         self.ctx.check(rc, "bfe_field_eval_batch")
         return tuple(outs)
 

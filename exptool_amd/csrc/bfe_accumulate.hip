@@ -336,7 +336,7 @@ __global__ __launch_bounds__(256) void sl_accumulate_kernel(
 }
 
 // Segment plan of one accumulate launch.  Single snapshot: at most
-// acc_max_blocks segments of whole tiles, one per block.  Batch: every
+// `cap` segments of whole tiles, one per block.  Batch: every
 // snapshot is cut into segments of seg_tiles tiles (never spanning two
 // snapshots); the blocks loop over all segments and rowb[s] is the first
 // segment of snapshot s, so each snapshot's partial rows are reduced on
@@ -347,10 +347,10 @@ struct AccPlan {
   std::vector<int64_t> segb, rowb;  // batch only (host copies)
 };
 
-static AccPlan plan_single(const Context* c, int64_t n, int tile) {
+static AccPlan plan_single(int64_t cap, int64_t n, int tile) {
   AccPlan P;
   const int64_t ntiles = (n + tile - 1) / tile;
-  int64_t nseg = ntiles < c->acc_max_blocks ? ntiles : c->acc_max_blocks;
+  int64_t nseg = ntiles < cap ? ntiles : cap;
   if (nseg < 1) nseg = 1;
   const int64_t per = (ntiles + nseg - 1) / nseg;
   P.S.segb = nullptr;
@@ -362,13 +362,12 @@ static AccPlan plan_single(const Context* c, int64_t n, int tile) {
   return P;
 }
 
-static AccPlan plan_batch(const Context* c, const int64_t* off, int nsnap, int tile) {
+static AccPlan plan_batch(int64_t cap, int64_t seg_tiles, const int64_t* off, int nsnap, int tile) {
   AccPlan P;
   int64_t total_tiles = 0;
   for (int q = 0; q < nsnap; ++q) total_tiles += (off[q + 1] - off[q] + tile - 1) / tile;
-  int64_t seg_tiles = c->batch_seg_tiles;
   if (seg_tiles <= 0) {
-    seg_tiles = (total_tiles + c->acc_max_blocks - 1) / c->acc_max_blocks;
+    seg_tiles = (total_tiles + cap - 1) / cap;
     if (seg_tiles < 1) seg_tiles = 1;
   }
   P.rowb.push_back(0);
@@ -386,7 +385,7 @@ static AccPlan plan_batch(const Context* c, const int64_t* off, int nsnap, int t
   P.S.seglen = 0;
   P.S.n = off[nsnap];
   P.S.nseg = (int64_t)P.segb.size() - 1;
-  P.grid = (int)(P.S.nseg < c->acc_max_blocks ? P.S.nseg : c->acc_max_blocks);
+  P.grid = (int)(P.S.nseg < cap ? P.S.nseg : cap);
   return P;
 }
 
@@ -417,7 +416,8 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
                               const float* m, int64_t n, const int64_t* off, int nsnap,
                               double* cos_out, double* sin_out, hipStream_t s) {
   const int K = c->eof_K;
-  AccPlan P = off ? plan_batch(c, off, nsnap, kTile) : plan_single(c, n, kTile);
+  AccPlan P = off ? plan_batch(c->eof_acc_blocks, c->eof_seg_tiles, off, nsnap, kTile)
+                  : plan_single(c->eof_acc_blocks, n, kTile);
   const size_t pbytes = (size_t)P.S.nseg * K * sizeof(double);
   const size_t sums_off = (pbytes + 255) & ~(size_t)255;
   const size_t need = sums_off + (size_t)nsnap * K * sizeof(double);
@@ -462,7 +462,8 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
   if (K > 1024) return set_error(c, BFE_ELIMIT, "sl_accumulate: (lmax+1)^2 nmax = %d > 1024", K);
   const size_t lds = ((size_t)R * (kSLTile + 1) + (size_t)kSLTile * LN + 3 * kSLTile) * 4;
   if (lds > 64 * 1024) return set_error(c, BFE_ELIMIT, "sl_accumulate: LDS %zu B > 64 KiB", lds);
-  AccPlan P = off ? plan_batch(c, off, nsnap, kSLTile) : plan_single(c, n, kSLTile);
+  AccPlan P = off ? plan_batch(c->sl_acc_blocks, c->sl_seg_tiles, off, nsnap, kSLTile)
+                  : plan_single(c->sl_acc_blocks, n, kSLTile);
   const size_t pbytes = (size_t)P.S.nseg * K * sizeof(double);
   const int64_t* rowb = nullptr;
   int rc = off ? upload_plan(c, P, pbytes, &rowb) : ensure_scratch(c, pbytes);

@@ -236,12 +236,12 @@ extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, in
 extern "C" int bfe_set_tuning(bfe_ctx* h, const char* name, int64_t value) {
   if (!h || !name) return BFE_EINVAL;
   Context* c = &h->c;
-  if (!strcmp(name, "acc_max_blocks")) {
-    if (value < 1 || value > (1 << 20)) return set_error(c, BFE_EINVAL, "acc_max_blocks outside [1, 2^20]");
-    c->acc_max_blocks = value;
-  } else if (!strcmp(name, "batch_seg_tiles")) {
-    if (value < 0) return set_error(c, BFE_EINVAL, "batch_seg_tiles < 0");
-    c->batch_seg_tiles = value;
+  if (!strcmp(name, "eof_acc_blocks") || !strcmp(name, "sl_acc_blocks")) {
+    if (value < 1 || value > (1 << 20)) return set_error(c, BFE_EINVAL, "%s outside [1, 2^20]", name);
+    (name[0] == 'e' ? c->eof_acc_blocks : c->sl_acc_blocks) = value;
+  } else if (!strcmp(name, "eof_seg_tiles") || !strcmp(name, "sl_seg_tiles")) {
+    if (value < 0) return set_error(c, BFE_EINVAL, "%s < 0", name);
+    (name[0] == 'e' ? c->eof_seg_tiles : c->sl_seg_tiles) = value;
   } else if (!strcmp(name, "order_min")) {
     if (value < 0) return set_error(c, BFE_EINVAL, "order_min < 0");
     c->order_min = value;
@@ -254,8 +254,10 @@ extern "C" int bfe_set_tuning(bfe_ctx* h, const char* name, int64_t value) {
 extern "C" int bfe_get_tuning(const bfe_ctx* h, const char* name, int64_t* value) {
   if (!h || !name || !value) return BFE_EINVAL;
   const Context* c = &h->c;
-  if (!strcmp(name, "acc_max_blocks")) *value = c->acc_max_blocks;
-  else if (!strcmp(name, "batch_seg_tiles")) *value = c->batch_seg_tiles;
+  if (!strcmp(name, "eof_acc_blocks")) *value = c->eof_acc_blocks;
+  else if (!strcmp(name, "sl_acc_blocks")) *value = c->sl_acc_blocks;
+  else if (!strcmp(name, "eof_seg_tiles")) *value = c->eof_seg_tiles;
+  else if (!strcmp(name, "sl_seg_tiles")) *value = c->sl_seg_tiles;
   else if (!strcmp(name, "order_min")) *value = c->order_min;
   else return BFE_EINVAL;
   return 0;

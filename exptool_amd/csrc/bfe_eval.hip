@@ -441,34 +441,55 @@ extern "C" int bfe_field_eval_cart(bfe_ctx* h, const float* x, const float* y, c
   return 0;
 }
 
-extern "C" int bfe_field_eval_batch(bfe_ctx* h, const float* x, const float* y, const float* z,
-                                    const int64_t* offsets, int nsnap, const double* eof_cos,
-                                    const double* eof_sin, const double* sl_coef, double halofac,
-                                    int disk_m, int disk_n, int halo_l, int halo_n,
-                                    uint32_t flags, double rotpos, float* fx, float* fy, float* fz,
-                                    float* pot, void* stream) {
+extern "C" int bfe_field_set_batch(bfe_ctx* h, const double* eof_cos, const double* eof_sin,
+                                   const double* sl_coef, int nsnap, double halofac, int disk_m,
+                                   int disk_n, int halo_l, int halo_n, uint32_t flags,
+                                   void* stream) {
   if (!h) return BFE_EINVAL;
   Context* c = &h->c;
-  if (!c->sl_ok || !c->eof_ok) return set_error(c, BFE_ENOTABLE, "field_eval_batch: tables not set");
-  if (!eof_cos || !eof_sin || !sl_coef) return set_error(c, BFE_EINVAL, "field_eval_batch: NULL coef");
-  if (!offsets || nsnap < 1 || nsnap > 65535)
-    return set_error(c, BFE_EINVAL, "field_eval_batch: need offsets and 1 <= nsnap <= 65535");
-  if (offsets[0] != 0) return set_error(c, BFE_EINVAL, "field_eval_batch: offsets[0] must be 0");
-  for (int q = 0; q < nsnap; ++q)
-    if (offsets[q + 1] < offsets[q]) return set_error(c, BFE_EINVAL, "field_eval_batch: offsets must not decrease");
-  const int64_t n = offsets[nsnap];
-  if (n > 0 && (!x || !y || !z)) return set_error(c, BFE_EINVAL, "field_eval_batch: NULL input");
+  if (!c->sl_ok || !c->eof_ok) return set_error(c, BFE_ENOTABLE, "field_set_batch: tables not set");
+  if (!eof_cos || !eof_sin || !sl_coef) return set_error(c, BFE_EINVAL, "field_set_batch: NULL coef");
+  if (nsnap < 1 || nsnap > 65535) return set_error(c, BFE_EINVAL, "field_set_batch: nsnap outside [1, 65535]");
   const int mm = disk_m < 0 ? c->mmax : std::min(disk_m, c->mmax);
   const int nn = disk_n < 0 ? c->norder : std::min(disk_n, c->norder);
   const int ll = halo_l < 0 ? c->lmax : std::min(halo_l, c->lmax);
   const int hn = halo_n < 0 ? c->nmax : std::min(halo_n, c->nmax);
   BFE_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
+  c->batch_nsnap = 0;
   const size_t slf = sl_fold_floats(c), eoff = eof_fold_floats(c);
   int rc = ensure_batch(c, &c->batch_sl, &c->batch_sl_floats, slf * nsnap);
   if (rc) return rc;
   rc = ensure_batch(c, &c->batch_eof, &c->batch_eof_floats, eoff * nsnap);
   if (rc) return rc;
+  const bool no_odd = (flags & BFE_NO_ODD) != 0;
+  rc = eof_fold_into(c, eof_cos, eof_sin, nsnap, mm, nn, 0, 1 << 30, no_odd,
+                     (flags & BFE_COMPAT_EOF_MASK) != 0, false, c->batch_eof, s);
+  if (rc) return rc;
+  rc = sl_fold_into(c, sl_coef, nsnap, halofac, ll, hn, -(1 << 30), 1 << 30, no_odd, c->batch_sl, s);
+  if (rc) return rc;
+  c->batch_nsnap = nsnap;
+  c->batch_flags = (int)(flags & (BFE_COMPAT_SL_COS_L | BFE_COMPAT_SL_EDGE));
+  return 0;
+}
+
+extern "C" int bfe_field_eval_batch(bfe_ctx* h, const float* x, const float* y, const float* z,
+                                    const int64_t* offsets, int nsnap, double rotpos, float* fx,
+                                    float* fy, float* fz, float* pot, void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (c->batch_nsnap < 1) return set_error(c, BFE_ENOTABLE, "field_eval_batch: no batch field (bfe_field_set_batch)");
+  if (!offsets || nsnap != c->batch_nsnap)
+    return set_error(c, BFE_EINVAL, "field_eval_batch: need offsets for the %d snapshots of the batch field",
+                     c->batch_nsnap);
+  if (offsets[0] != 0) return set_error(c, BFE_EINVAL, "field_eval_batch: offsets[0] must be 0");
+  for (int q = 0; q < nsnap; ++q)
+    if (offsets[q + 1] < offsets[q]) return set_error(c, BFE_EINVAL, "field_eval_batch: offsets must not decrease");
+  const int64_t n = offsets[nsnap];
+  if (n > 0 && (!x || !y || !z)) return set_error(c, BFE_EINVAL, "field_eval_batch: NULL input");
+  if (n == 0) return 0;
+  BFE_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
   if (c->batch_off_cap < (size_t)nsnap + 1) {
     if (c->batch_off) {
       BFE_HIP(c, hipDeviceSynchronize());
@@ -478,28 +499,24 @@ extern "C" int bfe_field_eval_batch(bfe_ctx* h, const float* x, const float* y, 
     BFE_HIP(c, hipMalloc(&c->batch_off, ((size_t)nsnap + 1) * sizeof(int64_t)));
     c->batch_off_cap = (size_t)nsnap + 1;
   }
+  // the offsets buffer may still be read by an earlier call on this stream
+  BFE_HIP(c, hipStreamSynchronize(s));
   BFE_HIP(c, hipMemcpy(c->batch_off, offsets, ((size_t)nsnap + 1) * sizeof(int64_t),
                        hipMemcpyHostToDevice));
-  const bool no_odd = (flags & BFE_NO_ODD) != 0;
-  rc = eof_fold_into(c, eof_cos, eof_sin, nsnap, mm, nn, 0, 1 << 30, no_odd,
-                     (flags & BFE_COMPAT_EOF_MASK) != 0, false, c->batch_eof, s);
-  if (rc) return rc;
-  rc = sl_fold_into(c, sl_coef, nsnap, halofac, ll, hn, -(1 << 30), 1 << 30, no_odd, c->batch_sl, s);
-  if (rc) return rc;
-  if (n == 0) return 0;
+  const size_t slf = sl_fold_floats(c), eoff = eof_fold_floats(c);
   FieldTables T;
   T.slg = sl_geom(c);
   T.eg = eof_geom(c);
   T.slF = c->batch_sl;
   T.eofF = c->batch_eof;
   T.lmax = c->lmax;
-  T.flags = flags & (BFE_COMPAT_SL_COS_L | BFE_COMPAT_SL_EDGE);
+  T.flags = (uint32_t)c->batch_flags;
   T.ximin = *std::min_element(c->h_xi.begin(), c->h_xi.end());
   T.crot = cos(rotpos);
   T.srot = sin(rotpos);
   const int* perm = nullptr;
   const uint32_t* sorted = nullptr;
-  rc = order_points(c, T, x, y, z, n, s, &perm, c->batch_off, nsnap, &sorted);
+  int rc = order_points(c, T, x, y, z, n, s, &perm, c->batch_off, nsnap, &sorted);
   if (rc) return rc;
   hipLaunchKernelGGL(field_eval_batch_kernel, dim3(xcd_grid(nblocks256(n))), dim3(256), 0, s, x, y,
                      z, n, perm, sorted, order_cell_count(T), T, c->batch_sl, (int64_t)slf,

@@ -100,8 +100,10 @@ struct Context {
   size_t scratch_bytes = 0;
 
   // ---- tuning (bfe_set_tuning) --------------------------------------------
-  int64_t acc_max_blocks = 2048;   // accumulate: blocks (single) / segment budget (batch)
-  int64_t batch_seg_tiles = 0;     // batch accumulate: tiles per segment, 0 = from acc_max_blocks
+  int64_t eof_acc_blocks = 2048;   // EOF accumulate: blocks (single) / segment budget (batch)
+  int64_t sl_acc_blocks = 2048;    // SL accumulate: the same
+  int64_t eof_seg_tiles = 0;       // batch accumulate: tiles per segment, 0 = from the budget
+  int64_t sl_seg_tiles = 0;
   int64_t order_min = 65536;       // evaluate in cell order from this many points
 
   // ---- spatial ordering of evaluation points (bfe_order.hip) -------------
@@ -114,6 +116,8 @@ struct Context {
   size_t batch_sl_floats = 0, batch_eof_floats = 0;
   int64_t* batch_off = nullptr;  // device copy of the snapshot offsets
   size_t batch_off_cap = 0;
+  int batch_nsnap = 0;           // coefficient sets folded by bfe_field_set_batch
+  int batch_flags = 0;
 };
 
 // Table geometry passed to kernels by value.

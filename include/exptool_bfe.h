@@ -150,19 +150,26 @@ int bfe_eof_accumulate_batch(bfe_ctx* ctx, const float* x, const float* y, const
 int bfe_sl_accumulate_batch(bfe_ctx* ctx, const float* x, const float* y, const float* z,
                             const float* m, const int64_t* offsets, int nsnap, uint32_t flags,
                             double* coef_out, void* stream);
-/* Fields.return_forces_cart (potential.py:445-497) for nsnap frozen fields at
- * once: snapshot s is evaluated with eof_cos/eof_sin[s] ((mmax+1) norder each)
- * and sl_coef[s] ((lmax+1)^2 nmax) at its own points; truncation and flags as
- * bfe_field_set.  Does not change the field set by bfe_field_set. */
+/* Fields.prep_tables + set_field_parameters (potential.py:257-294, :365-378)
+ * for nsnap frozen fields at once: eof_cos/eof_sin [nsnap][mmax+1][norder],
+ * sl_coef [nsnap][(lmax+1)^2][nmax]; truncation and flags as bfe_field_set.
+ * Independent of the field set by bfe_field_set. */
+int bfe_field_set_batch(bfe_ctx* ctx, const double* eof_cos, const double* eof_sin,
+                        const double* sl_coef, int nsnap, double halofac, int disk_m, int disk_n,
+                        int halo_l, int halo_n, uint32_t flags, void* stream);
+/* Fields.return_forces_cart (potential.py:445-497) of the batch fields: the
+ * points of snapshot s, [offsets[s], offsets[s+1]) (HOST offsets, nsnap as
+ * given to bfe_field_set_batch), are evaluated in field s.  Outputs as
+ * bfe_field_eval_cart (sums; no components).  Blocks the host until earlier
+ * work on `stream` is done (the offsets are copied to the device). */
 int bfe_field_eval_batch(bfe_ctx* ctx, const float* x, const float* y, const float* z,
-                         const int64_t* offsets, int nsnap, const double* eof_cos,
-                         const double* eof_sin, const double* sl_coef, double halofac, int disk_m,
-                         int disk_n, int halo_l, int halo_n, uint32_t flags, double rotpos,
-                         float* fx, float* fy, float* fz, float* pot, void* stream);
+                         const int64_t* offsets, int nsnap, double rotpos, float* fx, float* fy,
+                         float* fz, float* pot, void* stream);
 
 /* ---- launch-shape tuning (no counterpart in the reference) ---------------- */
-/* Knobs: "acc_max_blocks" (accumulate blocks / batch segment budget, default
- * 2048), "batch_seg_tiles" (tiles per batch segment, 0 = derived),
+/* Knobs: "eof_acc_blocks", "sl_acc_blocks" (accumulate blocks / batch
+ * segment budget per kernel, default 2048), "eof_seg_tiles", "sl_seg_tiles"
+ * (tiles per batch segment, 0 = derived from the budget),
  * "order_min" (points from which evaluation runs in table-cell order, 65536).
  * Results do not depend on "order_min"; accumulate sums depend on the segment
  * plan only through the float32 tile sums. */

@@ -48,7 +48,8 @@ def test_batch_accumulate_matches_oracle_and_single(sysm):
     s, slo, eo = sysm
     pd, ph, D, H = _snapshots()
     for tiles in (0, 1, 7):
-        s.set_tuning("batch_seg_tiles", tiles)
+        s.set_tuning("eof_seg_tiles", tiles)
+        s.set_tuning("sl_seg_tiles", tiles)
         c, sn = (v.cpu().numpy() for v in s.eof_accumulate_batch(*D, COUNTS))
         h = s.sl_accumulate_batch(*H, COUNTS).cpu().numpy()
         assert c.shape == (len(COUNTS), 7, 18) and h.shape == (len(COUNTS), 49, 20)
@@ -64,7 +65,8 @@ def test_batch_accumulate_matches_oracle_and_single(sysm):
             assert block_rel_sl(h[k], O.sl_accumulate(slo, *hh), 6) <= 1e-6, (tiles, k)
             c1, s1 = (v.cpu().numpy() for v in s.eof_accumulate(*d))
             assert block_rel_eof(c[k], c1) <= 1e-6
-    s.set_tuning("batch_seg_tiles", 0)
+    s.set_tuning("eof_seg_tiles", 0)
+    s.set_tuning("sl_seg_tiles", 0)
 
 
 def test_batch_field_equals_per_snapshot_field(sysm):
@@ -75,8 +77,8 @@ def test_batch_field_equals_per_snapshot_field(sysm):
     # evaluate each snapshot's field at its disc particles
     x, y, z = D[0], D[1], D[2]
     for no_odd in (False, True):
-        got = [v.cpu().numpy() for v in s.field_forces_batch(x, y, z, COUNTS, c, sn, h,
-                                                             no_odd=no_odd, rotpos=0.3)]
+        s.set_field_batch(c, sn, h, no_odd=no_odd)
+        got = [v.cpu().numpy() for v in s.field_forces_batch(x, y, z, COUNTS, rotpos=0.3)]
         off = np.concatenate([[0], np.cumsum(COUNTS)])
         for k, n in enumerate(COUNTS):
             if n == 0:
@@ -107,6 +109,27 @@ def test_batch_sampler_and_errors(sysm):
     with pytest.raises(ValueError):
         s.eof_accumulate_batch(*D, [50, 40])          # does not sum to n
     with pytest.raises(BFEError):
-        s.set_tuning("acc_max_blocks", 0)
+        s.set_tuning("eof_acc_blocks", 0)
     with pytest.raises(BFEError):
         s.set_tuning("no_such_knob", 1)
+
+
+def test_autotune_applies_choice_within_tolerance(sysm, tmp_path, monkeypatch):
+    s, slo, eo = sysm
+    from exptool_amd import autotune as A
+    monkeypatch.setenv("EXPTOOL_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    pd, ph, D, H = _snapshots()
+    base = s.eof_accumulate(*D)[0].cpu().numpy()
+    got = A.autotune(s, disc=[__import__("torch").tensor(v, dtype=__import__("torch").float32,
+                                                        device="cuda") for v in D], reps=1)
+    (key, rec), = got.items()
+    assert rec["eof_acc_blocks"] in A.BLOCK_CANDIDATES and s.get_tuning("eof_acc_blocks") == rec["eof_acc_blocks"]
+    assert (tmp_path / "tune.json").exists()
+    after = s.eof_accumulate(*D)[0].cpu().numpy()
+    assert block_rel_eof(after, base) <= 1e-6
+    s.set_tuning("eof_acc_blocks", 2048)
+    again = A.autotune(s, disc=[__import__("torch").tensor(v, dtype=__import__("torch").float32,
+                                                          device="cuda") for v in D], reps=1)
+    assert again[key]["eof_acc_blocks"] == rec["eof_acc_blocks"]      # from the cache
+    assert s.get_tuning("eof_acc_blocks") == rec["eof_acc_blocks"]
+    s.set_tuning("eof_acc_blocks", 2048)
