@@ -11,6 +11,7 @@
 // and every block writes float64 partials; a second kernel reduces the
 // partials over blocks in a fixed order with Kahan compensation, so results
 // are bitwise reproducible for a given n.
+#include <algorithm>
 #include <vector>
 
 #include "bfe_internal.hpp"
@@ -335,6 +336,314 @@ __global__ __launch_bounds__(256) void sl_accumulate_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Cell-ordered accumulate (particles sorted by cell, packed as float4)
+// ---------------------------------------------------------------------------
+// Each wave works on its own 64-particle sub-tiles of the block's segment:
+// lane q computes particle q's geometry, keeps its cell and weights in
+// registers (read by the other lanes with readlane, so uniform per particle)
+// and writes its per-row weights to a wave-private LDS slab.  Lanes then own
+// (row, chunk of radial orders) pairs and walk the 64 particles in order; as
+// the particles are sorted by cell, consecutive particles share the table rows
+// they need, so the sums are regrouped exactly per run of equal cells:
+//   EOF: sum_p w_j(p) sum_c c_c(p) T_c[k] = sum_c T_c[k] (sum_p w_j(p) c_c(p))
+//   SL : sum_p Y_row(p) (x1 P0 E[i] + x2 P0 E[i+1])[ln]
+//        = E[i][ln] sum_p Y_row x1 P0 + E[i+1][ln] sum_p Y_row x2 P0
+// The inner sums run in float32 over at most kFlush particles and are applied
+// to the table entries in float64.  At the end of a segment the four waves'
+// float64 sums are added in wave order.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ float lane_f(float v, int q) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q));
+}
+__device__ __forceinline__ int lane_i(int v, int q) { return __builtin_amdgcn_readlane(v, q); }
+
+constexpr int kDepChunkE = 2;   // EOF: radial orders per lane pair
+constexpr int kDepChunkS = 4;   // SL : radial orders per lane pair
+constexpr int kDepStride = 68;  // LDS row stride (floats): 16-byte rows on distinct banks
+
+// Applies the corner sums G of the current cell (node base) to the lane's
+// coefficients (float64) and clears them.
+template <int PPT>
+__device__ __forceinline__ void eof_flush(const float* __restrict__ T, int base, int K, int ny1,
+                                          int N, const int (&j_)[PPT], const int (&n0_)[PPT],
+                                          float (&G)[PPT][4], double (&acc)[PPT][kDepChunkE]) {
+#pragma unroll
+  for (int o = 0; o < PPT; ++o) {
+    if (j_[o] >= 0) {
+      const float* T00 = T + (size_t)base * K + j_[o] * N + n0_[o];
+      const float* T10 = T00 + (size_t)ny1 * K;
+#pragma unroll
+      for (int c = 0; c < kDepChunkE; ++c)
+        if (n0_[o] + c < N)
+          acc[o][c] += (double)G[o][0] * T00[c] + (double)G[o][1] * T10[c] +
+                       (double)G[o][2] * T00[K + c] + (double)G[o][3] * T10[K + c];
+    }
+    G[o][0] = G[o][1] = G[o][2] = G[o][3] = 0.f;
+  }
+}
+
+// Applies the radial sums A1, A2 of cell i to the lane's coefficients.
+template <int PPT>
+__device__ __forceinline__ void sl_flush(const float* __restrict__ E, int i, int LN, int N,
+                                         const int (&row_)[PPT], const int (&ln0_)[PPT],
+                                         const int (&n0_)[PPT], float (&A1)[PPT], float (&A2)[PPT],
+                                         double (&acc)[PPT][kDepChunkS]) {
+#pragma unroll
+  for (int o = 0; o < PPT; ++o) {
+    if (row_[o] >= 0) {
+      const float* e1 = E + (size_t)i * LN + ln0_[o];
+#pragma unroll
+      for (int c = 0; c < kDepChunkS; ++c)
+        if (n0_[o] + c < N) acc[o][c] += (double)A1[o] * e1[c] + (double)A2[o] * e1[LN + c];
+    }
+    A1[o] = A2[o] = 0.f;
+  }
+}
+
+template <int PPT>
+__global__ __launch_bounds__(256) void eof_deposit_kernel(
+    const float4* __restrict__ P, Segments S, EOFGeom g, const float* __restrict__ T, int K,
+    double* __restrict__ partials) {
+  extern __shared__ __align__(16) unsigned char dsm[];
+  const int J = 2 * g.mmax + 1;
+  const int N = g.norder;
+  const int NC = (N + kDepChunkE - 1) / kDepChunkE;
+  const int ny1 = g.numy + 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float* sW = reinterpret_cast<float*>(dsm) + (size_t)wave * J * kDepStride;  // [J][stride] per wave
+  double* sRed = reinterpret_cast<double*>(dsm);                           // [4][K] after the tiles
+  int j_[PPT], n0_[PPT];
+#pragma unroll
+  for (int o = 0; o < PPT; ++o) {
+    const int pr = lane + 64 * o;
+    j_[o] = pr / NC < J ? pr / NC : -1;
+    n0_[o] = (pr % NC) * kDepChunkE;
+  }
+  double acc[PPT][kDepChunkE];
+  for (int64_t sg = blockIdx.x; sg < S.nseg; sg += gridDim.x) {
+    int64_t s0, s1;
+    segment_range(S, sg, s0, s1);
+#pragma unroll
+    for (int o = 0; o < PPT; ++o)
+#pragma unroll
+      for (int c = 0; c < kDepChunkE; ++c) acc[o][c] = 0.0;
+    for (int64_t t0 = s0 + 64 * wave; t0 < s1; t0 += 256) {
+      const int64_t p = t0 + lane;
+      const bool live = p < s1;
+      int base = 0;
+      float c00 = 0.f, c10 = 0.f, c01 = 0.f, c11 = 0.f;
+      {
+        const float4 q4 = live ? P[p] : make_float4(1.f, 0.f, 0.f, 0.f);
+        const double x = q4.x, y = q4.y, z = q4.z, mass = live ? (double)q4.w : 0.0;
+        const double R = sqrt(x * x + y * y + 1.e-10);
+        double c1, s1v;
+        planar_angle(x, y, c1, s1v);
+        const EOFCell cell = eof_locate(R, z, g.cmap, g.ascale, g.hscale, g.xmin, g.dx, g.ymin,
+                                        g.dy, g.numx, g.numy);
+        base = cell.ix * ny1 + cell.iy;
+        c00 = (float)cell.c00; c10 = (float)cell.c10; c01 = (float)cell.c01; c11 = (float)cell.c11;
+        const double w = -kFourPi * mass;
+        double cm = 1.0, sm = 0.0, cm1 = c1, sm1 = -s1v;
+        sW[lane] = (float)w;
+        for (int m = 1; m <= g.mmax; ++m) {
+          const double cn = (m == 1) ? c1 : 2.0 * c1 * cm - cm1;
+          const double sn = (m == 1) ? s1v : 2.0 * c1 * sm - sm1;
+          cm1 = cm; sm1 = sm; cm = cn; sm = sn;
+          sW[m * kDepStride + lane] = (float)(w * cm);
+          sW[(g.mmax + m) * kDepStride + lane] = (float)(w * sm);
+        }
+      }
+      wave_sync();
+      const int64_t left = s1 - t0;
+      const int np = left < 64 ? (int)left : 64;
+      // walk the particles in order; per particle the cell and corner weights
+      // are read from its lane (uniform), the row weights 4 particles at a time
+      float G[PPT][4];
+#pragma unroll
+      for (int o = 0; o < PPT; ++o) G[o][0] = G[o][1] = G[o][2] = G[o][3] = 0.f;
+      int cur = lane_i(base, 0), cnt = 0;
+      for (int q0 = 0; q0 < np; q0 += 4) {
+        float4 wv[PPT];
+#pragma unroll
+        for (int o = 0; o < PPT; ++o)
+          wv[o] = *reinterpret_cast<const float4*>(sW + (j_[o] < 0 ? 0 : j_[o]) * kDepStride + q0);
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int q = q0 + jj;
+          if (q >= np) break;
+          const int b = lane_i(base, q);
+          if (b != cur || cnt == kFlush) {
+            eof_flush<PPT>(T, cur, K, ny1, N, j_, n0_, G, acc);
+            cur = b;
+            cnt = 0;
+          }
+          const float e00 = lane_f(c00, q), e10 = lane_f(c10, q), e01 = lane_f(c01, q),
+                      e11 = lane_f(c11, q);
+#pragma unroll
+          for (int o = 0; o < PPT; ++o) {
+            const float w = jj == 0 ? wv[o].x : jj == 1 ? wv[o].y : jj == 2 ? wv[o].z : wv[o].w;
+            G[o][0] = fmaf(w, e00, G[o][0]);
+            G[o][1] = fmaf(w, e10, G[o][1]);
+            G[o][2] = fmaf(w, e01, G[o][2]);
+            G[o][3] = fmaf(w, e11, G[o][3]);
+          }
+          ++cnt;
+        }
+      }
+      eof_flush<PPT>(T, cur, K, ny1, N, j_, n0_, G, acc);
+      wave_sync();
+    }
+    // fixed-order sum of the four waves' partial sums
+    __syncthreads();
+#pragma unroll
+    for (int o = 0; o < PPT; ++o)
+      if (j_[o] >= 0)
+#pragma unroll
+        for (int c = 0; c < kDepChunkE; ++c)
+          if (n0_[o] + c < N) sRed[(size_t)wave * K + j_[o] * N + n0_[o] + c] = acc[o][c];
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += 256)
+      partials[(size_t)sg * K + k] = ((sRed[k] + sRed[K + k]) + sRed[2 * K + k]) + sRed[3 * K + k];
+    __syncthreads();
+  }
+}
+
+template <int PPT>
+__global__ __launch_bounds__(256) void sl_deposit_kernel(
+    const float4* __restrict__ P, Segments S, SLGeom g, const float* __restrict__ E,
+    const double* __restrict__ lmc, int no_odd, double* __restrict__ partials) {
+  extern __shared__ __align__(16) unsigned char dsm[];
+  const int R = (g.lmax + 1) * (g.lmax + 1);
+  const int LN = (g.lmax + 1) * g.nmax;
+  const int K = R * g.nmax;
+  const int N = g.nmax;
+  const int NC = (N + kDepChunkS - 1) / kDepChunkS;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float* sY = reinterpret_cast<float*>(dsm) + (size_t)wave * R * kDepStride;  // [R][stride] per wave
+  double* sRed = reinterpret_cast<double*>(dsm);                            // [4][K] after the tiles
+  int row_[PPT], ln0_[PPT], n0_[PPT];
+#pragma unroll
+  for (int o = 0; o < PPT; ++o) {
+    const int pr = lane + 64 * o;
+    const int row = pr / NC;
+    row_[o] = row < R ? row : -1;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= row) ++l;
+    n0_[o] = (pr % NC) * kDepChunkS;
+    ln0_[o] = l * N + n0_[o];
+  }
+  double acc[PPT][kDepChunkS];
+  for (int64_t sg = blockIdx.x; sg < S.nseg; sg += gridDim.x) {
+    int64_t s0, s1;
+    segment_range(S, sg, s0, s1);
+#pragma unroll
+    for (int o = 0; o < PPT; ++o)
+#pragma unroll
+      for (int c = 0; c < kDepChunkS; ++c) acc[o][c] = 0.0;
+    for (int64_t t0 = s0 + 64 * wave; t0 < s1; t0 += 256) {
+      const int64_t p = t0 + lane;
+      const bool live = p < s1;
+      int ci = 0;
+      float w1 = 0.f, w2 = 0.f;
+      {
+        const float4 q4 = live ? P[p] : make_float4(1.f, 0.f, 0.f, 0.f);
+        const double x = q4.x, y = q4.y, z = q4.z, mass = live ? (double)q4.w : 0.0;
+        double r = sqrt(x * x + y * y + z * z);
+        r = (r == r) ? (r > 1.0e-10 ? r : 1.0e-10) : 1.0e-10;   // np.nanmax([r, 1e-10])
+        const double costh = z / r;
+        double c1, s1v;
+        planar_angle(x, y, c1, s1v);
+        const SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
+        const double P0 = cell.x1 * g.p0[cell.i] + cell.x2 * g.p0[cell.i + 1];
+        ci = cell.i;
+        w1 = live ? (float)(cell.x1 * P0) : 0.f;
+        w2 = live ? (float)(cell.x2 * P0) : 0.f;
+        // Y rows: P_m^m in closed form, legendre_R recurrence in l
+        // (spheresl.py:678-696), e^{i m phi} by the Chebyshev recurrence
+        const bool bad = !(costh == costh);
+        const double somx2 = sqrt((1.0 - costh) * (1.0 + costh));
+        double pw = 1.0, cmm = 1.0, smm = 0.0, cmm1 = c1, smm1 = -s1v;
+        for (int m = 0; m <= g.lmax; ++m) {
+          if (m > 0) {
+            pw *= somx2;
+            const double cn = (m == 1) ? c1 : 2.0 * c1 * cmm - cmm1;
+            const double sn = (m == 1) ? s1v : 2.0 * c1 * smm - smm1;
+            cmm1 = cmm; smm1 = smm; cmm = cn; smm = sn;
+          }
+          double plm1 = 0.0, pl = lmc[kLmcD + m] * pw;
+          for (int l = m; l <= g.lmax; ++l) {
+            const int lm = lm_index(l, m);
+            if (l > m) {
+              const double pn = costh * lmc[kLmcA + lm] * pl - lmc[kLmcB + lm] * plm1;
+              plm1 = pl;
+              pl = pn;
+            }
+            const bool skip = (no_odd && (l & 1)) || (bad && l > 0);
+            const double base = mass * lmc[kLmcY + lm] * pl;
+            if (m == 0) {
+              sY[(l * l) * kDepStride + lane] = skip ? 0.f : (float)base;
+            } else {
+              sY[(l * l + 2 * m - 1) * kDepStride + lane] = skip ? 0.f : (float)(base * cmm);
+              sY[(l * l + 2 * m) * kDepStride + lane] = skip ? 0.f : (float)(base * smm);
+            }
+          }
+        }
+      }
+      wave_sync();
+      const int64_t left = s1 - t0;
+      const int np = left < 64 ? (int)left : 64;
+      float A1[PPT], A2[PPT];
+#pragma unroll
+      for (int o = 0; o < PPT; ++o) A1[o] = A2[o] = 0.f;
+      int cur = lane_i(ci, 0), cnt = 0;
+      for (int q0 = 0; q0 < np; q0 += 4) {
+        float4 yv[PPT];
+#pragma unroll
+        for (int o = 0; o < PPT; ++o)
+          yv[o] = *reinterpret_cast<const float4*>(sY + (row_[o] < 0 ? 0 : row_[o]) * kDepStride + q0);
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+          const int q = q0 + jj;
+          if (q >= np) break;
+          const int i = lane_i(ci, q);
+          if (i != cur || cnt == kFlush) {
+            sl_flush<PPT>(E, cur, LN, N, row_, ln0_, n0_, A1, A2, acc);
+            cur = i;
+            cnt = 0;
+          }
+          const float a = lane_f(w1, q), b = lane_f(w2, q);
+#pragma unroll
+          for (int o = 0; o < PPT; ++o) {
+            const float yq = jj == 0 ? yv[o].x : jj == 1 ? yv[o].y : jj == 2 ? yv[o].z : yv[o].w;
+            A1[o] = fmaf(yq, a, A1[o]);
+            A2[o] = fmaf(yq, b, A2[o]);
+          }
+          ++cnt;
+        }
+      }
+      sl_flush<PPT>(E, cur, LN, N, row_, ln0_, n0_, A1, A2, acc);
+      wave_sync();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int o = 0; o < PPT; ++o)
+      if (row_[o] >= 0)
+#pragma unroll
+        for (int c = 0; c < kDepChunkS; ++c)
+          if (n0_[o] + c < N) sRed[(size_t)wave * K + row_[o] * N + n0_[o] + c] = acc[o][c];
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += 256)
+      partials[(size_t)sg * K + k] = ((sRed[k] + sRed[K + k]) + sRed[2 * K + k]) + sRed[3 * K + k];
+    __syncthreads();
+  }
+}
+
 // Segment plan of one accumulate launch.  Single snapshot: at most
 // `cap` segments of whole tiles, one per block.  Batch: every
 // snapshot is cut into segments of seg_tiles tiles (never spanning two
@@ -389,18 +698,25 @@ static AccPlan plan_batch(int64_t cap, int64_t seg_tiles, const int64_t* off, in
   return P;
 }
 
-// Copies the batch plan to device memory behind the partials in scratch.
-static int upload_plan(Context* c, AccPlan& P, size_t pbytes, const int64_t** rowb_dev) {
+// Copies the batch plan and the snapshot offsets to device memory behind the
+// partials in scratch (waiting first for earlier work that may still read it).
+static int upload_plan(Context* c, AccPlan& P, size_t pbytes, const int64_t* off, int nsnap,
+                       hipStream_t s, const int64_t** rowb_dev, const int64_t** off_dev) {
   const size_t segb_off = (pbytes + 255) & ~(size_t)255;
   const size_t rowb_off = segb_off + ((P.segb.size() * 8 + 255) & ~(size_t)255);
-  int rc = ensure_scratch(c, rowb_off + P.rowb.size() * 8 + 256);
+  const size_t offs_off = rowb_off + ((P.rowb.size() * 8 + 255) & ~(size_t)255);
+  int rc = ensure_scratch(c, offs_off + ((size_t)nsnap + 1) * 8 + 256);
   if (rc) return rc;
+  BFE_HIP(c, hipStreamSynchronize(s));
   int64_t* segb = (int64_t*)((char*)c->scratch + segb_off);
   int64_t* rowb = (int64_t*)((char*)c->scratch + rowb_off);
+  int64_t* offs = (int64_t*)((char*)c->scratch + offs_off);
   BFE_HIP(c, hipMemcpy(segb, P.segb.data(), P.segb.size() * 8, hipMemcpyHostToDevice));
   BFE_HIP(c, hipMemcpy(rowb, P.rowb.data(), P.rowb.size() * 8, hipMemcpyHostToDevice));
+  BFE_HIP(c, hipMemcpy(offs, off, ((size_t)nsnap + 1) * 8, hipMemcpyHostToDevice));
   P.S.segb = segb;
   *rowb_dev = rowb;
+  *off_dev = offs;
   return 0;
 }
 
@@ -422,9 +738,10 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
   const size_t sums_off = (pbytes + 255) & ~(size_t)255;
   const size_t need = sums_off + (size_t)nsnap * K * sizeof(double);
   const int64_t* rowb = nullptr;
+  const int64_t* off_dev = nullptr;
   int rc;
   if (off) {
-    rc = upload_plan(c, P, need, &rowb);   // partials | sums | segb | rowb
+    rc = upload_plan(c, P, need, off, nsnap, s, &rowb, &off_dev);   // partials | sums | plan
   } else {
     rc = ensure_scratch(c, need);
   }
@@ -433,7 +750,27 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
   double* sums = (double*)((char*)c->scratch + sums_off);
   EOFGeom g = eof_geom(c);
   const int opt = (K + 255) / 256;
-  if (n == 0) {
+  const float4* sorted = nullptr;
+  // cell-ordered form: (2 mmax + 1) x ceil(norder / 2) lane pairs, LDS for the
+  // per-wave weights and the float64 wave reduction
+  const size_t dlds = std::max((size_t)4 * (2 * c->mmax + 1) * kDepStride * sizeof(float),
+                               (size_t)4 * K * sizeof(double));
+  const int dpairs = (2 * c->mmax + 1) * ((c->norder + kDepChunkE - 1) / kDepChunkE);
+  if (n >= c->order_min && n > 0 && dpairs <= 512 && dlds <= 64 * 1024) {
+    rc = order_particles(c, 0, x, y, z, m, n, off_dev, nsnap, s, &sorted);
+    if (rc) return rc;
+  }
+  if (sorted) {
+    const int J = 2 * c->mmax + 1;
+    const int pairs = J * ((c->norder + kDepChunkE - 1) / kDepChunkE);
+    const size_t lds = std::max((size_t)4 * J * kDepStride * sizeof(float), (size_t)4 * K * sizeof(double));
+    if (pairs <= 128)
+      hipLaunchKernelGGL(eof_deposit_kernel<2>, dim3(P.grid), dim3(256), lds, s, sorted, P.S, g, c->eof_pot32, K, partials);
+    else if (pairs <= 256)
+      hipLaunchKernelGGL(eof_deposit_kernel<4>, dim3(P.grid), dim3(256), lds, s, sorted, P.S, g, c->eof_pot32, K, partials);
+    else
+      hipLaunchKernelGGL(eof_deposit_kernel<8>, dim3(P.grid), dim3(256), lds, s, sorted, P.S, g, c->eof_pot32, K, partials);
+  } else if (n == 0) {
     hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((P.S.nseg * K + 255) / 256)), dim3(256), 0, s,
                        partials, P.S.nseg * (int64_t)K);
   } else if (opt == 1) {
@@ -466,13 +803,30 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
                   : plan_single(c->sl_acc_blocks, n, kSLTile);
   const size_t pbytes = (size_t)P.S.nseg * K * sizeof(double);
   const int64_t* rowb = nullptr;
-  int rc = off ? upload_plan(c, P, pbytes, &rowb) : ensure_scratch(c, pbytes);
+  const int64_t* off_dev = nullptr;
+  int rc = off ? upload_plan(c, P, pbytes, off, nsnap, s, &rowb, &off_dev) : ensure_scratch(c, pbytes);
   if (rc) return rc;
   double* partials = (double*)c->scratch;
   SLGeom g = sl_geom(c);
   const int no_odd = (flags & BFE_NO_ODD) ? 1 : 0;
   const int opt = (K + 255) / 256;
-  if (n == 0) {
+  // cell-ordered form: (lmax+1)^2 x ceil(nmax / 4) lane pairs per wave, LDS
+  // for the per-wave Y rows and the float64 wave reduction
+  const int pairs = R * ((c->nmax + kDepChunkS - 1) / kDepChunkS);
+  const size_t dlds = std::max((size_t)4 * R * kDepStride * sizeof(float), (size_t)4 * K * sizeof(double));
+  const float4* sorted = nullptr;
+  if (n >= c->order_min && n > 0 && pairs <= 512 && dlds <= 64 * 1024) {
+    rc = order_particles(c, 1, x, y, z, m, n, off_dev, nsnap, s, &sorted);
+    if (rc) return rc;
+  }
+  if (sorted) {
+    if (pairs <= 128)
+      hipLaunchKernelGGL(sl_deposit_kernel<2>, dim3(P.grid), dim3(256), dlds, s, sorted, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
+    else if (pairs <= 256)
+      hipLaunchKernelGGL(sl_deposit_kernel<4>, dim3(P.grid), dim3(256), dlds, s, sorted, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
+    else
+      hipLaunchKernelGGL(sl_deposit_kernel<8>, dim3(P.grid), dim3(256), dlds, s, sorted, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
+  } else if (n == 0) {
     hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((P.S.nseg * K + 255) / 256)), dim3(256), 0, s,
                        partials, P.S.nseg * (int64_t)K);
   } else if (opt == 1) {

@@ -155,6 +155,13 @@ int order_points(Context* c, const FieldTables& T, const float* x, const float* 
                  const int64_t* off_dev = nullptr, int nsnap = 1,
                  const uint32_t** sorted_out = nullptr);
 uint32_t order_cell_count(const FieldTables& T);
+// Sorted copy of the particles packed as float4 (x, y, z, m) in accumulate
+// cell order: kind 0 EOF (ix, iy) cells, kind 1 SL radial cells; with off_dev
+// the key is (snapshot, cell), so snapshot s keeps its range
+// [off[s], off[s+1]).  The sort is stable, so the order is reproducible.
+int order_particles(Context* c, int kind, const float* x, const float* y, const float* z,
+                    const float* m, int64_t n, const int64_t* off_dev, int nsnap, hipStream_t s,
+                    const float4** sorted_out);
 // float64 per-point scratch in the ordering buffer (n entries), valid after
 // order_points for the same n.
 double* order_energy_buffer(Context* c, int64_t n);

@@ -32,11 +32,12 @@ def sysm():
 
 
 COUNTS = [3000, 0, 1500, 40000, 257, 9000]
+COUNTS_BIG = [50000, 30000, 0, 20000]        # >= 65536 in total: cell-ordered accumulate
 
 
-def _snapshots():
+def _snapshots(counts=COUNTS):
     parts_d, parts_h = [], []
-    for k, n in enumerate(COUNTS):
+    for k, n in enumerate(counts):
         d, h = disc_halo(max(n, 1), 100 + k)
         parts_d.append([v[:n] for v in d])
         parts_h.append([v[:n] for v in h])
@@ -44,16 +45,17 @@ def _snapshots():
     return parts_d, parts_h, f32(*cat(parts_d)), f32(*cat(parts_h))
 
 
-def test_batch_accumulate_matches_oracle_and_single(sysm):
+@pytest.mark.parametrize("counts", [COUNTS, COUNTS_BIG])
+def test_batch_accumulate_matches_oracle_and_single(sysm, counts):
     s, slo, eo = sysm
-    pd, ph, D, H = _snapshots()
+    pd, ph, D, H = _snapshots(counts)
     for tiles in (0, 1, 7):
         s.set_tuning("eof_seg_tiles", tiles)
         s.set_tuning("sl_seg_tiles", tiles)
-        c, sn = (v.cpu().numpy() for v in s.eof_accumulate_batch(*D, COUNTS))
-        h = s.sl_accumulate_batch(*H, COUNTS).cpu().numpy()
-        assert c.shape == (len(COUNTS), 7, 18) and h.shape == (len(COUNTS), 49, 20)
-        for k, n in enumerate(COUNTS):
+        c, sn = (v.cpu().numpy() for v in s.eof_accumulate_batch(*D, counts))
+        h = s.sl_accumulate_batch(*H, counts).cpu().numpy()
+        assert c.shape == (len(counts), 7, 18) and h.shape == (len(counts), 49, 20)
+        for k, n in enumerate(counts):
             if n == 0:
                 assert np.all(c[k] == 0) and np.all(sn[k] == 0) and np.all(h[k] == 0)
                 continue

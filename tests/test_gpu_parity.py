@@ -363,3 +363,31 @@ def test_leapfrog_independent_of_orbit_order(big):
     inv = np.argsort(p)
     assert np.array_equal(a, b[:, inv])
     assert np.array_equal(da, db[inv])
+
+
+def test_cell_ordered_accumulate_matches_oracle(big):
+    """From order_min (65536) particles on, accumulate sorts the particles by
+    table cell and regroups the sums per cell run; it must meet the same bound
+    against the oracle as the input-order kernels, and be reproducible."""
+    s, slo, eo = big
+    (dx, dy, dz, dm), (hx, hy, hz, hm) = disc_halo(100000, 51)
+    dx, dy, dz, dm = f32(dx, dy, dz, dm)
+    hx, hy, hz, hm = f32(hx, hy, hz, hm)
+    rc, rs = O.eof_accumulate(eo, dx, dy, dz, dm)
+    ref_h = O.sl_accumulate(slo, hx, hy, hz, hm)
+    out = {}
+    for omin in (65536, 1 << 40):           # cell-ordered, then input order
+        s.set_tuning("order_min", omin)
+        c, sn = (v.cpu().numpy() for v in s.eof_accumulate(dx, dy, dz, dm))
+        h = s.sl_accumulate(hx, hy, hz, hm).cpu().numpy()
+        assert block_rel_eof(c, rc) <= COEF_TOL, omin
+        assert block_rel_eof(sn[1:], rs[1:]) <= COEF_TOL, omin
+        assert block_rel_sl(h, ref_h, slo.lmax) <= COEF_TOL, omin
+        out[omin] = (c, sn, h)
+    s.set_tuning("order_min", 65536)
+    c2, s2 = (v.cpu().numpy() for v in s.eof_accumulate(dx, dy, dz, dm))
+    h2 = s.sl_accumulate(hx, hy, hz, hm).cpu().numpy()
+    c1, s1, h1 = out[65536]
+    assert np.array_equal(c1, c2) and np.array_equal(s1, s2) and np.array_equal(h1, h2)
+    hn = s.sl_accumulate(hx, hy, hz, hm, no_odd=True).cpu().numpy()
+    assert block_rel_sl(hn, O.sl_accumulate(slo, hx, hy, hz, hm, no_odd=True), slo.lmax) <= COEF_TOL
