@@ -59,6 +59,10 @@ def pmc_summary():
         return None
 
 
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +136,7 @@ def main():
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / a.steps
     value = ntot / ((t1 - t0) / a.steps)
+    progress(f"step: {ms_per_step:.3f} ms")
 
     # ---- per-kernel timings (HIP events on the launch stream) ---------------
     def timed(fn, reps):
@@ -163,9 +168,27 @@ def main():
         "sl_accumulate": (a.n_halo, sl_acc_flops(L, n), 16),
         "field_eval_cart": (ntot, field_ref_flops(M, N, L, n), 28),
     }
-    dom = max(units, key=lambda k: kt[k])
-    roof = roofline(dom, units[dom], kt[dom], pmc_summary())
+    # the compute kernel alone (HIP events recorded by the library on the
+    # launch stream around that kernel; cell sort and reductions excluded)
+    lib_name = {"eof_accumulate": "eof_accumulate", "sl_accumulate": "sl_accumulate",
+                "field_eval_cart": "field_eval"}
+    sysm.set_tuning("kernel_timing", 1)
+    for k in lib_name.values():
+        sysm.kernel_times(k)
+    for _ in range(reps):
+        sysm.eof_accumulate(dx, dy, dz, dm)
+        sysm.sl_accumulate(hx, hy, hz, hm)
+        sysm.field_forces(ax, ay, az)
+    ko = {}
+    for k, nm in lib_name.items():
+        tot, cnt = sysm.kernel_times(nm)
+        ko[k] = tot / max(cnt, 1)
+    sysm.set_tuning("kernel_timing", 0)
+    sysm.set_field(c, s, h)
+    dom = max(units, key=lambda k: ko[k])
+    roof = roofline(dom, units[dom], ko[dom], pmc_summary())
 
+    progress(f"kernels timed, dominant {dom}")
     # ---- orbit integration in the frozen field (plain BFE and MLP surrogate) --
     orbit = None
     surrogate = None
@@ -190,11 +213,15 @@ def main():
                  "energy_drift_median": float(np.median(dr)),
                  "energy_drift_p99": float(np.quantile(dr, 0.99)),
                  "energy_drift_max": float(np.max(dr))}
+        progress(f"orbits: {orbit['orbit_steps_per_s']:.3e} orbit-steps/s")
         if a.mlp:
             surrogate = mlp_stage(a, sysm, st0, orbit)
+            progress(f"surrogate: {surrogate['orbit_steps_per_s']:.3e} orbit-steps/s")
 
     # ---- batched snapshots ------------------------------------------------------
     batch = batch_stage(a, sysm) if a.batch else None
+    if batch:
+        progress(f"batch: {batch['particles_per_s']:.3e} particles/s")
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample --------------------
     cpu = None
@@ -210,7 +237,7 @@ def main():
         "config": {"workload": "disc EOF mmax6 norder18 256x128 + halo SL lmax6 nmax20 numr2000, "
                                "1M+1M particles, accumulate + return_forces_cart at 2M",
                    "n_disc": a.n_disc, "n_halo": a.n_halo, "parallelism": "single GPU"},
-        "kernel_seconds": kt, "roofline": roof, "tuning": tuned, "orbits": orbit,
+        "call_seconds": kt, "kernel_seconds": ko, "roofline": roof, "tuning": tuned, "orbits": orbit,
         "surrogate": surrogate, "batch": batch,
         "cpu_baseline": cpu,
         "setup_seconds": setup_s,

@@ -215,6 +215,15 @@ class BFESystem:
                        "bfe_get_tuning")
         return int(v.value)
 
+    def kernel_times(self, name):
+        """(total seconds, launches) of the timed compute kernel `name` since the
+        last call (needs set_tuning("kernel_timing", 1))."""
+        import ctypes as C
+        ms, cnt = C.c_double(), C.c_int64()
+        self.ctx.check(self.ctx.lib.bfe_kernel_times(self.ctx.h, name.encode(), C.byref(ms),
+                                                     C.byref(cnt)), "bfe_kernel_times")
+        return ms.value * 1e-3, int(cnt.value)
+
     # --------------------------------------------------------------- evaluate
     def sl_eval(self, x, y, z, coef, L1=-1000, L2=1000, no_odd=False, compat=False,
                 density=True, cos_l=False, stream=None):

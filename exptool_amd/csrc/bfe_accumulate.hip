@@ -760,6 +760,7 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
     rc = order_particles(c, 0, x, y, z, m, n, off_dev, nsnap, s, &sorted);
     if (rc) return rc;
   }
+  timer_begin(c, "eof_accumulate", s);
   if (sorted) {
     const int J = 2 * c->mmax + 1;
     const int pairs = J * ((c->norder + kDepChunkE - 1) / kDepChunkE);
@@ -780,6 +781,7 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
   } else {
     hipLaunchKernelGGL(eof_accumulate_kernel<4>, dim3(P.grid), dim3(256), 0, s, x, y, z, m, P.S, g, c->eof_pot32, K, partials);
   }
+  timer_end(c, "eof_accumulate", s);
   BFE_HIP(c, hipGetLastError());
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(K, nsnap), dim3(256), 0, s, partials,
                      (int)P.S.nseg, K, sums, rowb);
@@ -819,6 +821,7 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
     rc = order_particles(c, 1, x, y, z, m, n, off_dev, nsnap, s, &sorted);
     if (rc) return rc;
   }
+  timer_begin(c, "sl_accumulate", s);
   if (sorted) {
     if (pairs <= 128)
       hipLaunchKernelGGL(sl_deposit_kernel<2>, dim3(P.grid), dim3(256), dlds, s, sorted, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
@@ -836,6 +839,7 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
   } else {
     hipLaunchKernelGGL(sl_accumulate_kernel<4>, dim3(P.grid), dim3(256), lds, s, x, y, z, m, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
   }
+  timer_end(c, "sl_accumulate", s);
   BFE_HIP(c, hipGetLastError());
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(K, nsnap), dim3(256), 0, s, partials,
                      (int)P.S.nseg, K, coef_out, rowb);

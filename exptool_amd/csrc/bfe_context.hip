@@ -73,6 +73,34 @@ static void free_eof(Context* c) {
   c->eof_ok = false;
 }
 
+void timer_begin(Context* c, const char* name, hipStream_t s) {
+  if (!c->kernel_timing) return;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess) return;
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    return;
+  }
+  (void)hipEventRecord(a, s);
+  c->timer_events[name].push_back({a, b});
+}
+
+void timer_end(Context* c, const char* name, hipStream_t s) {
+  if (!c->kernel_timing) return;
+  auto it = c->timer_events.find(name);
+  if (it == c->timer_events.end() || it->second.empty()) return;
+  (void)hipEventRecord(it->second.back().second, s);
+}
+
+void timers_free(Context* c) {
+  for (auto& kv : c->timer_events)
+    for (auto& ab : kv.second) {
+      (void)hipEventDestroy(ab.first);
+      (void)hipEventDestroy(ab.second);
+    }
+  c->timer_events.clear();
+}
+
 }  // namespace bfe
 
 using namespace bfe;
@@ -105,6 +133,7 @@ extern "C" int bfe_destroy(bfe_ctx* h) {
   mlp_free(c);
   order_free(c);
   batch_free(c);
+  timers_free(c);
   if (c->scratch) (void)hipFree(c->scratch);
   delete h;
   return 0;
@@ -245,6 +274,8 @@ extern "C" int bfe_set_tuning(bfe_ctx* h, const char* name, int64_t value) {
   } else if (!strcmp(name, "order_min")) {
     if (value < 0) return set_error(c, BFE_EINVAL, "order_min < 0");
     c->order_min = value;
+  } else if (!strcmp(name, "kernel_timing")) {
+    c->kernel_timing = value != 0;
   } else {
     return set_error(c, BFE_EINVAL, "unknown tuning parameter '%s'", name);
   }
@@ -259,6 +290,29 @@ extern "C" int bfe_get_tuning(const bfe_ctx* h, const char* name, int64_t* value
   else if (!strcmp(name, "eof_seg_tiles")) *value = c->eof_seg_tiles;
   else if (!strcmp(name, "sl_seg_tiles")) *value = c->sl_seg_tiles;
   else if (!strcmp(name, "order_min")) *value = c->order_min;
+  else if (!strcmp(name, "kernel_timing")) *value = c->kernel_timing ? 1 : 0;
   else return BFE_EINVAL;
+  return 0;
+}
+
+// Total HIP-event time (ms) and count of the timed launches of kernel `name`
+// since the last call; waits for them to finish and forgets them.
+extern "C" int bfe_kernel_times(bfe_ctx* h, const char* name, double* total_ms, int64_t* count) {
+  if (!h || !name || !total_ms || !count) return BFE_EINVAL;
+  Context* c = &h->c;
+  *total_ms = 0.0;
+  *count = 0;
+  auto it = c->timer_events.find(name);
+  if (it == c->timer_events.end()) return 0;
+  for (auto& ab : it->second) {
+    BFE_HIP(c, hipEventSynchronize(ab.second));
+    float ms = 0.f;
+    BFE_HIP(c, hipEventElapsedTime(&ms, ab.first, ab.second));
+    *total_ms += ms;
+    *count += 1;
+    (void)hipEventDestroy(ab.first);
+    (void)hipEventDestroy(ab.second);
+  }
+  it->second.clear();
   return 0;
 }

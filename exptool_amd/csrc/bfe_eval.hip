@@ -243,18 +243,28 @@ __global__ __launch_bounds__(256) void eof_eval_kernel(
   }
 }
 
-// perm (may be NULL): cell order of the points; block b then takes logical
-// block xcd_block(nb) so that each XCD evaluates a contiguous range of cells.
+// pts (may be NULL): the points in cell order as float4 (x, y, z, input index
+// bits); block b then takes logical block xcd_block(nb) so that each XCD
+// evaluates a contiguous range of cells.  Without pts, point j of px/py/pz.
 __global__ __launch_bounds__(256) void field_eval_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-    int64_t n, const int* __restrict__ perm, FieldTables T, float* fx, float* fy, float* fz,
+    int64_t n, const float4* __restrict__ pts, FieldTables T, float* fx, float* fy, float* fz,
     float* pot, float* comp) {
   const int64_t nb = (n + 255) / 256;
-  const int64_t j = (perm ? xcd_block(nb) : (int64_t)blockIdx.x) * 256 + threadIdx.x;
+  const int64_t j = (pts ? xcd_block(nb) : (int64_t)blockIdx.x) * 256 + threadIdx.x;
   if (j >= n) return;
-  const int64_t i = perm ? (int64_t)perm[j] : j;
+  float qx, qy, qz;
+  int64_t i;
+  if (pts) {
+    const float4 q = pts[j];
+    qx = q.x; qy = q.y; qz = q.z;
+    i = (int64_t)__float_as_int(q.w);
+  } else {
+    qx = px[j]; qy = py[j]; qz = pz[j];
+    i = j;
+  }
   double cv[8];
-  field_eval_point(T, px[i], py[i], pz[i], cv);
+  field_eval_point(T, qx, qy, qz, cv);
   if (fx) fx[i] = (float)(cv[0] + cv[1]);
   if (fy) fy[i] = (float)(cv[2] + cv[3]);
   if (fz) fz[i] = (float)(cv[4] + cv[5]);
@@ -269,20 +279,20 @@ __global__ __launch_bounds__(256) void field_eval_kernel(
 // (snapshot, cell) order belongs to snapshot sorted[j] / ncell and uses that
 // snapshot's folded tables.
 __global__ __launch_bounds__(256) void field_eval_batch_kernel(
-    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
-    int64_t n, const int* __restrict__ perm, const uint32_t* __restrict__ sorted, uint32_t ncell,
+    int64_t n, const float4* __restrict__ pts, const uint32_t* __restrict__ sorted, uint32_t ncell,
     FieldTables T, const float* __restrict__ slB, int64_t sl_stride,
     const float* __restrict__ eofB, int64_t eof_stride, float* fx, float* fy, float* fz,
     float* pot) {
   const int64_t nb = (n + 255) / 256;
   const int64_t j = xcd_block(nb) * 256 + threadIdx.x;
   if (j >= n) return;
-  const int64_t i = perm[j];
+  const float4 q = pts[j];
+  const int64_t i = (int64_t)__float_as_int(q.w);
   const size_t snap = sorted[j] / ncell;
   T.slF = slB + snap * sl_stride;
   T.eofF = eofB + snap * eof_stride;
   double cv[8];
-  field_eval_point(T, px[i], py[i], pz[i], cv);
+  field_eval_point(T, q.x, q.y, q.z, cv);
   if (fx) fx[i] = (float)(cv[0] + cv[1]);
   if (fy) fy[i] = (float)(cv[2] + cv[3]);
   if (fz) fz[i] = (float)(cv[4] + cv[5]);
@@ -430,13 +440,16 @@ extern "C" int bfe_field_eval_cart(bfe_ctx* h, const float* x, const float* y, c
   BFE_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
   const int* perm = nullptr;
+  const float4* pts = nullptr;
   if (n >= c->order_min) {
-    rc = order_points(c, T, x, y, z, n, s, &perm);
+    rc = order_points(c, T, x, y, z, n, s, &perm, nullptr, 1, nullptr, &pts);
     if (rc) return rc;
   }
-  const unsigned grid = perm ? xcd_grid(nblocks256(n)) : nblocks256(n);
-  hipLaunchKernelGGL(field_eval_kernel, dim3(grid), dim3(256), 0, s, x, y, z, n, perm, T, fx, fy,
+  const unsigned grid = pts ? xcd_grid(nblocks256(n)) : nblocks256(n);
+  timer_begin(c, "field_eval", s);
+  hipLaunchKernelGGL(field_eval_kernel, dim3(grid), dim3(256), 0, s, x, y, z, n, pts, T, fx, fy,
                      fz, pot, comp);
+  timer_end(c, "field_eval", s);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }
@@ -516,11 +529,12 @@ extern "C" int bfe_field_eval_batch(bfe_ctx* h, const float* x, const float* y, 
   T.srot = sin(rotpos);
   const int* perm = nullptr;
   const uint32_t* sorted = nullptr;
-  int rc = order_points(c, T, x, y, z, n, s, &perm, c->batch_off, nsnap, &sorted);
+  const float4* pts = nullptr;
+  int rc = order_points(c, T, x, y, z, n, s, &perm, c->batch_off, nsnap, &sorted, &pts);
   if (rc) return rc;
-  hipLaunchKernelGGL(field_eval_batch_kernel, dim3(xcd_grid(nblocks256(n))), dim3(256), 0, s, x, y,
-                     z, n, perm, sorted, order_cell_count(T), T, c->batch_sl, (int64_t)slf,
-                     c->batch_eof, (int64_t)eoff, fx, fy, fz, pot);
+  hipLaunchKernelGGL(field_eval_batch_kernel, dim3(xcd_grid(nblocks256(n))), dim3(256), 0, s, n,
+                     pts, sorted, order_cell_count(T), T, c->batch_sl, (int64_t)slf, c->batch_eof,
+                     (int64_t)eoff, fx, fy, fz, pot);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }

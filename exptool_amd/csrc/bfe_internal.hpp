@@ -6,7 +6,9 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "bfe_math.hpp"
@@ -106,6 +108,10 @@ struct Context {
   int64_t sl_seg_tiles = 0;
   int64_t order_min = 65536;       // evaluate in cell order from this many points
 
+  // ---- optional HIP-event timing of the main compute kernels ---------------
+  bool kernel_timing = false;
+  std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> timer_events;
+
   // ---- spatial ordering of evaluation points (bfe_order.hip) -------------
   void* order = nullptr;       // keys[n] | sorted keys[n] | perm[n] | per-point f64[n] | sort temp
   size_t order_bytes = 0;
@@ -149,11 +155,12 @@ struct FieldTables;
 // returned pointer stays valid until the next call on this context.
 // With off_dev (device int64[nsnap + 1], concatenated snapshots) the key is
 // (snapshot, cell), and sorted_out (may be NULL) receives the sorted keys:
-// snapshot = key / order_cell_count(T).
+// snapshot = key / order_cell_count(T).  points_out (may be NULL) receives
+// the points in key order as float4 (x, y, z, input index as int bits).
 int order_points(Context* c, const FieldTables& T, const float* x, const float* y,
                  const float* z, int64_t n, hipStream_t s, const int** perm_out,
                  const int64_t* off_dev = nullptr, int nsnap = 1,
-                 const uint32_t** sorted_out = nullptr);
+                 const uint32_t** sorted_out = nullptr, const float4** points_out = nullptr);
 uint32_t order_cell_count(const FieldTables& T);
 // Sorted copy of the particles packed as float4 (x, y, z, m) in accumulate
 // cell order: kind 0 EOF (ix, iy) cells, kind 1 SL radial cells; with off_dev
@@ -179,6 +186,12 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nb) {
   const int64_t per = (nb + 7) / 8;
   return (int64_t)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
+
+// With kernel_timing on, record HIP events around a compute kernel launch on
+// stream s (read back by bfe_kernel_times).
+void timer_begin(Context* c, const char* name, hipStream_t s);
+void timer_end(Context* c, const char* name, hipStream_t s);
+void timers_free(Context* c);
 
 int set_error(Context* c, int code, const char* fmt, ...);
 int check_hip(Context* c, hipError_t e, const char* what);

@@ -52,21 +52,31 @@ __device__ __forceinline__ int snapshot_of(const int64_t* __restrict__ off, int 
   return lo;
 }
 
-// key = snapshot * nkeys + cell (snapshot 0 without offsets)
+// key = snapshot * nkeys + cell (snapshot 0 without offsets); with packed,
+// also (x, y, z, index bits) for a gather in key order
 __global__ __launch_bounds__(256) void order_key_kernel(
     const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
     int64_t n, FieldTables T, uint32_t neof, uint32_t nkeys, const int64_t* __restrict__ off,
-    int nsnap, uint32_t* __restrict__ keys) {
+    int nsnap, uint32_t* __restrict__ keys, float4* __restrict__ packed) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const float xf = x[i], yf = y[i], zf = z[i];
   const uint32_t snap = off ? (uint32_t)snapshot_of(off, nsnap, i) : 0u;
-  keys[i] = snap * nkeys + cell_key(T, x[i], y[i], z[i], neof);
+  keys[i] = snap * nkeys + cell_key(T, xf, yf, zf, neof);
+  if (packed) packed[i] = make_float4(xf, yf, zf, __int_as_float((int)i));
+}
+
+__global__ __launch_bounds__(256) void gather_points_kernel(const float4* __restrict__ in,
+                                                            const int* __restrict__ perm, int64_t n,
+                                                            float4* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = in[perm[j]];
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 struct OrderLayout {
-  size_t keys, sorted, perm, energy, temp, total;
+  size_t keys, sorted, perm, energy, pin, pout, temp, total;
 };
 
 static OrderLayout order_layout(int64_t n, size_t temp_bytes) {
@@ -75,7 +85,9 @@ static OrderLayout order_layout(int64_t n, size_t temp_bytes) {
   L.sorted = L.keys + align256((size_t)n * 4);
   L.perm = L.sorted + align256((size_t)n * 4);
   L.energy = L.perm + align256((size_t)n * 4);
-  L.temp = L.energy + align256((size_t)n * 8);
+  L.pin = L.energy + align256((size_t)n * 8);
+  L.pout = L.pin + align256((size_t)n * 16);
+  L.temp = L.pout + align256((size_t)n * 16);
   L.total = L.temp + align256(temp_bytes);
   return L;
 }
@@ -102,9 +114,10 @@ uint32_t order_cell_count(const FieldTables& T) {
 
 int order_points(Context* c, const FieldTables& T, const float* x, const float* y, const float* z,
                  int64_t n, hipStream_t s, const int** perm_out, const int64_t* off_dev,
-                 int nsnap, const uint32_t** sorted_out) {
+                 int nsnap, const uint32_t** sorted_out, const float4** points_out) {
   *perm_out = nullptr;
   if (sorted_out) *sorted_out = nullptr;
+  if (points_out) *points_out = nullptr;
   if (n > (int64_t)0x7fffffff) return set_error(c, BFE_ELIMIT, "order: n = %lld > 2^31 - 1", (long long)n);
   const uint64_t ncell = (uint64_t)T.eg.numx * T.eg.numy + (uint64_t)(T.slg.numr - 1);
   const uint64_t nkeys = ncell * (uint64_t)(off_dev ? nsnap : 1);
@@ -124,13 +137,21 @@ int order_points(Context* c, const FieldTables& T, const float* x, const float* 
   uint32_t* sorted = (uint32_t*)(base + L.sorted);
   int* perm = (int*)(base + L.perm);
   const uint32_t neof = (uint32_t)T.eg.numx * (uint32_t)T.eg.numy;
+  float4* pin = points_out ? (float4*)(base + L.pin) : nullptr;
   hipLaunchKernelGGL(order_key_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z,
-                     n, T, neof, (uint32_t)ncell, off_dev, nsnap, keys);
+                     n, T, neof, (uint32_t)ncell, off_dev, nsnap, keys, pin);
   BFE_HIP(c, hipGetLastError());
   BFE_HIP(c, rocprim::radix_sort_pairs<SortConfig>((void*)(base + L.temp), temp_bytes, keys, sorted, iota, perm,
                                        ni, 0u, bits, s));
   *perm_out = perm;
   if (sorted_out) *sorted_out = sorted;
+  if (points_out) {
+    float4* pout = (float4*)(base + L.pout);
+    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pin,
+                       perm, n, pout);
+    BFE_HIP(c, hipGetLastError());
+    *points_out = pout;
+  }
   return 0;
 }
 

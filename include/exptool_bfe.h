@@ -175,6 +175,13 @@ int bfe_field_eval_batch(bfe_ctx* ctx, const float* x, const float* y, const flo
  * plan only through the float32 tile sums. */
 int bfe_set_tuning(bfe_ctx* ctx, const char* name, int64_t value);
 int bfe_get_tuning(const bfe_ctx* ctx, const char* name, int64_t* value);
+/* With tuning knob "kernel_timing" = 1, the main compute kernel of
+ * bfe_field_eval_cart ("field_eval"), bfe_eof_accumulate ("eof_accumulate")
+ * and bfe_sl_accumulate ("sl_accumulate") is bracketed by HIP events on the
+ * call's stream (cell sort and reductions excluded).  Returns the total time
+ * (ms) and number of such launches since the last call for `name`; waits for
+ * them. */
+int bfe_kernel_times(bfe_ctx* ctx, const char* name, double* total_ms, int64_t* count);
 
 /* ---- MLP neural field (no counterpart in the reference) ----------------- */
 /* A 3 -> 128 -> 128 -> 128 -> 128 -> 4 SiLU network mapping (x, y, z) to
