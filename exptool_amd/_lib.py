@@ -23,6 +23,7 @@ COMPAT_SL_EDGE = 0x2
 COMPAT_SL_DENS = 0x4
 COMPAT_EOF_MASK = 0x8
 NO_ODD = 0x10
+ACC_BF16 = 0x20
 COMPAT_ALL = 0xF
 
 EXPORTS = (

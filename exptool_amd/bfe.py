@@ -97,8 +97,9 @@ class BFESystem:
             raise BFEError(f"{what} tables are not set on this BFESystem")
 
     # ------------------------------------------------------------- accumulate
-    def sl_accumulate(self, x, y, z, m, no_odd=False, stream=None):
-        """Spherical coefficients ((lmax+1)^2, nmax) float64 (device)."""
+    def sl_accumulate(self, x, y, z, m, no_odd=False, bf16=False, stream=None):
+        """Spherical coefficients ((lmax+1)^2, nmax) float64 (device).  bf16:
+        the two-piece bf16 MFMA mode (about 1e-5 relative)."""
         self._need("sl")
         x, y, z, n = self._xyz(x, y, z)
         m = f32(m, self.device)
@@ -107,7 +108,8 @@ class BFESystem:
         out = torch.empty(((self.sl["lmax"] + 1) ** 2, self.sl["nmax"]), dtype=torch.float64,
                           device=self.device)
         rc = self.ctx.lib.bfe_sl_accumulate(self.ctx.h, ptr(x), ptr(y), ptr(z), ptr(m), n,
-                                            _lib.NO_ODD if no_odd else 0, ptr(out),
+                                            (_lib.NO_ODD if no_odd else 0) |
+                                            (_lib.ACC_BF16 if bf16 else 0), ptr(out),
                                             stream_ptr(stream))
         self.ctx.check(rc, "bfe_sl_accumulate")
         return out

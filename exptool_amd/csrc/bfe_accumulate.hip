@@ -644,6 +644,199 @@ __global__ __launch_bounds__(256) void sl_deposit_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// SL accumulate as a matrix product on the bf16 MFMA units
+// ---------------------------------------------------------------------------
+// For each l the tile contribution is C_l[row][n] += sum_p Y_row(p) Phi_ln(p),
+// a (2l+1) x 64 by 64 x nmax product.  Operands are split into NP bf16 pieces
+// x = x_0 + x_1 (+ x_2) (each the bf16 rounding of the remainder) and the
+// products x_a y_b with a + b < NP are accumulated on
+// v_mfma_f32_16x16x32_bf16 (smallest first): NP = 3 keeps about 24 bits of
+// each product (float32-class), NP = 2 about 16 bits (the bf16 mode).  Sums
+// are float32 inside the MFMA over 32 particles, then float64.
+//   A (16 x 32): A[c][8g + j] = Y_{l^2 + c}(particle 32 s + 8 g + j), c < 2l+1
+//   B (32 x 16): B[8g + j][c] = Phi_{l, 16 nt + c}(particle 32 s + 8 g + j)
+//   D (16 x 16): D[4g + i][c]
+// Y rows are built per particle (four threads by m column) and stored as
+// bf16 pieces in LDS; each lane computes its own B entries from the radial
+// table (every Phi value is used by exactly one output tile).
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kMT = 64;        // particles per tile (two MFMA K-steps)
+constexpr int kMS = 72;        // LDS row stride (bf16): 144 B rows spread over banks
+constexpr int kMSlots = 4;     // output tiles per wave
+
+__device__ __forceinline__ uint16_t bf16_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_float(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+template <int NP>
+__device__ __forceinline__ void bf16_split(float x, uint16_t (&p)[NP]) {
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    p[k] = bf16_bits(x);
+    x -= bf16_float(p[k]);
+  }
+}
+
+template <int NP>
+__global__ __launch_bounds__(256) void sl_mfma_kernel(
+    const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
+    const float* __restrict__ pm, Segments S, SLGeom g, const float* __restrict__ E,
+    const double* __restrict__ lmc, int no_odd, double* __restrict__ partials) {
+  extern __shared__ __align__(16) unsigned char msm[];
+  const int L = g.lmax, R = (L + 1) * (L + 1), N = g.nmax, LN = (L + 1) * N;
+  const int K = R * N;
+  const int NT = (N + 15) / 16;
+  const int ntiles = (L + 1) * NT;
+  uint16_t* sY = reinterpret_cast<uint16_t*>(msm);                  // [NP][R][kMS]
+  int* s_i = reinterpret_cast<int*>(msm + (size_t)NP * R * kMS * 2);   // [kMT]
+  float* s_w1 = reinterpret_cast<float*>(s_i + kMT);
+  float* s_w2 = s_w1 + kMT;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6, gq = lane >> 4, cc = lane & 15;
+  const int qp = tid & (kMT - 1), h = tid >> 6;
+  double accd[kMSlots][4];
+  for (int64_t sg = blockIdx.x; sg < S.nseg; sg += gridDim.x) {
+    int64_t s0, s1;
+    segment_range(S, sg, s0, s1);
+#pragma unroll
+    for (int t = 0; t < kMSlots; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) accd[t][i] = 0.0;
+    for (int64_t t0 = s0; t0 < s1; t0 += kMT) {
+      // ---- geometry: Y rows (bf16 pieces), radial cell and weights ----------
+      {
+        const int64_t p = t0 + qp;
+        const bool live = p < s1;
+        const double x = live ? (double)px[p] : 1.0, y = live ? (double)py[p] : 0.0;
+        const double z = live ? (double)pz[p] : 0.0, mass = live ? (double)pm[p] : 0.0;
+        double r = sqrt(x * x + y * y + z * z);
+        r = (r == r) ? (r > 1.0e-10 ? r : 1.0e-10) : 1.0e-10;   // np.nanmax([r, 1e-10])
+        const double costh = z / r;
+        double c1, s1v;
+        planar_angle(x, y, c1, s1v);
+        if (h == 0) {
+          SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
+          const double P0 = cell.x1 * g.p0[cell.i] + cell.x2 * g.p0[cell.i + 1];
+          s_i[qp] = cell.i;
+          s_w1[qp] = live ? (float)(cell.x1 * P0) : 0.f;
+          s_w2[qp] = live ? (float)(cell.x2 * P0) : 0.f;
+        }
+        const bool bad = !(costh == costh);
+        const double somx2 = sqrt((1.0 - costh) * (1.0 + costh));
+        double ch = 1.0, sh = 0.0, pw = 1.0;
+        for (int j = 0; j < h; ++j) {
+          const double cn = ch * c1 - sh * s1v;
+          sh = sh * c1 + ch * s1v;
+          ch = cn;
+          pw *= somx2;
+        }
+        const double c2 = c1 * c1 - s1v * s1v, s2 = 2.0 * c1 * s1v;
+        const double c4 = c2 * c2 - s2 * s2, s4 = 2.0 * c2 * s2;
+        const double sq2 = somx2 * somx2, p4 = sq2 * sq2;
+        for (int m = h; m <= L; m += 4) {
+          double plm1 = 0.0, pl = lmc[kLmcD + m] * pw;
+          for (int l = m; l <= L; ++l) {
+            const int lm = lm_index(l, m);
+            if (l > m) {
+              const double pn = costh * lmc[kLmcA + lm] * pl - lmc[kLmcB + lm] * plm1;
+              plm1 = pl;
+              pl = pn;
+            }
+            const bool skip = (no_odd && (l & 1)) || (bad && l > 0);
+            const double base = mass * lmc[kLmcY + lm] * pl;
+            const int nrow = m == 0 ? 1 : 2;
+            for (int k = 0; k < nrow; ++k) {
+              const int row = m == 0 ? l * l : l * l + 2 * m - 1 + k;
+              const float v = skip ? 0.f : (float)(m == 0 ? base : base * (k ? sh : ch));
+              uint16_t pc[NP];
+              bf16_split<NP>(v, pc);
+#pragma unroll
+              for (int q = 0; q < NP; ++q) sY[((size_t)q * R + row) * kMS + qp] = pc[q];
+            }
+          }
+          const double cn = ch * c4 - sh * s4;
+          sh = sh * c4 + ch * s4;
+          ch = cn;
+          pw *= p4;
+        }
+      }
+      __syncthreads();
+      // ---- products: wave w takes output tiles w, w + 4, ... ------------------
+#pragma unroll
+      for (int slot = 0; slot < kMSlots; ++slot) {
+        const int t = wave + 4 * slot;
+        if (t >= ntiles) break;
+        const int l = t / NT, nt = t - l * NT;
+        const bool arow = cc < 2 * l + 1;
+        const int n = 16 * nt + cc;
+        const bool bcol = n < N;
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+          const int q0 = 32 * ks + 8 * gq;
+          bf16x8_t a[NP], b[NP];
+#pragma unroll
+          for (int q = 0; q < NP; ++q) {
+            if (arow) {
+              a[q] = *reinterpret_cast<const bf16x8_t*>(sY + ((size_t)q * R + l * l + cc) * kMS + q0);
+            } else {
+#pragma unroll
+              for (int j = 0; j < 8; ++j) a[q][j] = (__bf16)0.f;
+            }
+          }
+          uint16_t bp[NP][8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            float phi = 0.f;
+            if (bcol) {
+              const int qq = q0 + j;
+              const float* e = E + (size_t)s_i[qq] * LN + l * N + n;
+              phi = s_w1[qq] * e[0] + s_w2[qq] * e[LN];
+            }
+            uint16_t pc[NP];
+            bf16_split<NP>(phi, pc);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) bp[q][j] = pc[q];
+          }
+#pragma unroll
+          for (int q = 0; q < NP; ++q)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) b[q][j] = __builtin_bit_cast(__bf16, bp[q][j]);
+          floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
+          if (NP == 3) {
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
+          }
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) accd[slot][i] += (double)acc[i];
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int slot = 0; slot < kMSlots; ++slot) {
+      const int t = wave + 4 * slot;
+      if (t >= ntiles) break;
+      const int l = t / NT, nt = t - l * NT;
+      const int n = 16 * nt + cc;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int rl = 4 * gq + i;
+        if (rl < 2 * l + 1 && n < N) partials[(size_t)sg * K + (size_t)(l * l + rl) * N + n] = accd[slot][i];
+      }
+    }
+  }
+}
+
 // Segment plan of one accumulate launch.  Single snapshot: at most
 // `cap` segments of whole tiles, one per block.  Batch: every
 // snapshot is cut into segments of seg_tiles tiles (never spanning two
@@ -812,6 +1005,31 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
   SLGeom g = sl_geom(c);
   const int no_odd = (flags & BFE_NO_ODD) ? 1 : 0;
   const int opt = (K + 255) / 256;
+  // matrix-product form on the MFMA units (lmax <= 7, at most 16 output tiles)
+  const int mtiles = (c->lmax + 1) * ((c->nmax + 15) / 16);
+  const bool bf16_mode = (flags & BFE_ACC_BF16) != 0;
+  const bool mfma_ok = c->lmax <= 7 && mtiles <= 4 * kMSlots;
+  if (bf16_mode && !mfma_ok)
+    return set_error(c, BFE_ELIMIT, "sl_accumulate: BFE_ACC_BF16 needs lmax <= 7 and (lmax+1) ceil(nmax/16) <= 16");
+  // measured at lmax 6, nmax 20, 1M particles on MI355X: MFMA 0.59 ms, the
+  // cell-ordered vector-ALU path 0.19 ms + 0.09 ms sort, so "auto" takes the
+  // vector-ALU paths (building the bf16 operand pieces costs more than the
+  // products save at these sizes)
+  if (n > 0 && mfma_ok && (bf16_mode || c->sl_acc_path == 1)) {
+    const int np = bf16_mode ? 2 : 3;
+    const size_t mlds = (size_t)np * R * kMS * 2 + 3 * kMT * 4;
+    timer_begin(c, "sl_accumulate", s);
+    if (np == 3)
+      hipLaunchKernelGGL(sl_mfma_kernel<3>, dim3(P.grid), dim3(256), mlds, s, x, y, z, m, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
+    else
+      hipLaunchKernelGGL(sl_mfma_kernel<2>, dim3(P.grid), dim3(256), mlds, s, x, y, z, m, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
+    timer_end(c, "sl_accumulate", s);
+    BFE_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(K, nsnap), dim3(256), 0, s, partials,
+                       (int)P.S.nseg, K, coef_out, rowb);
+    BFE_HIP(c, hipGetLastError());
+    return 0;
+  }
   // cell-ordered form: (lmax+1)^2 x ceil(nmax / 4) lane pairs per wave, LDS
   // for the per-wave Y rows and the float64 wave reduction
   const int pairs = R * ((c->nmax + kDepChunkS - 1) / kDepChunkS);

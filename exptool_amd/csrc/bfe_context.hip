@@ -276,6 +276,9 @@ extern "C" int bfe_set_tuning(bfe_ctx* h, const char* name, int64_t value) {
     c->order_min = value;
   } else if (!strcmp(name, "kernel_timing")) {
     c->kernel_timing = value != 0;
+  } else if (!strcmp(name, "sl_acc_path")) {
+    if (value < 0 || value > 2) return set_error(c, BFE_EINVAL, "sl_acc_path outside [0, 2]");
+    c->sl_acc_path = value;
   } else {
     return set_error(c, BFE_EINVAL, "unknown tuning parameter '%s'", name);
   }
@@ -291,6 +294,7 @@ extern "C" int bfe_get_tuning(const bfe_ctx* h, const char* name, int64_t* value
   else if (!strcmp(name, "sl_seg_tiles")) *value = c->sl_seg_tiles;
   else if (!strcmp(name, "order_min")) *value = c->order_min;
   else if (!strcmp(name, "kernel_timing")) *value = c->kernel_timing ? 1 : 0;
+  else if (!strcmp(name, "sl_acc_path")) *value = c->sl_acc_path;
   else return BFE_EINVAL;
   return 0;
 }

@@ -107,6 +107,7 @@ struct Context {
   int64_t eof_seg_tiles = 0;       // batch accumulate: tiles per segment, 0 = from the budget
   int64_t sl_seg_tiles = 0;
   int64_t order_min = 65536;       // evaluate in cell order from this many points
+  int64_t sl_acc_path = 0;         // SL accumulate: 0 auto (vector ALU), 1 MFMA, 2 vector ALU
 
   // ---- optional HIP-event timing of the main compute kernels ---------------
   bool kernel_timing = false;

@@ -42,6 +42,8 @@ typedef struct bfe_ctx bfe_ctx;
 #define BFE_COMPAT_SL_DENS 0x4u    /* all_eval_particles den1 P_1^m + 0.25*pi, spheresl.py:1323,1351 */
 #define BFE_COMPAT_EOF_MASK 0x8u   /* eof.force_eval odd-m mask |cos(pi m/2)|, eof.py:833 */
 #define BFE_NO_ODD 0x10u           /* skip odd l (SL) / odd m (EOF eval) */
+#define BFE_ACC_BF16 0x20u         /* SL accumulate in the bf16 mode: two-piece bf16 MFMA
+                                      products (about 1e-5 relative), see bfe_sl_accumulate */
 #define BFE_COMPAT_ALL 0xFu
 
 /* ---- context ------------------------------------------------------------ */
@@ -73,7 +75,12 @@ int bfe_eof_tables_set(bfe_ctx* ctx, int mmax, int norder, int numx, int numy, i
 /* ---- accumulate --------------------------------------------------------- */
 /* replaces: spheresl.compute_coefficients_solitary (spheresl.py:567-656) and
  * its Pool driver multi_compute_coefficients + np.sum (spheresl.py:411-471).
- * coef_out: device ((lmax+1)^2, nmax) float64.  flags: BFE_NO_ODD. */
+ * coef_out: device ((lmax+1)^2, nmax) float64.  flags: BFE_NO_ODD, BFE_ACC_BF16.
+ * The sums over particles run on the vector ALUs in float32 with float64
+ * partial sums (cell-ordered from "order_min" particles on).  For lmax <= 7
+ * they can run as bf16 MFMA products with float32 accumulation instead:
+ * three-piece split operands with tuning knob sl_acc_path = 1 (float32-class,
+ * <= 1e-6 relative), two pieces with BFE_ACC_BF16 (the bf16 mode, <= 1e-4). */
 int bfe_sl_accumulate(bfe_ctx* ctx, const float* x, const float* y, const float* z,
                       const float* m, int64_t n, uint32_t flags, double* coef_out,
                       void* stream);
@@ -167,7 +174,7 @@ int bfe_field_eval_batch(bfe_ctx* ctx, const float* x, const float* y, const flo
                          float* fz, float* pot, void* stream);
 
 /* ---- launch-shape tuning (no counterpart in the reference) ---------------- */
-/* Knobs: "eof_acc_blocks", "sl_acc_blocks" (accumulate blocks / batch
+/* Knobs: "sl_acc_path" (0 auto, 1 MFMA, 2 vector ALU), "eof_acc_blocks", "sl_acc_blocks" (accumulate blocks / batch
  * segment budget per kernel, default 2048), "eof_seg_tiles", "sl_seg_tiles"
  * (tiles per batch segment, 0 = derived from the budget),
  * "order_min" (points from which evaluation runs in table-cell order, 65536).

@@ -391,3 +391,26 @@ def test_cell_ordered_accumulate_matches_oracle(big):
     assert np.array_equal(c1, c2) and np.array_equal(s1, s2) and np.array_equal(h1, h2)
     hn = s.sl_accumulate(hx, hy, hz, hm, no_odd=True).cpu().numpy()
     assert block_rel_sl(hn, O.sl_accumulate(slo, hx, hy, hz, hm, no_odd=True), slo.lmax) <= COEF_TOL
+
+
+@pytest.mark.parametrize("n", [3000, 100000])
+def test_sl_accumulate_mfma_valu_and_bf16_modes(big, n):
+    """SL accumulate on the MFMA units (three-piece bf16 split, default for
+    lmax <= 7), on the vector ALUs (sl_acc_path = 2, cell-ordered from 65536
+    particles) and in the two-piece bf16 mode (BFE_ACC_BF16, <= 1e-4)."""
+    s, slo, _ = big
+    _, (hx, hy, hz, hm) = disc_halo(n, 61)
+    hx, hy, hz, hm = f32(hx, hy, hz, hm)
+    for no_odd in (False, True):
+        ref = O.sl_accumulate(slo, hx, hy, hz, hm, no_odd=no_odd)
+        for path in (1, 2):
+            s.set_tuning("sl_acc_path", path)
+            got = s.sl_accumulate(hx, hy, hz, hm, no_odd=no_odd).cpu().numpy()
+            assert block_rel_sl(got, ref, slo.lmax) <= COEF_TOL, (path, no_odd)
+        s.set_tuning("sl_acc_path", 0)
+        b = s.sl_accumulate(hx, hy, hz, hm, no_odd=no_odd, bf16=True).cpu().numpy()
+        e = block_rel_sl(b, ref, slo.lmax)
+        assert e <= 1e-4, e
+    a1 = s.sl_accumulate(hx, hy, hz, hm).cpu().numpy()
+    a2 = s.sl_accumulate(hx, hy, hz, hm).cpu().numpy()
+    assert np.array_equal(a1, a2)
