@@ -146,19 +146,22 @@ __global__ __launch_bounds__(256) void eof_accumulate_kernel(
   }
 }
 
-// Fixed-order reduction of partials[nb][K] -> out[K]: one block per k; lane t
-// sums b = t, t + 256, ... with Kahan compensation, then a fixed pairwise tree
-// in LDS combines the 256 lane sums.  The order depends only on nb, so results
-// are bitwise reproducible.
-// With rowb (nsnap + 1 row boundaries), block (k, snapshot) sums the rows of
-// its snapshot into out[snapshot * K + k].
+// Column sums of the segment partials [rows][K] in a fixed order.  A block
+// covers 8 adjacent columns with 32 row lanes (each row read is 64
+// contiguous bytes): row lane j Kahan-sums rows j, j + 32, ... of its
+// column, then a fixed tree adds the 32 lanes.  The order depends only on
+// the row count, so results are bitwise reproducible.  With rowb, grid.y runs
+// over snapshots: snapshot q sums rows [rowb[q], rowb[q+1]) into out + q * K.
+constexpr int kRedCols = 8;
+constexpr int kRedLanes = 32;
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __restrict__ partials,
                                                               int nb, int K,
                                                               double* __restrict__ out,
                                                               const int64_t* __restrict__ rowb) {
-  __shared__ double sh[256];
-  const int k = blockIdx.x;
-  const int t = threadIdx.x;
+  __shared__ double sh[kRedLanes][kRedCols];
+  const int c = threadIdx.x % kRedCols;
+  const int j = threadIdx.x / kRedCols;
+  const int k = blockIdx.x * kRedCols + c;
   int64_t r0 = 0, r1 = nb;
   if (rowb) {
     r0 = rowb[blockIdx.y];
@@ -166,19 +169,22 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __re
     out += (size_t)blockIdx.y * K;
   }
   double s = 0.0, comp = 0.0;
-  for (int64_t b = r0 + t; b < r1; b += 256) {
-    const double yv = partials[(size_t)b * K + k] - comp;
-    const double tv = s + yv;
-    comp = (tv - s) - yv;
-    s = tv;
+  if (k < K) {
+#pragma unroll 4
+    for (int64_t b = r0 + j; b < r1; b += kRedLanes) {
+      const double yv = partials[(size_t)b * K + k] - comp;
+      const double tv = s + yv;
+      comp = (tv - s) - yv;
+      s = tv;
+    }
   }
-  sh[t] = s;
+  sh[j][c] = s;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) sh[t] += sh[t + w];
+  for (int w = kRedLanes / 2; w > 0; w >>= 1) {
+    if (j < w) sh[j][c] += sh[j + w][c];
     __syncthreads();
   }
-  if (t == 0) out[k] = sh[0];
+  if (j == 0 && k < K) out[k] = sh[0][c];
 }
 
 __global__ void fill_f64_kernel(double* __restrict__ a, int64_t n) {
@@ -976,7 +982,7 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
   }
   timer_end(c, "eof_accumulate", s);
   BFE_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(K, nsnap), dim3(256), 0, s, partials,
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3((K + kRedCols - 1) / kRedCols, nsnap), dim3(256), 0, s, partials,
                      (int)P.S.nseg, K, sums, rowb);
   const int nc = (c->mmax + 1) * c->norder;
   hipLaunchKernelGGL(eof_split_kernel, dim3((nc + 255) / 256, nsnap), dim3(256), 0, s, sums, c->mmax,
@@ -1025,7 +1031,7 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
       hipLaunchKernelGGL(sl_mfma_kernel<2>, dim3(P.grid), dim3(256), mlds, s, x, y, z, m, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
     timer_end(c, "sl_accumulate", s);
     BFE_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(K, nsnap), dim3(256), 0, s, partials,
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((K + kRedCols - 1) / kRedCols, nsnap), dim3(256), 0, s, partials,
                        (int)P.S.nseg, K, coef_out, rowb);
     BFE_HIP(c, hipGetLastError());
     return 0;
@@ -1059,7 +1065,7 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
   }
   timer_end(c, "sl_accumulate", s);
   BFE_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3(K, nsnap), dim3(256), 0, s, partials,
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3((K + kRedCols - 1) / kRedCols, nsnap), dim3(256), 0, s, partials,
                      (int)P.S.nseg, K, coef_out, rowb);
   BFE_HIP(c, hipGetLastError());
   return 0;

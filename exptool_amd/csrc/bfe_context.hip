@@ -57,8 +57,8 @@ static void free_ptr(void* p) {
 
 static void free_sl(Context* c) {
   free_ptr(c->sl_xi); free_ptr(c->sl_p0); free_ptr(c->sl_d0);
-  free_ptr(c->sl_ef64); free_ptr(c->sl_ev64); free_ptr(c->sl_e32); free_ptr(c->sl_lmc);
-  c->sl_xi = c->sl_p0 = c->sl_d0 = c->sl_ef64 = c->sl_ev64 = nullptr;
+  free_ptr(c->sl_fold64); free_ptr(c->sl_e32); free_ptr(c->sl_lmc);
+  c->sl_xi = c->sl_p0 = c->sl_d0 = c->sl_fold64 = nullptr;
   c->sl_e32 = nullptr;
   c->sl_lmc = nullptr;
   for (auto& f : c->slf) { free_ptr(f.d); f = SLFolded(); }
@@ -165,6 +165,21 @@ extern "C" int bfe_sl_tables_set(bfe_ctx* h, int lmax, int nmax, int numr, int c
     double inv = 1.0 / sqrt(evtable[ln]);
     for (int i = 0; i < numr; ++i) e32[(size_t)i * LN + ln] = (float)(eftable[ln * numr + i] * inv);
   }
+  // the three scaled tables the coefficient fold sums over n, with n fastest
+  const size_t plane = LN * numr;
+  std::vector<double> fold64(3 * plane);
+  for (int l = 0; l <= lmax; ++l)
+    for (int n = 0; n < nmax; ++n) {
+      const size_t ln = (size_t)l * nmax + n;
+      const double sev = sqrt(evtable[ln]);
+      for (int i = 0; i < numr; ++i) {
+        const double e = eftable[ln * numr + i];
+        const size_t o = ((size_t)l * numr + i) * nmax + n;
+        fold64[o] = e / sev;
+        fold64[plane + o] = e * sev;
+        fold64[2 * plane + o] = e * p0[i] / sev;
+      }
+    }
   // recurrence and normalisation constants (spheresl.py:664-700, :823-863);
   // N_lm^2 = (2l+1)/(4 pi) (l-m)!/(l+m)! from an exact product of integers
   std::vector<double> lmc(kLmcSize, 0.0);
@@ -193,8 +208,7 @@ extern "C" int bfe_sl_tables_set(bfe_ctx* h, int lmax, int nmax, int numr, int c
   if ((rc = dev_upload(c, &c->sl_xi, xi, numr))) return rc;
   if ((rc = dev_upload(c, &c->sl_p0, p0, numr))) return rc;
   if ((rc = dev_upload(c, &c->sl_d0, d0, numr))) return rc;
-  if ((rc = dev_upload(c, &c->sl_ev64, evtable, LN))) return rc;
-  if ((rc = dev_upload(c, &c->sl_ef64, eftable, LN * numr))) return rc;
+  if ((rc = dev_upload(c, &c->sl_fold64, fold64.data(), fold64.size()))) return rc;
   if ((rc = dev_upload(c, &c->sl_e32, e32.data(), e32.size()))) return rc;
   c->h_xi.assign(xi, xi + numr);
   c->lmax = lmax; c->nmax = nmax; c->numr = numr; c->cmap = cmap; c->scale = scale;

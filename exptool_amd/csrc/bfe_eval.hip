@@ -14,14 +14,16 @@ struct SLFoldArgs {
 };
 
 // One thread per (radial node i, (l, m) pair p in m-major order).  E, Dn and
-// G at i-1, i, i+1 are summed over n in float64 from the float64 master
-// tables, D1/D2 formed, and the 8-float block rounded once to float32.
+// G at i-1, i, i+1 are summed over n in float64 from the float64 scaled
+// tables S = [ef/sqrt(ev), ef sqrt(ev), ef p0/sqrt(ev)] ([l][i][n] each),
+// D1/D2 formed, and the 8-float block rounded once to float32.
 // grid.y runs over coefficient sets: set q reads coef + q * coef_stride and
 // writes out + q * out_stride.
-__global__ void sl_fold_kernel(const double* __restrict__ ef, const double* __restrict__ ev,
-                               const double* __restrict__ p0, const double* __restrict__ coef,
-                               int nmax, int numr, SLFoldArgs a, float* __restrict__ out,
-                               int64_t coef_stride, int64_t out_stride) {
+__global__ __launch_bounds__(256) void sl_fold_kernel(const double* __restrict__ S,
+                                                      const double* __restrict__ coef, int nmax,
+                                                      int numr, SLFoldArgs a,
+                                                      float* __restrict__ out, int64_t coef_stride,
+                                                      int64_t out_stride) {
   coef += (size_t)blockIdx.y * coef_stride;
   out += (size_t)blockIdx.y * out_stride;
   const int np = sl_npairs(a.lmax);
@@ -38,24 +40,32 @@ __global__ void sl_fold_kernel(const double* __restrict__ ef, const double* __re
   double N = sqrt((0.5 * l + 0.25) / kPi * ratio);
   if (m > 0) N *= 1.4142135623730951;
   const double mul = a.lmul[l] * N;
-  const int rc = l * l + (m > 0 ? 2 * m - 1 : 0);
-  const int rs = l * l + 2 * m;
+  const double* cc = coef + (size_t)(l * l + (m > 0 ? 2 * m - 1 : 0)) * nmax;
+  const double* cs = coef + (size_t)(l * l + 2 * m) * nmax;
+  const bool two = m > 0;
   const bool inner = (i >= 1 && i <= numr - 2);
+  const size_t plane = (size_t)(a.lmax + 1) * numr * nmax;
+  const double* E = S + ((size_t)l * numr + i) * nmax;
+  const double* D = E + plane;
+  const double* G = E + 2 * plane;
   double v[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};  // E, Dn, Gm, G0, Gp
-  for (int cs = 0; cs < (m > 0 ? 2 : 1); ++cs) {
-    const int row = cs ? rs : rc;
-    for (int n = 0; n < a.nmax_use; ++n) {
-      const double c = coef[(size_t)row * nmax + n] * mul;
-      const size_t base = ((size_t)l * nmax + n) * numr;
-      const double sev = sqrt(ev[l * nmax + n]);
-      const double e = ef[base + i];
-      v[cs][0] += c * e / sev;
-      v[cs][1] += c * e * sev;
-      v[cs][3] += c * e * p0[i] / sev;
-      if (inner) {
-        v[cs][2] += c * ef[base + i - 1] * p0[i - 1] / sev;
-        v[cs][4] += c * ef[base + i + 1] * p0[i + 1] / sev;
-      }
+#pragma unroll 4
+  for (int n = 0; n < a.nmax_use; ++n) {
+    const double c0 = cc[n] * mul;
+    const double c1 = two ? cs[n] * mul : 0.0;
+    const double e = E[n], d = D[n], g0 = G[n];
+    v[0][0] += c0 * e;
+    v[1][0] += c1 * e;
+    v[0][1] += c0 * d;
+    v[1][1] += c1 * d;
+    v[0][3] += c0 * g0;
+    v[1][3] += c1 * g0;
+    if (inner) {
+      const double gm = G[n - nmax], gp = G[n + nmax];
+      v[0][2] += c0 * gm;
+      v[1][2] += c1 * gm;
+      v[0][4] += c0 * gp;
+      v[1][4] += c1 * gp;
     }
   }
   float* o = out + ((size_t)i * np + p) * 8;
@@ -74,36 +84,56 @@ struct EOFFoldArgs {
   int nuse, dens;
 };
 
-// One thread per (node, m), nodes fastest so the function-major float64
-// tables are read coalesced; writes [potC, rfC, zfC, dC, potS, rfS, zfS, dS].
-__global__ void eof_fold_kernel(const double* __restrict__ T0, const double* __restrict__ T1,
-                                const double* __restrict__ T2, const double* __restrict__ T3,
-                                const double* __restrict__ ccos, const double* __restrict__ csin,
-                                int mmax, int norder, int64_t nodes, EOFFoldArgs a,
-                                float* __restrict__ out, int64_t coef_stride, int64_t out_stride) {
-  ccos += (size_t)blockIdx.y * coef_stride;
-  csin += (size_t)blockIdx.y * coef_stride;
-  out += (size_t)blockIdx.y * out_stride;
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nodes * (mmax + 1)) return;
-  const int m = (int)(idx / nodes);
-  const int64_t node = idx - (int64_t)m * nodes;
-  const double* tabs[4] = {T0, T1, T2, T3};
+// One thread per (row group q, node): q = m for cos m = 0..mmax, mmax + m
+// for sin m = 1..mmax; nodes fastest so the function-major float64 tables
+// are read coalesced, and q uniform per block so the coefficients are
+// scalar loads.  Writes its half [pot, rf, zf, dens] of the 8-float block
+// [potC, rfC, zfC, dC, potS, rfS, zfS, dS]; cos m = 0 also zeroes the sin
+// half.  grid.x = (2 mmax + 1) * chunks, grid.y = coefficient sets.
+__global__ __launch_bounds__(256) void eof_fold_kernel(
+    const double* __restrict__ T0, const double* __restrict__ T1, const double* __restrict__ T2,
+    const double* __restrict__ T3, const double* __restrict__ ccos, const double* __restrict__ csin,
+    int mmax, int norder, int64_t nodes, int chunks, EOFFoldArgs a, float* __restrict__ out,
+    int64_t coef_stride, int64_t out_stride) {
+  const int q = (int)(blockIdx.x / chunks);
+  const int64_t node = (int64_t)(blockIdx.x - (unsigned)q * chunks) * 256 + threadIdx.x;
+  if (node >= nodes) return;
+  const bool sn = q > mmax;
+  const int m = sn ? q - mmax : q;
+  const double* cf = (sn ? csin : ccos) + (size_t)blockIdx.y * coef_stride + (size_t)m * norder;
   const double mul = a.mmul[m];
-  float* o = out + ((size_t)node * (mmax + 1) + m) * 8;
-  for (int cs = 0; cs < 2; ++cs) {
-    const bool have = (cs == 0) || (m > 0);
-    const int j = cs ? mmax + m : m;
-    const double* cf = (cs ? csin : ccos) + (size_t)m * norder;
-    for (int g = 0; g < 4; ++g) {
-      double s = 0.0;
-      if (have && (g < 3 || a.dens)) {
-        const double* t = tabs[g] + (size_t)j * norder * nodes + node;
-        for (int n = 0; n < a.nuse; ++n) s += cf[n] * t[(size_t)n * nodes];
-      }
-      o[cs * 4 + g] = (float)(s * mul);
+  const size_t row = (size_t)q * norder * nodes + node;
+  const double* t0 = T0 + row;
+  const double* t1 = T1 + row;
+  const double* t2 = T2 + row;
+  const double* t3 = T3 + row;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  if (a.dens) {
+#pragma unroll 6
+    for (int n = 0; n < a.nuse; ++n) {
+      const double c = cf[n];
+      const size_t o = (size_t)n * nodes;
+      s0 += c * t0[o];
+      s1 += c * t1[o];
+      s2 += c * t2[o];
+      s3 += c * t3[o];
+    }
+  } else {
+#pragma unroll 6
+    for (int n = 0; n < a.nuse; ++n) {
+      const double c = cf[n];
+      const size_t o = (size_t)n * nodes;
+      s0 += c * t0[o];
+      s1 += c * t1[o];
+      s2 += c * t2[o];
     }
   }
+  float* o = out + (size_t)blockIdx.y * out_stride + ((size_t)node * (mmax + 1) + m) * 8 + (sn ? 4 : 0);
+  o[0] = (float)(s0 * mul);
+  o[1] = (float)(s1 * mul);
+  o[2] = (float)(s2 * mul);
+  o[3] = (float)(s3 * mul);
+  if (q == 0) o[4] = o[5] = o[6] = o[7] = 0.f;
 }
 
 static int ensure_folded(Context* c, float** d, size_t* have, size_t floats, hipStream_t s) {
@@ -139,7 +169,7 @@ static int sl_fold_into(Context* c, const double* coef, int nset, double halofac
   a.nmax_use = nmax_use;
   const int64_t total = (int64_t)c->numr * np;
   hipLaunchKernelGGL(sl_fold_kernel, dim3((unsigned)((total + 255) / 256), nset), dim3(256), 0, s,
-                     c->sl_ef64, c->sl_ev64, c->sl_p0, coef, c->nmax, c->numr, a, out,
+                     c->sl_fold64, coef, c->nmax, c->numr, a, out,
                      (int64_t)(c->lmax + 1) * (c->lmax + 1) * c->nmax, (int64_t)sl_fold_floats(c));
   BFE_HIP(c, hipGetLastError());
   return 0;
@@ -172,10 +202,11 @@ static int eof_fold_into(Context* c, const double* ccos, const double* csin, int
   }
   a.nuse = nuse;
   a.dens = dens ? 1 : 0;
-  const int64_t total = nodes * (c->mmax + 1);
-  hipLaunchKernelGGL(eof_fold_kernel, dim3((unsigned)((total + 255) / 256), nset), dim3(256), 0, s,
-                     c->eof64[0], c->eof64[1], c->eof64[2], dens ? c->eof64[3] : c->eof64[0],
-                     ccos, csin, c->mmax, c->norder, nodes, a, out,
+  const int chunks = (int)((nodes + 255) / 256);
+  hipLaunchKernelGGL(eof_fold_kernel, dim3((unsigned)((2 * c->mmax + 1) * chunks), nset), dim3(256),
+                     0, s, c->eof64[0], c->eof64[1], c->eof64[2],
+                     dens ? c->eof64[3] : c->eof64[0], ccos, csin, c->mmax, c->norder, nodes,
+                     chunks, a, out,
                      (int64_t)(c->mmax + 1) * c->norder, (int64_t)eof_fold_floats(c));
   BFE_HIP(c, hipGetLastError());
   return 0;
@@ -439,10 +470,9 @@ extern "C" int bfe_field_eval_cart(bfe_ctx* h, const float* x, const float* y, c
   if (n == 0) return 0;
   BFE_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
-  const int* perm = nullptr;
   const float4* pts = nullptr;
   if (n >= c->order_min) {
-    rc = order_points(c, T, x, y, z, n, s, &perm, nullptr, 1, nullptr, &pts);
+    rc = order_points(c, T, x, y, z, n, s, nullptr, nullptr, 1, nullptr, &pts);
     if (rc) return rc;
   }
   const unsigned grid = pts ? xcd_grid(nblocks256(n)) : nblocks256(n);
@@ -527,10 +557,9 @@ extern "C" int bfe_field_eval_batch(bfe_ctx* h, const float* x, const float* y, 
   T.ximin = *std::min_element(c->h_xi.begin(), c->h_xi.end());
   T.crot = cos(rotpos);
   T.srot = sin(rotpos);
-  const int* perm = nullptr;
   const uint32_t* sorted = nullptr;
   const float4* pts = nullptr;
-  int rc = order_points(c, T, x, y, z, n, s, &perm, c->batch_off, nsnap, &sorted, &pts);
+  int rc = order_points(c, T, x, y, z, n, s, nullptr, c->batch_off, nsnap, &sorted, &pts);
   if (rc) return rc;
   hipLaunchKernelGGL(field_eval_batch_kernel, dim3(xcd_grid(nblocks256(n))), dim3(256), 0, s, n,
                      pts, sorted, order_cell_count(T), T, c->batch_sl, (int64_t)slf, c->batch_eof,

@@ -73,8 +73,7 @@ struct Context {
   double* sl_xi = nullptr;     // numr
   double* sl_p0 = nullptr;     // numr
   double* sl_d0 = nullptr;     // numr
-  double* sl_ef64 = nullptr;   // (lmax+1, nmax, numr) float64 master copy
-  double* sl_ev64 = nullptr;   // (lmax+1, nmax)
+  double* sl_fold64 = nullptr; // [3][lmax+1][numr][nmax] float64: ef/sqrt(ev), ef sqrt(ev), ef p0/sqrt(ev)
   float* sl_e32 = nullptr;     // [numr][(lmax+1)*nmax]  ef/sqrt(ev), node-major
   double* sl_lmc = nullptr;    // per-(l, m) constants of the accumulate recurrences
   std::vector<double> h_xi;    // host copy (min xi for the field guard)
@@ -114,7 +113,7 @@ struct Context {
   std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> timer_events;
 
   // ---- spatial ordering of evaluation points (bfe_order.hip) -------------
-  void* order = nullptr;       // keys[n] | sorted keys[n] | perm[n] | per-point f64[n] | sort temp
+  void* order = nullptr;       // sort keys and packed points (two buffers each) | perm | per-point f64 | tile counts
   size_t order_bytes = 0;
 
   // ---- batched field evaluation: per-snapshot folded tables ---------------
@@ -152,8 +151,9 @@ inline EOFGeom eof_geom(const Context* c) {
 struct FieldTables;
 // Permutation perm[j] = i listing the points in order of their table cell
 // (EOF (R, z) cell inside the disc grid, else the SL radial cell), so that
-// the lanes of a wave read the same table rows.  Rebuilt by each call; the
-// returned pointer stays valid until the next call on this context.
+// the lanes of a wave read the same table rows (perm_out may be NULL).
+// Rebuilt by each call; the returned pointers stay valid until the next call
+// on this context.
 // With off_dev (device int64[nsnap + 1], concatenated snapshots) the key is
 // (snapshot, cell), and sorted_out (may be NULL) receives the sorted keys:
 // snapshot = key / order_cell_count(T).  points_out (may be NULL) receives

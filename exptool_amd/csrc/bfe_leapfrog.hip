@@ -213,8 +213,8 @@ extern "C" int bfe_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float* vx,
   if (rc) return rc;
   OrbitAux* aux = (OrbitAux*)c->scratch;
   double* partE = (double*)((char*)c->scratch + ((aux_bytes + 255) & ~(size_t)255));
-  // cell order of the initial positions; without a graph it is rebuilt from
-  // the current positions before every chunk
+  // cell order of the initial positions, rebuilt from the current positions
+  // before every chunk
   const int* perm = nullptr;
   rc = order_points(c, T, x, y, z, n, s, &perm);
   if (rc) return rc;
@@ -254,9 +254,9 @@ extern "C" int bfe_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float* vx,
     BFE_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   }
   for (const LeapArgs& b : chunks) {
-    // re-sort by the current cells between chunks (not inside a captured
-    // graph: the sort issues hipMemsetAsync)
-    if (b.k0 > 0 && !use_graph) {
+    // re-sort by the current cells between chunks (the sort allocates
+    // nothing here: the buffer was sized by the first sort, before any capture)
+    if (b.k0 > 0) {
       rc = order_points(c, T, x, y, z, n, s, &perm);
       if (rc) return rc;
     }

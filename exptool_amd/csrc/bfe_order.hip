@@ -1,46 +1,39 @@
-// Cell ordering of evaluation points.
+// Cell ordering of evaluation points and particles.
 //
 // The folded field tables (SL 1.8 MB, EOF 7.4 MB at the workload size) do
 // not fit in one XCD's 4 MB L2, and points in input order send every lane of
 // a wave to a different table cell.  Sorting the points by table cell lets a
 // wave's lanes share cells and, with the XCD-aware block mapping, each XCD
-// work on one contiguous range of cells.
+// work on one contiguous range of cells.  Accumulate sorts its particles the
+// same way so that runs of equal cells can be summed before the table reads.
 //
 // Key: the EOF (ix, iy) cell for points inside the disc grid, else
-// numx * numy + the SL radial cell.  The keys are sorted with rocPRIM's radix
-// sort (values: the point indices); a counting sort with global atomics was
-// slower here because thousands of points share the busiest cells.  No
-// result depends on the order (evaluation and orbit integration are per
-// point, and energy sums are taken in input order).  The sort issues
-// hipMemsetAsync, so it is never placed inside a captured graph.
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/iterator/counting_iterator.hpp>
-
+// numx * numy + the SL radial cell (evaluation); the accumulate cell of the
+// particle (accumulate); plus snapshot * cells for concatenated snapshots.
+//
+// The sort is a stable least-significant-digit counting sort over 8-bit
+// digits, ceil(bits / 8) passes of three kernels:
+//   count   - per 1024-item tile, the digit histogram (the first pass also
+//             computes the keys from the coordinates);
+//   rowscan - per digit, the exclusive prefix of the tile counts;
+//   scatter - each tile ranks its items per digit, stages them in LDS in
+//             digit order and writes them out in runs.
+// Within a wave, items with equal digits are found with eight ballots and
+// ranked by popcount; a per-wave histogram in LDS carries the ranks across
+// the wave's item slots.  The order inside a tile is (wave, slot, lane),
+// which is input order, so the sort is stable and the result reproducible.
+// There are no global atomics (a counting sort with them was slow here
+// because thousands of points share the busiest cells), no spin-waits and no
+// memsets: every counter is written by a kernel.  The packed float4 (x, y,
+// z, mass or input index) travels with the key, so no gather pass is needed.
 #include "bfe_field.hpp"
 
 namespace bfe {
 
-// Onesweep radix sort at every size: rocPRIM's default switches to a merge
-// sort up to 2^20 items, which took several times longer for our keys.
-using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
-                                              rocprim::default_config, 0>;
-
-__device__ __forceinline__ uint32_t cell_key(const FieldTables& T, float xf, float yf, float zf,
-                                             uint32_t neof) {
-  const double x = xf, y = yf, z = zf;
-  const double R = sqrt(x * x + y * y);
-  const EOFGeom& g = T.eg;
-  const double X = (r_to_xi(R, g.cmap, g.ascale) - g.xmin) / g.dx;
-  const double Y = (z_to_y(z, g.hscale) - g.ymin) / g.dy;
-  if (X >= 0.0 && X < (double)g.numx && Y >= 0.0 && Y < (double)g.numy)
-    return (uint32_t)X * (uint32_t)g.numy + (uint32_t)Y;
-  const SLGeom& s = T.slg;
-  const double xi = r_to_xi(sqrt(R * R + z * z), s.cmap, s.scale);
-  const double f = floor((xi - s.xi[0]) / (s.xi[1] - s.xi[0]));
-  int i = 0;
-  if (f == f) i = f < 0.0 ? 0 : (f > (double)(s.numr - 2) ? s.numr - 2 : (int)f);
-  return neof + (uint32_t)i;
-}
+constexpr int kSortThreads = 256;
+constexpr int kSortIPT = 4;                              // items per thread
+constexpr int kSortTile = kSortThreads * kSortIPT;       // items per tile (block)
+constexpr int kRadix = 256;                              // 8-bit digits
 
 // Snapshot of point i for concatenated snapshots: the last s with off[s] <= i.
 __device__ __forceinline__ int snapshot_of(const int64_t* __restrict__ off, int nsnap, int64_t i) {
@@ -52,43 +45,293 @@ __device__ __forceinline__ int snapshot_of(const int64_t* __restrict__ off, int 
   return lo;
 }
 
-// key = snapshot * nkeys + cell (snapshot 0 without offsets); with packed,
-// also (x, y, z, index bits) for a gather in key order
-__global__ __launch_bounds__(256) void order_key_kernel(
-    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-    int64_t n, FieldTables T, uint32_t neof, uint32_t nkeys, const int64_t* __restrict__ off,
-    int nsnap, uint32_t* __restrict__ keys, float4* __restrict__ packed) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float xf = x[i], yf = y[i], zf = z[i];
-  const uint32_t snap = off ? (uint32_t)snapshot_of(off, nsnap, i) : 0u;
-  keys[i] = snap * nkeys + cell_key(T, xf, yf, zf, neof);
-  if (packed) packed[i] = make_float4(xf, yf, zf, __int_as_float((int)i));
+// Table geometry for the sort keys, in float32.  A key only decides the
+// order: the accumulate kernels recompute each particle's cell in float64
+// and start a new run wherever it changes, and evaluation is per point, so a
+// key that differs from the float64 cell next to a cell edge costs a shorter
+// run, never a different result.
+struct KeyGeom {
+  int ecmap, numx, numy;
+  float ea, eih, exmin, eidx, eymin, eidy;   // EOF: ascale, 1/hscale, xmin, 1/dx, ymin, 1/dy
+  int scmap, numr;
+  float ss, sxi0, sidxi;                     // SL: scale, xi[0], 1/(xi[1] - xi[0])
+};
+
+static KeyGeom key_geom(const EOFGeom& e, const SLGeom& s, const double* h_xi) {
+  KeyGeom g;
+  g.ecmap = e.cmap;
+  g.numx = e.numx;
+  g.numy = e.numy;
+  g.ea = (float)e.ascale;
+  g.eih = (float)(1.0 / e.hscale);
+  g.exmin = (float)e.xmin;
+  g.eidx = (float)(1.0 / e.dx);
+  g.eymin = (float)e.ymin;
+  g.eidy = (float)(1.0 / e.dy);
+  g.scmap = s.cmap;
+  g.numr = s.numr;
+  g.ss = (float)s.scale;
+  g.sxi0 = h_xi ? (float)h_xi[0] : 0.f;
+  g.sidxi = h_xi ? (float)(1.0 / (h_xi[1] - h_xi[0])) : 0.f;
+  return g;
 }
 
-__global__ __launch_bounds__(256) void gather_points_kernel(const float4* __restrict__ in,
-                                                            const int* __restrict__ perm, int64_t n,
-                                                            float4* __restrict__ out) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) out[j] = in[perm[j]];
+// r -> xi of the maps (compatibility.py r_to_xi): (r/a - 1)/(r/a + 1), ln r, r
+__device__ __forceinline__ float key_xi(float r, int cmap, float a) {
+  if (cmap == 1) return __fdividef(r - a, r + a);
+  if (cmap == 2) return __logf(r);
+  return r;
+}
+
+// z -> y = sign(z) asinh(|z| / h)
+__device__ __forceinline__ float key_y(float z, float ih) {
+  const float q = fabsf(z) * ih;
+  const float v = __logf(q + sqrtf(fmaf(q, q, 1.f)));
+  return z < 0.f ? -v : v;
+}
+
+// floor(v) clamped to [0, hi]; NaN -> 0
+__device__ __forceinline__ uint32_t key_clamp(float v, int hi) {
+  return v >= 0.f ? (uint32_t)fminf(v, (float)hi) : 0u;
+}
+
+__device__ __forceinline__ uint32_t key_sl_cell(const KeyGeom& g, float r) {
+  return key_clamp((key_xi(r, g.scmap, g.ss) - g.sxi0) * g.sidxi, g.numr - 2);
+}
+
+// Where the keys and packed values of the first pass come from.
+struct KeySrc {
+  const float* x;
+  const float* y;
+  const float* z;
+  const float* m;        // mass for the packed .w; NULL: the input index as int bits
+  const int64_t* off;    // device snapshot offsets [nsnap + 1], NULL: one snapshot
+  int nsnap;
+  uint32_t ncell;        // cells per snapshot
+  int kind;              // 0 EOF accumulate cell, 1 SL accumulate cell, 2 evaluation cell
+  KeyGeom g;
+};
+
+// kind 0: the EOF (ix, iy) cell of (R = sqrt(x^2 + y^2 + 1e-10), z), clamped
+// to the grid as eof.return_bins; kind 1: the SL radial cell of
+// r = max(|x|, 1e-10); kind 2: the EOF cell for points inside the disc grid,
+// else numx * numy + the SL radial cell.
+__device__ __forceinline__ uint32_t source_key(const KeySrc& S, int64_t i) {
+  const float x = S.x[i], y = S.y[i], z = S.z[i];
+  const KeyGeom& g = S.g;
+  uint32_t cell;
+  if (S.kind == 1) {
+    cell = key_sl_cell(g, fmaxf(sqrtf(fmaf(x, x, fmaf(y, y, z * z))), 1.e-10f));
+  } else {
+    const float R2 = fmaf(x, x, y * y);
+    const float R = sqrtf(S.kind == 0 ? R2 + 1.e-10f : R2);
+    const float X = (key_xi(R, g.ecmap, g.ea) - g.exmin) * g.eidx;
+    const float Y = (key_y(z, g.eih) - g.eymin) * g.eidy;
+    if (S.kind == 0)
+      cell = key_clamp(X, g.numx - 1) * (uint32_t)g.numy + key_clamp(Y, g.numy - 1);
+    else if (X >= 0.f && X < (float)g.numx && Y >= 0.f && Y < (float)g.numy)
+      cell = (uint32_t)X * (uint32_t)g.numy + (uint32_t)Y;
+    else
+      cell = (uint32_t)g.numx * (uint32_t)g.numy + key_sl_cell(g, sqrtf(fmaf(z, z, R2)));
+  }
+  const uint32_t snap = S.off ? (uint32_t)snapshot_of(S.off, S.nsnap, i) : 0u;
+  return snap * S.ncell + cell;
+}
+
+// Lanes of the wave (among the live ones) whose digit equals this lane's.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool live) {
+  uint64_t mask = __ballot(live);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t v = __ballot(live && bit);
+    mask &= bit ? v : ~v;
+  }
+  return mask;
+}
+
+// Inclusive scan over the 256 threads of the block (all must call it).
+__device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* wsum) {
+  const int t = threadIdx.x, l = t & 63, w = t >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o, 64);
+    if (l >= o) v += u;
+  }
+  if (l == 63) wsum[w] = v;
+  __syncthreads();
+  uint32_t add = 0;
+  for (int q = 0; q < w; ++q) add += wsum[q];
+  __syncthreads();
+  return v + add;
+}
+
+// Item slot i of thread (w, l) in tile b: b * kSortTile + (w * kSortIPT + i) * 64 + l.
+__device__ __forceinline__ int64_t sort_item(int i) {
+  const int t = threadIdx.x;
+  return (int64_t)blockIdx.x * kSortTile + (int64_t)((t >> 6) * kSortIPT + i) * 64 + (t & 63);
+}
+
+// Digit histogram of each tile: counts[d * nb + tile].  First pass (kin ==
+// NULL): keys computed from the source and written to kfirst.
+__global__ __launch_bounds__(kSortThreads) void sort_count_kernel(
+    KeySrc S, const uint32_t* __restrict__ kin, uint32_t* __restrict__ kfirst, int64_t n, int shift,
+    uint32_t* __restrict__ counts, int nb) {
+  __shared__ uint32_t hist[4][kRadix];
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  for (int j = t; j < 4 * kRadix; j += kSortThreads) (&hist[0][0])[j] = 0u;
+  uint32_t key[kSortIPT];
+#pragma unroll
+  for (int i = 0; i < kSortIPT; ++i) {
+    const int64_t idx = sort_item(i);
+    key[i] = 0u;
+    if (idx < n) {
+      if (kin) {
+        key[i] = kin[idx];
+      } else {
+        key[i] = source_key(S, idx);
+        kfirst[idx] = key[i];
+      }
+    }
+  }
+  __syncthreads();
+  const uint64_t lt = (1ull << l) - 1ull;
+#pragma unroll
+  for (int i = 0; i < kSortIPT; ++i) {
+    const bool live = sort_item(i) < n;
+    const uint32_t d = (key[i] >> shift) & (kRadix - 1);
+    const uint64_t mk = match_digit(d, live);
+    if (live && (mk & lt) == 0ull) hist[w][d] += (uint32_t)__popcll(mk);
+  }
+  __syncthreads();
+  counts[(int64_t)t * nb + blockIdx.x] = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];
+}
+
+// Per digit (block), the exclusive prefix of its tile counts, in place, and
+// the digit total.
+__global__ __launch_bounds__(kSortThreads) void sort_rowscan_kernel(uint32_t* __restrict__ counts,
+                                                                    int nb,
+                                                                    uint32_t* __restrict__ totals) {
+  __shared__ uint32_t wsum[4];
+  const int t = threadIdx.x;
+  uint32_t* row = counts + (int64_t)blockIdx.x * nb;
+  const int per = (nb + kSortThreads - 1) / kSortThreads;
+  const int j0 = min(t * per, nb), j1 = min(j0 + per, nb);
+  uint32_t sum = 0u;
+  for (int j = j0; j < j1; ++j) sum += row[j];
+  const uint32_t incl = block_scan_incl(sum, wsum);
+  uint32_t run = incl - sum;
+  for (int j = j0; j < j1; ++j) {
+    const uint32_t v = row[j];
+    row[j] = run;
+    run += v;
+  }
+  if (t == kSortThreads - 1) totals[blockIdx.x] = incl;
+}
+
+// Stable scatter of one tile by digit.  vin == NULL: the values are packed
+// from the source (first pass).  kout and perm may be NULL on the last pass.
+__global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
+    KeySrc S, const uint32_t* __restrict__ kin, const float4* __restrict__ vin, int64_t n,
+    int shift, const uint32_t* __restrict__ rowpre, const uint32_t* __restrict__ totals, int nb,
+    uint32_t* __restrict__ kout, float4* __restrict__ vout, int* __restrict__ perm) {
+  __shared__ uint32_t hist[4][kRadix];   // per-wave running counts, then wave prefixes
+  __shared__ uint32_t dstart[kRadix];    // tile-local start of each digit
+  __shared__ uint32_t gpos[kRadix];      // output position of the tile's first item of each digit
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t skey[kSortTile];
+  __shared__ float4 sval[kSortTile];
+  const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  for (int j = t; j < 4 * kRadix; j += kSortThreads) (&hist[0][0])[j] = 0u;
+  uint32_t key[kSortIPT];
+  float4 val[kSortIPT];
+#pragma unroll
+  for (int i = 0; i < kSortIPT; ++i) {
+    const int64_t idx = sort_item(i);
+    key[i] = 0u;
+    val[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (idx < n) {
+      key[i] = kin[idx];
+      val[i] = vin ? vin[idx]
+                   : make_float4(S.x[idx], S.y[idx], S.z[idx],
+                                 S.m ? S.m[idx] : __int_as_float((int)idx));
+    }
+  }
+  __syncthreads();
+  // rank within the wave: items of earlier slots, then lower lanes of this slot
+  const uint64_t lt = (1ull << l) - 1ull;
+  uint32_t rank[kSortIPT];
+#pragma unroll
+  for (int i = 0; i < kSortIPT; ++i) {
+    const bool live = sort_item(i) < n;
+    const uint32_t d = (key[i] >> shift) & (kRadix - 1);
+    const uint64_t mk = match_digit(d, live);
+    uint32_t before = 0u;
+    if (live) before = hist[w][d];
+    if (live && (mk & lt) == 0ull) hist[w][d] = before + (uint32_t)__popcll(mk);
+    rank[i] = before + (uint32_t)__popcll(mk & lt);
+  }
+  __syncthreads();
+  const uint32_t c0 = hist[0][t], c1 = hist[1][t], c2 = hist[2][t], c3 = hist[3][t];
+  const uint32_t tot = c0 + c1 + c2 + c3;
+  const uint32_t gt = totals[t];
+  const uint32_t pre = rowpre[(int64_t)t * nb + blockIdx.x];
+  __syncthreads();
+  hist[0][t] = 0u;
+  hist[1][t] = c0;
+  hist[2][t] = c0 + c1;
+  hist[3][t] = c0 + c1 + c2;
+  const uint32_t ds = block_scan_incl(tot, wsum) - tot;
+  const uint32_t gb = block_scan_incl(gt, wsum) - gt;
+  dstart[t] = ds;
+  gpos[t] = gb + pre;
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kSortIPT; ++i) {
+    if (sort_item(i) < n) {
+      const uint32_t d = (key[i] >> shift) & (kRadix - 1);
+      const uint32_t lp = dstart[d] + hist[w][d] + rank[i];
+      skey[lp] = key[i];
+      sval[lp] = val[i];
+    }
+  }
+  __syncthreads();
+  const int64_t rem = n - (int64_t)blockIdx.x * kSortTile;
+  const int cnt = rem < kSortTile ? (int)rem : kSortTile;
+  for (int j = t; j < cnt; j += kSortThreads) {
+    const uint32_t k = skey[j];
+    const uint32_t d = (k >> shift) & (kRadix - 1);
+    const uint32_t pos = gpos[d] + (uint32_t)j - dstart[d];
+    const float4 v = sval[j];
+    if (kout) kout[pos] = k;
+    vout[pos] = v;
+    if (perm) perm[pos] = __float_as_int(v.w);
+  }
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Ordering buffer: first-pass keys | keys A | keys B | values A | values B |
+// perm | per-point f64 | tile counts | digit totals.
 struct OrderLayout {
-  size_t keys, sorted, perm, energy, pin, pout, temp, total;
+  size_t k0, ka, kb, va, vb, perm, energy, counts, totals, total;
 };
 
-static OrderLayout order_layout(int64_t n, size_t temp_bytes) {
+static int sort_tiles(int64_t n) { return (int)((n + kSortTile - 1) / kSortTile); }
+
+static OrderLayout order_layout(int64_t n) {
   OrderLayout L;
-  L.keys = 0;
-  L.sorted = L.keys + align256((size_t)n * 4);
-  L.perm = L.sorted + align256((size_t)n * 4);
-  L.energy = L.perm + align256((size_t)n * 4);
-  L.pin = L.energy + align256((size_t)n * 8);
-  L.pout = L.pin + align256((size_t)n * 16);
-  L.temp = L.pout + align256((size_t)n * 16);
-  L.total = L.temp + align256(temp_bytes);
+  const size_t kbytes = align256((size_t)n * 4), vbytes = align256((size_t)n * 16);
+  L.k0 = 0;
+  L.ka = L.k0 + kbytes;
+  L.kb = L.ka + kbytes;
+  L.va = L.kb + kbytes;
+  L.vb = L.va + vbytes;
+  L.perm = L.vb + vbytes;
+  L.energy = L.perm + kbytes;
+  L.counts = L.energy + align256((size_t)n * 8);
+  L.totals = L.counts + align256((size_t)sort_tiles(n) * kRadix * 4);
+  L.total = L.totals + align256((size_t)kRadix * 4);
   return L;
 }
 
@@ -108,6 +351,58 @@ static int ensure_order(Context* c, size_t bytes) {
   return 0;
 }
 
+struct SortOut {
+  const float4* values;
+  const uint32_t* keys;
+  const int* perm;
+};
+
+// Sort the n items of S by key (nkeys distinct keys at most); writes the
+// sorted values, and when asked the sorted keys and the permutation.
+static int sort_source(Context* c, const KeySrc& S, int64_t n, uint64_t nkeys, bool want_keys,
+                       bool want_perm, hipStream_t s, SortOut* out) {
+  out->values = nullptr;
+  out->keys = nullptr;
+  out->perm = nullptr;
+  if (n > (int64_t)0x7fffffff) return set_error(c, BFE_ELIMIT, "order: n = %lld > 2^31 - 1", (long long)n);
+  if (nkeys > 0xffffffffull) return set_error(c, BFE_ELIMIT, "order: too many table cells");
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < nkeys) ++bits;
+  const int passes = (bits + 7) / 8;
+  const OrderLayout L = order_layout(n);
+  int rc = ensure_order(c, L.total);
+  if (rc) return rc;
+  char* base = (char*)c->order;
+  uint32_t* k0 = (uint32_t*)(base + L.k0);
+  uint32_t* kbuf[2] = {(uint32_t*)(base + L.ka), (uint32_t*)(base + L.kb)};
+  float4* vbuf[2] = {(float4*)(base + L.va), (float4*)(base + L.vb)};
+  int* perm = (int*)(base + L.perm);
+  uint32_t* counts = (uint32_t*)(base + L.counts);
+  uint32_t* totals = (uint32_t*)(base + L.totals);
+  const int nb = sort_tiles(n);
+  for (int p = 0; p < passes && n > 0; ++p) {
+    const int shift = 8 * p;
+    const bool last = p == passes - 1;
+    const uint32_t* kin = p == 0 ? (const uint32_t*)k0 : kbuf[(p - 1) & 1];
+    const float4* vin = p == 0 ? nullptr : vbuf[(p - 1) & 1];
+    uint32_t* kout = (!last || want_keys) ? kbuf[p & 1] : nullptr;
+    hipLaunchKernelGGL(sort_count_kernel, dim3((unsigned)nb), dim3(kSortThreads), 0, s, S,
+                       p == 0 ? nullptr : kin, k0, n, shift, counts, nb);
+    BFE_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(sort_rowscan_kernel, dim3(kRadix), dim3(kSortThreads), 0, s, counts, nb,
+                       totals);
+    BFE_HIP(c, hipGetLastError());
+    hipLaunchKernelGGL(sort_scatter_kernel, dim3((unsigned)nb), dim3(kSortThreads), 0, s, S, kin,
+                       vin, n, shift, counts, totals, nb, kout, vbuf[p & 1],
+                       (last && want_perm) ? perm : nullptr);
+    BFE_HIP(c, hipGetLastError());
+  }
+  out->values = vbuf[(passes - 1) & 1];
+  if (want_keys) out->keys = kbuf[(passes - 1) & 1];
+  if (want_perm) out->perm = perm;
+  return 0;
+}
+
 uint32_t order_cell_count(const FieldTables& T) {
   return (uint32_t)T.eg.numx * (uint32_t)T.eg.numy + (uint32_t)(T.slg.numr - 1);
 }
@@ -115,127 +410,42 @@ uint32_t order_cell_count(const FieldTables& T) {
 int order_points(Context* c, const FieldTables& T, const float* x, const float* y, const float* z,
                  int64_t n, hipStream_t s, const int** perm_out, const int64_t* off_dev,
                  int nsnap, const uint32_t** sorted_out, const float4** points_out) {
-  *perm_out = nullptr;
+  if (perm_out) *perm_out = nullptr;
   if (sorted_out) *sorted_out = nullptr;
   if (points_out) *points_out = nullptr;
-  if (n > (int64_t)0x7fffffff) return set_error(c, BFE_ELIMIT, "order: n = %lld > 2^31 - 1", (long long)n);
   const uint64_t ncell = (uint64_t)T.eg.numx * T.eg.numy + (uint64_t)(T.slg.numr - 1);
-  const uint64_t nkeys = ncell * (uint64_t)(off_dev ? nsnap : 1);
-  if (nkeys > 0xffffffffull) return set_error(c, BFE_ELIMIT, "order: too many table cells");
-  unsigned bits = 1;
-  while (bits < 32 && (1ull << bits) < nkeys) ++bits;
-  const int ni = (int)n;
-  rocprim::counting_iterator<int> iota(0);
-  size_t temp_bytes = 0;
-  BFE_HIP(c, rocprim::radix_sort_pairs<SortConfig>(nullptr, temp_bytes, (const uint32_t*)nullptr,
-                                       (uint32_t*)nullptr, iota, (int*)nullptr, ni, 0u, bits, s));
-  const OrderLayout L = order_layout(n, temp_bytes);
-  int rc = ensure_order(c, L.total);
+  if (ncell > 0xffffffffull) return set_error(c, BFE_ELIMIT, "order: too many table cells");
+  KeySrc S{x, y, z, nullptr, off_dev, off_dev ? nsnap : 1, (uint32_t)ncell, 2,
+           key_geom(T.eg, T.slg, c->h_xi.size() >= 2 ? c->h_xi.data() : nullptr)};
+  SortOut o;
+  int rc = sort_source(c, S, n, ncell * (uint64_t)(off_dev ? nsnap : 1), sorted_out != nullptr,
+                       perm_out != nullptr, s, &o);
   if (rc) return rc;
-  char* base = (char*)c->order;
-  uint32_t* keys = (uint32_t*)(base + L.keys);
-  uint32_t* sorted = (uint32_t*)(base + L.sorted);
-  int* perm = (int*)(base + L.perm);
-  const uint32_t neof = (uint32_t)T.eg.numx * (uint32_t)T.eg.numy;
-  float4* pin = points_out ? (float4*)(base + L.pin) : nullptr;
-  hipLaunchKernelGGL(order_key_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, z,
-                     n, T, neof, (uint32_t)ncell, off_dev, nsnap, keys, pin);
-  BFE_HIP(c, hipGetLastError());
-  BFE_HIP(c, rocprim::radix_sort_pairs<SortConfig>((void*)(base + L.temp), temp_bytes, keys, sorted, iota, perm,
-                                       ni, 0u, bits, s));
-  *perm_out = perm;
-  if (sorted_out) *sorted_out = sorted;
-  if (points_out) {
-    float4* pout = (float4*)(base + L.pout);
-    hipLaunchKernelGGL(gather_points_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, pin,
-                       perm, n, pout);
-    BFE_HIP(c, hipGetLastError());
-    *points_out = pout;
-  }
+  if (perm_out) *perm_out = o.perm;
+  if (sorted_out) *sorted_out = o.keys;
+  if (points_out) *points_out = o.values;
   return 0;
 }
 
-// ---- accumulate: a sorted, packed copy of the particles ---------------------
-
-// key = snapshot * ncell + accumulate cell: kind 0 the EOF (ix, iy) cell of
-// (R = sqrt(x^2 + y^2 + 1e-10), z) as eof.return_bins; kind 1 the SL radial
-// cell of r = max(|x|, 1e-10).
-__global__ __launch_bounds__(256) void pack_key_kernel(
-    const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-    const float* __restrict__ m, int64_t n, int kind, EOFGeom eg, SLGeom sg, uint32_t ncell,
-    const int64_t* __restrict__ off, int nsnap, uint32_t* __restrict__ keys,
-    float4* __restrict__ packed) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float xf = x[i], yf = y[i], zf = z[i];
-  packed[i] = make_float4(xf, yf, zf, m[i]);
-  const double xd = xf, yd = yf, zd = zf;
-  uint32_t cell;
-  if (kind == 0) {
-    const double R = sqrt(xd * xd + yd * yd + 1.e-10);
-    const EOFCell ce = eof_locate(R, zd, eg.cmap, eg.ascale, eg.hscale, eg.xmin, eg.dx, eg.ymin,
-                                  eg.dy, eg.numx, eg.numy);
-    cell = (uint32_t)ce.ix * (uint32_t)eg.numy + (uint32_t)ce.iy;
-  } else {
-    double r = sqrt(xd * xd + yd * yd + zd * zd);
-    r = (r == r) ? (r > 1.0e-10 ? r : 1.0e-10) : 1.0e-10;
-    cell = (uint32_t)sl_locate(r, sg.cmap, sg.scale, sg.xi, sg.numr).i;
-  }
-  const uint32_t snap = off ? (uint32_t)snapshot_of(off, nsnap, i) : 0u;
-  keys[i] = snap * ncell + cell;
-}
-
-__global__ __launch_bounds__(256) void gather_packed_kernel(const float4* __restrict__ in,
-                                                            const int* __restrict__ perm, int64_t n,
-                                                            float4* __restrict__ out) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) out[j] = in[perm[j]];
-}
-
-// The keys are sorted with the point indices as values (rocPRIM's fast path
-// for 4-byte values; 16-byte values took several times longer), then the
-// packed particles are gathered in sorted order.
 int order_particles(Context* c, int kind, const float* x, const float* y, const float* z,
                     const float* m, int64_t n, const int64_t* off_dev, int nsnap, hipStream_t s,
                     const float4** sorted_out) {
   *sorted_out = nullptr;
-  if (n > (int64_t)0x7fffffff) return set_error(c, BFE_ELIMIT, "order: n = %lld > 2^31 - 1", (long long)n);
   const EOFGeom eg = eof_geom(c);
   const SLGeom sg = sl_geom(c);
   const uint64_t ncell = kind == 0 ? (uint64_t)eg.numx * eg.numy : (uint64_t)(sg.numr - 1);
-  const uint64_t nkeys = ncell * (uint64_t)(off_dev ? nsnap : 1);
-  if (nkeys > 0xffffffffull) return set_error(c, BFE_ELIMIT, "order: too many table cells");
-  unsigned bits = 1;
-  while (bits < 32 && (1ull << bits) < nkeys) ++bits;
-  const int ni = (int)n;
-  rocprim::counting_iterator<int> iota(0);
-  size_t temp_bytes = 0;
-  BFE_HIP(c, rocprim::radix_sort_pairs<SortConfig>(nullptr, temp_bytes, (const uint32_t*)nullptr,
-                                       (uint32_t*)nullptr, iota, (int*)nullptr, ni, 0u, bits, s));
-  const size_t kb = align256((size_t)n * 4), pb = align256((size_t)n * 16);
-  const size_t total = 3 * kb + 2 * pb + align256(temp_bytes);
-  int rc = ensure_order(c, total);
+  if (ncell > 0xffffffffull) return set_error(c, BFE_ELIMIT, "order: too many table cells");
+  KeySrc S{x, y, z, m, off_dev, off_dev ? nsnap : 1, (uint32_t)ncell, kind,
+           key_geom(eg, sg, c->h_xi.size() >= 2 ? c->h_xi.data() : nullptr)};
+  SortOut o;
+  int rc = sort_source(c, S, n, ncell * (uint64_t)(off_dev ? nsnap : 1), false, false, s, &o);
   if (rc) return rc;
-  char* base = (char*)c->order;
-  uint32_t* keys = (uint32_t*)base;
-  uint32_t* sorted = (uint32_t*)(base + kb);
-  int* perm = (int*)(base + 2 * kb);
-  float4* pin = (float4*)(base + 3 * kb);
-  float4* pout = (float4*)(base + 3 * kb + pb);
-  void* temp = base + 3 * kb + 2 * pb;
-  const unsigned grid = (unsigned)((n + 255) / 256);
-  hipLaunchKernelGGL(pack_key_kernel, dim3(grid), dim3(256), 0, s, x, y, z, m, n, kind, eg, sg,
-                     (uint32_t)ncell, off_dev, nsnap, keys, pin);
-  BFE_HIP(c, hipGetLastError());
-  BFE_HIP(c, rocprim::radix_sort_pairs<SortConfig>(temp, temp_bytes, keys, sorted, iota, perm, ni, 0u, bits, s));
-  hipLaunchKernelGGL(gather_packed_kernel, dim3(grid), dim3(256), 0, s, pin, perm, n, pout);
-  BFE_HIP(c, hipGetLastError());
-  *sorted_out = pout;
+  *sorted_out = o.values;
   return 0;
 }
 
 double* order_energy_buffer(Context* c, int64_t n) {
-  return (double*)((char*)c->order + order_layout(n, 0).energy);
+  return (double*)((char*)c->order + order_layout(n).energy);
 }
 
 void order_free(Context* c) {
