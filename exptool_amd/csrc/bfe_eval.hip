@@ -13,12 +13,13 @@ struct SLFoldArgs {
   int lmax, nmax_use;
 };
 
-// One thread per (radial node i, (l, m) pair p in m-major order).  E, Dn and
-// G at i-1, i, i+1 are summed over n in float64 from the float64 scaled
-// tables S = [ef/sqrt(ev), ef sqrt(ev), ef p0/sqrt(ev)] ([l][i][n] each),
-// D1/D2 formed, and the 8-float block rounded once to float32.
-// grid.y runs over coefficient sets: set q reads coef + q * coef_stride and
-// writes out + q * out_stride.
+// One thread per (radial node i, (l, m) pair p in m-major order): E and Dn at
+// i and i+1 and G at c-1, c, c+1 (c = max(i, 1)) are summed over n in
+// float64 from the float64 scaled tables S = [ef/sqrt(ev), ef sqrt(ev),
+// ef p0/sqrt(ev)] ([l][i][n] each), D1/D2 formed, and the blocks rounded once
+// to float32 (layout in bfe_field.hpp; the last node, never a cell start,
+// is zero).  grid.y runs over coefficient sets: set q reads
+// coef + q * coef_stride and writes out + q * out_stride.
 __global__ __launch_bounds__(256) void sl_fold_kernel(const double* __restrict__ S,
                                                       const double* __restrict__ coef, int nmax,
                                                       int numr, SLFoldArgs a,
@@ -31,6 +32,13 @@ __global__ __launch_bounds__(256) void sl_fold_kernel(const double* __restrict__
   if (idx >= (int64_t)numr * np) return;
   const int i = (int)(idx / np);
   const int p = (int)(idx - (int64_t)i * np);
+  float* o = out + ((size_t)i * np + p) * 8;
+  float* od = out + (size_t)numr * np * 8 + ((size_t)i * np + p) * 4;
+  if (i == numr - 1) {
+    for (int k = 0; k < 8; ++k) o[k] = 0.f;
+    for (int k = 0; k < 4; ++k) od[k] = 0.f;
+    return;
+  }
   int m = 0, rem = p;
   while (rem > a.lmax - m) { rem -= a.lmax - m + 1; ++m; }
   const int l = m + rem;
@@ -43,40 +51,36 @@ __global__ __launch_bounds__(256) void sl_fold_kernel(const double* __restrict__
   const double* cc = coef + (size_t)(l * l + (m > 0 ? 2 * m - 1 : 0)) * nmax;
   const double* cs = coef + (size_t)(l * l + 2 * m) * nmax;
   const bool two = m > 0;
-  const bool inner = (i >= 1 && i <= numr - 2);
+  const int ctr = i == 0 ? 1 : i;
   const size_t plane = (size_t)(a.lmax + 1) * numr * nmax;
-  const double* E = S + ((size_t)l * numr + i) * nmax;
+  const double* E = S + ((size_t)l * numr + i) * nmax;   // rows i, i+1
   const double* D = E + plane;
-  const double* G = E + 2 * plane;
-  double v[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};  // E, Dn, Gm, G0, Gp
-#pragma unroll 4
+  const double* G = S + 2 * plane + ((size_t)l * numr + ctr) * nmax;  // rows c-1, c, c+1
+  // v[cs]: E_i, E_i+1, Dn_i, Dn_i+1, G_c-1, G_c, G_c+1
+  double v[2][7] = {{0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0}};
+#pragma unroll 2
   for (int n = 0; n < a.nmax_use; ++n) {
     const double c0 = cc[n] * mul;
     const double c1 = two ? cs[n] * mul : 0.0;
-    const double e = E[n], d = D[n], g0 = G[n];
-    v[0][0] += c0 * e;
-    v[1][0] += c1 * e;
-    v[0][1] += c0 * d;
-    v[1][1] += c1 * d;
-    v[0][3] += c0 * g0;
-    v[1][3] += c1 * g0;
-    if (inner) {
-      const double gm = G[n - nmax], gp = G[n + nmax];
-      v[0][2] += c0 * gm;
-      v[1][2] += c1 * gm;
-      v[0][4] += c0 * gp;
-      v[1][4] += c1 * gp;
+    const double t[7] = {E[n], E[n + nmax], D[n], D[n + nmax], G[n - nmax], G[n], G[n + nmax]};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      v[0][k] += c0 * t[k];
+      v[1][k] += c1 * t[k];
     }
   }
-  float* o = out + ((size_t)i * np + p) * 8;
   o[0] = (float)v[0][0];
   o[1] = (float)v[1][0];
   o[2] = (float)v[0][1];
   o[3] = (float)v[1][1];
-  o[4] = inner ? (float)(0.5 * (v[0][4] - v[0][2])) : 0.f;
-  o[5] = inner ? (float)(0.5 * (v[1][4] - v[1][2])) : 0.f;
-  o[6] = inner ? (float)(v[0][2] - 2.0 * v[0][3] + v[0][4]) : 0.f;
-  o[7] = inner ? (float)(v[1][2] - 2.0 * v[1][3] + v[1][4]) : 0.f;
+  o[4] = (float)(0.5 * (v[0][6] - v[0][4]));
+  o[5] = (float)(0.5 * (v[1][6] - v[1][4]));
+  o[6] = (float)(v[0][4] - 2.0 * v[0][5] + v[0][6]);
+  o[7] = (float)(v[1][4] - 2.0 * v[1][5] + v[1][6]);
+  od[0] = (float)v[0][2];
+  od[1] = (float)v[1][2];
+  od[2] = (float)v[0][3];
+  od[3] = (float)v[1][3];
 }
 
 struct EOFFoldArgs {
@@ -149,7 +153,7 @@ static int ensure_folded(Context* c, float** d, size_t* have, size_t floats, hip
   return 0;
 }
 
-static size_t sl_fold_floats(const Context* c) { return (size_t)c->numr * sl_npairs(c->lmax) * 8; }
+static size_t sl_fold_floats(const Context* c) { return (size_t)c->numr * sl_npairs(c->lmax) * 12; }
 static size_t eof_fold_floats(const Context* c) {
   return (size_t)(c->numx + 1) * (c->numy + 1) * (c->mmax + 1) * 8;
 }

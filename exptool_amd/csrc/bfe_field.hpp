@@ -19,13 +19,23 @@ struct EOFOut {
 };
 
 // Folded table layouts (float32, streamed as float4):
-//  SL : per radial node, one 8-float block per (l, m) pair in m-major order
-//       (m = 0..L, l = m..L):  [E_c, E_s, Dn_c, Dn_s, D1_c, D1_s, D2_c, D2_s]
-//       E = sum_n c N_lm ef/sqrt(ev), Dn = sum_n c N_lm ef sqrt(ev),
-//       D1 = (G_{i+1} - G_{i-1})/2, D2 = G_{i-1} - 2 G_i + G_{i+1},
-//       G = sum_n c N_lm ef p0/sqrt(ev)   (S entries are 0 for m = 0).
+//  SL : per radial node i, one 8-float block per (l, m) pair in m-major order
+//       (m = 0..L, l = m..L):
+//         [E_c(i), E_s(i), E_c(i+1), E_s(i+1), D1_c, D1_s, D2_c, D2_s]
+//       with D1, D2 at the stencil centre c = max(i, 1); after the numr
+//       nodes of these, the density blocks per node and pair:
+//         [Dn_c(i), Dn_s(i), Dn_c(i+1), Dn_s(i+1)].
+//       E = sum_n a N_lm ef/sqrt(ev), Dn = sum_n a N_lm ef sqrt(ev),
+//       D1 = (G_{c+1} - G_{c-1})/2, D2 = G_{c-1} - 2 G_c + G_{c+1},
+//       G = sum_n a N_lm ef p0/sqrt(ev)   (S entries are 0 for m = 0).
+//       Forces read two float4 per pair, both from node i.
 //  EOF: per grid node, one 8-float block per m = 0..M:
 //       [potC, rforceC, zforceC, densC, potS, rforceS, zforceS, densS].
+// SL table values are interpolated in float32 (they are stored in float32;
+// the weights are convex) and combined in float64.  EOF values are
+// interpolated in float64: far from the disc plane the mapped grid cells are
+// wide and the force is small against the corner values, where float32
+// interpolation gave force errors of 3e-5.
 __host__ __device__ inline int sl_npairs(int L) { return (L + 1) * (L + 2) / 2; }
 
 // SL evaluation at (r, cos theta, phi): get_halo_dens_pot_force
@@ -44,16 +54,16 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
   const SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
   const int i = cell.i;
   const double x1 = cell.x1, x2 = cell.x2;
+  const float x1f = (float)x1, x2f = (float)x2;
   const double P0 = x1 * g.p0[i] + x2 * g.p0[i + 1];
   const double D0 = DENS ? x1 * g.d0[i] + x2 * g.d0[i + 1] : 0.0;
   const double dxi = g.xi[1] - g.xi[0];
   const double fac = d_xi_to_r(cell.x, g.cmap, g.scale) / dxi;
-  const int ctr = (i == 0) ? 1 : i;
-  const double ps = (i == 0) ? ((flags & BFE_COMPAT_SL_EDGE) ? x2 : x2 - 1.0) : x2;
-  const int ns4 = 2 * sl_npairs(L);  // float4 per node
-  const float4* __restrict__ Ai = reinterpret_cast<const float4*>(F) + (size_t)i * ns4;
-  const float4* __restrict__ Ai1 = Ai + ns4;
-  const float4* __restrict__ Bc = reinterpret_cast<const float4*>(F) + (size_t)ctr * ns4 + 1;
+  const float ps = (float)((i == 0) ? ((flags & BFE_COMPAT_SL_EDGE) ? x2 : x2 - 1.0) : x2);
+  const int np = sl_npairs(L);
+  const float4* __restrict__ A = reinterpret_cast<const float4*>(F) + (size_t)i * 2 * np;
+  const float4* __restrict__ Nd =
+      reinterpret_cast<const float4*>(F + (size_t)g.numr * np * 8) + (size_t)i * np;
 
   const double x = costh;
   const bool good = (x == x);
@@ -81,8 +91,9 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
 #pragma unroll 1
     for (int l = m; l <= L; ++l, ++p) {
       if (l > m) {
-        const double pn = (l == m + 1) ? x * (2.0 * m + 1.0) * pmm
-                                       : (x * (2 * l - 1) * pl - (l + m - 1) * plm1) / (l - m);
+        // (x (2l-1) P_{l-1} - (l+m-1) P_{l-2}) / (l-m), constants precomputed
+        const int lm = lm_index(l, m);
+        const double pn = x * g.lmc[kLmcA + lm] * pl - g.lmc[kLmcB + lm] * plm1;
         plm1 = pl;
         pl = pn;
         if (cosl) {
@@ -90,13 +101,16 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
           clm1 = cl; slm1 = sl; cl = cn; sl = sn;
         }
       }
-      const float4 a0 = Ai[2 * p], a1 = Ai1[2 * p], b = Bc[2 * p];
-      const double ec = x1 * (double)a0.x + x2 * (double)a1.x;
-      const double dc = (double)b.x + ps * (double)b.z;
+      const float4 a = A[2 * p], b = A[2 * p + 1];
+      const double ec = (double)fmaf(x2f, a.z, x1f * a.x);
+      const double dc = (double)fmaf(ps, b.z, b.x);
       if (l == 0) {
         pot0 = ec;
         potr0 = dc;
-        if (DENS) den0 = x1 * (double)a0.z + x2 * (double)a1.z;
+        if (DENS) {
+          const float4 nd = Nd[p];
+          den0 = (double)fmaf(x2f, nd.z, x1f * nd.x);
+        }
         continue;
       }
       if (!good) continue;  // P_l^m = 0 for l > 0 when cos(theta) is NaN (spheresl.py:698)
@@ -106,10 +120,13 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
         pot1 += pl * ec;
         potr += pl * dc;
         pott += dpl * ec;
-        if (DENS) den1 += (dq ? x : pl) * (x1 * (double)a0.z + x2 * (double)a1.z);
+        if (DENS) {
+          const float4 nd = Nd[p];
+          den1 += (dq ? x : pl) * (double)fmaf(x2f, nd.z, x1f * nd.x);
+        }
       } else {
-        const double es = x1 * (double)a0.y + x2 * (double)a1.y;
-        const double ds = (double)b.y + ps * (double)b.w;
+        const double es = (double)fmaf(x2f, a.w, x1f * a.y);
+        const double ds = (double)fmaf(ps, b.w, b.y);
         const double ca = cosl ? cl : cmm, sa = cosl ? sl : smm;
         const double pa = ec * ca + es * sa;
         pot1 += pl * pa;
@@ -117,8 +134,9 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
         pott += dpl * pa;
         potp += pl * m * (-ec * sa + es * ca);
         if (DENS) {
-          const double nc = x1 * (double)a0.z + x2 * (double)a1.z;
-          const double ns = x1 * (double)a0.w + x2 * (double)a1.w;
+          const float4 nd = Nd[p];
+          const double nc = (double)fmaf(x2f, nd.z, x1f * nd.x);
+          const double ns = (double)fmaf(x2f, nd.w, x1f * nd.y);
           den1 += pl * (nc * ca + ns * sa);
         }
       }

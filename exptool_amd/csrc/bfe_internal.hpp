@@ -133,6 +133,7 @@ struct SLGeom {
   const double* d0;
   int numr, cmap, lmax, nmax;
   double scale;
+  const double* lmc;   // Context::sl_lmc
 };
 
 struct EOFGeom {
@@ -141,7 +142,8 @@ struct EOFGeom {
 };
 
 inline SLGeom sl_geom(const Context* c) {
-  return SLGeom{c->sl_xi, c->sl_p0, c->sl_d0, c->numr, c->cmap, c->lmax, c->nmax, c->scale};
+  return SLGeom{c->sl_xi, c->sl_p0, c->sl_d0, c->numr, c->cmap, c->lmax, c->nmax, c->scale,
+                c->sl_lmc};
 }
 inline EOFGeom eof_geom(const Context* c) {
   return EOFGeom{c->ecmap, c->numx, c->numy, c->mmax, c->norder,
