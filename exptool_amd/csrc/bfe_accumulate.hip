@@ -345,100 +345,132 @@ __global__ __launch_bounds__(256) void sl_accumulate_kernel(
 // ---------------------------------------------------------------------------
 // Cell-ordered accumulate (particles sorted by cell, packed as float4)
 // ---------------------------------------------------------------------------
-// Each wave works on its own 64-particle sub-tiles of the block's segment:
-// lane q computes particle q's geometry, keeps its cell and weights in
-// registers (read by the other lanes with readlane, so uniform per particle)
-// and writes its per-row weights to a wave-private LDS slab.  Lanes then own
-// (row, chunk of radial orders) pairs and walk the 64 particles in order; as
-// the particles are sorted by cell, consecutive particles share the table rows
-// they need, so the sums are regrouped exactly per run of equal cells:
+// Each wave works on its own 64-particle tiles of the block's segment: lane q
+// computes particle q's geometry and writes its per-row weights, its cell and
+// its cell weights to a wave-private LDS slab.  Lanes then own (row, chunk of
+// CH radial orders) slots and walk the particles in order; as the particles
+// are sorted by cell, consecutive particles share the table rows they need,
+// so the sums are regrouped exactly per run of equal cells:
 //   EOF: sum_p w_j(p) sum_c c_c(p) T_c[k] = sum_c T_c[k] (sum_p w_j(p) c_c(p))
 //   SL : sum_p Y_row(p) (x1 P0 E[i] + x2 P0 E[i+1])[ln]
 //        = E[i][ln] sum_p Y_row x1 P0 + E[i+1][ln] sum_p Y_row x2 P0
-// The inner sums run in float32 over at most kFlush particles and are applied
-// to the table entries in float64.  At the end of a segment the four waves'
-// float64 sums are added in wave order.
+// The per-particle sums run in float32 over at most kFlush particles and are
+// then added to float64 run sums; the run sums are applied to the table
+// entries (float64) when the cell changes and at the end of the segment, so
+// table rows are read once per run.  A run may continue across the wave's
+// tiles.  Per-particle values are read from LDS at wave-uniform addresses
+// (broadcast), four particles at a time; a group of four in the current cell
+// takes a branch-free path.  At the end of a segment the four waves' float64
+// sums are added in wave order.
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ float lane_f(float v, int q) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), q));
+constexpr int kDepStride = 68;     // LDS row stride (floats): 16-byte rows on distinct banks
+constexpr int kDepExtra = 5 * 64;  // EOF: per-wave LDS after the rows (corner weights, cells)
+// SL: rows of the per-wave LDS region (the padding of 48 rows holds three
+// per-particle arrays)
+__host__ __device__ constexpr int sl_dep_rows(int R) { return R > 48 ? R : 48; }
+
+// CH consecutive floats of a padded deposit-table row (16-byte aligned for
+// CH % 4 == 0, 8-byte for CH == 2).
+template <int CH>
+__device__ __forceinline__ void dep_row(const float* __restrict__ p, float (&v)[CH]) {
+  if constexpr (CH % 4 == 0) {
+#pragma unroll
+    for (int k = 0; k < CH / 4; ++k) {
+      const float4 q = reinterpret_cast<const float4*>(p)[k];
+      v[4 * k] = q.x;
+      v[4 * k + 1] = q.y;
+      v[4 * k + 2] = q.z;
+      v[4 * k + 3] = q.w;
+    }
+  } else {
+    static_assert(CH == 2, "chunk of 2 or a multiple of 4");
+    const float2 q = reinterpret_cast<const float2*>(p)[0];
+    v[0] = q.x;
+    v[1] = q.y;
+  }
 }
-__device__ __forceinline__ int lane_i(int v, int q) { return __builtin_amdgcn_readlane(v, q); }
 
-constexpr int kDepChunkE = 2;   // EOF: radial orders per lane pair
-constexpr int kDepChunkS = 4;   // SL : radial orders per lane pair
-constexpr int kDepStride = 68;  // LDS row stride (floats): 16-byte rows on distinct banks
-
-// Applies the corner sums G of the current cell (node base) to the lane's
-// coefficients (float64) and clears them.
-template <int PPT>
-__device__ __forceinline__ void eof_flush(const float* __restrict__ T, int base, int K, int ny1,
-                                          int N, const int (&j_)[PPT], const int (&n0_)[PPT],
-                                          float (&G)[PPT][4], double (&acc)[PPT][kDepChunkE]) {
+// Applies the float64 corner run sums GD of the cell at node base to the
+// lane's coefficients and clears them.  T: padded table [node][J][np].
+template <int PPT, int CH>
+__device__ __forceinline__ void eof_apply(const float* __restrict__ T, int base, int rowlen,
+                                          int ny1, int np, const int (&j_)[PPT],
+                                          const int (&n0_)[PPT], double (&GD)[PPT][4],
+                                          double (&acc)[PPT][CH]) {
 #pragma unroll
   for (int o = 0; o < PPT; ++o) {
     if (j_[o] >= 0) {
-      const float* T00 = T + (size_t)base * K + j_[o] * N + n0_[o];
-      const float* T10 = T00 + (size_t)ny1 * K;
+      const float* T00 = T + (size_t)base * rowlen + j_[o] * np + n0_[o];
+      float t00[CH], t10[CH], t01[CH], t11[CH];
+      dep_row<CH>(T00, t00);
+      dep_row<CH>(T00 + (size_t)ny1 * rowlen, t10);
+      dep_row<CH>(T00 + rowlen, t01);
+      dep_row<CH>(T00 + (size_t)(ny1 + 1) * rowlen, t11);
 #pragma unroll
-      for (int c = 0; c < kDepChunkE; ++c)
-        if (n0_[o] + c < N)
-          acc[o][c] += (double)G[o][0] * T00[c] + (double)G[o][1] * T10[c] +
-                       (double)G[o][2] * T00[K + c] + (double)G[o][3] * T10[K + c];
+      for (int c = 0; c < CH; ++c)
+        acc[o][c] += GD[o][0] * t00[c] + GD[o][1] * t10[c] + GD[o][2] * t01[c] + GD[o][3] * t11[c];
     }
-    G[o][0] = G[o][1] = G[o][2] = G[o][3] = 0.f;
-  }
-}
-
-// Applies the radial sums A1, A2 of cell i to the lane's coefficients.
-template <int PPT>
-__device__ __forceinline__ void sl_flush(const float* __restrict__ E, int i, int LN, int N,
-                                         const int (&row_)[PPT], const int (&ln0_)[PPT],
-                                         const int (&n0_)[PPT], float (&A1)[PPT], float (&A2)[PPT],
-                                         double (&acc)[PPT][kDepChunkS]) {
-#pragma unroll
-  for (int o = 0; o < PPT; ++o) {
-    if (row_[o] >= 0) {
-      const float* e1 = E + (size_t)i * LN + ln0_[o];
-#pragma unroll
-      for (int c = 0; c < kDepChunkS; ++c)
-        if (n0_[o] + c < N) acc[o][c] += (double)A1[o] * e1[c] + (double)A2[o] * e1[LN + c];
-    }
-    A1[o] = A2[o] = 0.f;
+    GD[o][0] = GD[o][1] = GD[o][2] = GD[o][3] = 0.0;
   }
 }
 
 template <int PPT>
+__device__ __forceinline__ void eof_to_run(float (&G)[PPT][4], double (&GD)[PPT][4]) {
+#pragma unroll
+  for (int o = 0; o < PPT; ++o)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      GD[o][k] += (double)G[o][k];
+      G[o][k] = 0.f;
+    }
+}
+
+template <int PPT, int CH>
 __global__ __launch_bounds__(256) void eof_deposit_kernel(
     const float4* __restrict__ P, Segments S, EOFGeom g, const float* __restrict__ T, int K,
     double* __restrict__ partials) {
+  const int rp = dep_np(g.norder);  // padded row length of T
+  const int rowlen = (2 * g.mmax + 1) * rp;
   extern __shared__ __align__(16) unsigned char dsm[];
   const int J = 2 * g.mmax + 1;
   const int N = g.norder;
-  const int NC = (N + kDepChunkE - 1) / kDepChunkE;
+  const int NC = (N + CH - 1) / CH;
   const int ny1 = g.numy + 1;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  float* sW = reinterpret_cast<float*>(dsm) + (size_t)wave * J * kDepStride;  // [J][stride] per wave
-  double* sRed = reinterpret_cast<double*>(dsm);                           // [4][K] after the tiles
+  // per wave: row weights [J][stride], then corner weights [4][64] and cells [64]
+  float* sW = reinterpret_cast<float*>(dsm) + (size_t)wave * (J * kDepStride + kDepExtra);
+  float* sE = sW + J * kDepStride;
+  int* sB = reinterpret_cast<int*>(sE + 4 * 64);
+  double* sRed = reinterpret_cast<double*>(dsm);  // [4][K] after the tiles
   int j_[PPT], n0_[PPT];
 #pragma unroll
   for (int o = 0; o < PPT; ++o) {
     const int pr = lane + 64 * o;
     j_[o] = pr / NC < J ? pr / NC : -1;
-    n0_[o] = (pr % NC) * kDepChunkE;
+    n0_[o] = (pr % NC) * CH;
   }
-  double acc[PPT][kDepChunkE];
+  double acc[PPT][CH];
   for (int64_t sg = blockIdx.x; sg < S.nseg; sg += gridDim.x) {
     int64_t s0, s1;
     segment_range(S, sg, s0, s1);
+    float G[PPT][4];
+    double GD[PPT][4];
 #pragma unroll
-    for (int o = 0; o < PPT; ++o)
+    for (int o = 0; o < PPT; ++o) {
 #pragma unroll
-      for (int c = 0; c < kDepChunkE; ++c) acc[o][c] = 0.0;
+      for (int c = 0; c < CH; ++c) acc[o][c] = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        G[o][k] = 0.f;
+        GD[o][k] = 0.0;
+      }
+    }
+    int cur = -1, cnt = 0;
     for (int64_t t0 = s0 + 64 * wave; t0 < s1; t0 += 256) {
       const int64_t p = t0 + lane;
       const bool live = p < s1;
@@ -457,6 +489,11 @@ __global__ __launch_bounds__(256) void eof_deposit_kernel(
         const double w = -kFourPi * mass;
         double cm = 1.0, sm = 0.0, cm1 = c1, sm1 = -s1v;
         sW[lane] = (float)w;
+        sE[lane] = c00;
+        sE[64 + lane] = c10;
+        sE[128 + lane] = c01;
+        sE[192 + lane] = c11;
+        sB[lane] = base;
         for (int m = 1; m <= g.mmax; ++m) {
           const double cn = (m == 1) ? c1 : 2.0 * c1 * cm - cm1;
           const double sn = (m == 1) ? s1v : 2.0 * c1 * sm - sm1;
@@ -468,42 +505,73 @@ __global__ __launch_bounds__(256) void eof_deposit_kernel(
       wave_sync();
       const int64_t left = s1 - t0;
       const int np = left < 64 ? (int)left : 64;
-      // walk the particles in order; per particle the cell and corner weights
-      // are read from its lane (uniform), the row weights 4 particles at a time
-      float G[PPT][4];
-#pragma unroll
-      for (int o = 0; o < PPT; ++o) G[o][0] = G[o][1] = G[o][2] = G[o][3] = 0.f;
-      int cur = lane_i(base, 0), cnt = 0;
+      if (cur < 0) cur = sB[0];
+#pragma unroll 1
       for (int q0 = 0; q0 < np; q0 += 4) {
-        float4 wv[PPT];
-#pragma unroll
-        for (int o = 0; o < PPT; ++o)
-          wv[o] = *reinterpret_cast<const float4*>(sW + (j_[o] < 0 ? 0 : j_[o]) * kDepStride + q0);
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-          const int q = q0 + jj;
-          if (q >= np) break;
-          const int b = lane_i(base, q);
-          if (b != cur || cnt == kFlush) {
-            eof_flush<PPT>(T, cur, K, ny1, N, j_, n0_, G, acc);
-            cur = b;
+        const int4 bb = *reinterpret_cast<const int4*>(sB + q0);
+        const bool run = q0 + 4 <= np && bb.x == cur && bb.y == cur && bb.z == cur && bb.w == cur;
+        if (__builtin_amdgcn_readfirstlane((int)run)) {
+          if (cnt + 4 > kFlush) {
+            eof_to_run<PPT>(G, GD);
             cnt = 0;
           }
-          const float e00 = lane_f(c00, q), e10 = lane_f(c10, q), e01 = lane_f(c01, q),
-                      e11 = lane_f(c11, q);
+          const float4 e00 = *reinterpret_cast<const float4*>(sE + q0);
+          const float4 e10 = *reinterpret_cast<const float4*>(sE + 64 + q0);
+          const float4 e01 = *reinterpret_cast<const float4*>(sE + 128 + q0);
+          const float4 e11 = *reinterpret_cast<const float4*>(sE + 192 + q0);
 #pragma unroll
           for (int o = 0; o < PPT; ++o) {
-            const float w = jj == 0 ? wv[o].x : jj == 1 ? wv[o].y : jj == 2 ? wv[o].z : wv[o].w;
-            G[o][0] = fmaf(w, e00, G[o][0]);
-            G[o][1] = fmaf(w, e10, G[o][1]);
-            G[o][2] = fmaf(w, e01, G[o][2]);
-            G[o][3] = fmaf(w, e11, G[o][3]);
+            const float4 wv = *reinterpret_cast<const float4*>(sW + (j_[o] < 0 ? 0 : j_[o]) * kDepStride + q0);
+            G[o][0] = fmaf(wv.x, e00.x, G[o][0]);
+            G[o][1] = fmaf(wv.x, e10.x, G[o][1]);
+            G[o][2] = fmaf(wv.x, e01.x, G[o][2]);
+            G[o][3] = fmaf(wv.x, e11.x, G[o][3]);
+            G[o][0] = fmaf(wv.y, e00.y, G[o][0]);
+            G[o][1] = fmaf(wv.y, e10.y, G[o][1]);
+            G[o][2] = fmaf(wv.y, e01.y, G[o][2]);
+            G[o][3] = fmaf(wv.y, e11.y, G[o][3]);
+            G[o][0] = fmaf(wv.z, e00.z, G[o][0]);
+            G[o][1] = fmaf(wv.z, e10.z, G[o][1]);
+            G[o][2] = fmaf(wv.z, e01.z, G[o][2]);
+            G[o][3] = fmaf(wv.z, e11.z, G[o][3]);
+            G[o][0] = fmaf(wv.w, e00.w, G[o][0]);
+            G[o][1] = fmaf(wv.w, e10.w, G[o][1]);
+            G[o][2] = fmaf(wv.w, e01.w, G[o][2]);
+            G[o][3] = fmaf(wv.w, e11.w, G[o][3]);
+          }
+          cnt += 4;
+          continue;
+        }
+        // one particle at a time: cell changes
+#pragma unroll 1
+        for (int q = q0; q < q0 + 4 && q < np; ++q) {
+          const int b = sB[q];
+          if (b != cur) {
+            eof_to_run<PPT>(G, GD);
+            eof_apply<PPT, CH>(T, cur, rowlen, ny1, rp, j_, n0_, GD, acc);
+            cur = b;
+            cnt = 0;
+          } else if (cnt == kFlush) {
+            eof_to_run<PPT>(G, GD);
+            cnt = 0;
+          }
+          const float f00 = sE[q], f10 = sE[64 + q], f01 = sE[128 + q], f11 = sE[192 + q];
+#pragma unroll
+          for (int o = 0; o < PPT; ++o) {
+            const float w = sW[(j_[o] < 0 ? 0 : j_[o]) * kDepStride + q];
+            G[o][0] = fmaf(w, f00, G[o][0]);
+            G[o][1] = fmaf(w, f10, G[o][1]);
+            G[o][2] = fmaf(w, f01, G[o][2]);
+            G[o][3] = fmaf(w, f11, G[o][3]);
           }
           ++cnt;
         }
       }
-      eof_flush<PPT>(T, cur, K, ny1, N, j_, n0_, G, acc);
       wave_sync();
+    }
+    if (cur >= 0) {
+      eof_to_run<PPT>(G, GD);
+      eof_apply<PPT, CH>(T, cur, rowlen, ny1, rp, j_, n0_, GD, acc);
     }
     // fixed-order sum of the four waves' partial sums
     __syncthreads();
@@ -511,7 +579,7 @@ __global__ __launch_bounds__(256) void eof_deposit_kernel(
     for (int o = 0; o < PPT; ++o)
       if (j_[o] >= 0)
 #pragma unroll
-        for (int c = 0; c < kDepChunkE; ++c)
+        for (int c = 0; c < CH; ++c)
           if (n0_[o] + c < N) sRed[(size_t)wave * K + j_[o] * N + n0_[o] + c] = acc[o][c];
     __syncthreads();
     for (int k = threadIdx.x; k < K; k += 256)
@@ -520,7 +588,39 @@ __global__ __launch_bounds__(256) void eof_deposit_kernel(
   }
 }
 
+// Applies the float64 radial run sums D1, D2 of cell i to the lane's
+// coefficients and clears them.  E: padded table [node][lmax + 1][np].
+template <int PPT, int CH>
+__device__ __forceinline__ void sl_apply(const float* __restrict__ E, int i, int rowlen, int np,
+                                         const int (&row_)[PPT], const int (&l_)[PPT],
+                                         const int (&n0_)[PPT], double (&D1)[PPT],
+                                         double (&D2)[PPT], double (&acc)[PPT][CH]) {
+#pragma unroll
+  for (int o = 0; o < PPT; ++o) {
+    if (row_[o] >= 0) {
+      const float* e1 = E + (size_t)i * rowlen + l_[o] * np + n0_[o];
+      float a[CH], b[CH];
+      dep_row<CH>(e1, a);
+      dep_row<CH>(e1 + rowlen, b);
+#pragma unroll
+      for (int c = 0; c < CH; ++c) acc[o][c] += D1[o] * a[c] + D2[o] * b[c];
+    }
+    D1[o] = D2[o] = 0.0;
+  }
+}
+
 template <int PPT>
+__device__ __forceinline__ void sl_to_run(float (&A1)[PPT], float (&A2)[PPT], double (&D1)[PPT],
+                                          double (&D2)[PPT]) {
+#pragma unroll
+  for (int o = 0; o < PPT; ++o) {
+    D1[o] += (double)A1[o];
+    D2[o] += (double)A2[o];
+    A1[o] = A2[o] = 0.f;
+  }
+}
+
+template <int PPT, int CH>
 __global__ __launch_bounds__(256) void sl_deposit_kernel(
     const float4* __restrict__ P, Segments S, SLGeom g, const float* __restrict__ E,
     const double* __restrict__ lmc, int no_odd, double* __restrict__ partials) {
@@ -529,11 +629,17 @@ __global__ __launch_bounds__(256) void sl_deposit_kernel(
   const int LN = (g.lmax + 1) * g.nmax;
   const int K = R * g.nmax;
   const int N = g.nmax;
-  const int NC = (N + kDepChunkS - 1) / kDepChunkS;
+  const int NC = (N + CH - 1) / CH;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  float* sY = reinterpret_cast<float*>(dsm) + (size_t)wave * R * kDepStride;  // [R][stride] per wave
-  double* sRed = reinterpret_cast<double*>(dsm);                            // [4][K] after the tiles
-  int row_[PPT], ln0_[PPT], n0_[PPT];
+  // per wave: Y rows [max(R, 48)][stride]; the per-particle radial weights
+  // and cells live in the row padding (columns 64..67): value q of array k
+  // at row 16 k + q / 4, column 64 + q % 4, so four particles are one float4
+  float* sY = reinterpret_cast<float*>(dsm) + (size_t)wave * sl_dep_rows(R) * kDepStride;
+  float* sP = sY + 64;
+  double* sRed = reinterpret_cast<double*>(dsm);  // [4][K] after the tiles
+  const int rp = dep_np(N);  // padded row length of E
+  const int rowlen = (g.lmax + 1) * rp;
+  int row_[PPT], l_[PPT], n0_[PPT];
 #pragma unroll
   for (int o = 0; o < PPT; ++o) {
     const int pr = lane + 64 * o;
@@ -541,17 +647,23 @@ __global__ __launch_bounds__(256) void sl_deposit_kernel(
     row_[o] = row < R ? row : -1;
     int l = 0;
     while ((l + 1) * (l + 1) <= row) ++l;
-    n0_[o] = (pr % NC) * kDepChunkS;
-    ln0_[o] = l * N + n0_[o];
+    n0_[o] = (pr % NC) * CH;
+    l_[o] = l;
   }
-  double acc[PPT][kDepChunkS];
+  double acc[PPT][CH];
   for (int64_t sg = blockIdx.x; sg < S.nseg; sg += gridDim.x) {
     int64_t s0, s1;
     segment_range(S, sg, s0, s1);
+    float A1[PPT], A2[PPT];
+    double D1[PPT], D2[PPT];
 #pragma unroll
-    for (int o = 0; o < PPT; ++o)
+    for (int o = 0; o < PPT; ++o) {
 #pragma unroll
-      for (int c = 0; c < kDepChunkS; ++c) acc[o][c] = 0.0;
+      for (int c = 0; c < CH; ++c) acc[o][c] = 0.0;
+      A1[o] = A2[o] = 0.f;
+      D1[o] = D2[o] = 0.0;
+    }
+    int cur = -1, cnt = 0;
     for (int64_t t0 = s0 + 64 * wave; t0 < s1; t0 += 256) {
       const int64_t p = t0 + lane;
       const bool live = p < s1;
@@ -570,6 +682,10 @@ __global__ __launch_bounds__(256) void sl_deposit_kernel(
         ci = cell.i;
         w1 = live ? (float)(cell.x1 * P0) : 0.f;
         w2 = live ? (float)(cell.x2 * P0) : 0.f;
+        const int pad = (lane >> 2) * kDepStride + (lane & 3);
+        sP[pad] = w1;
+        sP[16 * kDepStride + pad] = w2;
+        reinterpret_cast<int*>(sP)[32 * kDepStride + pad] = ci;
         // Y rows: P_m^m in closed form, legendre_R recurrence in l
         // (spheresl.py:678-696), e^{i m phi} by the Chebyshev recurrence
         const bool bad = !(costh == costh);
@@ -604,44 +720,70 @@ __global__ __launch_bounds__(256) void sl_deposit_kernel(
       wave_sync();
       const int64_t left = s1 - t0;
       const int np = left < 64 ? (int)left : 64;
-      float A1[PPT], A2[PPT];
-#pragma unroll
-      for (int o = 0; o < PPT; ++o) A1[o] = A2[o] = 0.f;
-      int cur = lane_i(ci, 0), cnt = 0;
+      if (cur < 0) cur = reinterpret_cast<const int*>(sP)[32 * kDepStride];
+#pragma unroll 1
       for (int q0 = 0; q0 < np; q0 += 4) {
-        float4 yv[PPT];
-#pragma unroll
-        for (int o = 0; o < PPT; ++o)
-          yv[o] = *reinterpret_cast<const float4*>(sY + (row_[o] < 0 ? 0 : row_[o]) * kDepStride + q0);
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-          const int q = q0 + jj;
-          if (q >= np) break;
-          const int i = lane_i(ci, q);
-          if (i != cur || cnt == kFlush) {
-            sl_flush<PPT>(E, cur, LN, N, row_, ln0_, n0_, A1, A2, acc);
-            cur = i;
+        const float* pq = sP + (q0 >> 2) * kDepStride;
+        const int4 cc = *reinterpret_cast<const int4*>(pq + 32 * kDepStride);
+        const bool run = q0 + 4 <= np && cc.x == cur && cc.y == cur && cc.z == cur && cc.w == cur;
+        if (__builtin_amdgcn_readfirstlane((int)run)) {
+          if (cnt + 4 > kFlush) {
+            sl_to_run<PPT>(A1, A2, D1, D2);
             cnt = 0;
           }
-          const float a = lane_f(w1, q), b = lane_f(w2, q);
+          const float4 wa = *reinterpret_cast<const float4*>(pq);
+          const float4 wb = *reinterpret_cast<const float4*>(pq + 16 * kDepStride);
 #pragma unroll
           for (int o = 0; o < PPT; ++o) {
-            const float yq = jj == 0 ? yv[o].x : jj == 1 ? yv[o].y : jj == 2 ? yv[o].z : yv[o].w;
+            const float4 yv = *reinterpret_cast<const float4*>(sY + (row_[o] < 0 ? 0 : row_[o]) * kDepStride + q0);
+            A1[o] = fmaf(yv.x, wa.x, A1[o]);
+            A2[o] = fmaf(yv.x, wb.x, A2[o]);
+            A1[o] = fmaf(yv.y, wa.y, A1[o]);
+            A2[o] = fmaf(yv.y, wb.y, A2[o]);
+            A1[o] = fmaf(yv.z, wa.z, A1[o]);
+            A2[o] = fmaf(yv.z, wb.z, A2[o]);
+            A1[o] = fmaf(yv.w, wa.w, A1[o]);
+            A2[o] = fmaf(yv.w, wb.w, A2[o]);
+          }
+          cnt += 4;
+          continue;
+        }
+        // one particle at a time: cell changes
+#pragma unroll 1
+        for (int q = q0; q < q0 + 4 && q < np; ++q) {
+          const float* pj = sP + (q >> 2) * kDepStride + (q & 3);
+          const int i = reinterpret_cast<const int*>(pj)[32 * kDepStride];
+          if (i != cur) {
+            sl_to_run<PPT>(A1, A2, D1, D2);
+            sl_apply<PPT, CH>(E, cur, rowlen, rp, row_, l_, n0_, D1, D2, acc);
+            cur = i;
+            cnt = 0;
+          } else if (cnt == kFlush) {
+            sl_to_run<PPT>(A1, A2, D1, D2);
+            cnt = 0;
+          }
+          const float a = pj[0], b = pj[16 * kDepStride];
+#pragma unroll
+          for (int o = 0; o < PPT; ++o) {
+            const float yq = sY[(row_[o] < 0 ? 0 : row_[o]) * kDepStride + q];
             A1[o] = fmaf(yq, a, A1[o]);
             A2[o] = fmaf(yq, b, A2[o]);
           }
           ++cnt;
         }
       }
-      sl_flush<PPT>(E, cur, LN, N, row_, ln0_, n0_, A1, A2, acc);
       wave_sync();
+    }
+    if (cur >= 0) {
+      sl_to_run<PPT>(A1, A2, D1, D2);
+      sl_apply<PPT, CH>(E, cur, rowlen, rp, row_, l_, n0_, D1, D2, acc);
     }
     __syncthreads();
 #pragma unroll
     for (int o = 0; o < PPT; ++o)
       if (row_[o] >= 0)
 #pragma unroll
-        for (int c = 0; c < kDepChunkS; ++c)
+        for (int c = 0; c < CH; ++c)
           if (n0_[o] + c < N) sRed[(size_t)wave * K + row_[o] * N + n0_[o] + c] = acc[o][c];
     __syncthreads();
     for (int k = threadIdx.x; k < K; k += 256)
@@ -927,6 +1069,23 @@ static int check_offsets(Context* c, const int64_t* off, int nsnap, const char* 
   return 0;
 }
 
+// (slots per lane, radial orders per slot) of the cell-ordered kernels.  The
+// shape must give rows * ceil(N / CH) slots <= 64 * PPT.  With a fitting
+// tuned index it is used; otherwise the first fitting shape of the list
+// (ordered by preference).  {0, 0}: none fits.
+struct DepShape {
+  int ppt, ch;
+};
+static bool dep_fits(int rows, int N, DepShape d) { return rows * ((N + d.ch - 1) / d.ch) <= 64 * d.ppt; }
+static DepShape dep_shape(int rows, int N, const DepShape* cands, int ncand, int64_t tuned) {
+  if (tuned >= 0 && tuned < ncand && dep_fits(rows, N, cands[tuned])) return cands[tuned];
+  for (int k = 0; k < ncand; ++k)
+    if (dep_fits(rows, N, cands[k])) return cands[k];
+  return DepShape{0, 0};
+}
+static const DepShape kEofShapes[kEofDepShapes] = {{1, 4}, {1, 8}, {1, 16}, {2, 2}, {2, 4}, {2, 8}, {2, 16}, {4, 8}};
+static const DepShape kSlShapes[kSlDepShapes] = {{1, 8}, {1, 16}, {1, 24}, {2, 8}, {2, 16}, {2, 24}, {4, 16}};
+
 static int eof_accumulate_run(Context* c, const float* x, const float* y, const float* z,
                               const float* m, int64_t n, const int64_t* off, int nsnap,
                               double* cos_out, double* sin_out, hipStream_t s) {
@@ -950,26 +1109,24 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
   EOFGeom g = eof_geom(c);
   const int opt = (K + 255) / 256;
   const float4* sorted = nullptr;
-  // cell-ordered form: (2 mmax + 1) x ceil(norder / 2) lane pairs, LDS for the
-  // per-wave weights and the float64 wave reduction
-  const size_t dlds = std::max((size_t)4 * (2 * c->mmax + 1) * kDepStride * sizeof(float),
+  // cell-ordered form: (2 mmax + 1) rows x chunks of radial orders per lane,
+  // LDS for the per-wave weights and the float64 wave reduction
+  const size_t dlds = std::max((size_t)4 * ((2 * c->mmax + 1) * kDepStride + kDepExtra) * sizeof(float),
                                (size_t)4 * K * sizeof(double));
-  const int dpairs = (2 * c->mmax + 1) * ((c->norder + kDepChunkE - 1) / kDepChunkE);
-  if (n >= c->order_min && n > 0 && dpairs <= 512 && dlds <= 64 * 1024) {
+  const DepShape dsh = dep_shape(2 * c->mmax + 1, c->norder, kEofShapes, kEofDepShapes, c->eof_dep_shape);
+  if (n >= c->order_min && n > 0 && dsh.ppt > 0 && dlds <= 64 * 1024) {
     rc = order_particles(c, 0, x, y, z, m, n, off_dev, nsnap, s, &sorted);
     if (rc) return rc;
   }
   timer_begin(c, "eof_accumulate", s);
   if (sorted) {
-    const int J = 2 * c->mmax + 1;
-    const int pairs = J * ((c->norder + kDepChunkE - 1) / kDepChunkE);
-    const size_t lds = std::max((size_t)4 * J * kDepStride * sizeof(float), (size_t)4 * K * sizeof(double));
-    if (pairs <= 128)
-      hipLaunchKernelGGL(eof_deposit_kernel<2>, dim3(P.grid), dim3(256), lds, s, sorted, P.S, g, c->eof_pot32, K, partials);
-    else if (pairs <= 256)
-      hipLaunchKernelGGL(eof_deposit_kernel<4>, dim3(P.grid), dim3(256), lds, s, sorted, P.S, g, c->eof_pot32, K, partials);
-    else
-      hipLaunchKernelGGL(eof_deposit_kernel<8>, dim3(P.grid), dim3(256), lds, s, sorted, P.S, g, c->eof_pot32, K, partials);
+#define BFE_EOF_DEP(PP, CC)                                                                   \
+  else if (dsh.ppt == PP && dsh.ch == CC) hipLaunchKernelGGL((eof_deposit_kernel<PP, CC>),    \
+      dim3(P.grid), dim3(256), dlds, s, sorted, P.S, g, c->eof_dep32, K, partials);
+    if (false) {}
+    BFE_EOF_DEP(1, 4) BFE_EOF_DEP(1, 8) BFE_EOF_DEP(1, 16) BFE_EOF_DEP(2, 2) BFE_EOF_DEP(2, 4)
+    BFE_EOF_DEP(2, 8) BFE_EOF_DEP(2, 16) BFE_EOF_DEP(4, 8)
+#undef BFE_EOF_DEP
   } else if (n == 0) {
     hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((P.S.nseg * K + 255) / 256)), dim3(256), 0, s,
                        partials, P.S.nseg * (int64_t)K);
@@ -1036,23 +1193,25 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
     BFE_HIP(c, hipGetLastError());
     return 0;
   }
-  // cell-ordered form: (lmax+1)^2 x ceil(nmax / 4) lane pairs per wave, LDS
-  // for the per-wave Y rows and the float64 wave reduction
-  const int pairs = R * ((c->nmax + kDepChunkS - 1) / kDepChunkS);
-  const size_t dlds = std::max((size_t)4 * R * kDepStride * sizeof(float), (size_t)4 * K * sizeof(double));
+  // cell-ordered form: (lmax+1)^2 rows x chunks of radial orders per lane,
+  // LDS for the per-wave Y rows and the float64 wave reduction
+  const DepShape dsh = dep_shape(R, c->nmax, kSlShapes, kSlDepShapes, c->sl_dep_shape);
+  const size_t dlds = std::max((size_t)4 * sl_dep_rows(R) * kDepStride * sizeof(float),
+                               (size_t)4 * K * sizeof(double));
   const float4* sorted = nullptr;
-  if (n >= c->order_min && n > 0 && pairs <= 512 && dlds <= 64 * 1024) {
+  if (n >= c->order_min && n > 0 && dsh.ppt > 0 && dlds <= 64 * 1024) {
     rc = order_particles(c, 1, x, y, z, m, n, off_dev, nsnap, s, &sorted);
     if (rc) return rc;
   }
   timer_begin(c, "sl_accumulate", s);
   if (sorted) {
-    if (pairs <= 128)
-      hipLaunchKernelGGL(sl_deposit_kernel<2>, dim3(P.grid), dim3(256), dlds, s, sorted, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
-    else if (pairs <= 256)
-      hipLaunchKernelGGL(sl_deposit_kernel<4>, dim3(P.grid), dim3(256), dlds, s, sorted, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
-    else
-      hipLaunchKernelGGL(sl_deposit_kernel<8>, dim3(P.grid), dim3(256), dlds, s, sorted, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
+#define BFE_SL_DEP(PP, CC)                                                                    \
+  else if (dsh.ppt == PP && dsh.ch == CC) hipLaunchKernelGGL((sl_deposit_kernel<PP, CC>),     \
+      dim3(P.grid), dim3(256), dlds, s, sorted, P.S, g, c->sl_dep32, c->sl_lmc, no_odd, partials);
+    if (false) {}
+    BFE_SL_DEP(1, 8) BFE_SL_DEP(1, 16) BFE_SL_DEP(1, 24) BFE_SL_DEP(2, 8) BFE_SL_DEP(2, 16)
+    BFE_SL_DEP(2, 24) BFE_SL_DEP(4, 16)
+#undef BFE_SL_DEP
   } else if (n == 0) {
     hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((P.S.nseg * K + 255) / 256)), dim3(256), 0, s,
                        partials, P.S.nseg * (int64_t)K);

@@ -57,7 +57,8 @@ static void free_ptr(void* p) {
 
 static void free_sl(Context* c) {
   free_ptr(c->sl_xi); free_ptr(c->sl_p0); free_ptr(c->sl_d0);
-  free_ptr(c->sl_fold64); free_ptr(c->sl_e32); free_ptr(c->sl_lmc);
+  free_ptr(c->sl_fold64); free_ptr(c->sl_e32); free_ptr(c->sl_dep32); free_ptr(c->sl_lmc);
+  c->sl_dep32 = nullptr;
   c->sl_xi = c->sl_p0 = c->sl_d0 = c->sl_fold64 = nullptr;
   c->sl_e32 = nullptr;
   c->sl_lmc = nullptr;
@@ -68,7 +69,9 @@ static void free_sl(Context* c) {
 static void free_eof(Context* c) {
   for (auto& p : c->eof64) { free_ptr(p); p = nullptr; }
   free_ptr(c->eof_pot32);
+  free_ptr(c->eof_dep32);
   c->eof_pot32 = nullptr;
+  c->eof_dep32 = nullptr;
   for (auto& f : c->eoff) { free_ptr(f.d); f = EOFFolded(); }
   c->eof_ok = false;
 }
@@ -210,6 +213,15 @@ extern "C" int bfe_sl_tables_set(bfe_ctx* h, int lmax, int nmax, int numr, int c
   if ((rc = dev_upload(c, &c->sl_d0, d0, numr))) return rc;
   if ((rc = dev_upload(c, &c->sl_fold64, fold64.data(), fold64.size()))) return rc;
   if ((rc = dev_upload(c, &c->sl_e32, e32.data(), e32.size()))) return rc;
+  {
+    const int np = dep_np(nmax);
+    std::vector<float> d32((size_t)numr * (lmax + 1) * np, 0.f);
+    for (int i = 0; i < numr; ++i)
+      for (int l = 0; l <= lmax; ++l)
+        for (int n = 0; n < nmax; ++n)
+          d32[((size_t)i * (lmax + 1) + l) * np + n] = e32[(size_t)i * LN + (size_t)l * nmax + n];
+    if ((rc = dev_upload(c, &c->sl_dep32, d32.data(), d32.size()))) return rc;
+  }
   c->h_xi.assign(xi, xi + numr);
   c->lmax = lmax; c->nmax = nmax; c->numr = numr; c->cmap = cmap; c->scale = scale;
   c->sl_ok = true;
@@ -265,6 +277,13 @@ extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, in
         for (size_t node = 0; node < nodes; ++node)
           pot32[node * K + k] = (float)buf[(size_t)k * nodes + node];
       if ((rc = dev_upload(c, &c->eof_pot32, pot32.data(), pot32.size()))) return rc;
+      const int J = 2 * mmax + 1, np = dep_np(norder);
+      std::vector<float> d32(nodes * J * np, 0.f);
+      for (size_t node = 0; node < nodes; ++node)
+        for (int j = 0; j < J; ++j)
+          for (int n = 0; n < norder; ++n)
+            d32[(node * J + j) * np + n] = pot32[node * K + (size_t)j * norder + n];
+      if ((rc = dev_upload(c, &c->eof_dep32, d32.data(), d32.size()))) return rc;
     }
   }
   c->mmax = mmax; c->norder = norder; c->numx = numx; c->numy = numy; c->ecmap = cmap;
@@ -293,6 +312,12 @@ extern "C" int bfe_set_tuning(bfe_ctx* h, const char* name, int64_t value) {
   } else if (!strcmp(name, "sl_acc_path")) {
     if (value < 0 || value > 2) return set_error(c, BFE_EINVAL, "sl_acc_path outside [0, 2]");
     c->sl_acc_path = value;
+  } else if (!strcmp(name, "eof_dep_shape") || !strcmp(name, "sl_dep_shape")) {
+    const bool e = name[0] == 'e';
+    const int64_t count = e ? kEofDepShapes : kSlDepShapes;
+    if (value < -1 || value >= count)
+      return set_error(c, BFE_EINVAL, "%s outside [-1, %lld]", name, (long long)(count - 1));
+    (e ? c->eof_dep_shape : c->sl_dep_shape) = value;
   } else {
     return set_error(c, BFE_EINVAL, "unknown tuning parameter '%s'", name);
   }
@@ -309,6 +334,8 @@ extern "C" int bfe_get_tuning(const bfe_ctx* h, const char* name, int64_t* value
   else if (!strcmp(name, "order_min")) *value = c->order_min;
   else if (!strcmp(name, "kernel_timing")) *value = c->kernel_timing ? 1 : 0;
   else if (!strcmp(name, "sl_acc_path")) *value = c->sl_acc_path;
+  else if (!strcmp(name, "eof_dep_shape")) *value = c->eof_dep_shape;
+  else if (!strcmp(name, "sl_dep_shape")) *value = c->sl_dep_shape;
   else return BFE_EINVAL;
   return 0;
 }

@@ -47,28 +47,16 @@ __host__ __device__ inline int sl_npairs(int L) { return (L + 1) * (L + 2) / 2; 
 //   flags: BFE_COMPAT_SL_EDGE  first-cell stencil one cell outward (:150-153)
 //          BFE_COMPAT_SL_COS_L cos/sin(l phi) for 0 < m (:1222-1225)
 //          BFE_COMPAT_SL_DENS  den1 uses P_1^0 for m = 0 (:1323)
-template <bool DENS>
-__device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __restrict__ F,
-                                               int L, double r, double costh, double c1,
-                                               double s1, uint32_t flags, SLOut& o) {
-  const SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
-  const int i = cell.i;
-  const double x1 = cell.x1, x2 = cell.x2;
-  const float x1f = (float)x1, x2f = (float)x2;
-  const double P0 = x1 * g.p0[i] + x2 * g.p0[i + 1];
-  const double D0 = DENS ? x1 * g.d0[i] + x2 * g.d0[i + 1] : 0.0;
-  const double dxi = g.xi[1] - g.xi[0];
-  const double fac = d_xi_to_r(cell.x, g.cmap, g.scale) / dxi;
-  const float ps = (float)((i == 0) ? ((flags & BFE_COMPAT_SL_EDGE) ? x2 : x2 - 1.0) : x2);
-  const int np = sl_npairs(L);
-  const float4* __restrict__ A = reinterpret_cast<const float4*>(F) + (size_t)i * 2 * np;
-  const float4* __restrict__ Nd =
-      reinterpret_cast<const float4*>(F + (size_t)g.numr * np * 8) + (size_t)i * np;
-
-  const double x = costh;
+// Angular sums of sl_eval_folded.  Without COSL the azimuthal factors
+// cos/sin(m phi) do not depend on l, so the l sums are taken per m first
+// (sum_l P_l^m E, sum_l P_l^m D, sum_l dP_l^m E for cos and sin) and
+// combined with the factors once per m.
+template <bool DENS, bool COSL>
+__device__ __forceinline__ void sl_angular(const SLGeom& g, const float4* __restrict__ A,
+                                           const float4* __restrict__ Nd, int L, float x1f,
+                                           float x2f, float ps, double x, double c1, double s1,
+                                           bool dq, double* acc) {
   const bool good = (x == x);
-  const bool cosl = (flags & BFE_COMPAT_SL_COS_L) != 0;
-  const bool dq = (flags & BFE_COMPAT_SL_DENS) != 0;
   double xc = x;  // dlegendre_R clamps x for the derivative only
   if (good && 1.0 - fabs(xc) < 1.e-8) xc = (xc > 0) ? 1.0 - 1.e-8 : -(1.0 - 1.e-8);
   const double somx2d = 1.0 / (xc * xc - 1.0);
@@ -88,6 +76,7 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
     }
     double plm1 = 0.0, pl = pmm;
     double cl = cmm, sl = smm, clm1 = cmm1, slm1 = smm1;
+    double Pc = 0, Ps = 0, Rc = 0, Rs = 0, Tc = 0, Ts = 0, Nc = 0, Ns = 0;
 #pragma unroll 1
     for (int l = m; l <= L; ++l, ++p) {
       if (l > m) {
@@ -96,7 +85,7 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
         const double pn = x * g.lmc[kLmcA + lm] * pl - g.lmc[kLmcB + lm] * plm1;
         plm1 = pl;
         pl = pn;
-        if (cosl) {
+        if (COSL) {
           const double cn = 2.0 * c1 * cl - clm1, sn = 2.0 * c1 * sl - slm1;
           clm1 = cl; slm1 = sl; cl = cn; sl = sn;
         }
@@ -124,31 +113,82 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
           const float4 nd = Nd[p];
           den1 += (dq ? x : pl) * (double)fmaf(x2f, nd.z, x1f * nd.x);
         }
-      } else {
-        const double es = (double)fmaf(x2f, a.w, x1f * a.y);
-        const double ds = (double)fmaf(ps, b.w, b.y);
-        const double ca = cosl ? cl : cmm, sa = cosl ? sl : smm;
-        const double pa = ec * ca + es * sa;
+        continue;
+      }
+      const double es = (double)fmaf(x2f, a.w, x1f * a.y);
+      const double ds = (double)fmaf(ps, b.w, b.y);
+      if (COSL) {
+        const double pa = ec * cl + es * sl;
         pot1 += pl * pa;
-        potr += pl * (dc * ca + ds * sa);
+        potr += pl * (dc * cl + ds * sl);
         pott += dpl * pa;
-        potp += pl * m * (-ec * sa + es * ca);
-        if (DENS) {
-          const float4 nd = Nd[p];
-          const double nc = (double)fmaf(x2f, nd.z, x1f * nd.x);
-          const double ns = (double)fmaf(x2f, nd.w, x1f * nd.y);
-          den1 += pl * (nc * ca + ns * sa);
+        potp += pl * m * (-ec * sl + es * cl);
+      } else {
+        Pc += pl * ec;
+        Ps += pl * es;
+        Rc += pl * dc;
+        Rs += pl * ds;
+        Tc += dpl * ec;
+        Ts += dpl * es;
+      }
+      if (DENS) {
+        const float4 nd = Nd[p];
+        const double nc = (double)fmaf(x2f, nd.z, x1f * nd.x);
+        const double ns = (double)fmaf(x2f, nd.w, x1f * nd.y);
+        if (COSL) {
+          den1 += pl * (nc * cl + ns * sl);
+        } else {
+          Nc += pl * nc;
+          Ns += pl * ns;
         }
       }
     }
+    if (!COSL && m > 0) {
+      pot1 += cmm * Pc + smm * Ps;
+      potr += cmm * Rc + smm * Rs;
+      pott += cmm * Tc + smm * Ts;
+      potp += m * (cmm * Ps - smm * Pc);
+      if (DENS) den1 += cmm * Nc + smm * Ns;
+    }
   }
-  o.pot0 = pot0 * P0;
-  o.potr = (potr0 + potr) * fac;
-  o.den0 = den0 * D0;
-  o.pot1 = pot1 * P0;
-  o.pott = pott * P0;
-  o.potp = potp * P0;
-  o.den1 = den1 * D0;
+  acc[0] = pot0;
+  acc[1] = potr0 + potr;
+  acc[2] = den0;
+  acc[3] = pot1;
+  acc[4] = pott;
+  acc[5] = potp;
+  acc[6] = den1;
+}
+
+template <bool DENS>
+__device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __restrict__ F,
+                                               int L, double r, double costh, double c1,
+                                               double s1, uint32_t flags, SLOut& o) {
+  const SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
+  const int i = cell.i;
+  const double x1 = cell.x1, x2 = cell.x2;
+  const double P0 = x1 * g.p0[i] + x2 * g.p0[i + 1];
+  const double D0 = DENS ? x1 * g.d0[i] + x2 * g.d0[i + 1] : 0.0;
+  const double dxi = g.xi[1] - g.xi[0];
+  const double fac = d_xi_to_r(cell.x, g.cmap, g.scale) / dxi;
+  const float ps = (float)((i == 0) ? ((flags & BFE_COMPAT_SL_EDGE) ? x2 : x2 - 1.0) : x2);
+  const int np = sl_npairs(L);
+  const float4* __restrict__ A = reinterpret_cast<const float4*>(F) + (size_t)i * 2 * np;
+  const float4* __restrict__ Nd =
+      reinterpret_cast<const float4*>(F + (size_t)g.numr * np * 8) + (size_t)i * np;
+  const bool dq = (flags & BFE_COMPAT_SL_DENS) != 0;
+  double acc[7];
+  if (flags & BFE_COMPAT_SL_COS_L)
+    sl_angular<DENS, true>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
+  else
+    sl_angular<DENS, false>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
+  o.pot0 = acc[0] * P0;
+  o.potr = acc[1] * fac;
+  o.den0 = acc[2] * D0;
+  o.pot1 = acc[3] * P0;
+  o.pott = acc[4] * P0;
+  o.potp = acc[5] * P0;
+  o.den1 = acc[6] * D0;
 }
 
 // EOF evaluation at planar radius R, height z, angle phi from the folded

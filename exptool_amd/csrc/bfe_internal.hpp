@@ -28,6 +28,18 @@ constexpr int kMaxLM = (kMaxL + 1) * (kMaxL + 2) / 2;
 constexpr int kLmcY = 0, kLmcA = kMaxLM, kLmcB = 2 * kMaxLM, kLmcD = 3 * kMaxLM;
 constexpr int kLmcSize = 3 * kMaxLM + kMaxL + 1;
 
+// Row length of the padded deposit tables: radial orders rounded up to a
+// multiple of 48, so that for every chunk length of the lane shapes (2, 4, 8,
+// 16, 24: divisors of 48) all chunks lie inside the row and start on a
+// 16-byte boundary (chunks of 2: 8 bytes).
+__host__ __device__ constexpr int dep_np(int n) { return (n + 47) / 48 * 48; }
+
+// Lane shapes (slots per lane, radial orders per slot) instantiated for the
+// cell-ordered accumulate kernels (bfe_accumulate.hip), selectable by index
+// with the tuning knobs eof_dep_shape / sl_dep_shape.
+constexpr int kEofDepShapes = 8;
+constexpr int kSlDepShapes = 7;
+
 // Folded tables (layouts documented in bfe_field.hpp): coefficients summed
 // into the basis tables once per coefficient set.
 struct SLFolded {
@@ -75,6 +87,7 @@ struct Context {
   double* sl_d0 = nullptr;     // numr
   double* sl_fold64 = nullptr; // [3][lmax+1][numr][nmax] float64: ef/sqrt(ev), ef sqrt(ev), ef p0/sqrt(ev)
   float* sl_e32 = nullptr;     // [numr][(lmax+1)*nmax]  ef/sqrt(ev), node-major
+  float* sl_dep32 = nullptr;   // [numr][lmax+1][dep_np(nmax)] the same, rows zero-padded
   double* sl_lmc = nullptr;    // per-(l, m) constants of the accumulate recurrences
   std::vector<double> h_xi;    // host copy (min xi for the field guard)
 
@@ -86,6 +99,7 @@ struct Context {
   int eof_K = 0;               // (2*mmax+1)*norder rows of the node-major tables
   double* eof64[4] = {nullptr, nullptr, nullptr, nullptr};  // pot, rf, zf, dens: [K][node]
   float* eof_pot32 = nullptr;  // [node][K] potential for accumulate
+  float* eof_dep32 = nullptr;  // [node][2 mmax + 1][dep_np(norder)] the same, rows zero-padded
 
   // ---- folded tables: slot 0 = batch evaluators, slot 1 = frozen field ----
   SLFolded slf[2];
@@ -107,6 +121,8 @@ struct Context {
   int64_t sl_seg_tiles = 0;
   int64_t order_min = 65536;       // evaluate in cell order from this many points
   int64_t sl_acc_path = 0;         // SL accumulate: 0 auto (vector ALU), 1 MFMA, 2 vector ALU
+  int64_t eof_dep_shape = -1;      // cell-ordered EOF accumulate lane shape (index), -1 auto
+  int64_t sl_dep_shape = -1;       // cell-ordered SL accumulate lane shape (index), -1 auto
 
   // ---- optional HIP-event timing of the main compute kernels ---------------
   bool kernel_timing = false;

@@ -174,12 +174,15 @@ int bfe_field_eval_batch(bfe_ctx* ctx, const float* x, const float* y, const flo
                          float* fz, float* pot, void* stream);
 
 /* ---- launch-shape tuning (no counterpart in the reference) ---------------- */
-/* Knobs: "sl_acc_path" (0 auto, 1 MFMA, 2 vector ALU), "eof_acc_blocks", "sl_acc_blocks" (accumulate blocks / batch
- * segment budget per kernel, default 2048), "eof_seg_tiles", "sl_seg_tiles"
- * (tiles per batch segment, 0 = derived from the budget),
+/* Knobs: "sl_acc_path" (0 auto, 1 MFMA, 2 vector ALU), "eof_acc_blocks",
+ * "sl_acc_blocks" (accumulate blocks / batch segment budget per kernel,
+ * default 2048), "eof_seg_tiles", "sl_seg_tiles" (tiles per batch segment,
+ * 0 = derived from the budget), "eof_dep_shape" (0..7), "sl_dep_shape" (0..6)
+ * (lane shape of the cell-ordered accumulate kernels, -1 = automatic; a
+ * shape that does not fit the tables falls back to the automatic one),
  * "order_min" (points from which evaluation runs in table-cell order, 65536).
  * Results do not depend on "order_min"; accumulate sums depend on the segment
- * plan only through the float32 tile sums. */
+ * plan and the lane shape only through the float32 partial sums. */
 int bfe_set_tuning(bfe_ctx* ctx, const char* name, int64_t value);
 int bfe_get_tuning(const bfe_ctx* ctx, const char* name, int64_t* value);
 /* With tuning knob "kernel_timing" = 1, the main compute kernel of
