@@ -393,6 +393,35 @@ def test_cell_ordered_accumulate_matches_oracle(big):
     assert block_rel_sl(hn, O.sl_accumulate(slo, hx, hy, hz, hm, no_odd=True), slo.lmax) <= COEF_TOL
 
 
+def test_cell_ordered_accumulate_every_lane_shape(big):
+    """Each lane shape of the cell-ordered kernels (slots per lane, radial
+    orders per slot; shapes that do not fit the tables fall back to the
+    automatic one) meets the oracle bound; an index outside the list raises."""
+    from exptool_amd._lib import BFEError
+    s, slo, eo = big
+    (dx, dy, dz, dm), (hx, hy, hz, hm) = disc_halo(80000, 53)
+    dx, dy, dz, dm = f32(dx, dy, dz, dm)
+    hx, hy, hz, hm = f32(hx, hy, hz, hm)
+    rc, rs = O.eof_accumulate(eo, dx, dy, dz, dm)
+    ref_h = O.sl_accumulate(slo, hx, hy, hz, hm)
+    for k in range(-1, 8):
+        s.set_tuning("eof_dep_shape", k)
+        c, sn = (v.cpu().numpy() for v in s.eof_accumulate(dx, dy, dz, dm))
+        assert block_rel_eof(c, rc) <= COEF_TOL, k
+        assert block_rel_eof(sn[1:], rs[1:]) <= COEF_TOL, k
+    for k in range(-1, 7):
+        s.set_tuning("sl_dep_shape", k)
+        h = s.sl_accumulate(hx, hy, hz, hm).cpu().numpy()
+        assert block_rel_sl(h, ref_h, slo.lmax) <= COEF_TOL, k
+    s.set_tuning("eof_dep_shape", -1)
+    s.set_tuning("sl_dep_shape", -1)
+    assert s.get_tuning("eof_dep_shape") == -1
+    with pytest.raises(BFEError):
+        s.set_tuning("eof_dep_shape", 8)
+    with pytest.raises(BFEError):
+        s.set_tuning("sl_dep_shape", -2)
+
+
 @pytest.mark.parametrize("n", [3000, 100000])
 def test_sl_accumulate_mfma_valu_and_bf16_modes(big, n):
     """SL accumulate on the MFMA units (three-piece bf16 split, default for
