@@ -121,8 +121,9 @@ def main():
         tuned = autotune(sysm, disc=(dx, dy, dz, dm), halo=(hx, hy, hz, hm), cache=False)
 
     def step():
-        c, s = sysm.eof_accumulate(dx, dy, dz, dm)
-        h = sysm.sl_accumulate(hx, hy, hz, hm)
+        # Fields.total_coefficients (disc and halo accumulate, run concurrently),
+        # prep_tables/set_field_parameters, return_forces_cart at every particle
+        c, s, h = sysm.accumulate_disc_halo((dx, dy, dz, dm), (hx, hy, hz, hm))
         sysm.set_field(c, s, h)
         return sysm.field_forces(ax, ay, az)
 

@@ -126,13 +126,15 @@ class Fields:
             self.xcen_disk = self.ycen_disk = self.zcen_disk = 0.0
             self.xcen_halo = self.ycen_halo = self.zcen_halo = 0.0
         dx, dy, dz = eof_mod._nan_to_origin(dx, dy, dz)
-        c, s = self.system.eof_accumulate(dx, dy, dz, dm)
+        # disc and halo accumulate run concurrently on the device
+        c, s, hcoef = self.system.accumulate_disc_halo((dx, dy, dz, dm), (hx, hy, hz, hm),
+                                                       no_odd=self.no_odd)
         self.EOF = eof_mod.EOF_Object()
         self.EOF.cos, self.EOF.sin = c.cpu().numpy(), s.cpu().numpy()
         self.EOF.mmax, self.EOF.nmax, self.EOF.eof_file = self.mmax, self.norder, self.eof_file
         self.EOF.time, self.EOF.nbodies = self.time, dm.size
         self.SL = spheresl.SL_Object()
-        self.SL.expcoef = self.system.sl_accumulate(hx, hy, hz, hm, no_odd=self.no_odd).cpu().numpy()
+        self.SL.expcoef = hcoef.cpu().numpy()
         self.SL.lmax, self.SL.nmax = self.lmaxhalo, self.nmaxhalo
         self.SL.sph_file, self.SL.model_file = self.sph_file, self.model_file
         self.SL.time, self.SL.nbodies = self.time, hm.size

@@ -129,6 +129,30 @@ class BFESystem:
         self.ctx.check(rc, "bfe_eof_accumulate")
         return cos, sin
 
+    def accumulate_disc_halo(self, disc, halo, no_odd=False, bf16=False, stream=None):
+        """Fields.total_coefficients for one snapshot: disc (x, y, z, m) -> EOF
+        (cos, sin) and halo (x, y, z, m) -> SL coefficients in one call; the
+        two accumulates run concurrently.  Returns (cos, sin, coef)."""
+        self._need("eof")
+        self._need("sl")
+        dx, dy, dz, nd = self._xyz(*disc[:3])
+        dm = f32(disc[3], self.device)
+        hx, hy, hz, nh = self._xyz(*halo[:3])
+        hm = f32(halo[3], self.device)
+        if dm.numel() != nd or hm.numel() != nh:
+            raise ValueError("mass must have the same length as x")
+        shape = (self.eof["mmax"] + 1, self.eof["norder"])
+        cos = torch.empty(shape, dtype=torch.float64, device=self.device)
+        sin = torch.empty(shape, dtype=torch.float64, device=self.device)
+        coef = torch.empty(((self.sl["lmax"] + 1) ** 2, self.sl["nmax"]), dtype=torch.float64,
+                           device=self.device)
+        rc = self.ctx.lib.bfe_accumulate_disc_halo(
+            self.ctx.h, ptr(dx), ptr(dy), ptr(dz), ptr(dm), nd, ptr(hx), ptr(hy), ptr(hz), ptr(hm),
+            nh, (_lib.NO_ODD if no_odd else 0) | (_lib.ACC_BF16 if bf16 else 0), ptr(cos), ptr(sin),
+            ptr(coef), stream_ptr(stream))
+        self.ctx.check(rc, "bfe_accumulate_disc_halo")
+        return cos, sin, coef
+
     # ------------------------------------------------------------- batches
     @staticmethod
     def _offsets(counts_or_offsets, n=None, offsets=False):

@@ -1287,6 +1287,38 @@ extern "C" int bfe_sl_accumulate(bfe_ctx* h, const float* x, const float* y, con
   return sl_accumulate_run(c, x, y, z, m, n, nullptr, 1, flags, coef_out, (hipStream_t)stream);
 }
 
+// Fields.total_coefficients (potential.py:98-252) for one snapshot: the disc
+// EOF and halo SL coefficients in one call, the two accumulates running
+// concurrently (the SL half on the context's internal stream with its own
+// work buffers).  Results are those of bfe_eof_accumulate and
+// bfe_sl_accumulate.
+extern "C" int bfe_accumulate_disc_halo(bfe_ctx* h, const float* dx, const float* dy,
+                                        const float* dz, const float* dm, int64_t nd,
+                                        const float* hx, const float* hy, const float* hz,
+                                        const float* hm, int64_t nh, uint32_t sl_flags,
+                                        double* cos_out, double* sin_out, double* coef_out,
+                                        void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (!c->eof_ok || !c->sl_ok) return set_error(c, BFE_ENOTABLE, "accumulate_disc_halo: tables not set");
+  if (!cos_out || !sin_out || !coef_out)
+    return set_error(c, BFE_EINVAL, "accumulate_disc_halo: NULL output");
+  int rc = acc_common_checks(c, dx, dy, dz, dm, nd, "accumulate_disc_halo (disc)");
+  if (rc) return rc;
+  rc = acc_common_checks(c, hx, hy, hz, hm, nh, "accumulate_disc_halo (halo)");
+  if (rc) return rc;
+  BFE_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream, s2 = nullptr;
+  rc = fork_begin(c, s, &s2);
+  if (rc) return rc;
+  swap_work(c);
+  const int rs = sl_accumulate_run(c, hx, hy, hz, hm, nh, nullptr, 1, sl_flags, coef_out, s2);
+  swap_work(c);
+  const int re = rs ? 0 : eof_accumulate_run(c, dx, dy, dz, dm, nd, nullptr, 1, cos_out, sin_out, s);
+  rc = fork_end(c, s);
+  return rs ? rs : (re ? re : rc);
+}
+
 extern "C" int bfe_sl_accumulate_batch(bfe_ctx* h, const float* x, const float* y, const float* z,
                                        const float* m, const int64_t* offsets, int nsnap,
                                        uint32_t flags, double* coef_out, void* stream) {

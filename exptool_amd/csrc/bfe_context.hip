@@ -4,6 +4,8 @@
 
 #include <cmath>
 
+#include <utility>
+
 #include "bfe_internal.hpp"
 
 namespace bfe {
@@ -138,9 +140,43 @@ extern "C" int bfe_destroy(bfe_ctx* h) {
   batch_free(c);
   timers_free(c);
   if (c->scratch) (void)hipFree(c->scratch);
+  if (c->alt_scratch) (void)hipFree(c->alt_scratch);
+  if (c->alt_order) (void)hipFree(c->alt_order);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->aux) (void)hipStreamDestroy(c->aux);
   delete h;
   return 0;
 }
+
+namespace bfe {
+
+int fork_begin(Context* c, hipStream_t s, hipStream_t* aux) {
+  if (!c->aux) {
+    BFE_HIP(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+    BFE_HIP(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    BFE_HIP(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  }
+  BFE_HIP(c, hipEventRecord(c->ev_fork, s));
+  BFE_HIP(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+  *aux = c->aux;
+  return 0;
+}
+
+int fork_end(Context* c, hipStream_t s) {
+  BFE_HIP(c, hipEventRecord(c->ev_join, c->aux));
+  BFE_HIP(c, hipStreamWaitEvent(s, c->ev_join, 0));
+  return 0;
+}
+
+void swap_work(Context* c) {
+  std::swap(c->scratch, c->alt_scratch);
+  std::swap(c->scratch_bytes, c->alt_scratch_bytes);
+  std::swap(c->order, c->alt_order);
+  std::swap(c->order_bytes, c->alt_order_bytes);
+}
+
+}  // namespace bfe
 
 extern "C" const char* bfe_last_error(const bfe_ctx* h) {
   return h ? h->c.err.c_str() : g_create_err.c_str();

@@ -451,11 +451,15 @@ extern "C" int bfe_field_set(bfe_ctx* h, const double* eof_cos, const double* eo
   BFE_HIP(c, hipSetDevice(c->device));
   hipStream_t s = (hipStream_t)stream;
   const bool no_odd = (flags & BFE_NO_ODD) != 0;
-  int rc = eof_fold(c, 1, eof_cos, eof_sin, mm, nn, 0, 1 << 30, no_odd,
-                    (flags & BFE_COMPAT_EOF_MASK) != 0, false, s);
+  // the SL fold (a few thousand waves) runs beside the HBM-bound EOF fold
+  hipStream_t s2 = nullptr;
+  int rc = fork_begin(c, s, &s2);
   if (rc) return rc;
-  rc = sl_fold(c, 1, sl_coef, halofac, ll, hn, -(1 << 30), 1 << 30, no_odd, s);
-  if (rc) return rc;
+  const int rs = sl_fold(c, 1, sl_coef, halofac, ll, hn, -(1 << 30), 1 << 30, no_odd, s2);
+  const int re = rs ? 0 : eof_fold(c, 1, eof_cos, eof_sin, mm, nn, 0, 1 << 30, no_odd,
+                                   (flags & BFE_COMPAT_EOF_MASK) != 0, false, s);
+  rc = fork_end(c, s);
+  if (rs || re || rc) return rs ? rs : (re ? re : rc);
   c->field_halofac = halofac;
   c->field_flags = (int)(flags & (BFE_COMPAT_SL_COS_L | BFE_COMPAT_SL_EDGE));
   return 0;

@@ -128,6 +128,16 @@ struct Context {
   bool kernel_timing = false;
   std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> timer_events;
 
+  // ---- a second set of work buffers and an internal stream, for the two
+  // independent halves of a call that runs them concurrently (disc and
+  // halo accumulate, EOF and SL folds); see fork_begin / fork_end
+  void* alt_scratch = nullptr;
+  size_t alt_scratch_bytes = 0;
+  void* alt_order = nullptr;
+  size_t alt_order_bytes = 0;
+  hipStream_t aux = nullptr;   // created on first use, default priority
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+
   // ---- spatial ordering of evaluation points (bfe_order.hip) -------------
   void* order = nullptr;       // sort keys and packed points (two buffers each) | perm | per-point f64 | tile counts
   size_t order_bytes = 0;
@@ -211,6 +221,14 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nb) {
 void timer_begin(Context* c, const char* name, hipStream_t s);
 void timer_end(Context* c, const char* name, hipStream_t s);
 void timers_free(Context* c);
+
+// Concurrent halves of one call: fork_begin makes the internal stream wait
+// for the caller's stream s and returns it; work enqueued on it between
+// swap_work calls uses the second buffer set.  fork_end makes s wait for the
+// internal stream again (always call it once fork_begin succeeded).
+int fork_begin(Context* c, hipStream_t s, hipStream_t* aux);
+int fork_end(Context* c, hipStream_t s);
+void swap_work(Context* c);
 
 int set_error(Context* c, int code, const char* fmt, ...);
 int check_hip(Context* c, hipError_t e, const char* what);

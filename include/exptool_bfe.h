@@ -157,6 +157,15 @@ int bfe_eof_accumulate_batch(bfe_ctx* ctx, const float* x, const float* y, const
 int bfe_sl_accumulate_batch(bfe_ctx* ctx, const float* x, const float* y, const float* z,
                             const float* m, const int64_t* offsets, int nsnap, uint32_t flags,
                             double* coef_out, void* stream);
+/* Fields.total_coefficients (potential.py:98-252) for one snapshot: disc
+ * (x, y, z, m)[nd] -> EOF cos/sin [mmax+1][norder] as bfe_eof_accumulate,
+ * halo [nh] -> SL coef [(lmax+1)^2][nmax] as bfe_sl_accumulate (sl_flags:
+ * BFE_NO_ODD, BFE_ACC_BF16).  The two run concurrently: the halo half on the
+ * context's internal stream, joined back into `stream` before return. */
+int bfe_accumulate_disc_halo(bfe_ctx* ctx, const float* dx, const float* dy, const float* dz,
+                             const float* dm, int64_t nd, const float* hx, const float* hy,
+                             const float* hz, const float* hm, int64_t nh, uint32_t sl_flags,
+                             double* cos_out, double* sin_out, double* coef_out, void* stream);
 /* Fields.prep_tables + set_field_parameters (potential.py:257-294, :365-378)
  * for nsnap frozen fields at once: eof_cos/eof_sin [nsnap][mmax+1][norder],
  * sl_coef [nsnap][(lmax+1)^2][nmax]; truncation and flags as bfe_field_set.

@@ -393,6 +393,27 @@ def test_cell_ordered_accumulate_matches_oracle(big):
     assert block_rel_sl(hn, O.sl_accumulate(slo, hx, hy, hz, hm, no_odd=True), slo.lmax) <= COEF_TOL
 
 
+def test_accumulate_disc_halo_equals_separate_calls(big):
+    """The combined disc + halo accumulate (concurrent halves, the halo half
+    with its own work buffers) returns exactly what the two single calls
+    return, for cell-ordered and input-order sizes and empty halves."""
+    s, slo, eo = big
+    (dx, dy, dz, dm), (hx, hy, hz, hm) = disc_halo(90000, 57)
+    D = f32(dx, dy, dz, dm)
+    H = f32(hx, hy, hz, hm)
+    for nd, nh in ((90000, 90000), (90000, 5000), (3000, 90000), (0, 7000), (7000, 0)):
+        d = [v[:nd] for v in D]
+        h = [v[:nh] for v in H]
+        c, sn, hc = (v.cpu().numpy() for v in s.accumulate_disc_halo(d, h))
+        c1, s1 = (v.cpu().numpy() for v in s.eof_accumulate(*d))
+        h1 = s.sl_accumulate(*h).cpu().numpy()
+        assert np.array_equal(c, c1) and np.array_equal(sn, s1) and np.array_equal(hc, h1), (nd, nh)
+    c, sn, hc = (v.cpu().numpy() for v in s.accumulate_disc_halo(D, H, no_odd=True))
+    assert np.array_equal(hc, s.sl_accumulate(*H, no_odd=True).cpu().numpy())
+    rc, rs = O.eof_accumulate(eo, *D)
+    assert block_rel_eof(c, rc) <= COEF_TOL
+
+
 def test_cell_ordered_accumulate_every_lane_shape(big):
     """Each lane shape of the cell-ordered kernels (slots per lane, radial
     orders per slot; shapes that do not fit the tables fall back to the
