@@ -278,28 +278,79 @@ __global__ __launch_bounds__(256) void eof_eval_kernel(
   }
 }
 
-// pts (may be NULL): the points in cell order as float4 (x, y, z, input index
-// bits); block b then takes logical block xcd_block(nb) so that each XCD
-// evaluates a contiguous range of cells.  Without pts, point j of px/py/pz.
+// LDS a block may fill with SL node force blocks (2 np float4 per node): 40
+// nodes of lmax 6 (35 KB, four blocks per CU); measured on the workload,
+// 16, 32, 40 and 48 nodes gave 243, 222, 219 and 230 us.
+constexpr int kStageFloat4 = 40 * 56;
+
+// Block-wide minimum and maximum of v (all 256 threads call it).
+__device__ __forceinline__ void block_minmax(int v_min, int v_max, int* smin, int* smax,
+                                             int& out_min, int& out_max) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v_min = min(v_min, __shfl_xor(v_min, o, 64));
+    v_max = max(v_max, __shfl_xor(v_max, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    smin[threadIdx.x >> 6] = v_min;
+    smax[threadIdx.x >> 6] = v_max;
+  }
+  __syncthreads();
+  out_min = min(min(smin[0], smin[1]), min(smin[2], smin[3]));
+  out_max = max(max(smax[0], smax[1]), max(smax[2], smax[3]));
+}
+
+// Fields.return_forces_cart at n points.  pts (may be NULL): the points in
+// table-cell order as float4 (x, y, z, input index bits), outputs scattered
+// back to input order; block b then takes logical block xcd_block(nb) so that
+// each XCD evaluates a contiguous range of cells.  Without pts, point j of
+// px/py/pz.  In cell order a block's points usually share a few SL radial
+// nodes: when their force blocks fit in kStageFloat4, the block copies them
+// to LDS and the SL sums read them there.
 __global__ __launch_bounds__(256) void field_eval_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
     int64_t n, const float4* __restrict__ pts, FieldTables T, float* fx, float* fy, float* fz,
     float* pot, float* comp) {
+  __shared__ float4 stage[kStageFloat4];
+  __shared__ int smin[4], smax[4];
   const int64_t nb = (n + 255) / 256;
   const int64_t j = (pts ? xcd_block(nb) : (int64_t)blockIdx.x) * 256 + threadIdx.x;
-  if (j >= n) return;
-  float qx, qy, qz;
-  int64_t i;
-  if (pts) {
-    const float4 q = pts[j];
-    qx = q.x; qy = q.y; qz = q.z;
-    i = (int64_t)__float_as_int(q.w);
-  } else {
-    qx = px[j]; qy = py[j]; qz = pz[j];
-    i = j;
+  if (pts && (j - threadIdx.x) >= n) return;   // whole block past the end (XCD mapping)
+  const bool live = j < n;
+  float qx = 0.f, qy = 0.f, qz = 0.f;
+  int64_t i = 0;
+  if (live) {
+    if (pts) {
+      const float4 q = pts[j];
+      qx = q.x; qy = q.y; qz = q.z;
+      i = (int64_t)__float_as_int(q.w);
+    } else {
+      qx = px[j]; qy = py[j]; qz = pz[j];
+      i = j;
+    }
   }
+  const int np2 = sl_npairs(T.lmax) * 2;   // float4 per SL node
+  bool staged = false;
+  int imin = 0;
+  SLCell cell{0.0, 0, 0.0, 0.0};
+  if (pts && np2 <= kStageFloat4) {
+    if (live) cell = field_sl_cell(T.slg, qx, qy, qz);
+    int imax;
+    block_minmax(live ? cell.i : 0x7fffffff, live ? cell.i : -1, smin, smax, imin, imax);
+    if ((imax - imin + 1) * np2 <= kStageFloat4) {
+      const float4* src = reinterpret_cast<const float4*>(T.slF) + (size_t)imin * np2;
+      const int cnt = (imax - imin + 1) * np2;
+      for (int k = threadIdx.x; k < cnt; k += 256) stage[k] = src[k];
+      staged = true;
+    }
+    __syncthreads();
+  }
+  if (!live) return;
   double cv[8];
-  field_eval_point(T, qx, qy, qz, cv);
+  if (staged)
+    field_eval_point(T, qx, qy, qz, cv, &cell, stage, imin);
+  else
+    field_eval_point(T, qx, qy, qz, cv);
   if (fx) fx[i] = (float)(cv[0] + cv[1]);
   if (fy) fy[i] = (float)(cv[2] + cv[3]);
   if (fz) fz[i] = (float)(cv[4] + cv[5]);

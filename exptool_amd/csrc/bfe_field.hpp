@@ -160,11 +160,17 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, const float4* __rest
   acc[6] = den1;
 }
 
+// pcell: the radial cell of r when the caller has located it already.
+// Abase/ibase: where node i's force blocks are read, Abase + (i - ibase)
+// * 2 np float4 (default: the folded table itself; a block may stage its
+// nodes in LDS and pass that copy).
 template <bool DENS>
 __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __restrict__ F,
                                                int L, double r, double costh, double c1,
-                                               double s1, uint32_t flags, SLOut& o) {
-  const SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
+                                               double s1, uint32_t flags, SLOut& o,
+                                               const SLCell* pcell = nullptr,
+                                               const float4* Abase = nullptr, int ibase = 0) {
+  const SLCell cell = pcell ? *pcell : sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
   const int i = cell.i;
   const double x1 = cell.x1, x2 = cell.x2;
   const double P0 = x1 * g.p0[i] + x2 * g.p0[i + 1];
@@ -173,7 +179,8 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
   const double fac = d_xi_to_r(cell.x, g.cmap, g.scale) / dxi;
   const float ps = (float)((i == 0) ? ((flags & BFE_COMPAT_SL_EDGE) ? x2 : x2 - 1.0) : x2);
   const int np = sl_npairs(L);
-  const float4* __restrict__ A = reinterpret_cast<const float4*>(F) + (size_t)i * 2 * np;
+  const float4* __restrict__ A =
+      Abase ? Abase + (size_t)(i - ibase) * 2 * np : reinterpret_cast<const float4*>(F) + (size_t)i * 2 * np;
   const float4* __restrict__ Nd =
       reinterpret_cast<const float4*>(F + (size_t)g.numr * np * 8) + (size_t)i * np;
   const bool dq = (flags & BFE_COMPAT_SL_DENS) != 0;
@@ -253,6 +260,13 @@ __device__ __forceinline__ void eof_eval_folded(const EOFGeom& g, const float* _
   o.d = d;
 }
 
+// The SL radial cell field_eval_point locates for the point (x, y, z).
+__device__ __forceinline__ SLCell field_sl_cell(const SLGeom& g, double x, double y, double z) {
+  const double r2 = sqrt(x * x + y * y) + 1.e-15;
+  const double r3 = sqrt(r2 * r2 + z * z) + 1.e-15;
+  return sl_locate(r3, g.cmap, g.scale, g.xi, g.numr);
+}
+
 // Frozen disc + halo field at a Cartesian point, Fields.return_forces_cart
 // (potential.py:445-497).  comp receives the 8 reference outputs.
 struct FieldTables {
@@ -266,8 +280,12 @@ struct FieldTables {
   double crot, srot; // cos/sin of rotpos
 };
 
+// pcell/slA/slbase: the point's SL cell (field_sl_cell) and SL node blocks
+// staged by the caller (see sl_eval_folded).
 __device__ __forceinline__ void field_eval_point(const FieldTables& T, double x, double y,
-                                                 double z, double comp[8]) {
+                                                 double z, double comp[8],
+                                                 const SLCell* pcell = nullptr,
+                                                 const float4* slA = nullptr, int slbase = 0) {
   const double r2 = sqrt(x * x + y * y) + 1.e-15;
   const double r3 = sqrt(r2 * r2 + z * z) + 1.e-15;
   const double costh = z / r3;
@@ -278,7 +296,7 @@ __device__ __forceinline__ void field_eval_point(const FieldTables& T, double x,
   EOFOut d;
   eof_eval_folded<false>(T.eg, T.eofF, r2, z, c1, s1, d);
   SLOut h;
-  sl_eval_folded<false>(T.slg, T.slF, T.lmax, r3, costh, c1, s1, T.flags, h);
+  sl_eval_folded<false>(T.slg, T.slF, T.lmax, r3, costh, c1, s1, T.flags, h, pcell, slA, slbase);
   double dfp = d.fp, hfp = h.potp;
   if (r3 < T.ximin) {
     hfp = 0.0;
