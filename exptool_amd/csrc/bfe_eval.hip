@@ -278,28 +278,6 @@ __global__ __launch_bounds__(256) void eof_eval_kernel(
   }
 }
 
-// LDS a block may fill with SL node force blocks (2 np float4 per node): 40
-// nodes of lmax 6 (35 KB, four blocks per CU); measured on the workload,
-// 16, 32, 40 and 48 nodes gave 243, 222, 219 and 230 us.
-constexpr int kStageFloat4 = 40 * 56;
-
-// Block-wide minimum and maximum of v (all 256 threads call it).
-__device__ __forceinline__ void block_minmax(int v_min, int v_max, int* smin, int* smax,
-                                             int& out_min, int& out_max) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    v_min = min(v_min, __shfl_xor(v_min, o, 64));
-    v_max = max(v_max, __shfl_xor(v_max, o, 64));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    smin[threadIdx.x >> 6] = v_min;
-    smax[threadIdx.x >> 6] = v_max;
-  }
-  __syncthreads();
-  out_min = min(min(smin[0], smin[1]), min(smin[2], smin[3]));
-  out_max = max(max(smax[0], smax[1]), max(smax[2], smax[3]));
-}
-
 // Fields.return_forces_cart at n points.  pts (may be NULL): the points in
 // table-cell order as float4 (x, y, z, input index bits), outputs scattered
 // back to input order; block b then takes logical block xcd_block(nb) so that
@@ -329,21 +307,12 @@ __global__ __launch_bounds__(256) void field_eval_kernel(
       i = j;
     }
   }
-  const int np2 = sl_npairs(T.lmax) * 2;   // float4 per SL node
   bool staged = false;
   int imin = 0;
   SLCell cell{0.0, 0, 0.0, 0.0};
-  if (pts && np2 <= kStageFloat4) {
+  if (pts) {
     if (live) cell = field_sl_cell(T.slg, qx, qy, qz);
-    int imax;
-    block_minmax(live ? cell.i : 0x7fffffff, live ? cell.i : -1, smin, smax, imin, imax);
-    if ((imax - imin + 1) * np2 <= kStageFloat4) {
-      const float4* src = reinterpret_cast<const float4*>(T.slF) + (size_t)imin * np2;
-      const int cnt = (imax - imin + 1) * np2;
-      for (int k = threadIdx.x; k < cnt; k += 256) stage[k] = src[k];
-      staged = true;
-    }
-    __syncthreads();
+    staged = stage_sl_nodes(T, live, cell.i, stage, smin, smax, imin);
   }
   if (!live) return;
   double cv[8];

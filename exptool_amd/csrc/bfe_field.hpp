@@ -315,4 +315,46 @@ __device__ __forceinline__ void field_eval_point(const FieldTables& T, double x,
   comp[7] = hp;
 }
 
+// LDS a block may fill with SL node force blocks (2 np float4 per node): 40
+// nodes of lmax 6 (35 KB, four blocks per CU); measured on the workload,
+// 16, 32, 40 and 48 nodes gave 243, 222, 219 and 230 us.
+constexpr int kStageFloat4 = 40 * 56;
+
+// Block-wide minimum and maximum of v (all 256 threads call it).
+__device__ __forceinline__ void block_minmax(int v_min, int v_max, int* smin, int* smax,
+                                             int& out_min, int& out_max) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    v_min = min(v_min, __shfl_xor(v_min, o, 64));
+    v_max = max(v_max, __shfl_xor(v_max, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    smin[threadIdx.x >> 6] = v_min;
+    smax[threadIdx.x >> 6] = v_max;
+  }
+  __syncthreads();
+  out_min = min(min(smin[0], smin[1]), min(smin[2], smin[3]));
+  out_max = max(max(smax[0], smax[1]), max(smax[2], smax[3]));
+}
+
+// Copies the force blocks of the SL nodes used by the block's points to
+// stage when they fit (all 256 threads call it; live: the thread has a
+// point, node: its SL node).  Returns whether they were staged; imin is then
+// the first staged node.  Ends with a barrier.
+__device__ __forceinline__ bool stage_sl_nodes(const FieldTables& T, bool live, int node,
+                                               float4* stage, int* smin, int* smax, int& imin) {
+  const int np2 = sl_npairs(T.lmax) * 2;   // float4 per SL node
+  int imax;
+  block_minmax(live ? node : 0x7fffffff, live ? node : -1, smin, smax, imin, imax);
+  bool staged = false;
+  if (imax >= imin && (imax - imin + 1) * np2 <= kStageFloat4) {
+    const float4* src = reinterpret_cast<const float4*>(T.slF) + (size_t)imin * np2;
+    const int cnt = (imax - imin + 1) * np2;
+    for (int k = threadIdx.x; k < cnt; k += 256) stage[k] = src[k];
+    staged = true;
+  }
+  __syncthreads();
+  return staged;
+}
+
 }  // namespace bfe

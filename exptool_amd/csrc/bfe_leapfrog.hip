@@ -99,7 +99,10 @@ __global__ __launch_bounds__(256) void leapfrog_chunk_kernel(
     float* __restrict__ x, float* __restrict__ y, float* __restrict__ z, float* __restrict__ vx,
     float* __restrict__ vy, float* __restrict__ vz, const int* __restrict__ perm, FieldTables T,
     LeapArgs a, OrbitAux* __restrict__ aux, double* __restrict__ ebuf, float* __restrict__ traj) {
+  __shared__ float4 stage[kStageFloat4];
+  __shared__ int smin[4], smax[4];
   const int64_t j = xcd_block(a.nblocks) * 256 + threadIdx.x;
+  if (j - threadIdx.x >= a.n) return;   // whole block past the end (XCD mapping)
   const bool live = j < a.n;
   const int64_t i = live ? (int64_t)perm[j] : 0;
   float px = 0, py = 0, pz = 0, ux = 0, uy = 0, uz = 0, ax = 0, ay = 0, az = 0;
@@ -114,15 +117,26 @@ __global__ __launch_bounds__(256) void leapfrog_chunk_kernel(
   const float hdt2 = 0.5f * dt * dt;
   const float hdt = 0.5f * dt;
   float pot = 0.f;
+  // every thread of the block runs the same steps: per step the block's SL
+  // nodes are staged in LDS when they fit (orbits start a chunk in cell order)
   for (int64_t k = a.k0 + 1; k <= a.k1; ++k) {
+    SLCell cell{0.0, 0, 0.0, 0.0};
     if (live) {
       // integrate.py:129-143
       px = px + ux * dt + ax * hdt2;
       py = py + uy * dt + ay * hdt2;
       pz = pz + uz * dt + az * hdt2;
+      cell = field_sl_cell(T.slg, px, py, pz);
+    }
+    int imin = 0;
+    const bool staged = stage_sl_nodes(T, live, cell.i, stage, smin, smax, imin);
+    if (live) {
       rot_of(a, k, T);
       double cv[8];
-      field_eval_point(T, px, py, pz, cv);
+      if (staged)
+        field_eval_point(T, px, py, pz, cv, &cell, stage, imin);
+      else
+        field_eval_point(T, px, py, pz, cv);
       const float nx = (float)(cv[0] + cv[1]), ny = (float)(cv[2] + cv[3]), nz = (float)(cv[4] + cv[5]);
       pot = (float)(cv[6] + cv[7]);
       ux = ux + (ax + nx) * hdt;
@@ -137,6 +151,7 @@ __global__ __launch_bounds__(256) void leapfrog_chunk_kernel(
         t[6 * n + i] = pot;
       }
     }
+    __syncthreads();   // the stage is rewritten by the next step
   }
   if (live) {
     x[i] = px; y[i] = py; z[i] = pz;
