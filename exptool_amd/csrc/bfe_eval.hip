@@ -338,16 +338,38 @@ __global__ __launch_bounds__(256) void field_eval_batch_kernel(
     FieldTables T, const float* __restrict__ slB, int64_t sl_stride,
     const float* __restrict__ eofB, int64_t eof_stride, float* fx, float* fy, float* fz,
     float* pot) {
+  __shared__ float4 stage[kStageFloat4];
+  __shared__ int smin[4], smax[4];
   const int64_t nb = (n + 255) / 256;
   const int64_t j = xcd_block(nb) * 256 + threadIdx.x;
-  if (j >= n) return;
-  const float4 q = pts[j];
+  if (j - threadIdx.x >= n) return;   // whole block past the end (XCD mapping)
+  const bool live = j < n;
+  float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+  int snap = 0;
+  if (live) {
+    q = pts[j];
+    snap = (int)(sorted[j] / ncell);
+  }
+  // SL nodes are staged when all the block's points belong to one snapshot
+  int s0, s1;
+  block_minmax(live ? snap : 0x7fffffff, live ? snap : -1, smin, smax, s0, s1);
+  __syncthreads();   // smin/smax are reused below
+  T.slF = slB + (size_t)(live ? snap : s0) * sl_stride;
+  T.eofF = eofB + (size_t)(live ? snap : s0) * eof_stride;
+  SLCell cell{0.0, 0, 0.0, 0.0};
+  bool staged = false;
+  int imin = 0;
+  if (s0 == s1) {
+    if (live) cell = field_sl_cell(T.slg, q.x, q.y, q.z);
+    staged = stage_sl_nodes(T, live, cell.i, stage, smin, smax, imin);
+  }
+  if (!live) return;
   const int64_t i = (int64_t)__float_as_int(q.w);
-  const size_t snap = sorted[j] / ncell;
-  T.slF = slB + snap * sl_stride;
-  T.eofF = eofB + snap * eof_stride;
   double cv[8];
-  field_eval_point(T, q.x, q.y, q.z, cv);
+  if (staged)
+    field_eval_point(T, q.x, q.y, q.z, cv, &cell, stage, imin);
+  else
+    field_eval_point(T, q.x, q.y, q.z, cv);
   if (fx) fx[i] = (float)(cv[0] + cv[1]);
   if (fy) fy[i] = (float)(cv[2] + cv[3]);
   if (fz) fz[i] = (float)(cv[4] + cv[5]);
