@@ -29,7 +29,7 @@ COMPAT_ALL = 0xF
 EXPORTS = (
     "bfe_create", "bfe_destroy", "bfe_last_error", "bfe_sl_tables_set", "bfe_eof_tables_set",
     "bfe_sl_accumulate", "bfe_eof_accumulate", "bfe_sl_eval", "bfe_eof_eval", "bfe_field_set",
-    "bfe_field_eval_cart", "bfe_leapfrog", "bfe_mlp_set_params", "bfe_mlp_get_params",
+    "bfe_field_eval_cart", "bfe_leapfrog", "bfe_leapfrog_dt", "bfe_mlp_set_params", "bfe_mlp_get_params",
     "bfe_mlp_forward", "bfe_mlp_gradient", "bfe_mlp_train_step", "bfe_mlp_leapfrog",
     "bfe_eof_accumulate_batch", "bfe_sl_accumulate_batch", "bfe_field_set_batch",
     "bfe_field_eval_batch", "bfe_accumulate_disc_halo",
@@ -59,6 +59,8 @@ def _declare(lib):
         "bfe_field_eval_cart": (_I32, [_VP, _VP, _VP, _VP, _I64, _F64, _VP, _VP, _VP, _VP, _VP, _VP]),
         "bfe_leapfrog": (_I32, [_VP] + [_VP] * 6 + [_I64, _F64, _I64, _F64, _F64, _I32, _VP, _VP, _I32, _VP,
                                                   _I32, _VP]),
+        "bfe_leapfrog_dt": (_I32, [_VP] + [_VP] * 6 + [_I64, _VP, _I64, _F64, _F64, _I32, _VP, _VP, _I32,
+                                                     _VP, _I32, _VP]),
         "bfe_eof_accumulate_batch": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _I32, _U32, _VP, _VP, _VP]),
         "bfe_sl_accumulate_batch": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _I32, _U32, _VP, _VP]),
         "bfe_field_set_batch": (_I32, [_VP, _VP, _VP, _VP, _I32, _F64, _I32, _I32, _I32, _I32, _U32, _VP]),

@@ -320,6 +320,8 @@ class BFESystem:
     def leapfrog(self, state, dt, nsteps, t0=0.0, rotfreq=0.0, energy_every=0,
                  record_every=0, use_graph=False, stream=None):
         """Advance state = (x, y, z, vx, vy, vz) float32 device tensors in place.
+        dt is one step size for every orbit, or a float64 tensor (n,) of
+        per-orbit step sizes (bfe_leapfrog_dt).
         Returns dict(drift=(n,) float64 max relative energy drift per orbit,
         energy=(checks,) float64 total energy log or None,
         traj=(nrec, 7, n) float32 or None)."""
@@ -339,9 +341,19 @@ class BFESystem:
         s = stream if stream is not None else torch.cuda.current_stream()
         if use_graph and s.cuda_stream == 0:
             raise ValueError("use_graph needs a non-default stream (torch.cuda.Stream())")
-        rc = self.ctx.lib.bfe_leapfrog(self.ctx.h, *[ptr(t) for t in state], n, float(dt),
-                                       int(nsteps), float(t0), float(rotfreq), int(energy_every),
-                                       ptr(drift), ptr(elog), int(record_every), ptr(traj),
-                                       1 if use_graph else 0, stream_ptr(s))
-        self.ctx.check(rc, "bfe_leapfrog")
+        if torch.is_tensor(dt):
+            if dt.dtype != torch.float64 or dt.numel() != n or dt.device != self.device:
+                raise ValueError("per-orbit dt must be a float64 tensor of n values on the context device")
+            dt = dt.contiguous()
+            rc = self.ctx.lib.bfe_leapfrog_dt(self.ctx.h, *[ptr(t) for t in state], n, ptr(dt),
+                                              int(nsteps), float(t0), float(rotfreq), int(energy_every),
+                                              ptr(drift), ptr(elog), int(record_every), ptr(traj),
+                                              1 if use_graph else 0, stream_ptr(s))
+            self.ctx.check(rc, "bfe_leapfrog_dt")
+        else:
+            rc = self.ctx.lib.bfe_leapfrog(self.ctx.h, *[ptr(t) for t in state], n, float(dt),
+                                           int(nsteps), float(t0), float(rotfreq), int(energy_every),
+                                           ptr(drift), ptr(elog), int(record_every), ptr(traj),
+                                           1 if use_graph else 0, stream_ptr(s))
+            self.ctx.check(rc, "bfe_leapfrog")
         return dict(drift=drift, energy=elog, traj=traj)

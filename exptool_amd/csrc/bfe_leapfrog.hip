@@ -25,6 +25,7 @@ struct OrbitAux {
 
 struct LeapArgs {
   double dt, t0, omega;  // omega = 2 pi rotfreq
+  const double* dtv;     // per-orbit step sizes [n] (NULL: every orbit uses dt)
   int64_t n;
   int64_t k0, k1;        // steps (k0, k1] are taken; step k is at time t0 + k dt
   int energy_every;      // 0 = no checks
@@ -33,9 +34,14 @@ struct LeapArgs {
   int nblocks;
 };
 
-__device__ __forceinline__ void rot_of(const LeapArgs& a, int64_t k, FieldTables& T) {
+__device__ __forceinline__ double step_of(const LeapArgs& a, int64_t i) {
+  return a.dtv ? a.dtv[i] : a.dt;
+}
+
+// rotation of the field at step k of an orbit with step size dt
+__device__ __forceinline__ void rot_of(const LeapArgs& a, double dt, int64_t k, FieldTables& T) {
   if (a.omega != 0.0) {
-    const double ang = a.omega * (a.t0 + (double)k * a.dt);
+    const double ang = a.omega * (a.t0 + (double)k * dt);
     T.crot = cos(ang);
     T.srot = sin(ang);
   } else {
@@ -72,7 +78,7 @@ __global__ __launch_bounds__(256) void leapfrog_init_kernel(
   if (j < a.n) {
     const int64_t i = perm[j];
     double e;
-    rot_of(a, a.k0, T);
+    rot_of(a, step_of(a, i), a.k0, T);
     double cv[8];
     field_eval_point(T, x[i], y[i], z[i], cv);
     const float fx = (float)(cv[0] + cv[1]), fy = (float)(cv[2] + cv[3]), fz = (float)(cv[4] + cv[5]);
@@ -107,13 +113,15 @@ __global__ __launch_bounds__(256) void leapfrog_chunk_kernel(
   const int64_t i = live ? (int64_t)perm[j] : 0;
   float px = 0, py = 0, pz = 0, ux = 0, uy = 0, uz = 0, ax = 0, ay = 0, az = 0;
   OrbitAux o{};
+  double dtd = a.dt;
   if (live) {
     px = x[i]; py = y[i]; pz = z[i];
     ux = vx[i]; uy = vy[i]; uz = vz[i];
     o = aux[i];
     ax = o.ax; ay = o.ay; az = o.az;
+    dtd = step_of(a, i);
   }
-  const float dt = (float)a.dt;
+  const float dt = (float)dtd;
   const float hdt2 = 0.5f * dt * dt;
   const float hdt = 0.5f * dt;
   float pot = 0.f;
@@ -131,7 +139,7 @@ __global__ __launch_bounds__(256) void leapfrog_chunk_kernel(
     int imin = 0;
     const bool staged = stage_sl_nodes(T, live, cell.i, stage, smin, smax, imin);
     if (live) {
-      rot_of(a, k, T);
+      rot_of(a, dtd, k, T);
       double cv[8];
       if (staged)
         field_eval_point(T, px, py, pz, cv, &cell, stage, imin);
@@ -201,11 +209,11 @@ __global__ void energy_reduce_kernel(const double* __restrict__ partE, int nchec
 
 using namespace bfe;
 
-extern "C" int bfe_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float* vx, float* vy,
-                            float* vz, int64_t n, double dt, int64_t nsteps, double t0,
-                            double rotfreq, int energy_every, double* drift_out,
-                            double* energy_log, int record_every, float* traj, int use_graph,
-                            void* stream) {
+static int leapfrog_run(bfe_ctx* h, float* x, float* y, float* z, float* vx, float* vy,
+                        float* vz, int64_t n, double dt, const double* dtv, int64_t nsteps,
+                        double t0, double rotfreq, int energy_every, double* drift_out,
+                        double* energy_log, int record_every, float* traj, int use_graph,
+                        void* stream) {
   if (!h) return BFE_EINVAL;
   Context* c = &h->c;
   if (n < 0 || nsteps < 0) return set_error(c, BFE_EINVAL, "leapfrog: n or nsteps < 0");
@@ -237,6 +245,7 @@ extern "C" int bfe_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float* vx,
   const unsigned grid = xcd_grid(nb);
   LeapArgs a;
   a.dt = dt;
+  a.dtv = dtv;
   a.t0 = t0;
   a.omega = 2.0 * kPi * rotfreq;
   a.n = n;
@@ -306,4 +315,23 @@ extern "C" int bfe_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float* vx,
                        nchecks, nb, energy_log);
   BFE_HIP(c, hipGetLastError());
   return 0;
+}
+
+extern "C" int bfe_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float* vx, float* vy,
+                            float* vz, int64_t n, double dt, int64_t nsteps, double t0,
+                            double rotfreq, int energy_every, double* drift_out,
+                            double* energy_log, int record_every, float* traj, int use_graph,
+                            void* stream) {
+  return leapfrog_run(h, x, y, z, vx, vy, vz, n, dt, nullptr, nsteps, t0, rotfreq, energy_every,
+                      drift_out, energy_log, record_every, traj, use_graph, stream);
+}
+
+extern "C" int bfe_leapfrog_dt(bfe_ctx* h, float* x, float* y, float* z, float* vx, float* vy,
+                               float* vz, int64_t n, const double* dt, int64_t nsteps, double t0,
+                               double rotfreq, int energy_every, double* drift_out,
+                               double* energy_log, int record_every, float* traj, int use_graph,
+                               void* stream) {
+  if (h && n > 0 && !dt) return set_error(&h->c, BFE_EINVAL, "leapfrog_dt: NULL dt");
+  return leapfrog_run(h, x, y, z, vx, vy, vz, n, 0.0, dt, nsteps, t0, rotfreq, energy_every,
+                      drift_out, energy_log, record_every, traj, use_graph, stream);
 }

@@ -141,6 +141,16 @@ int bfe_leapfrog(bfe_ctx* ctx, float* x, float* y, float* z, float* vx, float* v
                  int64_t n, double dt, int64_t nsteps, double t0, double rotfreq,
                  int energy_every, double* drift_out, double* energy_log, int record_every,
                  float* traj, int use_graph, void* stream);
+/* replaces: the per-orbit loop of utils/integrate.integrate_grid (integrate.py:
+ * 848-889, and its launch-from-y and 3D forms :760-846), where every orbit of
+ * the grid runs leapfrog_integrate with its own dtime (:871).  Same as
+ * bfe_leapfrog with dt (device float64 [n]) giving each orbit its step size:
+ * orbit i is at time t0 + k dt[i] after k steps and sees the field rotated by
+ * 2 pi rotfreq (t0 + k dt[i]). */
+int bfe_leapfrog_dt(bfe_ctx* ctx, float* x, float* y, float* z, float* vx, float* vy, float* vz,
+                    int64_t n, const double* dt, int64_t nsteps, double t0, double rotfreq,
+                    int energy_every, double* drift_out, double* energy_log, int record_every,
+                    float* traj, int use_graph, void* stream);
 
 /* ---- batched snapshots ---------------------------------------------------- */
 /* Many snapshots in one launch: the particles of nsnap snapshots are

@@ -18,6 +18,8 @@ source tree).  Where the reference has an arithmetic quirk, the function takes
   * EOF ``force_eval`` no_odd mask is |cos(pi m/2)| (~6e-17, not 0) for odd m
     (eof.py:833)                                              -> compat
   * EOF upper-edge linear extrapolation (eof.py:414-422): always reproduced.
+  * ``compute_timestep`` uses dfz*hfz as the vertical force (integrate.py:249,
+    251)                                                      -> compat_dt
   * cmap=2 (xi = ln r) raises NameError in the reference
     (halo_methods.py:195, spheresl.py:128-130): the intended map is used and
     xi is clamped to the table range.
@@ -614,16 +616,22 @@ def leapfrog(F: FieldSpec, nint, dt, pos, vel, rotfreq=0.0, compat=False):
     """integrate.py:117-155 for many orbits at once (no apse stop):
     x_k+1 = x_k + v_k dt + a_k dt^2/2;  a_k+1 = F(x_k+1, 2 pi rotfreq t_k+1);
     v_k+1 = v_k + (a_k + a_k+1) dt/2.  Returns dict of (norb, nint) arrays
-    T, X, Y, Z, VX, VY, VZ, P, FX, FY, FZ."""
+    T, X, Y, Z, VX, VY, VZ, P, FX, FY, FZ.  dt may be an (norb,) array of
+    per-orbit step sizes (then T is (norb, nint) too)."""
     pos = np.atleast_2d(np.asarray(pos, dtype=np.float64))
     vel = np.atleast_2d(np.asarray(vel, dtype=np.float64))
     norb = pos.shape[0]
-    times = np.arange(0, nint) * dt
-    barpos = 2.0 * np.pi * rotfreq * times
+    if np.ndim(dt) == 0:
+        times = np.arange(0, nint) * dt
+        barpos = np.broadcast_to(2.0 * np.pi * rotfreq * times, (norb, nint))
+    else:
+        dt = np.asarray(dt, dtype=np.float64)
+        times = dt[:, None] * np.arange(0, nint)[None, :]
+        barpos = 2.0 * np.pi * rotfreq * times
     out = {k: np.zeros((norb, nint)) for k in ("X", "Y", "Z", "VX", "VY", "VZ", "P", "FX", "FY", "FZ")}
     x, y, z = pos[:, 0].copy(), pos[:, 1].copy(), pos[:, 2].copy()
     vx, vy, vz = vel[:, 0].copy(), vel[:, 1].copy(), vel[:, 2].copy()
-    fx, fy, fz, p = total_force(F, x, y, z, barpos[0], compat)
+    fx, fy, fz, p = total_force(F, x, y, z, barpos[:, 0], compat)
     for k, v in zip(("X", "Y", "Z", "VX", "VY", "VZ", "P", "FX", "FY", "FZ"),
                     (x, y, z, vx, vy, vz, p, fx, fy, fz)):
         out[k][:, 0] = v
@@ -631,7 +639,7 @@ def leapfrog(F: FieldSpec, nint, dt, pos, vel, rotfreq=0.0, compat=False):
         x = x + vx * dt + 0.5 * fx * dt ** 2.0
         y = y + vy * dt + 0.5 * fy * dt ** 2.0
         z = z + vz * dt + 0.5 * fz * dt ** 2.0
-        nfx, nfy, nfz, p = total_force(F, x, y, z, barpos[step], compat)
+        nfx, nfy, nfz, p = total_force(F, x, y, z, barpos[:, step], compat)
         vx = vx + 0.5 * (fx + nfx) * dt
         vy = vy + 0.5 * (fy + nfy) * dt
         vz = vz + 0.5 * (fz + nfz) * dt
@@ -641,3 +649,71 @@ def leapfrog(F: FieldSpec, nint, dt, pos, vel, rotfreq=0.0, compat=False):
             out[k][:, step] = v
     out["T"] = times
     return out
+
+
+# ----------------------------------------------------------------------------
+# R24: compute_timestep and the orbit-grid drivers (utils/integrate.py:217-272,
+# 760-967)
+# ----------------------------------------------------------------------------
+
+def compute_timestep(F: FieldSpec, pos, vel, dyn_res=100.0, compat_dt=True, compat=False):
+    """integrate.py:217-272 for many start points (pos, vel (norb, 3)):
+    eps = 1e-10 (:241); vtot = |v| + eps; forces from return_forces_cart at
+    the unrotated start (:247); dtr = x.(fx, fy, fz') and
+    atot = |(fx, fy, fz')| + eps with fz' = dfz*hfz when compat_dt (the
+    reference's product, :249, :251), else dfz+hfz; ptot = |Phi_d + Phi_h|;
+    criteria dtd = 1/sqrt(vtot+eps), dtv = sqrt(vtot/(atot+eps)),
+    dta = ptot/(|dtr|+eps), dtA = sqrt(ptot/(atot^2+eps)) (:256-259); the
+    smallest over dyn_res (:270).  Returns (norb,) dt and (4, norb) criteria."""
+    pos = np.atleast_2d(np.asarray(pos, dtype=np.float64))
+    vel = np.atleast_2d(np.asarray(vel, dtype=np.float64))
+    eps = 1.e-10
+    vtot = np.sum(vel ** 2.0, axis=1) ** 0.5 + eps
+    dfx, hfx, dfy, hfy, dfz, hfz, dp, hp = return_forces_cart(F, pos[:, 0], pos[:, 1], pos[:, 2],
+                                                              0.0, compat)
+    fz = dfz * hfz if compat_dt else dfz + hfz
+    dtr = pos[:, 0] * (dfx + hfx) + pos[:, 1] * (dfy + hfy) + pos[:, 2] * fz
+    atot = ((dfx + hfx) ** 2.0 + (dfy + hfy) ** 2.0 + fz ** 2.0) ** 0.5 + eps
+    ptot = np.abs(dp + hp)
+    crit = np.array([1.0 / np.sqrt(vtot + eps), np.sqrt(vtot / (atot + eps)),
+                     ptot / (np.abs(dtr) + eps), np.sqrt(ptot / (atot * atot + eps))])
+    return np.min(crit, axis=0) / dyn_res, crit
+
+
+def integrate_grid(F: FieldSpec, rads, vels, nint, dt, rotfreq=0.0, dyn_res=100.0, zs=None,
+                   vzs=None, launch="x", compat_dt=True, compat=False):
+    """integrate_grid / _launchy / _3D / _3D_launchy (integrate.py:760-922):
+    for each rad, vel (, z, vz) in loop order, start [rad, 0, z] (launch x)
+    or [0, rad, z] (launch y) with velocity [0, vel, vz]; dtime =
+    max(compute_timestep, dt) (:871); the leapfrog for nint steps; rows
+    X, Y, TX, TY, VX, VY, T (2D) or X, Y, Z, TX, TY, VX, VY, VZ, T (3D), with
+    (TX, TY) the counter-clockwise rotation by barpos when barpos has a
+    negative value, else the clockwise one (:180-188).  Returns the grid
+    array (nrad, nvel, [nz, nvz,] 7 | 9, nint) and the per-orbit dtime."""
+    three = zs is not None
+    rads, vels = np.atleast_1d(rads).astype(np.float64), np.atleast_1d(vels).astype(np.float64)
+    zl = np.atleast_1d(zs).astype(np.float64) if three else np.zeros(1)
+    vzl = np.atleast_1d(vzs).astype(np.float64) if three else np.zeros(1)
+    starts = []
+    for rad in rads:
+        for vel in vels:
+            for z in zl:
+                for vz in vzl:
+                    p = [rad, 0.0, z] if launch == "x" else [0.0, rad, z]
+                    starts.append((p, [0.0, vel, vz]))
+    pos = np.array([s[0] for s in starts])
+    vel = np.array([s[1] for s in starts])
+    dtc, _ = compute_timestep(F, pos, vel, dyn_res, compat_dt, compat)
+    dtime = np.maximum(dtc, dt)
+    o = leapfrog(F, nint, dtime, pos, vel, rotfreq, compat)
+    barpos = 2.0 * np.pi * rotfreq * o["T"]
+    c, s = np.cos(barpos), np.sin(barpos)
+    ccw = np.min(barpos, axis=1, keepdims=True) < 0.0
+    TX = np.where(ccw, c * o["X"] - s * o["Y"], c * o["X"] + s * o["Y"])
+    TY = np.where(ccw, s * o["X"] + c * o["Y"], -1.0 * s * o["X"] + c * o["Y"])
+    rows = dict(o, TX=TX, TY=TY)
+    keys = ("X", "Y", "Z", "TX", "TY", "VX", "VY", "VZ", "T") if three else \
+        ("X", "Y", "TX", "TY", "VX", "VY", "T")
+    arr = np.stack([rows[k] for k in keys], 1)
+    shape = (rads.size, vels.size, zl.size, vzl.size) if three else (rads.size, vels.size)
+    return arr.reshape(shape + arr.shape[1:]), dtime

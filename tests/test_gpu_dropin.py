@@ -204,3 +204,101 @@ def test_leapfrog_integrate_apse(files, golden):
     cut = leapfrog_integrate(F, 200, float(golden["leap_dt"]), init[:3], init[3:], apse=True, ap_max=1)
     assert cut["X"].size == n
     assert np.array_equal(cut["X"], full["X"][:n])
+
+
+def _spec(golden):
+    return O.FieldSpec(eof_table(golden), golden["eof_cos"], golden["eof_sin"], sl_table(golden),
+                       golden["sl_coef"])
+
+
+def _grid_starts_f32(rads, vels, zs=None, vzs=None, launch="x"):
+    from exptool_amd.utils.integrate import _grid_starts
+    shape, pos, vel = _grid_starts(rads, vels, zs, vzs, launch)
+    return shape, *_f32(pos, vel)
+
+
+def test_compute_timestep_matches_oracle(files, golden):
+    from exptool_amd.utils.integrate import compute_timestep
+    F = _fields(files, golden)                         # compat=True: dfz*hfz as the reference
+    _, pos, vel = _grid_starts_f32([0.3, 1.0, 2.0, 3.5], [0.1, 0.35, 0.6], zs=[0.0, 0.08], vzs=[0.0])
+    spec = _spec(golden)
+    for compat in (True, False):
+        got = compute_timestep(F, pos, vel, dyn_res=100., compat=None if compat else False)
+        ref, _ = O.compute_timestep(spec, pos.T, vel.T, 100.0, compat_dt=compat, compat=True)
+        assert np.max(np.abs(got / ref - 1.0)) <= 1e-4, compat
+        one = compute_timestep(F, pos[:, 5], vel[:, 5], dyn_res=100., compat=None if compat else False)
+        assert isinstance(one, float) and one == got[5]
+
+
+def test_integrate_grid_matches_oracle(files, golden):
+    from exptool_amd.utils import integrate as I
+    F = _fields(files, golden)
+    spec = _spec(golden)
+    rads, vels, nint = np.float32([0.6, 1.5, 3.0]), np.float32([0.25, 0.45]), 200
+    _, pos, vel = _grid_starts_f32(rads, vels)
+    crit, _ = O.compute_timestep(spec, pos.T, vel.T, 100.0, compat_dt=True, compat=True)
+    guard = float(np.median(crit))                     # some orbits take the guard, some their own
+    got = I.integrate_grid(rads, vels, F, nint, guard, -0.05, False, -1, -1, 100., 1000, 0)
+    ref, dtime = O.integrate_grid(spec, rads, vels, nint, guard, rotfreq=-0.05, compat_dt=True,
+                                  compat=True)
+    assert got.shape == ref.shape == (3, 2, 7, nint)
+    for row in range(7):
+        assert relmax(got[:, :, row], ref[:, :, row]) <= 1e-4, row
+    assert np.max(np.abs(got[:, :, 6, 1].ravel() / dtime - 1)) <= 1e-4
+    # 3D grid launched from the y axis, clockwise pattern frame
+    zs, vzs = np.float32([0.0, 0.1]), np.float32([0.0, 0.03])
+    got = I.integrate_grid_3D_launchy(rads[:2], vels, F, 120, guard, 0.05, False, -1, -1, 100., 1000,
+                                      zs, vzs, 0)
+    ref, _ = O.integrate_grid(spec, rads[:2], vels, 120, guard, rotfreq=0.05, zs=zs, vzs=vzs,
+                              launch="y", compat_dt=True, compat=True)
+    assert got.shape == ref.shape == (2, 2, 2, 2, 9, 120)
+    for row in range(9):
+        assert relmax(got[..., row, :], ref[..., row, :]) <= 1e-4, row
+    with pytest.raises(ValueError):
+        I.integrate_grid_3D(rads, vels, F, 10, guard, 0.0, False, -1, -1, 100., 1000, None, vzs, 0)
+
+
+def test_leapfrog_dt_equals_single_step_runs(files, golden):
+    """bfe_leapfrog_dt: every orbit bit for bit as a bfe_leapfrog run of that
+    orbit alone with its step size."""
+    import torch
+    F = _fields(files, golden)
+    F.freeze()
+    s = F.system
+    init = golden["leap_init"]
+    dts = np.array([0.01, 0.0037, 0.02])
+    st = [torch.tensor(init[:, k], dtype=torch.float32, device="cuda") for k in range(6)]
+    out = s.leapfrog(st, dt=torch.tensor(dts, device="cuda"), nsteps=150, rotfreq=0.05,
+                     energy_every=50, record_every=1)
+    traj = out["traj"].cpu().numpy()
+    for i in range(3):
+        one = [torch.tensor(init[i:i + 1, k], dtype=torch.float32, device="cuda") for k in range(6)]
+        r = s.leapfrog(one, dt=float(dts[i]), nsteps=150, rotfreq=0.05, energy_every=50,
+                       record_every=1)
+        assert np.array_equal(traj[:, :, i], r["traj"].cpu().numpy()[:, :, 0]), i
+        assert out["drift"][i].item() == r["drift"][0].item()
+    with pytest.raises(ValueError):
+        s.leapfrog(st, dt=torch.tensor(dts[:2], device="cuda"), nsteps=5)
+
+
+def test_do_integrate_multi_and_orbit_map(files, golden, tmp_path):
+    from exptool_amd.utils import integrate as I
+    F = _fields(files, golden)
+    rads, vels = np.linspace(0.5, 3.0, 7), np.array([0.2, 0.4, 0.6])
+    full = I.do_integrate_multi(rads, vels, F, 64, 0.002, -0.03, False, -1, -1, 50., 1000, nprocs=3)
+    assert full.dtype == np.float32 and full.shape == (7, 3, 7, 64)
+    one = I.integrate_grid(rads, vels, F, 64, 0.002, -0.03, False, -1, -1, 50., 1000, 0)
+    assert np.array_equal(full, one.astype("f4"))
+    parts = I.multi_compute_integration(I.redistribute_arrays(rads, 3), 3, vels, F, 64, 0.002, -0.03,
+                                        False, -1, -1, 50., 1000, "x")
+    assert [p.shape[0] for p in parts] == [3, 2, 2]
+    assert np.array_equal(I.re_form_orbit_arrays(parts), full)
+    with open(tmp_path / "omap.txt", "w") as f:
+        I.print_orbit_array(f, full)
+    D = I.read_integrations(str(tmp_path / "omap.txt"))
+    for line, (i, j) in enumerate(np.ndindex(7, 3)):
+        assert np.float32(D["R_0"][line]) == np.float32(rads[i])
+        assert np.float32(D["V_0"][line]) == np.float32(vels[j])
+        assert np.array_equal(np.float32(D["TX"][line]), full[i, j, 2])
+    with pytest.raises(ValueError):
+        I.do_integrate_multi(rads, vels, F, 8, 0.002, 0.0, False, -1, -1, 50., 1000, threedee=True)

@@ -1,0 +1,77 @@
+"""Host side of the orbit-grid drivers (utils/integrate.py:196-214, 440-568,
+970-1039): start points, the radius split, the float32 stacking and the text
+orbit-map format, checked against values written out by hand.  No device."""
+import numpy as np
+import pytest
+
+from exptool_amd.utils import integrate as I
+
+
+def test_gen_init_step_and_grid_starts():
+    assert I.gen_init_step(2.0, 0.5) == ([2.0, 0.0, 0.0], [0.0, 0.5, 0.0])
+    assert I.gen_init_step(2.0, 0.5, z0=0.1, zvel0=0.2) == ([2.0, 0.0, 0.1], [0.0, 0.5, 0.2])
+    shape, pos, vel = I._grid_starts([1.0, 2.0, 3.0], [0.1, 0.2])
+    assert shape == (3, 2)
+    # radius outermost, then velocity (integrate.py:862-869)
+    np.testing.assert_array_equal(pos[0], [1, 1, 2, 2, 3, 3])
+    np.testing.assert_array_equal(vel[1], [0.1, 0.2] * 3)
+    assert not pos[1].any() and not pos[2].any() and not vel[0].any() and not vel[2].any()
+    shape, pos, vel = I._grid_starts([1.0], [0.1, 0.2], zs=[0.0, 0.5], vzs=[0.3], launch="y")
+    assert shape == (1, 2, 2, 1)
+    np.testing.assert_array_equal(pos, [[0, 0, 0, 0], [1, 1, 1, 1], [0, 0.5, 0, 0.5]])
+    np.testing.assert_array_equal(vel, [[0, 0, 0, 0], [0.1, 0.1, 0.2, 0.2], [0.3] * 4])
+    with pytest.raises(ValueError):
+        I._grid_starts([1.0], [0.1], launch="z")
+
+
+def test_redistribute_and_re_form(capsys):
+    parts = I.redistribute_arrays(np.arange(10.0), 3)
+    assert [list(p) for p in parts] == [[0, 1, 2, 3], [4, 5, 6], [7, 8, 9]]
+    assert "Each processor has 3 radii." in capsys.readouterr().out
+    parts = I.redistribute_arrays(np.arange(4.0), 4)
+    assert [list(p) for p in parts] == [[0], [1], [2], [3]]
+    a = [np.full((n, 2, 7, 3), float(n)) for n in (4, 3, 3)]
+    out = I.re_form_orbit_arrays(a)
+    assert out.dtype == np.float32 and out.shape == (10, 2, 7, 3)
+    assert np.all(out[:4] == 4) and np.all(out[4:] == 3)
+    out3 = I.re_form_orbit_arrays_3D([np.zeros((2, 1, 2, 2, 9, 3)), np.ones((1, 1, 2, 2, 9, 3))])
+    assert out3.shape == (3, 1, 2, 2, 9, 3) and out3.dtype == np.float32
+
+
+def test_orbit_map_text_format(tmp_path):
+    """print_orbit_array writes nsteps, R_0 (X[0]), V_0 (VY[0]), dT (T[1]),
+    then X, Y, TX, TY, VX, VY, each value followed by a space."""
+    o = np.zeros((1, 1, 7, 2), np.float32)
+    o[0, 0] = [[1.5, 1.25], [0.0, 0.5], [1.5, 1.0], [0.0, -0.75], [0.0, -0.125], [0.5, 0.5],
+               [0.0, 0.25]]
+    path = tmp_path / "omap.txt"
+    with open(path, "w") as f:
+        I.print_orbit_array(f, o)
+    assert path.read_text() == ("2 1.5 0.5 0.25 1.5 1.25 0.0 0.5 1.5 1.0 0.0 -0.75 0.0 -0.125 "
+                                "0.5 0.5 \n")
+    D = I.read_integrations(str(path))
+    assert D["R_0"][0] == 1.5 and D["V_0"][0] == 0.5 and D["dT"][0] == 0.25
+    assert D["X"][0] == [1.5, 1.25] and D["TY"][0] == [0.0, -0.75] and D["VY"][0] == [0.5, 0.5]
+    o3 = np.zeros((1, 1, 1, 1, 9, 2), np.float32)
+    o3[0, 0, 0, 0, :, 0] = [1.0, 0.0, 0.25, 1.0, 0.0, 0.0, 0.75, 0.125, 0.0]
+    o3[0, 0, 0, 0, :, 1] = [0.5, 0.5, 0.0, 0.5, 0.5, -0.5, 0.5, 0.0, 0.5]
+    path3 = tmp_path / "omap3.txt"
+    with open(path3, "w") as f:
+        I.print_orbit_array_3D(f, o3)
+    assert path3.read_text() == ("2 1.0 0.75 0.25 0.125 0.5 1.0 0.5 0.0 0.5 0.25 0.0 1.0 0.5 "
+                                 "0.0 0.5 0.0 -0.5 0.75 0.5 0.125 0.0 \n")
+    D = I.read_integrations_3D(str(path3))
+    assert (D["R_0"][0], D["V_0"][0], D["Z_0"][0], D["VZ_0"][0], D["dT"][0]) == (1.0, 0.75, 0.25, 0.125, 0.5)
+    assert D["Z"][0] == [0.25, 0.0] and D["VZ"][0] == [0.125, 0.0]
+
+
+def test_orbit_map_many_lines(tmp_path):
+    rng = np.random.default_rng(0)
+    o = rng.normal(size=(3, 2, 7, 5)).astype(np.float32)
+    with open(tmp_path / "m.txt", "w") as f:
+        I.print_orbit_array(f, o)
+    D = I.read_integrations(str(tmp_path / "m.txt"))
+    for line, (i, j) in enumerate(np.ndindex(3, 2)):
+        for row, k in enumerate(("X", "Y", "TX", "TY", "VX", "VY")):
+            assert np.array_equal(np.float32(D[k][line]), o[i, j, row])
+        assert np.float32(D["dT"][line]) == o[i, j, 6, 1]

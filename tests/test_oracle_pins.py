@@ -264,3 +264,56 @@ def test_leapfrog_reversible_and_second_order(small_field):
     ref = end(0.0025)
     e1, e2 = np.abs(end(0.02) - ref).max(), np.abs(end(0.01) - ref).max()
     assert 2.5 < e1 / e2 < 6.0
+
+
+def test_compute_timestep_hand_values(monkeypatch):
+    """integrate.py:217-272 on forces chosen so each criterion is known by
+    hand.  Start 0: x = (2, 0, 0), |v| = 16, F = (-30, 0, dfz=1 / hfz=40),
+    Phi = -10: with the reference's dfz*hfz |a| = 50 and dtA = sqrt(10)/50
+    is the smallest; with dfz+hfz |a| = sqrt(2581).  Start 1: x = (0, 0, 3),
+    |v| = 1, F = (0, 0, dfz=2 / hfz=-1), Phi = -0.5: dta = 0.5/|x.a| wins,
+    1/12 with the product (x.a = -6) and 1/6 with the sum (x.a = 3)."""
+    vals = np.array([[-10.0, 0.0], [-20.0, 0.0], [0.0, 0.0], [0.0, 0.0], [1.0, 2.0], [40.0, -1.0],
+                     [-4.0, -0.25], [-6.0, -0.25]])
+    monkeypatch.setattr(O, "return_forces_cart", lambda F, x, y, z, rotpos=0.0, compat=False: tuple(vals))
+    pos = np.array([[2.0, 0.0, 0.0], [0.0, 0.0, 3.0]])
+    vel = np.array([[0.0, 16.0, 0.0], [0.0, 1.0, 0.0]])
+    dt, crit = O.compute_timestep(None, pos, vel, dyn_res=100.0, compat_dt=True)
+    np.testing.assert_allclose(dt, [np.sqrt(10.0) / 5000.0, 1.0 / 1200.0], rtol=1e-8)
+    np.testing.assert_allclose(crit[:, 0], [0.25, np.sqrt(16 / 50), 10 / 60, np.sqrt(10) / 50], rtol=1e-8)
+    dt, _ = O.compute_timestep(None, pos, vel, dyn_res=100.0, compat_dt=False)
+    np.testing.assert_allclose(dt, [np.sqrt(10.0 / 2581.0) / 100.0, 1.0 / 600.0], rtol=1e-8)
+
+
+def test_orbit_grid_layout_and_per_orbit_steps(small_field):
+    """integrate_grid = one leapfrog per start with its own dtime =
+    max(compute_timestep, dt); rows and start points in the reference's
+    order; (TX, TY) = e^{+-i barpos} (X + iY) by the sign of barpos."""
+    F = small_field
+    rads, vels, nint = np.array([1.0, 2.5]), np.array([0.3, 0.6]), 40
+    crit_dt, _ = O.compute_timestep(F, [[1.0, 0, 0], [1.0, 0, 0], [2.5, 0, 0], [2.5, 0, 0]],
+                                    [[0, 0.3, 0], [0, 0.6, 0], [0, 0.3, 0], [0, 0.6, 0]])
+    guard = float(np.median(crit_dt))
+    for rotfreq in (-0.05, 0.05):
+        arr, dtime = O.integrate_grid(F, rads, vels, nint, guard, rotfreq=rotfreq)
+        assert arr.shape == (2, 2, 7, nint)
+        np.testing.assert_array_equal(dtime, np.maximum(crit_dt, guard))
+        assert np.any(crit_dt < guard) and np.any(crit_dt > guard)
+        for i, r in enumerate(rads):
+            for j, v in enumerate(vels):
+                k = 2 * i + j
+                o = O.leapfrog(F, nint, dtime[k], [[r, 0, 0]], [[0, v, 0]], rotfreq=rotfreq)
+                for row, key in enumerate(("X", "Y", None, None, "VX", "VY")):
+                    if key:
+                        np.testing.assert_allclose(arr[i, j, row], o[key][0], rtol=0, atol=1e-13)
+                np.testing.assert_allclose(arr[i, j, 6], dtime[k] * np.arange(nint), rtol=1e-15)
+                b = 2 * np.pi * rotfreq * arr[i, j, 6]
+                rot = np.exp(1j * b if rotfreq < 0 else -1j * b) * (arr[i, j, 0] + 1j * arr[i, j, 1])
+                np.testing.assert_allclose(arr[i, j, 2] + 1j * arr[i, j, 3], rot, atol=1e-12)
+    zs, vzs = np.array([0.0, 0.1]), np.array([0.0, 0.02, 0.04])
+    arr, _ = O.integrate_grid(F, rads, vels, 5, guard, zs=zs, vzs=vzs, launch="y")
+    assert arr.shape == (2, 2, 2, 3, 9, 5)
+    for i, j, a, b in np.ndindex(2, 2, 2, 3):
+        start = arr[i, j, a, b, :, 0]
+        np.testing.assert_array_equal(start[[0, 1, 2, 5, 6, 7, 8]],
+                                      [0.0, rads[i], zs[a], 0.0, vels[j], vzs[b], 0.0])
