@@ -215,6 +215,23 @@ def main():
                  "energy_drift_p99": float(np.quantile(dr, 0.99)),
                  "energy_drift_max": float(np.max(dr))}
         progress(f"orbits: {orbit['orbit_steps_per_s']:.3e} orbit-steps/s")
+        # per-orbit step sizes (orbit grids) and RK4 (four evaluations a step)
+        st = [t.clone() for t in st0]
+        dtv = torch.full((no,), 1e-3, dtype=torch.float64, device=st[0].device)
+        t0 = time.perf_counter()
+        sysm.leapfrog(st, dt=dtv, nsteps=a.orbit_steps, energy_every=100)
+        torch.cuda.synchronize()
+        orbit["per_orbit_dt_steps_per_s"] = no * a.orbit_steps / (time.perf_counter() - t0)
+        nrk = max(1, a.orbit_steps // 4)
+        st = [t.clone() for t in st0]
+        sysm.rk4([t.clone() for t in st0], dt=4e-3, nsteps=4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sysm.rk4(st, dt=4e-3, nsteps=nrk)
+        torch.cuda.synchronize()
+        orbit["rk4_steps_per_s"] = no * nrk / (time.perf_counter() - t0)
+        progress(f"per-orbit dt: {orbit['per_orbit_dt_steps_per_s']:.3e} orbit-steps/s, "
+                 f"rk4: {orbit['rk4_steps_per_s']:.3e} steps/s")
         if a.mlp:
             surrogate = mlp_stage(a, sysm, st0, orbit)
             progress(f"surrogate: {surrogate['orbit_steps_per_s']:.3e} orbit-steps/s")

@@ -25,11 +25,12 @@ COMPAT_EOF_MASK = 0x8
 NO_ODD = 0x10
 ACC_BF16 = 0x20
 COMPAT_ALL = 0xF
+COMPAT_RK4_FRAME = 0x40
 
 EXPORTS = (
     "bfe_create", "bfe_destroy", "bfe_last_error", "bfe_sl_tables_set", "bfe_eof_tables_set",
     "bfe_sl_accumulate", "bfe_eof_accumulate", "bfe_sl_eval", "bfe_eof_eval", "bfe_field_set",
-    "bfe_field_eval_cart", "bfe_leapfrog", "bfe_leapfrog_dt", "bfe_mlp_set_params", "bfe_mlp_get_params",
+    "bfe_field_eval_cart", "bfe_leapfrog", "bfe_leapfrog_dt", "bfe_rk4", "bfe_mlp_set_params", "bfe_mlp_get_params",
     "bfe_mlp_forward", "bfe_mlp_gradient", "bfe_mlp_train_step", "bfe_mlp_leapfrog",
     "bfe_eof_accumulate_batch", "bfe_sl_accumulate_batch", "bfe_field_set_batch",
     "bfe_field_eval_batch", "bfe_accumulate_disc_halo",
@@ -61,6 +62,7 @@ def _declare(lib):
                                                   _I32, _VP]),
         "bfe_leapfrog_dt": (_I32, [_VP] + [_VP] * 6 + [_I64, _VP, _I64, _F64, _F64, _I32, _VP, _VP, _I32,
                                                      _VP, _I32, _VP]),
+        "bfe_rk4": (_I32, [_VP] + [_VP] * 6 + [_I64, _F64, _I64, _F64, _U32, _I32, _VP, _VP]),
         "bfe_eof_accumulate_batch": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _I32, _U32, _VP, _VP, _VP]),
         "bfe_sl_accumulate_batch": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _I32, _U32, _VP, _VP]),
         "bfe_field_set_batch": (_I32, [_VP, _VP, _VP, _VP, _I32, _F64, _I32, _I32, _I32, _I32, _U32, _VP]),

@@ -6,8 +6,9 @@ Table I/O (host):
   set_table_params  eof.py:316-347
   write_eof_cache   (writer of the same layout; the reference has none)
 Device work (libexptool_bfe.so through exptool_amd.bfe.BFESystem):
-  accumulate               eof.py:492-640
+  accumulate               eof.py:492-640 (VAR subsample sums: one batched launch)
   compute_coefficients     eof.py:1158-1260
+  compute_variance / compute_sn   eof.py:2135-2226 (host, coefficient-sized)
   accumulated_eval_particles / compute_forces   eof.py:989-1144, :1263-1317
   force_eval               eof.py:756-872 (many points per call)
 Return types follow the reference: float64 NumPy arrays and EOF_Object.
@@ -192,10 +193,35 @@ def _nan_to_origin(x, y, z):
 # device-backed API
 # ----------------------------------------------------------------------------
 
-def accumulate_tables(system, x, y, z, m):
-    """Coefficients (cos, sin) as float64 numpy arrays from a BFESystem."""
+def accumulate_tables(system, x, y, z, m, VAR=0):
+    """Coefficients (cos, sin) as float64 numpy arrays from a BFESystem; with
+    VAR > 0 also the subsample sums (cos2, sin2), each (VAR, mmax+1, norder)."""
+    from .._lib import f32
+    x, y, z, m = (f32(np.asarray(a, dtype=np.float64), system.device) for a in (x, y, z, m))
     c, s = system.eof_accumulate(x, y, z, m)
-    return c.cpu().numpy(), s.cpu().numpy()
+    if not VAR:
+        return c.cpu().numpy(), s.cpu().numpy()
+    c2, s2 = variance_samples(system, x, y, z, m, VAR)
+    return c.cpu().numpy(), s.cpu().numpy(), c2, s2
+
+
+def variance_samples(system, x, y, z, m, VAR):
+    """The VAR branch of eof.accumulate (eof.py:554-574): VAR subsamples of
+    floor(sqrt(n)) particles drawn with replacement by np.random.randint, in
+    the reference's call order on NumPy's global generator (so np.random.seed
+    gives the reference's draws), each summed like the full accumulate
+    (upscale 1).  The subsamples are gathered on the device and accumulated
+    in one batched launch.  x, y, z, m are float32 device tensors.  Returns
+    (cos2, sin2) float64 (VAR, mmax+1, norder) numpy arrays."""
+    import torch
+    VAR = int(VAR)
+    n = x.numel()
+    k = int(np.floor(np.sqrt(n)))
+    use = np.concatenate([np.random.randint(n, size=k) for _ in range(VAR)]).astype(np.int64)
+    idx = torch.from_numpy(use).to(system.device)
+    g = [t.index_select(0, idx) for t in (x, y, z, m)]
+    c2, s2 = system.eof_accumulate_batch(*g, [k] * VAR)
+    return c2.cpu().numpy(), s2.cpu().numpy()
 
 
 def compute_coefficients(PSPInput, eof_file, verbose=1, no_odd=False, nprocs_max=-1, VAR=False,
@@ -203,9 +229,9 @@ def compute_coefficients(PSPInput, eof_file, verbose=1, no_odd=False, nprocs_max
     """eof.py:1158-1260 on the GPU.  NaN particles are moved to the origin
     (the reference intends this at :1190-1204 but raises under NumPy >= 2);
     nanblock=True raises instead.  no_odd has no effect, as in the reference
-    accumulate (eof.py:545-551).  VAR (jackknife) is not implemented."""
-    if VAR:
-        raise NotImplementedError("VAR jackknife accumulation is not implemented")
+    accumulate (eof.py:545-551).  VAR > 0 (True counts as 1) adds the
+    subsample sums .cos2 / .sin2 of accumulate's VAR branch (eof.py:1256-1258,
+    the single-process form :1245-1246)."""
     x, y, z, m = _particle_arrays(PSPInput)
     if nanblock and (np.isnan(x) | np.isnan(y) | np.isnan(z)).any():
         raise ValueError(f"eof.compute_coefficients: NaN values in {getattr(PSPInput, 'filename', '')}")
@@ -219,7 +245,10 @@ def compute_coefficients(PSPInput, eof_file, verbose=1, no_odd=False, nprocs_max
     out.eof_file = eof_file
     out.mmax = sysm.eof["mmax"]
     out.nmax = sysm.eof["norder"]
-    out.cos, out.sin = accumulate_tables(sysm, x, y, z, m)
+    if VAR:
+        out.cos, out.sin, out.cos2, out.sin2 = accumulate_tables(sysm, x, y, z, m, VAR=int(VAR))
+    else:
+        out.cos, out.sin = accumulate_tables(sysm, x, y, z, m)
     return out
 
 
@@ -227,9 +256,8 @@ def accumulate(ParticleInstance, potC, potS, MMAX, NMAX, XMIN, dX, YMIN, dY, NUM
                HSCALE, CMAP, verbose=0, no_odd=False, VAR=0):
     """eof.py:492-640 with tables passed as arrays.  The tables are uploaded to
     a device context for the call (use compute_coefficients or a BFESystem to
-    keep them resident)."""
-    if VAR:
-        raise NotImplementedError("VAR jackknife accumulation is not implemented")
+    keep them resident).  VAR > 0 returns (cos, sin, cos2, sin2) as the
+    reference (eof.py:554-574)."""
     from ..bfe import BFESystem
     potC, potS = np.asarray(potC), np.asarray(potS)
     z0 = np.zeros(potC.shape)
@@ -237,8 +265,38 @@ def accumulate(ParticleInstance, potC, potS, MMAX, NMAX, XMIN, dX, YMIN, dY, NUM
              ascale=ASCALE, hscale=HSCALE, XMIN=XMIN, dX=dX, YMIN=YMIN, dY=dY, potC=potC,
              rforceC=z0, zforceC=z0, potS=potS, rforceS=z0, zforceS=z0, dens=0)
     x, y, z, m = _particle_arrays(ParticleInstance)
+    if VAR:
+        c, sn, c2, s2 = accumulate_tables(BFESystem(eof=t), x, y, z, m, VAR=int(VAR))
+        return (c[:MMAX + 1, :NMAX], sn[:MMAX + 1, :NMAX], c2[:, :MMAX + 1, :NMAX],
+                s2[:, :MMAX + 1, :NMAX])
     c, sn = accumulate_tables(BFESystem(eof=t), x, y, z, m)
     return c[:MMAX + 1, :NMAX], sn[:MMAX + 1, :NMAX]
+
+
+def compute_variance(ParticleInstance, accum_cos, accum_sin, accum_cos2, accum_sin2):
+    """eof.py:2135-2193: with w = 1/sum(m), varC = cos2 w^2 - (w cos)^2 / n
+    and b_Hall facC = (w cos)^2 / (varC/(n+1) + (w cos)^2 + 1e-10), the same
+    for sin.  Returns (varC, varS, facC, facS)."""
+    m = np.asarray(_particle_arrays(ParticleInstance)[3], dtype=np.float64)
+    wgt = 1. / np.sum(m)
+    nrm = wgt * wgt
+    srm = 1. / float(m.size)
+    sqrC = (np.asarray(accum_cos) * wgt) ** 2
+    sqrS = (np.asarray(accum_sin) * wgt) ** 2
+    varC = np.asarray(accum_cos2) * nrm - srm * sqrC
+    varS = np.asarray(accum_sin2) * nrm - srm * sqrS
+    facC = sqrC / (varC / (float(m.size) + 1.) + sqrC + 1.0e-10)
+    facS = sqrS / (varS / (float(m.size) + 1.) + sqrS + 1.0e-10)
+    return varC, varS, facC, facS
+
+
+def compute_sn(ParticleInstance, accum_cos, accum_sin, accum_cos2, accum_sin2):
+    """eof.py:2198-2226: signal to noise (coef^2 / var)^(1/2) with the
+    variance of compute_variance.  Returns (snC, snS)."""
+    varC, varS, _, _ = compute_variance(ParticleInstance, accum_cos, accum_sin, accum_cos2,
+                                        accum_sin2)
+    c, s = np.asarray(accum_cos), np.asarray(accum_sin)
+    return (c * c / varC) ** 0.5, (s * s / varS) ** 0.5
 
 
 def compute_forces(PSPInput, EOF_Object, verbose=1, nprocs=-1, m1=0, m2=1000, density=False):

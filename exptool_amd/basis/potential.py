@@ -196,6 +196,13 @@ class Fields:
             return tuple(float(v[0]) for v in c)
         return tuple(v.reshape(x.shape) for v in c)
 
+    def get_cartesian_forces_array(self, arr):
+        """Total force (fx, fy, fz) at arr = (x, y, z), the interface
+        utils/rk4.py expects of a potential (rk4.py:50); arrays of points
+        give (3, N)."""
+        fxd, fxh, fyd, fyh, fzd, fzh, _, _ = self.return_forces_cart(arr[0], arr[1], arr[2])
+        return np.array([fxd + fxh, fyd + fyh, fzd + fzh])
+
     def return_forces_cyl(self, xval, yval, zval, rotpos=0.0):
         """potential.py:389-440: (fr_disk, fr_halo, fp_disk, -fp_halo, fz_disk,
         fz_halo, -pot_disk, pot_halo) with R = sqrt(x^2+y^2)+1e-10 and

@@ -317,6 +317,27 @@ class BFESystem:
             return fx, fy, fz, pot, comp
         return fx, fy, fz, pot
 
+    def rk4(self, state, dt, nsteps, omegab=0.0, compat_frame=False, record_every=0, stream=None):
+        """Advance state = (x, y, z, vx, vy, vz) float32 device tensors in place
+        by nsteps RK4 steps in the frozen field (bfe_rk4); omegab != 0 adds the
+        pseudo-forces of a frame rotating about z (the reference's signs with
+        compat_frame).  Returns traj (nrec, 6, n) float32 or None."""
+        x = state[0]
+        n = x.numel()
+        for t in state:
+            if t.dtype != torch.float32 or not t.is_contiguous() or t.numel() != n or t.device != self.device:
+                raise ValueError("state tensors must be contiguous float32 on the context device")
+        traj = None
+        if record_every > 0:
+            traj = torch.empty((nsteps // record_every + 1, 6, n), dtype=torch.float32,
+                               device=self.device)
+        flags = _lib.COMPAT_RK4_FRAME if compat_frame else 0
+        rc = self.ctx.lib.bfe_rk4(self.ctx.h, *[ptr(t) for t in state], n, float(dt), int(nsteps),
+                                  float(omegab), flags, int(record_every), ptr(traj),
+                                  stream_ptr(stream))
+        self.ctx.check(rc, "bfe_rk4")
+        return traj
+
     def leapfrog(self, state, dt, nsteps, t0=0.0, rotfreq=0.0, energy_every=0,
                  record_every=0, use_graph=False, stream=None):
         """Advance state = (x, y, z, vx, vy, vz) float32 device tensors in place.

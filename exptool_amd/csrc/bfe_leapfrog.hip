@@ -101,7 +101,12 @@ __global__ __launch_bounds__(256) void leapfrog_init_kernel(
   }
 }
 
-__global__ __launch_bounds__(256) void leapfrog_chunk_kernel(
+// PER_ORBIT: each orbit reads its step size from a.dtv (an instantiation of
+// its own, so the common one keeps dt in scalar registers).  Two waves per
+// SIMD are asked for: the per-orbit step would otherwise take the register
+// count two past the 256 that allow it.
+template <bool PER_ORBIT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void leapfrog_chunk_kernel(
     float* __restrict__ x, float* __restrict__ y, float* __restrict__ z, float* __restrict__ vx,
     float* __restrict__ vy, float* __restrict__ vz, const int* __restrict__ perm, FieldTables T,
     LeapArgs a, OrbitAux* __restrict__ aux, double* __restrict__ ebuf, float* __restrict__ traj) {
@@ -119,7 +124,7 @@ __global__ __launch_bounds__(256) void leapfrog_chunk_kernel(
     ux = vx[i]; uy = vy[i]; uz = vz[i];
     o = aux[i];
     ax = o.ax; ay = o.ay; az = o.az;
-    dtd = step_of(a, i);
+    if (PER_ORBIT) dtd = a.dtv[i];
   }
   const float dt = (float)dtd;
   const float hdt2 = 0.5f * dt * dt;
@@ -205,6 +210,123 @@ __global__ void energy_reduce_kernel(const double* __restrict__ partE, int nchec
   log_out[c] = s;
 }
 
+// ---------------------------------------------------------------------------
+// Fourth-order Runge-Kutta (rk4.py:26-135): four field evaluations per step,
+// optionally with the pseudo-forces of a frame rotating at omegab about z.
+
+struct RK4Args {
+  double dt;
+  double omegab;         // 0 = inertial frame (rk4); else rk4_rotation
+  int frame_sign;        // +1 physical pseudo-forces, -1 the reference's signs
+  int64_t n;
+  int64_t k0, k1;        // steps (k0, k1] are taken
+  int record_every;      // 0 = no trajectory
+  int nblocks;
+};
+
+// Field force at (px, py, pz) for every live lane of the block, the block's
+// SL nodes staged in LDS when they fit.  Called by every thread of the block.
+__device__ __forceinline__ void rk4_force(const FieldTables& T, bool live, float px, float py,
+                                          float pz, float4* stage, int* smin, int* smax,
+                                          float& fx, float& fy, float& fz) {
+  SLCell cell{0.0, 0, 0.0, 0.0};
+  if (live) cell = field_sl_cell(T.slg, px, py, pz);
+  int imin = 0;
+  const bool staged = stage_sl_nodes(T, live, cell.i, stage, smin, smax, imin);
+  if (live) {
+    double cv[8];
+    if (staged)
+      field_eval_point(T, px, py, pz, cv, &cell, stage, imin);
+    else
+      field_eval_point(T, px, py, pz, cv);
+    fx = (float)(cv[0] + cv[1]);
+    fy = (float)(cv[2] + cv[3]);
+    fz = (float)(cv[4] + cv[5]);
+  }
+  __syncthreads();   // the stage is rewritten by the next evaluation
+}
+
+// Pseudo-forces of the rotating frame in the plane (rk4.py:66-90): physical
+// (Binney & Tremaine 3.117) Omega^2 R - 2 Omega x v; the reference returns
+// the negative of both terms (frame_sign = -1).
+__device__ __forceinline__ void rk4_frame(const RK4Args& a, float px, float py, float ux, float uy,
+                                          float& fx, float& fy) {
+  if (a.omegab == 0.0) return;
+  const float w = (float)a.omegab, s = (float)a.frame_sign;
+  fx += s * (w * w * px + 2.0f * w * uy);
+  fy += s * (w * w * py - 2.0f * w * ux);
+}
+
+__global__ __launch_bounds__(256) void rk4_chunk_kernel(
+    float* __restrict__ x, float* __restrict__ y, float* __restrict__ z, float* __restrict__ vx,
+    float* __restrict__ vy, float* __restrict__ vz, const int* __restrict__ perm, FieldTables T,
+    RK4Args a, float* __restrict__ traj) {
+  __shared__ float4 stage[kStageFloat4];
+  __shared__ int smin[4], smax[4];
+  const int64_t j = xcd_block(a.nblocks) * 256 + threadIdx.x;
+  if (j - threadIdx.x >= a.n) return;   // whole block past the end (XCD mapping)
+  const bool live = j < a.n;
+  const int64_t i = live ? (int64_t)perm[j] : 0;
+  float px = 0, py = 0, pz = 0, ux = 0, uy = 0, uz = 0;
+  if (live) {
+    px = x[i]; py = y[i]; pz = z[i];
+    ux = vx[i]; uy = vy[i]; uz = vz[i];
+  }
+  const float h = (float)a.dt, hh = 0.5f * h, h6 = h / 6.0f;
+  for (int64_t k = a.k0 + 1; k <= a.k1; ++k) {
+    // rk4.py:48-60: k1 at x, k2 and k3 at the half steps, k4 at the full step
+    float f1x = 0, f1y = 0, f1z = 0;
+    rk4_force(T, live, px, py, pz, stage, smin, smax, f1x, f1y, f1z);
+    rk4_frame(a, px, py, ux, uy, f1x, f1y);
+    const float r2x = ux + f1x * hh, r2y = uy + f1y * hh, r2z = uz + f1z * hh;
+    const float q2x = px + ux * hh, q2y = py + uy * hh, q2z = pz + uz * hh;
+    float f2x = 0, f2y = 0, f2z = 0;
+    rk4_force(T, live, q2x, q2y, q2z, stage, smin, smax, f2x, f2y, f2z);
+    rk4_frame(a, q2x, q2y, r2x, r2y, f2x, f2y);
+    const float r3x = ux + f2x * hh, r3y = uy + f2y * hh, r3z = uz + f2z * hh;
+    const float q3x = px + r2x * hh, q3y = py + r2y * hh, q3z = pz + r2z * hh;
+    float f3x = 0, f3y = 0, f3z = 0;
+    rk4_force(T, live, q3x, q3y, q3z, stage, smin, smax, f3x, f3y, f3z);
+    rk4_frame(a, q3x, q3y, r3x, r3y, f3x, f3y);
+    const float r4x = ux + f3x * h, r4y = uy + f3y * h, r4z = uz + f3z * h;
+    const float q4x = px + r3x * h, q4y = py + r3y * h, q4z = pz + r3z * h;
+    float f4x = 0, f4y = 0, f4z = 0;
+    rk4_force(T, live, q4x, q4y, q4z, stage, smin, smax, f4x, f4y, f4z);
+    rk4_frame(a, q4x, q4y, r4x, r4y, f4x, f4y);
+    if (live) {
+      const float nux = ux + h6 * (f1x + 2.0f * f2x + 2.0f * f3x + f4x);
+      const float nuy = uy + h6 * (f1y + 2.0f * f2y + 2.0f * f3y + f4y);
+      const float nuz = uz + h6 * (f1z + 2.0f * f2z + 2.0f * f3z + f4z);
+      px = px + h6 * (ux + 2.0f * r2x + 2.0f * r3x + r4x);
+      py = py + h6 * (uy + 2.0f * r2y + 2.0f * r3y + r4y);
+      pz = pz + h6 * (uz + 2.0f * r2z + 2.0f * r3z + r4z);
+      ux = nux; uy = nuy; uz = nuz;
+      if (traj && a.record_every > 0 && (k % a.record_every) == 0) {
+        const size_t n = (size_t)a.n;
+        float* t = traj + (size_t)(k / a.record_every) * 6 * n;
+        t[0 * n + i] = px; t[1 * n + i] = py; t[2 * n + i] = pz;
+        t[3 * n + i] = ux; t[4 * n + i] = uy; t[5 * n + i] = uz;
+      }
+    }
+  }
+  if (live) {
+    x[i] = px; y[i] = py; z[i] = pz;
+    vx[i] = ux; vy[i] = uy; vz[i] = uz;
+  }
+}
+
+// Record 0 of an RK4 trajectory: the initial state.
+__global__ void rk4_record0_kernel(const float* __restrict__ x, const float* __restrict__ y,
+                                   const float* __restrict__ z, const float* __restrict__ vx,
+                                   const float* __restrict__ vy, const float* __restrict__ vz,
+                                   int64_t n, float* __restrict__ traj) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    traj[0 * n + i] = x[i]; traj[1 * n + i] = y[i]; traj[2 * n + i] = z[i];
+    traj[3 * n + i] = vx[i]; traj[4 * n + i] = vy[i]; traj[5 * n + i] = vz[i];
+  }
+}
+
 }  // namespace bfe
 
 using namespace bfe;
@@ -285,8 +407,12 @@ static int leapfrog_run(bfe_ctx* h, float* x, float* y, float* z, float* vx, flo
       if (rc) return rc;
     }
     const bool check = ebuf && b.energy_every > 0;
-    hipLaunchKernelGGL(leapfrog_chunk_kernel, dim3(grid), dim3(256), 0, s, x, y, z, vx, vy, vz,
-                       perm, T, b, aux, check ? ebuf : nullptr, traj);
+    if (dtv)
+      hipLaunchKernelGGL(leapfrog_chunk_kernel<true>, dim3(grid), dim3(256), 0, s, x, y, z, vx, vy,
+                         vz, perm, T, b, aux, check ? ebuf : nullptr, traj);
+    else
+      hipLaunchKernelGGL(leapfrog_chunk_kernel<false>, dim3(grid), dim3(256), 0, s, x, y, z, vx, vy,
+                         vz, perm, T, b, aux, check ? ebuf : nullptr, traj);
     if (check)
       hipLaunchKernelGGL(energy_block_kernel, dim3(nb), dim3(256), 0, s, ebuf, n,
                          partE + (size_t)(b.check0 + 1) * nb);
@@ -334,4 +460,46 @@ extern "C" int bfe_leapfrog_dt(bfe_ctx* h, float* x, float* y, float* z, float* 
   if (h && n > 0 && !dt) return set_error(&h->c, BFE_EINVAL, "leapfrog_dt: NULL dt");
   return leapfrog_run(h, x, y, z, vx, vy, vz, n, 0.0, dt, nsteps, t0, rotfreq, energy_every,
                       drift_out, energy_log, record_every, traj, use_graph, stream);
+}
+
+extern "C" int bfe_rk4(bfe_ctx* h, float* x, float* y, float* z, float* vx, float* vy, float* vz,
+                       int64_t n, double dt, int64_t nsteps, double omegab, uint32_t flags,
+                       int record_every, float* traj, void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (n < 0 || nsteps < 0) return set_error(c, BFE_EINVAL, "rk4: n or nsteps < 0");
+  if (n > 0 && (!x || !y || !z || !vx || !vy || !vz)) return set_error(c, BFE_EINVAL, "rk4: NULL state");
+  if (record_every < 0) return set_error(c, BFE_EINVAL, "rk4: negative record_every");
+  if (traj && record_every == 0) return set_error(c, BFE_EINVAL, "rk4: traj needs record_every > 0");
+  if (flags & ~BFE_COMPAT_RK4_FRAME) return set_error(c, BFE_EINVAL, "rk4: unknown flags");
+  FieldTables T;
+  int rc = field_tables(c, 0.0, T);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  BFE_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  const int nb = (int)((n + 255) / 256);
+  const unsigned grid = xcd_grid(nb);
+  if (traj)
+    hipLaunchKernelGGL(rk4_record0_kernel, dim3(nb), dim3(256), 0, s, x, y, z, vx, vy, vz, n, traj);
+  RK4Args a;
+  a.dt = dt;
+  a.omegab = omegab;
+  a.frame_sign = (flags & BFE_COMPAT_RK4_FRAME) ? -1 : 1;
+  a.n = n;
+  a.record_every = record_every;
+  a.nblocks = nb;
+  // chunks of 25 steps (100 field evaluations), re-sorted by cell in between
+  const int64_t chunk = 25;
+  for (int64_t k0 = 0; k0 < nsteps; k0 += chunk) {
+    const int* perm = nullptr;
+    rc = order_points(c, T, x, y, z, n, s, &perm);
+    if (rc) return rc;
+    a.k0 = k0;
+    a.k1 = std::min<int64_t>(nsteps, k0 + chunk);
+    hipLaunchKernelGGL(rk4_chunk_kernel, dim3(grid), dim3(256), 0, s, x, y, z, vx, vy, vz, perm, T,
+                       a, traj);
+  }
+  BFE_HIP(c, hipGetLastError());
+  return 0;
 }

@@ -45,6 +45,7 @@ typedef struct bfe_ctx bfe_ctx;
 #define BFE_ACC_BF16 0x20u         /* SL accumulate in the bf16 mode: two-piece bf16 MFMA
                                       products (about 1e-5 relative), see bfe_sl_accumulate */
 #define BFE_COMPAT_ALL 0xFu
+#define BFE_COMPAT_RK4_FRAME 0x40u /* rk4_rotation pseudo-force signs, rk4.py:66-90 */
 
 /* ---- context ------------------------------------------------------------ */
 /* replaces: the Pool set-up of spheresl.compute_coefficients (spheresl.py:456)
@@ -151,6 +152,21 @@ int bfe_leapfrog_dt(bfe_ctx* ctx, float* x, float* y, float* z, float* vx, float
                     int64_t n, const double* dt, int64_t nsteps, double t0, double rotfreq,
                     int energy_every, double* drift_out, double* energy_log, int record_every,
                     float* traj, int use_graph, void* stream);
+
+/* replaces: utils/rk4.rk4 and rk4_rotation (rk4.py:26-63, 94-135) for many
+ * orbits in the frozen disc+halo field (not rotated: the field is fixed in
+ * the frame the orbits are integrated in).  State (x..vz) is advanced in
+ * place by nsteps classical RK4 steps of size dt: four field evaluations per
+ * step, at x, at the two half steps and at the full step.  omegab != 0 adds
+ * the in-plane pseudo-forces of a frame rotating at omegab about z:
+ * Omega^2 R - 2 Omega x v (Binney & Tremaine 3.117); with
+ * BFE_COMPAT_RK4_FRAME in flags the reference's add_rotation_acceleration
+ * signs, -Omega^2 R + 2 Omega x v.  traj (device float32 [nrec][6][n], may
+ * be NULL) records (x, y, z, vx, vy, vz) every record_every steps, step 0
+ * included. */
+int bfe_rk4(bfe_ctx* ctx, float* x, float* y, float* z, float* vx, float* vy, float* vz,
+            int64_t n, double dt, int64_t nsteps, double omegab, uint32_t flags,
+            int record_every, float* traj, void* stream);
 
 /* ---- batched snapshots ---------------------------------------------------- */
 /* Many snapshots in one launch: the particles of nsnap snapshots are

@@ -717,3 +717,49 @@ def integrate_grid(F: FieldSpec, rads, vels, nint, dt, rotfreq=0.0, dyn_res=100.
     arr = np.stack([rows[k] for k in keys], 1)
     shape = (rads.size, vels.size, zl.size, vzl.size) if three else (rads.size, vels.size)
     return arr.reshape(shape + arr.shape[1:]), dtime
+
+
+# ----------------------------------------------------------------------------
+# f4: RK4 integrator (utils/rk4.py:26-135), vectorised over orbits
+# ----------------------------------------------------------------------------
+
+def rk4(F: FieldSpec, nint, dt, pos, vel, omegab=0.0, frame_sign=1.0, compat=False):
+    """rk4 / rk4_rotation (rk4.py:26-63, 94-135) with the field unrotated:
+    k1 = (v, a(x, v)), k2 = (v + k1v dt/2, a(x + k1r dt/2, .)), k3 likewise
+    from k2, k4 = (v + k3v dt, a(x + k3r dt, .)); v += dt/6 (k1v + 2 k2v +
+    2 k3v + k4v), x likewise.  a = total force plus, for omegab != 0, the
+    planar pseudo-force frame_sign * (omegab^2 R - 2 omegab z^ x v):
+    frame_sign = +1 is Binney & Tremaine 3.117, -1 the reference's
+    add_rotation_acceleration (rk4.py:66-90).  Returns dict of (norb, nint)
+    arrays X, Y, Z, VX, VY, VZ."""
+    r = np.atleast_2d(np.asarray(pos, dtype=np.float64)).copy()
+    v = np.atleast_2d(np.asarray(vel, dtype=np.float64)).copy()
+    norb = r.shape[0]
+    out = {k: np.zeros((norb, nint)) for k in ("X", "Y", "Z", "VX", "VY", "VZ")}
+
+    def acc(q, u):
+        fx, fy, fz, _ = total_force(F, q[:, 0], q[:, 1], q[:, 2], 0.0, compat)
+        a = np.stack([fx, fy, fz], 1)
+        if omegab != 0.0:
+            a[:, 0] += frame_sign * (omegab * omegab * q[:, 0] + 2.0 * omegab * u[:, 1])
+            a[:, 1] += frame_sign * (omegab * omegab * q[:, 1] - 2.0 * omegab * u[:, 0])
+        return a
+
+    def rec(k):
+        for j, key in enumerate(("X", "Y", "Z")):
+            out[key][:, k] = r[:, j]
+            out["V" + key][:, k] = v[:, j]
+
+    rec(0)
+    for k in range(1, nint):
+        k1r, k1v = v, acc(r, v)
+        k2r = v + k1v * dt * 0.5
+        k2v = acc(r + k1r * dt * 0.5, k2r)
+        k3r = v + k2v * dt * 0.5
+        k3v = acc(r + k2r * dt * 0.5, k3r)
+        k4r = v + k3v * dt
+        k4v = acc(r + k3r * dt, k4r)
+        v = v + (1 / 6) * dt * (k1v + 2 * k2v + 2 * k3v + k4v)
+        r = r + (1 / 6) * dt * (k1r + 2 * k2r + 2 * k3r + k4r)
+        rec(k)
+    return out

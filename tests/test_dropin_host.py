@@ -94,3 +94,25 @@ def test_apse_stop_counts_apocentres():
     # first interior maximum of R at t = 2 pi (step 400): counted at step 401
     assert n1 == 402 and n2 == 802
     assert _apse_stop(X, Y, 100) == t.size
+
+
+def test_eof_variance_and_sn_hand_values():
+    """eof.compute_variance / compute_sn (eof.py:2135-2226) on values worked
+    by hand: masses (1, 1, 2) so w = 1/4, n = 3; cos = 8, cos2 = 32 give
+    var = 32/16 - 4/3 = 2/3, b_Hall = 4 / ((2/3)/4 + 4 + 1e-10) = 24/25 and
+    S/N = sqrt(64 / (2/3)) = sqrt(96); sin = -4, sin2 = 8 give
+    var = 1/2 - 1/3 = 1/6, b_Hall = 1 / ((1/6)/4 + 1 + 1e-10) and
+    S/N = sqrt(96)."""
+    from exptool_amd.basis import eof
+    from exptool_amd.io import particle
+    P = particle.make_holder(np.zeros(3), np.zeros(3), np.zeros(3), np.array([1.0, 1.0, 2.0]))
+    c, s = np.array([[8.0]]), np.array([[-4.0]])
+    c2, s2 = np.array([[[32.0]]]), np.array([[[8.0]]])
+    varC, varS, facC, facS = eof.compute_variance(P, c, s, c2, s2)
+    np.testing.assert_allclose(varC, [[[2 / 3]]], rtol=1e-14)
+    np.testing.assert_allclose(varS, [[[1 / 6]]], rtol=1e-13)
+    np.testing.assert_allclose(facC, [[[4 / (1 / 6 + 4 + 1e-10)]]], rtol=1e-14)
+    np.testing.assert_allclose(facS, [[[1 / (1 / 24 + 1 + 1e-10)]]], rtol=1e-14)
+    snC, snS = eof.compute_sn(P, c, s, c2, s2)
+    np.testing.assert_allclose(snC, [[[96 ** 0.5]]], rtol=1e-14)
+    np.testing.assert_allclose(snS, [[[96 ** 0.5]]], rtol=1e-13)

@@ -302,3 +302,59 @@ def test_do_integrate_multi_and_orbit_map(files, golden, tmp_path):
         assert np.array_equal(np.float32(D["TX"][line]), full[i, j, 2])
     with pytest.raises(ValueError):
         I.do_integrate_multi(rads, vels, F, 8, 0.002, 0.0, False, -1, -1, 50., 1000, threedee=True)
+
+
+def test_eof_var_subsample_sums(files, golden):
+    """accumulate's VAR branch (eof.py:554-574): with the same NumPy seed the
+    subsamples are the reference's draws, and each subsample sum equals the
+    oracle accumulate of those particles (<= 1e-6 per block)."""
+    from exptool_amd.basis import eof
+    from exptool_amd.io import particle
+    P = particle.Input(str(files / "OUT.run.00000"), comp="star")
+    eo = eof_table(golden)
+    x, y, z, m = _f32(*(golden[k] for k in ("px", "py", "pz", "pm")))
+    np.random.seed(11)
+    obj = eof.compute_coefficients(P, str(files / "eof.cache"), VAR=4)
+    plain = eof.compute_coefficients(P, str(files / "eof.cache"))
+    assert np.array_equal(obj.cos, plain.cos) and np.array_equal(obj.sin, plain.sin)
+    assert obj.cos2.shape == obj.sin2.shape == (4, eo.mmax + 1, eo.norder)
+    np.random.seed(11)
+    k = int(np.floor(np.sqrt(x.size)))
+    for T in range(4):
+        use = np.random.randint(x.size, size=k)
+        rc, rs = O.eof_accumulate(eo, x[use], y[use], z[use], m[use])
+        assert block_rel_eof(obj.cos2[T], rc) <= 1e-6, T
+        assert block_rel_eof(obj.sin2[T][1:], rs[1:]) <= 1e-6, T
+    one = eof.compute_coefficients(P, str(files / "eof.cache"), VAR=True)
+    assert one.cos2.shape == (1, eo.mmax + 1, eo.norder)
+    snC, snS = eof.compute_sn(P, obj.cos, obj.sin, obj.cos2, obj.sin2)
+    assert snC.shape == obj.cos2.shape and np.isfinite(snC[:, 0, 0]).all()
+
+
+def test_rk4_matches_oracle(files, golden):
+    """utils/rk4 (rk4.py:26-135) through bfe_rk4 against the oracle RK4:
+    <= 1e-4 over 150 steps, inertial and rotating frames, the reference's
+    pseudo-force signs (compat field) and the physical ones."""
+    from exptool_amd.utils import rk4 as R
+    F = _fields(files, golden)                         # compat=True
+    spec = _spec(golden)
+    init = golden["leap_init"].astype(np.float32).astype(np.float64)
+    pos, vel = init[:, :3].T.copy(), init[:, 3:].T.copy()
+    cases = ((0.0, None, -1.0), (0.25, None, -1.0), (0.25, False, 1.0))
+    for omegab, compat, sign in cases:
+        if omegab == 0.0:
+            r, v = R.rk4(F, pos, vel, 0.02, 150)
+        else:
+            r, v = R.rk4_rotation(F, pos, vel, 0.02, 150, omegab=omegab, compat=compat)
+        assert r.shape == v.shape == (3, 3, 150)
+        ref = O.rk4(spec, 150, 0.02, init[:, :3], init[:, 3:], omegab=omegab, frame_sign=sign,
+                    compat=True)
+        for j, k in enumerate("XYZ"):
+            assert relmax(r[j], ref[k]) <= 1e-4, (omegab, compat, k)
+            assert relmax(v[j], ref["V" + k]) <= 1e-4, (omegab, compat, k)
+    r1, v1 = R.rk4(F, init[1, :3], init[1, 3:], 0.02, 150)
+    r, v = R.rk4(F, pos, vel, 0.02, 150)
+    assert r1.shape == (3, 150) and np.array_equal(r1, r[:, 1]) and np.array_equal(v1, v[:, 1])
+    f = F.get_cartesian_forces_array(init[0, :3])
+    ref = O.total_force(spec, *init[0, :3], compat=True)[:3]
+    assert np.max(np.abs(f - np.ravel(ref))) <= 1e-5 * np.max(np.abs(ref))

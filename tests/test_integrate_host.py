@@ -75,3 +75,13 @@ def test_orbit_map_many_lines(tmp_path):
         for row, k in enumerate(("X", "Y", "TX", "TY", "VX", "VY")):
             assert np.array_equal(np.float32(D[k][line]), o[i, j, row])
         assert np.float32(D["dT"][line]) == o[i, j, 6, 1]
+
+
+def test_add_rotation_acceleration_hand_values():
+    """rk4.py:66-90 at x = (1, 2, .), v = (3, 4, .), omegab = 2:
+    -4 (1, 2, 0) + 2 (0, 0, 2) x (3, 4, 0) = (-4, -8, 0) + (-16, 12, 0)."""
+    from exptool_amd.utils import rk4
+    got = rk4.add_rotation_acceleration(np.array([1.0, 2.0, 5.0]), np.array([3.0, 4.0, 6.0]), omegab=2)
+    np.testing.assert_array_equal(got, [-20.0, 4.0, 0.0])
+    with pytest.raises(TypeError):
+        rk4.rk4(object(), [1.0, 0, 0], [0, 1.0, 0], 0.1, 3)
