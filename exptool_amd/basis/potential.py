@@ -7,6 +7,8 @@
   Fields.reset_field_parameters   potential.py:380-387
   Fields.return_forces_cart       potential.py:445-497 (libexptool_bfe field kernel)
   Fields.return_forces_cyl        potential.py:389-440 (batch evaluators + host assembly)
+  Fields.save_field, restore_field   potential.py:738-1041 (binary field file)
+  get_fields                      potential.py:1243-1289
 
 The tables and the folded field live in one BFESystem on the device; the field
 is re-frozen (bfe_field_set) whenever coefficients or field parameters change.
@@ -81,6 +83,7 @@ class Fields:
         self.evtablehalo, self.eftablehalo = st["ev"], st["ef"]
         self.xihalo, self.p0halo, self.d0halo = st["xi"], st["p0"], st["d0"]
         self.rarrhalo = st.get("r")
+        self.ltablehalo = np.asarray(st.get("ltable", np.arange(self.lmaxhalo + 1)), np.float64)
         self.halo_use_l, self.halo_use_n = self.lmaxhalo, self.nmaxhalo
         self._frozen = None
 
@@ -93,33 +96,43 @@ class Fields:
     def total_coefficients(self, disk=None, halo=None):
         """potential.py:98-252.  Particles come from the PSP file `infile`
         (components 'star' and 'dark') or from the `disk` / `halo` objects
-        (particle.holder, Input with .data).  centering subtracts the mass
-        centre of the 10 000 innermost particles of each component (the disc
-        centre for both with mutual_center).  halofac is 1 (the reference's
-        ratio of two identical reads, :138-143)."""
-        if self.transform:
-            raise NotImplementedError("transform=True (BarTransform) is outside this build")
+        (particle.holder, Input with .data).  transform rotates both into the
+        bar frame of the disc (pattern.BarTransform, :121-148).  centering
+        subtracts the mass centre of the 10 000 innermost particles of each
+        component (the disc centre for both with mutual_center; with compat
+        the halo centre uses the indices of the innermost disc particles, as
+        the reference does at :199-209).  halofac is 1 (the reference's ratio
+        of two identical reads, :138-143)."""
         if disk is None:
             disk = particle.Input(self.filename, comp="star", verbose=self.verbose)
         if halo is None:
             halo = particle.Input(self.filename, comp="dark", verbose=self.verbose)
         self.time = getattr(disk, "time", None)
+        if self.transform:
+            from ..analysis import pattern
+            disk = pattern.BarTransform(disk)
+            halo = pattern.BarTransform(halo, bar_angle=disk.bar_angle)
+            self.bar_angle = disk.bar_angle
         if self.system is None:
             self.prep_tables()
         dx, dy, dz, dm = (np.asarray(a, np.float64) for a in particle.particle_arrays(disk))
         hx, hy, hz, hm = (np.asarray(a, np.float64) for a in particle.particle_arrays(halo))
         self.halofac = 1.0
         if self.centering:
-            def centre(x, y, z, m, ncenter=10000):
-                idx = np.argsort(x * x + y * y + z * z)[:ncenter]
+            def inner(x, y, z, ncenter=10000):
+                return np.argsort(x * x + y * y + z * z)[:ncenter]
+
+            def centre(x, y, z, m, idx):
                 w = m[idx].sum()
                 return tuple(float((v[idx] * m[idx]).sum() / w) for v in (x, y, z))
-            self.xcen_disk, self.ycen_disk, self.zcen_disk = centre(dx, dy, dz, dm)
+            didx = inner(dx, dy, dz)
+            self.xcen_disk, self.ycen_disk, self.zcen_disk = centre(dx, dy, dz, dm, didx)
             if self.mutual_center:
                 self.xcen_halo, self.ycen_halo, self.zcen_halo = \
                     self.xcen_disk, self.ycen_disk, self.zcen_disk
             else:
-                self.xcen_halo, self.ycen_halo, self.zcen_halo = centre(hx, hy, hz, hm)
+                hidx = didx if self.compat else inner(hx, hy, hz)
+                self.xcen_halo, self.ycen_halo, self.zcen_halo = centre(hx, hy, hz, hm, hidx)
             dx, dy, dz = dx - self.xcen_disk, dy - self.ycen_disk, dz - self.zcen_disk
             hx, hy, hz = hx - self.xcen_halo, hy - self.ycen_halo, hz - self.zcen_halo
         else:
@@ -203,6 +216,46 @@ class Fields:
         fxd, fxh, fyd, fyh, fzd, fzh, _, _ = self.return_forces_cart(arr[0], arr[1], arr[2])
         return np.array([fxd + fxh, fyd + fyh, fzd + fzh])
 
+    # ------------------------------------------------------------ field files
+    def save_field(self, filename=''):
+        """potential.py:738-965: the field in the reference's binary layout:
+        4 x S100 file names; i4 nhalo, transform, no_odd, centering,
+        mutual_center, verbose; f4 time; i4 numx, numy, mmax, norder, cmap,
+        dens; f4 rmin, rmax, ascale, hscale, XMIN, dX, YMIN, dY and the disc
+        centre; f8 cos, sin, then potC, rforceC, zforceC, densC, potS,
+        rforceS, zforceS, densS; f4 halofac, rmin, rmax, scale and the halo
+        centre; i4 numr, cmap, lmax, nmax; f8 xi, p0, d0, ltable, ev, ef,
+        expcoef."""
+        if filename == '':
+            print('potential.Fields.save_field: No filename specified.')
+        shape = (self.mmax + 1, self.norder, self.numx + 1, self.numy + 1)
+
+        def dens(a):
+            return np.zeros(shape) if a is None else np.asarray(a, np.float64)
+        with open(filename, 'wb') as f:
+            np.array([self.filename, self.eof_file, self.sph_file, self.model_file],
+                     dtype='S100').tofile(f)
+            np.array([self.nhalo, self.transform, self.no_odd, self.centering, self.mutual_center,
+                      self.verbose], dtype='i4').tofile(f)
+            np.array([0.0 if getattr(self, 'time', None) is None else self.time],
+                     dtype='f4').tofile(f)
+            np.array([self.numx, self.numy, self.mmax, self.norder, self.cmapdisk, self.densdisk],
+                     dtype='i4').tofile(f)
+            cen = [getattr(self, k, 0.0) for k in ("xcen_disk", "ycen_disk", "zcen_disk",
+                                                    "xcen_halo", "ycen_halo", "zcen_halo")]
+            np.array([self.rmindisk, self.rmaxdisk, self.ascale, self.hscale, self.XMIN, self.dX,
+                      self.YMIN, self.dY] + cen[:3], dtype='f4').tofile(f)
+            for a in (self.EOF.cos, self.EOF.sin, self.potC, self.rforceC, self.zforceC,
+                      dens(self.densC), self.potS, self.rforceS, self.zforceS, dens(self.densS)):
+                np.asarray(a, dtype='f8').reshape(-1).tofile(f)
+            np.array([self.halofac, self.rminhalo, self.rmaxhalo, self.scalehalo] + cen[3:],
+                     dtype='f4').tofile(f)
+            np.array([self.numrhalo, self.cmaphalo, self.lmaxhalo, self.nmaxhalo],
+                     dtype='i4').tofile(f)
+            for a in (self.xihalo, self.p0halo, self.d0halo, self.ltablehalo, self.evtablehalo,
+                      self.eftablehalo, self.SL.expcoef):
+                np.asarray(a, dtype='f8').reshape(-1).tofile(f)
+
     def return_forces_cyl(self, xval, yval, zval, rotpos=0.0):
         """potential.py:389-440: (fr_disk, fr_halo, fp_disk, -fp_halo, fz_disk,
         fz_halo, -pot_disk, pot_halo) with R = sqrt(x^2+y^2)+1e-10 and
@@ -245,3 +298,87 @@ class Fields:
         if scalar:
             return tuple(float(v[0]) for v in out)
         return out
+
+
+def restore_field(filename='', compat=False, device=0):
+    """potential.py:968-1041: a Fields from a save_field file, ready to
+    evaluate (the state after total_coefficients + prep_tables).  As in the
+    reference the file names come back as bytes, the table parameters as
+    their float32 values (the disc grid uses the stored XMIN, dX, YMIN, dY)
+    and the field parameters are reset to all orders."""
+    with open(filename, 'rb') as f:
+        infile, eof_file, sph_file, model_file = np.fromfile(f, dtype='S100', count=4)
+        nhalo, transform, no_odd, centering, mutual_center, verbose = np.fromfile(f, dtype='i4',
+                                                                                  count=6)
+        [time] = np.fromfile(f, dtype='f4', count=1)
+        F = Fields(infile, eof_file, sph_file, model_file, nhalo=nhalo, transform=transform,
+                   no_odd=no_odd, centering=centering, mutual_center=mutual_center,
+                   verbose=verbose, compat=compat, device=device)
+        F.time = time
+        numx, numy, mmax, norder, cmapdisk, densdisk = (int(v) for v in
+                                                        np.fromfile(f, dtype='i4', count=6))
+        (rmindisk, rmaxdisk, ascale, hscale, XMIN, dX, YMIN, dY,
+         F.xcen_disk, F.ycen_disk, F.zcen_disk) = np.fromfile(f, dtype='f4', count=11)
+        k = (mmax + 1) * norder
+        cos = np.fromfile(f, dtype='f8', count=k).reshape(mmax + 1, norder)
+        sin = np.fromfile(f, dtype='f8', count=k).reshape(mmax + 1, norder)
+        shape = (mmax + 1, norder, numx + 1, numy + 1)
+        nt = int(np.prod(shape))
+        names = ("potC", "rforceC", "zforceC", "densC", "potS", "rforceS", "zforceS", "densS")
+        et = {nm: np.fromfile(f, dtype='f8', count=nt).reshape(shape) for nm in names}
+        et.update(mmax=mmax, norder=norder, numx=numx, numy=numy, cmap=cmapdisk, dens=densdisk,
+                  rmin=float(rmindisk), rmax=float(rmaxdisk), ascale=float(ascale),
+                  hscale=float(hscale), XMIN=float(XMIN), dX=float(dX), YMIN=float(YMIN),
+                  dY=float(dY))
+        (halofac, rminhalo, rmaxhalo, scalehalo,
+         F.xcen_halo, F.ycen_halo, F.zcen_halo) = np.fromfile(f, dtype='f4', count=7)
+        numr, cmaphalo, lmax, nmax = (int(v) for v in np.fromfile(f, dtype='i4', count=4))
+        xi = np.fromfile(f, dtype='f8', count=numr)
+        p0 = np.fromfile(f, dtype='f8', count=numr)
+        d0 = np.fromfile(f, dtype='f8', count=numr)
+        ltable = np.fromfile(f, dtype='f8', count=lmax + 1)
+        ev = np.fromfile(f, dtype='f8', count=(lmax + 1) * nmax).reshape(lmax + 1, nmax)
+        ef = np.fromfile(f, dtype='f8', count=(lmax + 1) * nmax * numr).reshape(lmax + 1, nmax, numr)
+        expcoef = np.fromfile(f, dtype='f8', count=(lmax + 1) ** 2 * nmax).reshape((lmax + 1) ** 2,
+                                                                                   nmax)
+    st = dict(lmax=lmax, nmax=nmax, numr=numr, cmap=cmaphalo, rmin=float(rminhalo),
+              rmax=float(rmaxhalo), scale=float(scalehalo), ltable=ltable, ev=ev, ef=ef, xi=xi,
+              p0=p0, d0=d0)
+    F._set_tables(et, st)
+    F.XMIN, F.dX, F.YMIN, F.dY = (float(v) for v in (XMIN, dX, YMIN, dY))
+    F.EOF = eof_mod.EOF_Object()
+    F.EOF.cos, F.EOF.sin = cos, sin
+    F.EOF.mmax, F.EOF.nmax, F.EOF.eof_file = mmax, norder, eof_file
+    F.SL = spheresl.SL_Object()
+    F.SL.expcoef = expcoef
+    F.SL.lmax, F.SL.nmax = lmax, nmax
+    F.SL.model_file, F.SL.sph_file = model_file, sph_file
+    F.halofac = float(halofac)
+    return F
+
+
+def get_fields(simulation_directory, simulation_name, intime, eof_file, sph_file, model_file,
+               bar_file='', nhalo=1000000, transform=True, fileprefix='OUT', compat=False,
+               device=0):
+    """potential.py:1243-1289: the field of dump <dir><prefix>.<name>.<intime
+    %05i> (centred, mutual centre) and its pattern speed from bar_file.  As
+    in the reference the pattern speed is the backward difference of the bar
+    position at the tabulated time nearest the dump (find_barpattern with
+    smth_order=None replaces the spline derivative computed just before).
+    Returns (F, patt, rotfreq = patt / 2 pi)."""
+    from ..analysis import pattern
+    infile = simulation_directory + fileprefix + '.' + simulation_name + '.%05i' % intime
+    if transform:
+        B = pattern.BarDetermine()
+        B.read_bar(bar_file)
+        B.frequency_and_derivative(spline_derivative=2)
+        P = particle.Input(infile, comp='star')
+        patt = pattern.find_barpattern(P.time, B, smth_order=None)
+        rotfreq = patt / (2. * np.pi)
+    else:
+        patt = 0.
+        rotfreq = 0.
+    F = Fields(infile, eof_file, sph_file, model_file, nhalo=nhalo, transform=transform,
+               no_odd=False, centering=True, mutual_center=True, compat=compat, device=device)
+    F.total_coefficients()      # reads the tables too (prep_tables)
+    return F, patt, rotfreq

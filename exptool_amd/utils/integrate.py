@@ -9,6 +9,7 @@
   redistribute_arrays, re_form_orbit_arrays(_3D)   utils/integrate.py:440-510
   print_orbit_array(_3D)   utils/integrate.py:513-568 (text orbit maps)
   read_integrations(_3D)   utils/integrate.py:970-1039
+  run_time, run_time_mod   utils/integrate.py:571-758 (grids from a dump)
   Orbits               orbits/orbit.py:50-56 (dict of arrays)
 
 The whole integration is one bfe_leapfrog call: the state stays in registers
@@ -486,3 +487,94 @@ def read_integrations_3D(infile):
     """integrate.py:1001-1039 (the print_orbit_array_3D format)."""
     return _read(infile, ["R_0", "V_0", "Z_0", "VZ_0", "dT"],
                  ["X", "Y", "Z", "TX", "TY", "VX", "VY", "VZ"])
+
+
+# ---------------------------------------------------------------------------
+# drivers from a simulation dump (integrate.py:571-758)
+# ---------------------------------------------------------------------------
+
+def run_time(simulation_directory, simulation_name, eof_file, sph_file, model_file, intime, rads,
+             vels, nint, dt, no_odd, halo_l, max_m, dyn_res, ap_max, verbose, nprocs=-1,
+             omegap=-1., orbitfile='', transform=True, fileprefix='OUT', launch='x'):
+    """integrate.py:571-635: the field of one dump (potential.get_fields),
+    the pattern speed (omegap >= 0 overrides it; the frame turns at
+    -|patt| / 2 pi), the orbit grid, written as an orbit map to orbitfile or
+    <dir>omap_<intime>.txt.  As in the reference no bar file is passed to
+    get_fields, so transform=True needs one and fails."""
+    from ..basis import potential
+    if verbose:
+        print('exptool.integrate.run_time: in directory {}, run {} at output {}, with transform={}'
+              .format(simulation_directory, simulation_name, intime, transform))
+    F, patt, rotfreq = potential.get_fields(simulation_directory, simulation_name, intime, eof_file,
+                                            sph_file, model_file, transform=transform,
+                                            fileprefix=fileprefix)
+    if omegap >= 0.:
+        patt = omegap
+    rotfreq = -1. * abs(patt / (2. * np.pi))
+    OrbitArray = do_integrate_multi(rads, vels, F, nint, dt, rotfreq, no_odd, halo_l, max_m, dyn_res,
+                                    ap_max, verbose=verbose, nprocs=nprocs, launch=launch)
+    path = orbitfile if orbitfile != '' else simulation_directory + 'omap_' + str(intime) + '.txt'
+    with open(path, 'w') as f:
+        print_orbit_array(f, OrbitArray)
+    return OrbitArray
+
+
+def run_time_mod(simulation_directory, simulation_name, eof_file, sph_file, model_file, intime,
+                 rads, vels, nint, dt, no_odd, halo_l, max_m, dyn_res, ap_max, verbose, nprocs=-1,
+                 omegap=-1., orbitfile='', transform=False, save_field=True, field_file_name='',
+                 field_file=None, bar_file='', fileprefix='OUT', threedee=False, zs=None, vzs=None,
+                 launch='x'):
+    """integrate.py:638-758: run_time with the field saved to (or, when
+    field_file is given, restored from) field_file_name, the pattern speed
+    from bar_file, and 2D or 3D grids (orbit map omap_<intime>.txt or
+    omap_3D_<intime>.txt)."""
+    from ..analysis import pattern
+    from ..basis import potential
+    from ..io import particle
+    if verbose:
+        print('exptool.integrate.run_time: in directory {}, run {} at output {}, with transform={}'
+              .format(simulation_directory, simulation_name, intime, transform))
+    if transform and bar_file == '':
+        print('error - no bar file supplied for transform')
+        return None
+    if field_file is None:
+        F, patt, rotfreq = potential.get_fields(simulation_directory, simulation_name, intime,
+                                                eof_file, sph_file, model_file, transform=transform,
+                                                bar_file=bar_file, fileprefix=fileprefix)
+        if save_field:
+            if str(field_file_name) != '':
+                F.save_field(str(field_file_name))
+            else:
+                print('saving field file with default name (field_file) to local directory')
+                F.save_field('field_file')
+    else:
+        print('field file supplied! File name:')
+        print(field_file_name)
+        F = potential.restore_field(str(field_file_name))
+        if transform:
+            B = pattern.BarDetermine()
+            B.read_bar(bar_file)
+            B.frequency_and_derivative(spline_derivative=2)
+            infile = simulation_directory + fileprefix + '.' + simulation_name + '.{0:05d}'.format(intime)
+            P = particle.Input(infile, comp='star')
+            patt = pattern.find_barpattern(P.time, B, smth_order=None)
+        else:
+            patt = 0.
+    if omegap >= 0.:
+        patt = omegap
+    rotfreq = -1. * abs(patt / (2. * np.pi))
+    if not threedee:
+        print('launching from ', launch, ' axis')
+        OrbitArray = do_integrate_multi(rads, vels, F, nint, dt, rotfreq, no_odd, halo_l, max_m,
+                                        dyn_res, ap_max, verbose=verbose, nprocs=nprocs, launch=launch)
+        path = orbitfile if orbitfile != '' else simulation_directory + 'omap_' + str(intime) + '.txt'
+        with open(path, 'w') as f:
+            print_orbit_array(f, OrbitArray)
+        return OrbitArray
+    OrbitArray = do_integrate_multi(rads, vels, F, nint, dt, rotfreq, no_odd, halo_l, max_m, dyn_res,
+                                    ap_max, verbose=verbose, nprocs=nprocs, threedee=threedee, zs=zs,
+                                    vzs=vzs, launch=launch)
+    path = orbitfile if orbitfile != '' else simulation_directory + 'omap_3D_' + str(intime) + '.txt'
+    with open(path, 'w') as f:
+        print_orbit_array_3D(f, OrbitArray)
+    return OrbitArray

@@ -116,3 +116,50 @@ def test_eof_variance_and_sn_hand_values():
     snC, snS = eof.compute_sn(P, c, s, c2, s2)
     np.testing.assert_allclose(snC, [[[96 ** 0.5]]], rtol=1e-14)
     np.testing.assert_allclose(snS, [[[96 ** 0.5]]], rtol=1e-13)
+
+
+BARFILE = os.path.join(HERE, "golden", "barfile001.dat")
+
+
+def test_bar_file_fixture_and_pattern_speed(tmp_path):
+    """barfile001.dat (the reference's own test data: time, bar position,
+    pattern speed) pins BarDetermine.read_bar, the backward differences of
+    frequency_and_derivative (pattern.py:362-402), unwrap_bar_position and
+    find_barpattern / find_barangle at tabulated times."""
+    from exptool_amd.analysis import pattern
+    ref = np.loadtxt(BARFILE)
+    B = pattern.BarDetermine(file=BARFILE)
+    assert np.array_equal(B.time, ref[:, 0]) and np.array_equal(B.pos, ref[:, 1])
+    B.frequency_and_derivative()
+    assert np.max(np.abs(B.deriv - ref[:, 2])) <= 1e-6      # file values printed to ~12 digits
+    assert B.deriv[0] == 0.0
+    i = 1234
+    assert pattern.find_barpattern(ref[i, 0] + 1e-4, B, smth_order=None) == B.deriv[i]
+    assert np.array_equal(pattern.find_barpattern(ref[[5, 9], 0], B, smth_order=None), B.deriv[[5, 9]])
+    assert abs(pattern.find_barangle(ref[i, 0], B) + ref[i, 1]) <= 1e-9
+    assert pattern.find_barangle(ref[i, 0] + 1e-4, B, interpolate=False) == -ref[i, 1]
+    fit = pattern.find_barpattern(ref[i, 0], B, smth_order=2)
+    d0 = np.zeros(len(ref)); d0[1:] = np.diff(ref[:, 1]) / np.diff(ref[:, 0])
+    assert fit == pytest.approx(np.polyval(np.polyfit(ref[:, 0], d0, 2), ref[i, 0]), rel=1e-9)
+    # the stored position is unwrapped: wrapping it into the m = 2 phase range
+    # and unwrapping again gives it back
+    W = pattern.BarDetermine()
+    W.time, W.pos = ref[:, 0].copy(), (ref[:, 1] + np.pi / 2) % np.pi - np.pi / 2
+    W.unwrap_bar_position()
+    assert np.max(np.abs(W.pos - ref[:, 1])) <= 1e-9
+    W.frequency_and_derivative()
+    W.print_bar(str(tmp_path / "bar.dat"))
+    R = pattern.BarDetermine(file=str(tmp_path / "bar.dat"))
+    assert np.array_equal(R.pos, W.pos) and np.array_equal(R.deriv, W.deriv)
+
+
+def test_bar_transform_given_angle():
+    from exptool_amd.analysis import pattern
+    from exptool_amd.io import particle
+    P = particle.make_holder(np.array([1.0, 0.0]), np.array([0.0, 2.0]), np.array([0.5, -0.5]),
+                             np.array([1.0, 2.0]))
+    T = pattern.BarTransform(P, bar_angle=np.pi / 2, rel_bar_angle=np.pi / 2)
+    assert T.bar_angle == np.pi
+    np.testing.assert_allclose(T.data['x'], [-1.0, 0.0], atol=1e-15)
+    np.testing.assert_allclose(T.data['y'], [0.0, -2.0], atol=1e-15)
+    assert np.array_equal(T.data['z'], [0.5, -0.5]) and np.array_equal(T.data['m'], [1.0, 2.0])

@@ -358,3 +358,96 @@ def test_rk4_matches_oracle(files, golden):
     f = F.get_cartesian_forces_array(init[0, :3])
     ref = O.total_force(spec, *init[0, :3], compat=True)[:3]
     assert np.max(np.abs(f - np.ravel(ref))) <= 1e-5 * np.max(np.abs(ref))
+
+
+def test_field_file_roundtrip(files, golden, tmp_path):
+    """save_field / restore_field (potential.py:738-1041): the reference's
+    byte layout, and a restored field that evaluates like the saved one
+    (its table parameters come back as float32, as in the reference)."""
+    from exptool_amd.basis import potential
+    F = _fields(files, golden)
+    path = tmp_path / "field.dat"
+    F.save_field(str(path))
+    raw = path.read_bytes()
+    off = 4 * 100 + 6 * 4 + 4 + 6 * 4 + 11 * 4          # names, flags, time, EOF ints, EOF floats
+    assert np.array_equal(np.frombuffer(raw[off:off + 8 * F.EOF.cos.size], "<f8"), F.EOF.cos.ravel())
+    nt = (F.mmax + 1) * F.norder * (F.numx + 1) * (F.numy + 1)
+    L, n, R = F.lmaxhalo, F.nmaxhalo, F.numrhalo
+    size = (off + 8 * (2 * (F.mmax + 1) * F.norder + 8 * nt) + 4 * 7 + 4 * 4
+            + 8 * (3 * R + (L + 1) + (L + 1) * n + (L + 1) * n * R + (L + 1) ** 2 * n))
+    assert len(raw) == size
+    assert np.array_equal(np.frombuffer(raw[-8 * F.SL.expcoef.size:], "<f8"), F.SL.expcoef.ravel())
+    G = potential.restore_field(str(path), compat=True)
+    assert G.filename == str(files / "OUT.run.00000").encode()[:100]
+    assert G.time == np.float32(F.time) and G.halofac == 1.0
+    x, y, z = (golden[k][:200] for k in ("px", "py", "pz"))
+    a = np.array(F.return_forces_cart(x, y, z))
+    b = np.array(G.return_forces_cart(x, y, z))
+    for j in range(8):
+        assert relmax(b[j], a[j]) <= 1e-5, j
+
+
+def test_bar_transform_and_transformed_field(files, golden):
+    """m = 2 phase on the device (pattern.py:159-177) of a bar tilted by 0.4,
+    and Fields(transform=True) equal to the untransformed field of the
+    particles rotated by the bar angle."""
+    from exptool_amd.analysis import pattern
+    from exptool_amd.basis.potential import Fields
+    from exptool_amd.io import particle
+    rng = np.random.default_rng(3)
+    R, t = rng.uniform(0.05, 0.95, 20000), rng.uniform(0, 2 * np.pi, 20000)
+    u, v = R * np.cos(t), 0.5 * R * np.sin(t)
+    x, y = u * np.cos(0.4) - v * np.sin(0.4), u * np.sin(0.4) + v * np.cos(0.4)
+    ph = pattern.m2_phase(x, y)
+    assert abs(ph - 0.4) < 0.02
+    w = np.hypot(x, y) < 1.0
+    f = 2 * np.arctan2(y[w], x[w])
+    assert abs(ph - np.arctan2(np.sin(f).sum(), np.cos(f).sum()) / 2) < 1e-12
+    args = (str(files / "OUT.run.00000"), str(files / "eof.cache"), str(files / "sl.cache"),
+            str(files / "sl.model"))
+    T = Fields(*args, transform=True, compat=True)
+    T.total_coefficients()
+    disk = particle.Input(args[0], comp="star")
+    halo = particle.Input(args[0], comp="dark")
+    assert T.bar_angle == -pattern.m2_phase(disk.data["x"], disk.data["y"])
+    rd = pattern.BarTransform(disk, bar_angle=T.bar_angle)
+    rh = pattern.BarTransform(halo, bar_angle=T.bar_angle)
+    P = Fields(*args, compat=True)
+    P.total_coefficients(disk=rd, halo=rh)
+    assert np.array_equal(T.EOF.cos, P.EOF.cos) and np.array_equal(T.SL.expcoef, P.SL.expcoef)
+
+
+def test_run_time_mod_end_to_end(files, golden, tmp_path):
+    """get_fields + run_time_mod (potential.py:1243-1289, integrate.py:
+    638-758): field from the dump, pattern speed from a bar file, saved
+    field, orbit grid and orbit map; then the same run from the saved
+    field."""
+    from exptool_amd.basis import potential
+    from exptool_amd.utils import integrate as I
+    t = np.round(np.arange(1.40, 1.6001, 0.01), 6)
+    with open(tmp_path / "bar.dat", "w") as f:
+        for ti in t:
+            print(ti, 0.3 + 2.0 * (ti - 1.4), file=f)
+    d = str(files) + "/"
+    rads, vels = np.array([0.8, 1.6]), np.array([0.3, 0.5])
+    kw = dict(nint=24, dt=0.002, no_odd=False, halo_l=-1, max_m=-1, dyn_res=50., ap_max=1000,
+              verbose=0)
+    arr = I.run_time_mod(d, "run", str(files / "eof.cache"), str(files / "sl.cache"),
+                         str(files / "sl.model"), 0, rads, vels, transform=True,
+                         bar_file=str(tmp_path / "bar.dat"),
+                         field_file_name=str(tmp_path / "field.dat"),
+                         orbitfile=str(tmp_path / "omap.txt"), **kw)
+    F, patt, rotfreq = potential.get_fields(d, "run", 0, str(files / "eof.cache"),
+                                            str(files / "sl.cache"), str(files / "sl.model"),
+                                            bar_file=str(tmp_path / "bar.dat"))
+    assert patt == pytest.approx(2.0, rel=1e-9)
+    ref = I.do_integrate_multi(rads, vels, F, kw["nint"], kw["dt"], -abs(patt) / (2 * np.pi), False,
+                               -1, -1, kw["dyn_res"], 1000)
+    assert np.array_equal(arr, ref)
+    D = I.read_integrations(str(tmp_path / "omap.txt"))
+    assert len(D["X"]) == 4 and np.array_equal(np.float32(D["VY"][3]), arr[1, 1, 5])
+    again = I.run_time_mod(d, "run", "", "", "", 0, rads, vels, transform=True,
+                           bar_file=str(tmp_path / "bar.dat"), field_file=True,
+                           field_file_name=str(tmp_path / "field.dat"),
+                           orbitfile=str(tmp_path / "omap2.txt"), **kw)
+    assert relmax(again, arr) <= 1e-4
