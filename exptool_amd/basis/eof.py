@@ -9,6 +9,8 @@ Device work (libexptool_bfe.so through exptool_amd.bfe.BFESystem):
   accumulate               eof.py:492-640 (VAR subsample sums: one batched launch)
   compute_coefficients     eof.py:1158-1260
   compute_variance / compute_sn   eof.py:2135-2226 (host, coefficient-sized)
+Coefficient files (host): save_eof_coefficients / restore_eof_coefficients
+  eof.py:1646-1763
   accumulated_eval_particles / compute_forces   eof.py:989-1144, :1263-1317
   force_eval               eof.py:756-872 (many points per call)
 Return types follow the reference: float64 NumPy arrays and EOF_Object.
@@ -297,6 +299,71 @@ def compute_sn(ParticleInstance, accum_cos, accum_sin, accum_cos2, accum_sin2):
                                         accum_sin2)
     c, s = np.asarray(accum_cos), np.asarray(accum_sin)
     return (c * c / varC) ** 0.5, (s * s / varS) ** 0.5
+
+
+# ----------------------------------------------------------------------------
+# coefficient files (eof.py:1646-1741)
+# ----------------------------------------------------------------------------
+
+_EOF_HDR = 4 + 100 + 8 + 4 + 100 + 8      # time, filename, comp, nbodies, eof_file, mmax/nmax
+
+
+def eof_coefficients_to_file(f, EOF_Object):
+    """eof.py:1646-1668: f4 time, S100 filename, S8 comp, i4 nbodies,
+    S100 eof_file, i4 mmax, nmax, then f8 cos and sin ((mmax+1) x nmax)."""
+    np.array([EOF_Object.time], dtype='<f4').tofile(f)
+    np.array([EOF_Object.filename], dtype='S100').tofile(f)
+    np.array([EOF_Object.comp], dtype='S8').tofile(f)
+    np.array([EOF_Object.nbodies], dtype='<i4').tofile(f)
+    np.array([EOF_Object.eof_file], dtype='S100').tofile(f)
+    np.array([EOF_Object.mmax, EOF_Object.nmax], dtype='<i4').tofile(f)
+    np.asarray(EOF_Object.cos, dtype='<f8').reshape(-1).tofile(f)
+    np.asarray(EOF_Object.sin, dtype='<f8').reshape(-1).tofile(f)
+
+
+def save_eof_coefficients(outfile, EOF_Object, verbose=0):
+    """eof.py:1672-1705: append one dump to outfile, whose leading i4 counts
+    the dumps (each 16 (mmax+1) nmax + 224 bytes)."""
+    if not os.path.exists(outfile):
+        with open(outfile, 'wb') as f:
+            np.array([0], dtype='<i4').tofile(f)
+    with open(outfile, 'rb+') as f:
+        nd = int(np.fromfile(f, dtype='<i4', count=1)[0]) + 1
+        f.seek(0)
+        np.array([nd], dtype='<i4').tofile(f)
+        if verbose:
+            print('eof.save_eof_coefficients: coefficient file currently has {0:d} dumps.'.format(nd))
+        f.seek(4 + (nd - 1) * (16 * (EOF_Object.mmax + 1) * EOF_Object.nmax + _EOF_HDR))
+        eof_coefficients_to_file(f, EOF_Object)
+
+
+def extract_eof_coefficients(f):
+    """eof.py:1743-1763: one dump from an open coefficient file."""
+    o = EOF_Object()
+    o.time = np.fromfile(f, dtype='<f4', count=1)[0]
+    o.filename = np.fromfile(f, dtype='S100', count=1)[0]
+    o.comp = np.fromfile(f, dtype='S8', count=1)[0]
+    o.nbodies = int(np.fromfile(f, dtype='<i4', count=1)[0])
+    o.eof_file = np.fromfile(f, dtype='S100', count=1)[0]
+    o.mmax, o.nmax = (int(v) for v in np.fromfile(f, dtype='<i4', count=2))
+    k = (o.mmax + 1) * o.nmax
+    o.cos = np.fromfile(f, dtype='<f8', count=k).reshape(o.mmax + 1, o.nmax)
+    o.sin = np.fromfile(f, dtype='<f8', count=k).reshape(o.mmax + 1, o.nmax)
+    return o
+
+
+def restore_eof_coefficients(infile):
+    """eof.py:1708-1740: (last EOF_Object, {time: EOF_Object}) in file
+    order."""
+    from collections import OrderedDict
+    with open(infile, 'rb') as f:
+        nd = int(np.fromfile(f, dtype='<i4', count=1)[0])
+        out = OrderedDict()
+        o = None
+        for _ in range(nd):
+            o = extract_eof_coefficients(f)
+            out[o.time] = o
+    return o, out
 
 
 def compute_forces(PSPInput, EOF_Object, verbose=1, nprocs=-1, m1=0, m2=1000, density=False):

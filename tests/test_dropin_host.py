@@ -163,3 +163,108 @@ def test_bar_transform_given_angle():
     np.testing.assert_allclose(T.data['x'], [-1.0, 0.0], atol=1e-15)
     np.testing.assert_allclose(T.data['y'], [0.0, -2.0], atol=1e-15)
     assert np.array_equal(T.data['z'], [0.5, -0.5]) and np.array_equal(T.data['m'], [1.0, 2.0])
+
+
+OUTCOEF = os.path.join(HERE, "golden", "outcoef.star.run0.dat")
+
+
+def test_outcoef_reference_fixture():
+    """outcoef.star.run0.dat (the reference's own test data, an EXP cylinder
+    coefficient file with YAML records): 2500 records of mmax 2, nmax 10;
+    values checked against the raw bytes of the first and last records."""
+    import struct
+    from exptool_amd.io import outcoef
+    O = outcoef.OutCoef(OUTCOEF)
+    assert O.basis == "Cylinder" and O.coefs.shape == (2500, 2, 3, 10)
+    raw = open(OUTCOEF, "rb").read()
+    magic, n = struct.unpack_from("<II", raw, 0)
+    assert magic == 202004387 and raw[8:8 + n].startswith(b"time: 0.0019531249999999998")
+    first = struct.unpack_from("<50d", raw, 8 + n)
+    assert O.T[0] == 0.0019531249999999998
+    # m = 0 cos, then (cos, sin) for m = 1, 2
+    assert list(O.coefs[0, 0, 0]) == list(first[0:10])
+    assert list(O.coefs[0, 0, 1]) == list(first[10:20]) and list(O.coefs[0, 1, 1]) == list(first[20:30])
+    assert list(O.coefs[0, 1, 2]) == list(first[40:50])
+    assert list(O.coefs[-1].ravel()[-10:]) == list(struct.unpack_from("<10d", raw, len(raw) - 80))
+    assert np.all(O.coefs[:, 1, 0] == 0) and np.all(np.diff(O.T) > 0)
+    O._repackage_cylindrical_coefficients()
+    assert np.array_equal(O.C[1]["sin"][3], O.coefs[:, 1, 1, 3])
+    D = O._repackage_cylindrical_coefficients_compatibility()
+    assert len(D) == 2500 and np.array_equal(D[O.T[7]].cos, O.coefs[7, 0])
+
+
+def test_outcoef_sphere_and_old_formats(tmp_path):
+    from exptool_amd.io import outcoef
+    rng = np.random.default_rng(0)
+    L, N = 2, 3
+    recs = [rng.normal(size=((L + 1) ** 2, N)) for _ in range(3)]
+    with open(tmp_path / "sph.dat", "wb") as f:
+        for t, a in enumerate(recs):
+            text = f"time: {0.5 * t}\nlmax: {L}\nnmax: {N}\n".encode()
+            np.array([202004386, len(text)], "<u4").tofile(f)
+            f.write(text)
+            np.ascontiguousarray(a.T).tofile(f)
+    S = outcoef.OutCoef(str(tmp_path / "sph.dat"))
+    assert S.basis == "SphereSL" and np.array_equal(S.coefs, np.array(recs)) and list(S.T) == [0, 0.5, 1]
+    S._repackage_spherical_coefficients()
+    assert np.array_equal(S.C[2][1]["sin"][2], S.coefs[:, 6, 2])
+    with open(tmp_path / "old.dat", "wb") as f:
+        for t in range(2):
+            np.array([0.25 * t], "<f8").tofile(f)
+            np.array([1, N], "<u4").tofile(f)
+            np.arange(3 * N, dtype="<f8").tofile(f)
+    C = outcoef.OutCoef(str(tmp_path / "old.dat"))
+    assert C.basis == "Cylinder" and C.coefs.shape == (2, 2, 2, N)
+    assert list(C.coefs[1, 0, 1]) == [3.0, 4.0, 5.0] and list(C.coefs[1, 1, 1]) == [6.0, 7.0, 8.0]
+
+
+@pytest.mark.parametrize("float_len,index", [(4, False), (8, True)])
+def test_spl_roundtrip(tmp_path, float_len, index):
+    from exptool_amd.io import spl_io
+    rng = np.random.default_rng(1)
+    n = 1001
+    d = {k: rng.normal(size=n) for k in ("m", "x", "y", "z", "vx", "vy", "vz", "potE")}
+    if index:
+        d["id"] = np.arange(n, dtype=np.int64) + 7
+    h = rng.normal(size=(8, 40))
+    halo = dict(zip(("m", "x", "y", "z", "vx", "vy", "vz", "potE"), h))
+    path = str(tmp_path / "SPL.run1.00003")
+    spl_io.write_spl(path, 2.5, [({"name": "star", "id": "cylinder"}, d),
+                                 ({"name": "dark", "id": "sphereSL"}, halo)], nprocs=3,
+                     float_len=float_len)
+    top = spl_io.Input(path)
+    assert top.time == 2.5 and list(top.header) == ["star", "dark"]
+    I = spl_io.Input(path, comp="star")
+    dt = np.float32 if float_len == 4 else np.float64
+    for k in ("m", "x", "y", "z", "vx", "vy", "vz", "potE"):
+        assert I.data[k].dtype == dt and np.array_equal(I.data[k], d[k].astype(dt))
+    assert ("id" in I.data) == index and (not index or np.array_equal(I.data["id"], d["id"]))
+    assert len(I.subfiles) == 3
+    J = spl_io.Input(path, comp="dark")
+    assert np.array_equal(J.data["x"], h[1].astype(dt))
+    with pytest.raises(IOError):
+        spl_io.Input(path, comp="gas")
+
+
+def test_eof_coefficient_file_roundtrip(tmp_path):
+    """save_eof_coefficients / restore_eof_coefficients (eof.py:1646-1763):
+    leading i4 dump count, then 224-byte headers and f8 cos, sin."""
+    from exptool_amd.basis import eof
+    rng = np.random.default_rng(2)
+    path = str(tmp_path / "coefs.dat")
+    objs = []
+    for t in (0.5, 1.0, 1.5):
+        o = eof.EOF_Object()
+        o.time, o.filename, o.comp, o.nbodies, o.eof_file = t, "OUT.run.00001", "star", 1234, "eof.cache"
+        o.mmax, o.nmax = 4, 6
+        o.cos, o.sin = rng.normal(size=(5, 6)), rng.normal(size=(5, 6))
+        eof.save_eof_coefficients(path, o)
+        objs.append(o)
+    raw = open(path, "rb").read()
+    assert len(raw) == 4 + 3 * (16 * 5 * 6 + 224)
+    assert np.frombuffer(raw[:4], "<i4")[0] == 3
+    assert np.array_equal(np.frombuffer(raw[4 + 224:4 + 224 + 240], "<f8"), objs[0].cos.ravel())
+    last, d = eof.restore_eof_coefficients(path)
+    assert list(d) == [np.float32(0.5), np.float32(1.0), np.float32(1.5)]
+    assert np.array_equal(last.sin, objs[2].sin) and last.comp == b"star" and last.nbodies == 1234
+    assert np.array_equal(d[np.float32(1.0)].cos, objs[1].cos)
