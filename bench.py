@@ -121,11 +121,10 @@ def main():
         tuned = autotune(sysm, disc=(dx, dy, dz, dm), halo=(hx, hy, hz, hm), cache=False)
 
     def step():
-        # Fields.total_coefficients (disc and halo accumulate, run concurrently),
-        # prep_tables/set_field_parameters, return_forces_cart at every particle
-        c, s, h = sysm.accumulate_disc_halo((dx, dy, dz, dm), (hx, hy, hz, hm))
-        sysm.set_field(c, s, h)
-        return sysm.field_forces(ax, ay, az)
+        # Fields.total_coefficients (disc and halo accumulate, run concurrently,
+        # each folded into the frozen field as soon as it is reduced), then
+        # return_forces_cart at every particle: one bfe_forces_disc_halo call
+        return sysm.forces_disc_halo((dx, dy, dz, dm), (hx, hy, hz, hm))
 
     for _ in range(a.warmup):
         step()

@@ -33,7 +33,7 @@ EXPORTS = (
     "bfe_field_eval_cart", "bfe_leapfrog", "bfe_leapfrog_dt", "bfe_rk4", "bfe_mlp_set_params", "bfe_mlp_get_params",
     "bfe_mlp_forward", "bfe_mlp_gradient", "bfe_mlp_train_step", "bfe_mlp_leapfrog",
     "bfe_eof_accumulate_batch", "bfe_sl_accumulate_batch", "bfe_field_set_batch",
-    "bfe_field_eval_batch", "bfe_accumulate_disc_halo",
+    "bfe_field_eval_batch", "bfe_accumulate_disc_halo", "bfe_forces_disc_halo",
     "bfe_set_tuning", "bfe_get_tuning", "bfe_kernel_times",
 )
 MLP_NPARAMS = 50564
@@ -69,6 +69,9 @@ def _declare(lib):
         "bfe_field_eval_batch": (_I32, [_VP, _VP, _VP, _VP, _VP, _I32, _F64, _VP, _VP, _VP, _VP, _VP]),
         "bfe_accumulate_disc_halo": (_I32, [_VP, _VP, _VP, _VP, _VP, _I64, _VP, _VP, _VP, _VP, _I64,
                                             _U32, _VP, _VP, _VP, _VP]),
+        "bfe_forces_disc_halo": (_I32, [_VP, _VP, _VP, _VP, _VP, _I64, _VP, _VP, _VP, _VP, _I64, _U32,
+                                        _VP, _VP, _VP, _F64, _I32, _I32, _I32, _I32, _U32, _F64,
+                                        _VP, _VP, _VP, _VP, _VP, _VP]),
         "bfe_set_tuning": (_I32, [_VP, C.c_char_p, _I64]),
         "bfe_get_tuning": (_I32, [_VP, C.c_char_p, C.POINTER(_I64)]),
         "bfe_kernel_times": (_I32, [_VP, C.c_char_p, C.POINTER(_F64), C.POINTER(_I64)]),

@@ -93,7 +93,7 @@ class Fields:
         self._set_tables(eof_mod.load_eof(self.eof_file),
                          halo_methods.load_sl(self.sph_file, self.model_file))
 
-    def total_coefficients(self, disk=None, halo=None):
+    def total_coefficients(self, disk=None, halo=None, return_forces=False):
         """potential.py:98-252.  Particles come from the PSP file `infile`
         (components 'star' and 'dark') or from the `disk` / `halo` objects
         (particle.holder, Input with .data).  transform rotates both into the
@@ -102,7 +102,11 @@ class Fields:
         component (the disc centre for both with mutual_center; with compat
         the halo centre uses the indices of the innermost disc particles, as
         the reference does at :199-209).  halofac is 1 (the reference's ratio
-        of two identical reads, :138-143)."""
+        of two identical reads, :138-143).  return_forces=True also returns
+        return_forces_cart (fx_disk, fx_halo, ..., pot_halo; potential.py:
+        445-497) at every particle used, disc particles then halo particles
+        (in the bar frame and centred coordinates the coefficients used), with
+        the current field parameters, from one bfe_forces_disc_halo call."""
         if disk is None:
             disk = particle.Input(self.filename, comp="star", verbose=self.verbose)
         if halo is None:
@@ -139,9 +143,19 @@ class Fields:
             self.xcen_disk = self.ycen_disk = self.zcen_disk = 0.0
             self.xcen_halo = self.ycen_halo = self.zcen_halo = 0.0
         dx, dy, dz = eof_mod._nan_to_origin(dx, dy, dz)
-        # disc and halo accumulate run concurrently on the device
-        c, s, hcoef = self.system.accumulate_disc_halo((dx, dy, dz, dm), (hx, hy, hz, hm),
-                                                       no_odd=self.no_odd)
+        forces = None
+        if return_forces:
+            ll, hn = self._halo_truncation()
+            got = self.system.forces_disc_halo(
+                (dx, dy, dz, dm), (hx, hy, hz, hm), halofac=self.halofac, no_odd=self.no_odd,
+                disk_m=min(self.disk_use_m, self.mmax), disk_n=min(self.disk_use_n, self.norder),
+                halo_l=ll, halo_n=hn, compat=self.compat, components=True)
+            c, s, hcoef = got["cos"], got["sin"], got["coef"]
+            forces = tuple(v for v in got["comp"].double().cpu().numpy())
+        else:
+            # disc and halo accumulate run concurrently on the device
+            c, s, hcoef = self.system.accumulate_disc_halo((dx, dy, dz, dm), (hx, hy, hz, hm),
+                                                           no_odd=self.no_odd)
         self.EOF = eof_mod.EOF_Object()
         self.EOF.cos, self.EOF.sin = c.cpu().numpy(), s.cpu().numpy()
         self.EOF.mmax, self.EOF.nmax, self.EOF.eof_file = self.mmax, self.norder, self.eof_file
@@ -152,6 +166,8 @@ class Fields:
         self.SL.sph_file, self.SL.model_file = self.sph_file, self.model_file
         self.SL.time, self.SL.nbodies = self.time, hm.size
         self._frozen = None
+        if return_forces:
+            return forces
 
     # ------------------------------------------------------------ parameters
     def set_field_parameters(self, no_odd=False, halo_l=-1, halo_n=-1, disk_m=-1, disk_n=-1):

@@ -153,6 +153,45 @@ class BFESystem:
         self.ctx.check(rc, "bfe_accumulate_disc_halo")
         return cos, sin, coef
 
+    def forces_disc_halo(self, disc, halo, halofac=1.0, no_odd=False, disk_m=-1, disk_n=-1,
+                         halo_l=-1, halo_n=-1, compat=False, rotpos=0.0, bf16=False,
+                         components=False, stream=None):
+        """Fields.total_coefficients then return_forces_cart at the same
+        particles, in one call (bfe_forces_disc_halo): disc (x, y, z, m) and
+        halo (x, y, z, m) give the coefficients, the frozen field and the
+        forces on every particle, disc first.  Returns dict(cos, sin, coef,
+        fx, fy, fz, pot[, comp]); the field stays set."""
+        self._need("eof")
+        self._need("sl")
+        dx, dy, dz, nd = self._xyz(*disc[:3])
+        dm = f32(disc[3], self.device)
+        hx, hy, hz, nh = self._xyz(*halo[:3])
+        hm = f32(halo[3], self.device)
+        if dm.numel() != nd or hm.numel() != nh:
+            raise ValueError("mass must have the same length as x")
+        shape = (self.eof["mmax"] + 1, self.eof["norder"])
+        out = dict(cos=torch.empty(shape, dtype=torch.float64, device=self.device),
+                   sin=torch.empty(shape, dtype=torch.float64, device=self.device),
+                   coef=torch.empty(((self.sl["lmax"] + 1) ** 2, self.sl["nmax"]),
+                                    dtype=torch.float64, device=self.device))
+        n = nd + nh
+        for k in ("fx", "fy", "fz", "pot"):
+            out[k] = torch.empty(n, dtype=torch.float32, device=self.device)
+        comp = torch.empty((8, n), dtype=torch.float32, device=self.device) if components else None
+        acc = (_lib.NO_ODD if no_odd else 0) | (_lib.ACC_BF16 if bf16 else 0)
+        fld = (_lib.NO_ODD if no_odd else 0) | (_lib.COMPAT_ALL if compat else 0)
+        rc = self.ctx.lib.bfe_forces_disc_halo(
+            self.ctx.h, ptr(dx), ptr(dy), ptr(dz), ptr(dm), nd, ptr(hx), ptr(hy), ptr(hz), ptr(hm),
+            nh, acc, ptr(out["cos"]), ptr(out["sin"]), ptr(out["coef"]), float(halofac),
+            int(disk_m), int(disk_n), int(halo_l), int(halo_n), fld, float(rotpos),
+            ptr(out["fx"]), ptr(out["fy"]), ptr(out["fz"]), ptr(out["pot"]), ptr(comp),
+            stream_ptr(stream))
+        self.ctx.check(rc, "bfe_forces_disc_halo")
+        self._field_keep = (out["cos"], out["sin"], out["coef"])
+        if components:
+            out["comp"] = comp
+        return out
+
     # ------------------------------------------------------------- batches
     @staticmethod
     def _offsets(counts_or_offsets, n=None, offsets=False):

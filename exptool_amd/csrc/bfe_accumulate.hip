@@ -1230,6 +1230,16 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
   return 0;
 }
 
+int eof_accumulate_one(Context* c, const float* x, const float* y, const float* z,
+                       const float* m, int64_t n, double* cos_out, double* sin_out, hipStream_t s) {
+  return eof_accumulate_run(c, x, y, z, m, n, nullptr, 1, cos_out, sin_out, s);
+}
+
+int sl_accumulate_one(Context* c, const float* x, const float* y, const float* z, const float* m,
+                      int64_t n, uint32_t flags, double* coef_out, hipStream_t s) {
+  return sl_accumulate_run(c, x, y, z, m, n, nullptr, 1, flags, coef_out, s);
+}
+
 }  // namespace bfe
 
 using namespace bfe;

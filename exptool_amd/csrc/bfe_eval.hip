@@ -13,13 +13,15 @@ struct SLFoldArgs {
   int lmax, nmax_use;
 };
 
-// One thread per (radial node i, (l, m) pair p in m-major order): E and Dn at
-// i and i+1 and G at c-1, c, c+1 (c = max(i, 1)) are summed over n in
-// float64 from the float64 scaled tables S = [ef/sqrt(ev), ef sqrt(ev),
-// ef p0/sqrt(ev)] ([l][i][n] each), D1/D2 formed, and the blocks rounded once
-// to float32 (layout in bfe_field.hpp; the last node, never a cell start,
-// is zero).  grid.y runs over coefficient sets: set q reads
-// coef + q * coef_stride and writes out + q * out_stride.
+// Four lanes per (radial node i, (l, m) pair p in m-major order), lane q
+// summing n = q, q + 4, ...: E and Dn at i and i+1 and G at c-1, c, c+1
+// (c = max(i, 1)) are summed over n in float64 from the float64 scaled
+// tables S = [ef/sqrt(ev), ef sqrt(ev), ef p0/sqrt(ev)] ([l][i][n] each),
+// the four partial sums are combined across the lanes, D1/D2 formed, and the
+// blocks rounded once to float32 (layout in bfe_field.hpp; the last node,
+// never a cell start, is zero).  grid.y runs over coefficient sets: set q
+// reads coef + q * coef_stride and writes out + q * out_stride.
+constexpr int kFoldLanes = 4;
 __global__ __launch_bounds__(256) void sl_fold_kernel(const double* __restrict__ S,
                                                       const double* __restrict__ coef, int nmax,
                                                       int numr, SLFoldArgs a,
@@ -28,15 +30,20 @@ __global__ __launch_bounds__(256) void sl_fold_kernel(const double* __restrict__
   coef += (size_t)blockIdx.y * coef_stride;
   out += (size_t)blockIdx.y * out_stride;
   const int np = sl_npairs(a.lmax);
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = (int)(gid & (kFoldLanes - 1));
+  const int64_t idx = gid / kFoldLanes;
+  // groups of four lanes are whole within a wave, so a group leaves together
   if (idx >= (int64_t)numr * np) return;
   const int i = (int)(idx / np);
   const int p = (int)(idx - (int64_t)i * np);
   float* o = out + ((size_t)i * np + p) * 8;
   float* od = out + (size_t)numr * np * 8 + ((size_t)i * np + p) * 4;
   if (i == numr - 1) {
-    for (int k = 0; k < 8; ++k) o[k] = 0.f;
-    for (int k = 0; k < 4; ++k) od[k] = 0.f;
+    if (q == 0)
+      for (int k = 0; k < 8; ++k) o[k] = 0.f;
+    if (q == 1)
+      for (int k = 0; k < 4; ++k) od[k] = 0.f;
     return;
   }
   int m = 0, rem = p;
@@ -59,7 +66,7 @@ __global__ __launch_bounds__(256) void sl_fold_kernel(const double* __restrict__
   // v[cs]: E_i, E_i+1, Dn_i, Dn_i+1, G_c-1, G_c, G_c+1
   double v[2][7] = {{0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0}};
 #pragma unroll 2
-  for (int n = 0; n < a.nmax_use; ++n) {
+  for (int n = q; n < a.nmax_use; n += kFoldLanes) {
     const double c0 = cc[n] * mul;
     const double c1 = two ? cs[n] * mul : 0.0;
     const double t[7] = {E[n], E[n + nmax], D[n], D[n + nmax], G[n - nmax], G[n], G[n + nmax]};
@@ -69,18 +76,31 @@ __global__ __launch_bounds__(256) void sl_fold_kernel(const double* __restrict__
       v[1][k] += c1 * t[k];
     }
   }
-  o[0] = (float)v[0][0];
-  o[1] = (float)v[1][0];
-  o[2] = (float)v[0][1];
-  o[3] = (float)v[1][1];
-  o[4] = (float)(0.5 * (v[0][6] - v[0][4]));
-  o[5] = (float)(0.5 * (v[1][6] - v[1][4]));
-  o[6] = (float)(v[0][4] - 2.0 * v[0][5] + v[0][6]);
-  o[7] = (float)(v[1][4] - 2.0 * v[1][5] + v[1][6]);
-  od[0] = (float)v[0][2];
-  od[1] = (float)v[1][2];
-  od[2] = (float)v[0][3];
-  od[3] = (float)v[1][3];
+  // fixed-order combination: (q0 + q1) + (q2 + q3) on every lane of the group
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      v[h][k] += __shfl_xor(v[h][k], 1, kFoldLanes);
+      v[h][k] += __shfl_xor(v[h][k], 2, kFoldLanes);
+    }
+  }
+  if (q == 0) {
+    o[0] = (float)v[0][0];
+    o[1] = (float)v[1][0];
+    o[2] = (float)v[0][1];
+    o[3] = (float)v[1][1];
+  } else if (q == 1) {
+    o[4] = (float)(0.5 * (v[0][6] - v[0][4]));
+    o[5] = (float)(0.5 * (v[1][6] - v[1][4]));
+    o[6] = (float)(v[0][4] - 2.0 * v[0][5] + v[0][6]);
+    o[7] = (float)(v[1][4] - 2.0 * v[1][5] + v[1][6]);
+  } else if (q == 2) {
+    od[0] = (float)v[0][2];
+    od[1] = (float)v[1][2];
+    od[2] = (float)v[0][3];
+    od[3] = (float)v[1][3];
+  }
 }
 
 struct EOFFoldArgs {
@@ -171,7 +191,7 @@ static int sl_fold_into(Context* c, const double* coef, int nset, double halofac
   }
   a.lmax = c->lmax;
   a.nmax_use = nmax_use;
-  const int64_t total = (int64_t)c->numr * np;
+  const int64_t total = (int64_t)c->numr * np * kFoldLanes;
   hipLaunchKernelGGL(sl_fold_kernel, dim3((unsigned)((total + 255) / 256), nset), dim3(256), 0, s,
                      c->sl_fold64, coef, c->nmax, c->numr, a, out,
                      (int64_t)(c->lmax + 1) * (c->lmax + 1) * c->nmax, (int64_t)sl_fold_floats(c));
@@ -479,19 +499,16 @@ extern "C" int bfe_eof_eval(bfe_ctx* h, const float* x, const float* y, const fl
   return 0;
 }
 
-extern "C" int bfe_field_set(bfe_ctx* h, const double* eof_cos, const double* eof_sin,
-                             const double* sl_coef, double halofac, int disk_m, int disk_n,
-                             int halo_l, int halo_n, uint32_t flags, void* stream) {
-  if (!h) return BFE_EINVAL;
-  Context* c = &h->c;
-  if (!c->sl_ok || !c->eof_ok) return set_error(c, BFE_ENOTABLE, "field_set: tables not set");
-  if (!eof_cos || !eof_sin || !sl_coef) return set_error(c, BFE_EINVAL, "field_set: NULL coef");
+namespace bfe {
+
+// Fold one coefficient set into the frozen field (bfe_field_set).
+static int field_set_run(Context* c, const double* eof_cos, const double* eof_sin,
+                         const double* sl_coef, double halofac, int disk_m, int disk_n,
+                         int halo_l, int halo_n, uint32_t flags, hipStream_t s) {
   const int mm = disk_m < 0 ? c->mmax : std::min(disk_m, c->mmax);
   const int nn = disk_n < 0 ? c->norder : std::min(disk_n, c->norder);
   const int ll = halo_l < 0 ? c->lmax : std::min(halo_l, c->lmax);
   const int hn = halo_n < 0 ? c->nmax : std::min(halo_n, c->nmax);
-  BFE_HIP(c, hipSetDevice(c->device));
-  hipStream_t s = (hipStream_t)stream;
   const bool no_odd = (flags & BFE_NO_ODD) != 0;
   // the SL fold (a few thousand waves) runs beside the HBM-bound EOF fold
   hipStream_t s2 = nullptr;
@@ -505,6 +522,34 @@ extern "C" int bfe_field_set(bfe_ctx* h, const double* eof_cos, const double* eo
   c->field_halofac = halofac;
   c->field_flags = (int)(flags & (BFE_COMPAT_SL_COS_L | BFE_COMPAT_SL_EDGE));
   return 0;
+}
+
+// Evaluate n points: sorted (pts, the index in .w) or in input order.
+static int field_eval_run(Context* c, const FieldTables& T, const float* x, const float* y,
+                          const float* z, int64_t n, const float4* pts, float* fx, float* fy,
+                          float* fz, float* pot, float* comp, hipStream_t s) {
+  if (n == 0) return 0;
+  const unsigned grid = pts ? xcd_grid(nblocks256(n)) : nblocks256(n);
+  timer_begin(c, "field_eval", s);
+  hipLaunchKernelGGL(field_eval_kernel, dim3(grid), dim3(256), 0, s, x, y, z, n, pts, T, fx, fy,
+                     fz, pot, comp);
+  timer_end(c, "field_eval", s);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
+}  // namespace bfe
+
+extern "C" int bfe_field_set(bfe_ctx* h, const double* eof_cos, const double* eof_sin,
+                             const double* sl_coef, double halofac, int disk_m, int disk_n,
+                             int halo_l, int halo_n, uint32_t flags, void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (!c->sl_ok || !c->eof_ok) return set_error(c, BFE_ENOTABLE, "field_set: tables not set");
+  if (!eof_cos || !eof_sin || !sl_coef) return set_error(c, BFE_EINVAL, "field_set: NULL coef");
+  BFE_HIP(c, hipSetDevice(c->device));
+  return field_set_run(c, eof_cos, eof_sin, sl_coef, halofac, disk_m, disk_n, halo_l, halo_n,
+                       flags, (hipStream_t)stream);
 }
 
 extern "C" int bfe_field_eval_cart(bfe_ctx* h, const float* x, const float* y, const float* z,
@@ -525,13 +570,66 @@ extern "C" int bfe_field_eval_cart(bfe_ctx* h, const float* x, const float* y, c
     rc = order_points(c, T, x, y, z, n, s, nullptr, nullptr, 1, nullptr, &pts);
     if (rc) return rc;
   }
-  const unsigned grid = pts ? xcd_grid(nblocks256(n)) : nblocks256(n);
-  timer_begin(c, "field_eval", s);
-  hipLaunchKernelGGL(field_eval_kernel, dim3(grid), dim3(256), 0, s, x, y, z, n, pts, T, fx, fy,
-                     fz, pot, comp);
-  timer_end(c, "field_eval", s);
-  BFE_HIP(c, hipGetLastError());
-  return 0;
+  return field_eval_run(c, T, x, y, z, n, pts, fx, fy, fz, pot, comp, s);
+}
+
+// Fields.total_coefficients followed by Fields.return_forces_cart at the
+// same particles (potential.py:98-252, :445-497).  Two concurrent branches,
+// each folding its coefficients as soon as they are reduced:
+//   caller stream  disc EOF accumulate, then the EOF fold;
+//   internal       halo SL accumulate (second buffer set), then the SL fold;
+// then, after the join, the evaluation order of all nd + nh points taken
+// as one set (no concatenated copy) and the evaluation.  Results equal
+// bfe_accumulate_disc_halo + bfe_field_set + bfe_field_eval_cart over the
+// concatenated points.
+extern "C" int bfe_forces_disc_halo(bfe_ctx* h, const float* dx, const float* dy,
+                                    const float* dz, const float* dm, int64_t nd, const float* hx,
+                                    const float* hy, const float* hz, const float* hm, int64_t nh,
+                                    uint32_t acc_flags, double* cos_out, double* sin_out,
+                                    double* coef_out, double halofac, int disk_m, int disk_n,
+                                    int halo_l, int halo_n, uint32_t field_flags, double rotpos,
+                                    float* fx, float* fy, float* fz, float* pot, float* comp,
+                                    void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (!c->eof_ok || !c->sl_ok) return set_error(c, BFE_ENOTABLE, "forces_disc_halo: tables not set");
+  if (!cos_out || !sin_out || !coef_out)
+    return set_error(c, BFE_EINVAL, "forces_disc_halo: NULL coefficient output");
+  if (nd < 0 || nh < 0) return set_error(c, BFE_EINVAL, "forces_disc_halo: n < 0");
+  if ((nd > 0 && (!dx || !dy || !dz || !dm)) || (nh > 0 && (!hx || !hy || !hz || !hm)))
+    return set_error(c, BFE_EINVAL, "forces_disc_halo: NULL input");
+  if (nd + nh > (int64_t)0x7fffffff) return set_error(c, BFE_ELIMIT, "forces_disc_halo: nd + nh > 2^31 - 1");
+  BFE_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream, s1 = nullptr;
+  const int64_t n = nd + nh;
+  const int mm = disk_m < 0 ? c->mmax : std::min(disk_m, c->mmax);
+  const int nn = disk_n < 0 ? c->norder : std::min(disk_n, c->norder);
+  const int ll = halo_l < 0 ? c->lmax : std::min(halo_l, c->lmax);
+  const int hn = halo_n < 0 ? c->nmax : std::min(halo_n, c->nmax);
+  const bool no_odd = (field_flags & BFE_NO_ODD) != 0;
+  int rc = fork_begin(c, s, &s1);
+  if (rc) return rc;
+  // halo: accumulate and fold on the first internal stream
+  swap_work(c);
+  int r1 = sl_accumulate_one(c, hx, hy, hz, hm, nh, acc_flags, coef_out, s1);
+  swap_work(c);
+  if (!r1) r1 = sl_fold(c, 1, coef_out, halofac, ll, hn, -(1 << 30), 1 << 30, no_odd, s1);
+  // disc: accumulate and fold on the caller's stream
+  int r0 = eof_accumulate_one(c, dx, dy, dz, dm, nd, cos_out, sin_out, s);
+  if (!r0) r0 = eof_fold(c, 1, cos_out, sin_out, mm, nn, 0, 1 << 30, no_odd,
+                         (field_flags & BFE_COMPAT_EOF_MASK) != 0, false, s);
+  rc = fork_end(c, s);
+  if (r0 || r1 || rc) return r0 ? r0 : (r1 ? r1 : rc);
+  c->field_halofac = halofac;
+  c->field_flags = (int)(field_flags & (BFE_COMPAT_SL_COS_L | BFE_COMPAT_SL_EDGE));
+  FieldTables T;
+  rc = field_tables(c, rotpos, T);
+  if (rc || n == 0) return rc;
+  // the two inputs are evaluated as one set, so they are ordered at every size
+  const float4* pts = nullptr;
+  rc = order_points2(c, T, dx, dy, dz, nd, hx, hy, hz, nh, s, &pts);
+  if (rc) return rc;
+  return field_eval_run(c, T, nullptr, nullptr, nullptr, n, pts, fx, fy, fz, pot, comp, s);
 }
 
 extern "C" int bfe_field_set_batch(bfe_ctx* h, const double* eof_cos, const double* eof_sin,
@@ -612,7 +710,7 @@ extern "C" int bfe_field_eval_batch(bfe_ctx* h, const float* x, const float* y, 
   int rc = order_points(c, T, x, y, z, n, s, nullptr, c->batch_off, nsnap, &sorted, &pts);
   if (rc) return rc;
   hipLaunchKernelGGL(field_eval_batch_kernel, dim3(xcd_grid(nblocks256(n))), dim3(256), 0, s, n,
-                     pts, sorted, order_cell_count(T), T, c->batch_sl, (int64_t)slf, c->batch_eof,
+                     pts, sorted, order_cell_count(T, nsnap), T, c->batch_sl, (int64_t)slf, c->batch_eof,
                      (int64_t)eoff, fx, fy, fz, pot);
   BFE_HIP(c, hipGetLastError());
   return 0;

@@ -177,9 +177,9 @@ inline EOFGeom eof_geom(const Context* c) {
 }
 
 struct FieldTables;
-// Permutation perm[j] = i listing the points in order of their table cell
-// (EOF (R, z) cell inside the disc grid, else the SL radial cell), so that
-// the lanes of a wave read the same table rows (perm_out may be NULL).
+// Permutation perm[j] = i listing the points in order of their SL radial
+// cell and cos(theta) bin, so that the lanes of a block share SL nodes and a
+// small patch of the EOF grid (perm_out may be NULL).
 // Rebuilt by each call; the returned pointers stay valid until the next call
 // on this context.
 // With off_dev (device int64[nsnap + 1], concatenated snapshots) the key is
@@ -190,14 +190,26 @@ int order_points(Context* c, const FieldTables& T, const float* x, const float* 
                  const float* z, int64_t n, hipStream_t s, const int** perm_out,
                  const int64_t* off_dev = nullptr, int nsnap = 1,
                  const uint32_t** sorted_out = nullptr, const float4** points_out = nullptr);
-uint32_t order_cell_count(const FieldTables& T);
+uint32_t order_cell_count(const FieldTables& T, int nsnap);
+// order_points over two point sets taken as one (n1 points of x, y, z then
+// n2 of x2, y2, z2): points_out in key order with the index over both.
+int order_points2(Context* c, const FieldTables& T, const float* x, const float* y,
+                  const float* z, int64_t n1, const float* x2, const float* y2, const float* z2,
+                  int64_t n2, hipStream_t s, const float4** points_out);
 // Sorted copy of the particles packed as float4 (x, y, z, m) in accumulate
-// cell order: kind 0 EOF (ix, iy) cells, kind 1 SL radial cells; with off_dev
-// the key is (snapshot, cell), so snapshot s keeps its range
-// [off[s], off[s+1]).  The sort is stable, so the order is reproducible.
+// cell order: kind 0 EOF (ix, iy) cells, kind 1 SL radial cells and
+// cos(theta) bins (the evaluation order); with off_dev the key is
+// (snapshot, cell), so snapshot s keeps its range [off[s], off[s+1]).  The
+// sort is stable, so the order is reproducible.
 int order_particles(Context* c, int kind, const float* x, const float* y, const float* z,
                     const float* m, int64_t n, const int64_t* off_dev, int nsnap, hipStream_t s,
                     const float4** sorted_out);
+// Single-snapshot accumulates (bfe_eof_accumulate / bfe_sl_accumulate
+// without the argument checks), enqueued on s with the current buffer set.
+int eof_accumulate_one(Context* c, const float* x, const float* y, const float* z,
+                       const float* m, int64_t n, double* cos_out, double* sin_out, hipStream_t s);
+int sl_accumulate_one(Context* c, const float* x, const float* y, const float* z, const float* m,
+                      int64_t n, uint32_t flags, double* coef_out, hipStream_t s);
 // float64 per-point scratch in the ordering buffer (n entries), valid after
 // order_points for the same n.
 double* order_energy_buffer(Context* c, int64_t n);

@@ -55,6 +55,7 @@ struct KeyGeom {
   float ea, eih, exmin, eidx, eymin, eidy;   // EOF: ascale, 1/hscale, xmin, 1/dx, ymin, 1/dy
   int scmap, numr;
   float ss, sxi0, sidxi;                     // SL: scale, xi[0], 1/(xi[1] - xi[0])
+  uint32_t tbins;                            // cos(theta) bins per SL cell (kind 1)
 };
 
 static KeyGeom key_geom(const EOFGeom& e, const SLGeom& s, const double* h_xi) {
@@ -73,7 +74,16 @@ static KeyGeom key_geom(const EOFGeom& e, const SLGeom& s, const double* h_xi) {
   g.ss = (float)s.scale;
   g.sxi0 = h_xi ? (float)h_xi[0] : 0.f;
   g.sidxi = h_xi ? (float)(1.0 / (h_xi[1] - h_xi[0])) : 0.f;
+  g.tbins = 1;
   return g;
+}
+
+// cos(theta) bins per SL cell: kThetaBins unless the keys of nsnap
+// snapshots would then pass 32 bits
+constexpr uint32_t kThetaBins = 32;
+static uint32_t theta_bins(int numr, int nsnap) {
+  const uint64_t full = (uint64_t)(numr - 1) * kThetaBins * (uint64_t)nsnap;
+  return full <= 0xffffffffull ? kThetaBins : 1u;
 }
 
 // r -> xi of the maps (compatibility.py r_to_xi): (r/a - 1)/(r/a + 1), ln r, r
@@ -99,6 +109,16 @@ __device__ __forceinline__ uint32_t key_sl_cell(const KeyGeom& g, float r) {
   return key_clamp((key_xi(r, g.scmap, g.ss) - g.sxi0) * g.sidxi, g.numr - 2);
 }
 
+// SL radial cell, then one of g.tbins equal bins of cos(theta): points of
+// one key share their SL nodes and lie in a small (R, z) patch of the EOF
+// grid, which serves the accumulate (runs of one SL cell) and evaluation.
+__device__ __forceinline__ uint32_t key_sl_theta(const KeyGeom& g, float x, float y, float z) {
+  const float r = sqrtf(fmaf(x, x, fmaf(y, y, z * z)));
+  const float ct = r > 0.f ? __fdividef(z, r) : 0.f;
+  return key_sl_cell(g, fmaxf(r, 1.e-10f)) * g.tbins +
+         key_clamp((ct + 1.f) * (0.5f * (float)g.tbins), (int)g.tbins - 1);
+}
+
 // Where the keys and packed values of the first pass come from.
 struct KeySrc {
   const float* x;
@@ -108,31 +128,47 @@ struct KeySrc {
   const int64_t* off;    // device snapshot offsets [nsnap + 1], NULL: one snapshot
   int nsnap;
   uint32_t ncell;        // cells per snapshot
-  int kind;              // 0 EOF accumulate cell, 1 SL accumulate cell, 2 evaluation cell
+  int kind;              // 0 EOF accumulate cell, 1 SL accumulate / evaluation cell
   KeyGeom g;
+  // items n1.. come from a second set of arrays (x2, y2, z2; no mass), so
+  // two point sets are ordered as one: the index in .w runs over both
+  const float* x2;
+  const float* y2;
+  const float* z2;
+  int64_t n1;
 };
 
+__device__ __forceinline__ void source_xyz(const KeySrc& S, int64_t i, float& x, float& y, float& z) {
+  if (i < S.n1) {
+    x = S.x[i]; y = S.y[i]; z = S.z[i];
+  } else {
+    const int64_t k = i - S.n1;
+    x = S.x2[k]; y = S.y2[k]; z = S.z2[k];
+  }
+}
+
+// Packed value of item i on the first pass: (x, y, z, mass or index bits).
+__device__ __forceinline__ float4 source_value(const KeySrc& S, int64_t i) {
+  float x, y, z;
+  source_xyz(S, i, x, y, z);
+  return make_float4(x, y, z, S.m ? S.m[i] : __int_as_float((int)i));
+}
+
 // kind 0: the EOF (ix, iy) cell of (R = sqrt(x^2 + y^2 + 1e-10), z), clamped
-// to the grid as eof.return_bins; kind 1: the SL radial cell of
-// r = max(|x|, 1e-10); kind 2: the EOF cell for points inside the disc grid,
-// else numx * numy + the SL radial cell.
+// to the grid as eof.return_bins; kind 1: key_sl_theta of r = max(|x|, 1e-10)
+// (the SL accumulate and evaluation order).
 __device__ __forceinline__ uint32_t source_key(const KeySrc& S, int64_t i) {
-  const float x = S.x[i], y = S.y[i], z = S.z[i];
+  float x, y, z;
+  source_xyz(S, i, x, y, z);
   const KeyGeom& g = S.g;
   uint32_t cell;
   if (S.kind == 1) {
-    cell = key_sl_cell(g, fmaxf(sqrtf(fmaf(x, x, fmaf(y, y, z * z))), 1.e-10f));
+    cell = key_sl_theta(g, x, y, z);
   } else {
-    const float R2 = fmaf(x, x, y * y);
-    const float R = sqrtf(S.kind == 0 ? R2 + 1.e-10f : R2);
+    const float R = sqrtf(fmaf(x, x, fmaf(y, y, 1.e-10f)));
     const float X = (key_xi(R, g.ecmap, g.ea) - g.exmin) * g.eidx;
     const float Y = (key_y(z, g.eih) - g.eymin) * g.eidy;
-    if (S.kind == 0)
-      cell = key_clamp(X, g.numx - 1) * (uint32_t)g.numy + key_clamp(Y, g.numy - 1);
-    else if (X >= 0.f && X < (float)g.numx && Y >= 0.f && Y < (float)g.numy)
-      cell = (uint32_t)X * (uint32_t)g.numy + (uint32_t)Y;
-    else
-      cell = (uint32_t)g.numx * (uint32_t)g.numy + key_sl_cell(g, sqrtf(fmaf(z, z, R2)));
+    cell = key_clamp(X, g.numx - 1) * (uint32_t)g.numy + key_clamp(Y, g.numy - 1);
   }
   const uint32_t snap = S.off ? (uint32_t)snapshot_of(S.off, S.nsnap, i) : 0u;
   return snap * S.ncell + cell;
@@ -252,9 +288,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
     val[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (idx < n) {
       key[i] = kin[idx];
-      val[i] = vin ? vin[idx]
-                   : make_float4(S.x[idx], S.y[idx], S.z[idx],
-                                 S.m ? S.m[idx] : __int_as_float((int)idx));
+      val[i] = vin ? vin[idx] : source_value(S, idx);
     }
   }
   __syncthreads();
@@ -358,7 +392,8 @@ struct SortOut {
 };
 
 // Sort the n items of S by key (nkeys distinct keys at most); writes the
-// sorted values, and when asked the sorted keys and the permutation.
+// sorted values, and when asked the sorted keys and the permutation (values
+// packed with the input index).
 static int sort_source(Context* c, const KeySrc& S, int64_t n, uint64_t nkeys, bool want_keys,
                        bool want_perm, hipStream_t s, SortOut* out) {
   out->values = nullptr;
@@ -403,8 +438,8 @@ static int sort_source(Context* c, const KeySrc& S, int64_t n, uint64_t nkeys, b
   return 0;
 }
 
-uint32_t order_cell_count(const FieldTables& T) {
-  return (uint32_t)T.eg.numx * (uint32_t)T.eg.numy + (uint32_t)(T.slg.numr - 1);
+uint32_t order_cell_count(const FieldTables& T, int nsnap) {
+  return (uint32_t)(T.slg.numr - 1) * theta_bins(T.slg.numr, nsnap);
 }
 
 int order_points(Context* c, const FieldTables& T, const float* x, const float* y, const float* z,
@@ -413,17 +448,38 @@ int order_points(Context* c, const FieldTables& T, const float* x, const float* 
   if (perm_out) *perm_out = nullptr;
   if (sorted_out) *sorted_out = nullptr;
   if (points_out) *points_out = nullptr;
-  const uint64_t ncell = (uint64_t)T.eg.numx * T.eg.numy + (uint64_t)(T.slg.numr - 1);
+  const int ns = off_dev ? nsnap : 1;
+  const uint32_t tb = theta_bins(T.slg.numr, ns);
+  const uint64_t ncell = (uint64_t)(T.slg.numr - 1) * tb;
   if (ncell > 0xffffffffull) return set_error(c, BFE_ELIMIT, "order: too many table cells");
-  KeySrc S{x, y, z, nullptr, off_dev, off_dev ? nsnap : 1, (uint32_t)ncell, 2,
-           key_geom(T.eg, T.slg, c->h_xi.size() >= 2 ? c->h_xi.data() : nullptr)};
+  KeySrc S{x, y, z, nullptr, off_dev, ns, (uint32_t)ncell, 1,
+           key_geom(T.eg, T.slg, c->h_xi.size() >= 2 ? c->h_xi.data() : nullptr),
+           x, y, z, n};
+  S.g.tbins = tb;
   SortOut o;
-  int rc = sort_source(c, S, n, ncell * (uint64_t)(off_dev ? nsnap : 1), sorted_out != nullptr,
-                       perm_out != nullptr, s, &o);
+  int rc = sort_source(c, S, n, ncell * (uint64_t)ns, sorted_out != nullptr, perm_out != nullptr,
+                       s, &o);
   if (rc) return rc;
   if (perm_out) *perm_out = o.perm;
   if (sorted_out) *sorted_out = o.keys;
   if (points_out) *points_out = o.values;
+  return 0;
+}
+
+int order_points2(Context* c, const FieldTables& T, const float* x, const float* y,
+                  const float* z, int64_t n1, const float* x2, const float* y2, const float* z2,
+                  int64_t n2, hipStream_t s, const float4** points_out) {
+  *points_out = nullptr;
+  const uint32_t tb = theta_bins(T.slg.numr, 1);
+  const uint64_t ncell = (uint64_t)(T.slg.numr - 1) * tb;
+  if (ncell > 0xffffffffull) return set_error(c, BFE_ELIMIT, "order: too many table cells");
+  KeySrc S{x, y, z, nullptr, nullptr, 1, (uint32_t)ncell, 1,
+           key_geom(T.eg, T.slg, c->h_xi.size() >= 2 ? c->h_xi.data() : nullptr), x2, y2, z2, n1};
+  S.g.tbins = tb;
+  SortOut o;
+  int rc = sort_source(c, S, n1 + n2, ncell, false, false, s, &o);
+  if (rc) return rc;
+  *points_out = o.values;
   return 0;
 }
 
@@ -433,10 +489,13 @@ int order_particles(Context* c, int kind, const float* x, const float* y, const 
   *sorted_out = nullptr;
   const EOFGeom eg = eof_geom(c);
   const SLGeom sg = sl_geom(c);
-  const uint64_t ncell = kind == 0 ? (uint64_t)eg.numx * eg.numy : (uint64_t)(sg.numr - 1);
+  const int ns = off_dev ? nsnap : 1;
+  const uint32_t tb = kind == 0 ? 1u : theta_bins(sg.numr, ns);
+  const uint64_t ncell = kind == 0 ? (uint64_t)eg.numx * eg.numy : (uint64_t)(sg.numr - 1) * tb;
   if (ncell > 0xffffffffull) return set_error(c, BFE_ELIMIT, "order: too many table cells");
-  KeySrc S{x, y, z, m, off_dev, off_dev ? nsnap : 1, (uint32_t)ncell, kind,
-           key_geom(eg, sg, c->h_xi.size() >= 2 ? c->h_xi.data() : nullptr)};
+  KeySrc S{x, y, z, m, off_dev, ns, (uint32_t)ncell, kind,
+           key_geom(eg, sg, c->h_xi.size() >= 2 ? c->h_xi.data() : nullptr), x, y, z, n};
+  S.g.tbins = tb;
   SortOut o;
   int rc = sort_source(c, S, n, ncell * (uint64_t)(off_dev ? nsnap : 1), false, false, s, &o);
   if (rc) return rc;

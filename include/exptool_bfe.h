@@ -192,6 +192,26 @@ int bfe_accumulate_disc_halo(bfe_ctx* ctx, const float* dx, const float* dy, con
                              const float* dm, int64_t nd, const float* hx, const float* hy,
                              const float* hz, const float* hm, int64_t nh, uint32_t sl_flags,
                              double* cos_out, double* sin_out, double* coef_out, void* stream);
+
+/* replaces: Fields.total_coefficients followed by Fields.return_forces_cart
+ * at the same particles (potential.py:98-252, :445-497) -- the force on
+ * every particle of the snapshot from the expansion of that snapshot.  The
+ * coefficients are those of bfe_accumulate_disc_halo (acc_flags: BFE_NO_ODD,
+ * BFE_ACC_BF16), the field that of bfe_field_set (halofac, truncations,
+ * field_flags: BFE_NO_ODD, BFE_COMPAT_*), and fx, fy, fz, pot, comp (device
+ * float32, [nd + nh] and [8][nd + nh], disc particles first, each may be
+ * NULL) those of bfe_field_eval_cart at rotpos over the concatenated points
+ * (no concatenated copy is needed).  Each half folds its coefficients as
+ * soon as its accumulate ends, so the fold of one overlaps the accumulate
+ * of the other.  The frozen field stays set afterwards, as after
+ * bfe_field_set. */
+int bfe_forces_disc_halo(bfe_ctx* ctx, const float* dx, const float* dy, const float* dz,
+                         const float* dm, int64_t nd, const float* hx, const float* hy,
+                         const float* hz, const float* hm, int64_t nh, uint32_t acc_flags,
+                         double* cos_out, double* sin_out, double* coef_out, double halofac,
+                         int disk_m, int disk_n, int halo_l, int halo_n, uint32_t field_flags,
+                         double rotpos, float* fx, float* fy, float* fz, float* pot, float* comp,
+                         void* stream);
 /* Fields.prep_tables + set_field_parameters (potential.py:257-294, :365-378)
  * for nsnap frozen fields at once: eof_cos/eof_sin [nsnap][mmax+1][norder],
  * sl_coef [nsnap][(lmax+1)^2][nmax]; truncation and flags as bfe_field_set.

@@ -451,3 +451,28 @@ def test_run_time_mod_end_to_end(files, golden, tmp_path):
                            field_file_name=str(tmp_path / "field.dat"),
                            orbitfile=str(tmp_path / "omap2.txt"), **kw)
     assert relmax(again, arr) <= 1e-4
+
+
+def test_total_coefficients_with_forces(files, golden):
+    """Fields.total_coefficients(return_forces=True): the same coefficients,
+    and return_forces_cart at every particle used, from one device call."""
+    from exptool_amd.basis.potential import Fields
+    from exptool_amd.io import particle
+    args = (str(files / "OUT.run.00000"), str(files / "eof.cache"), str(files / "sl.cache"),
+            str(files / "sl.model"))
+    A = Fields(*args, compat=True)
+    A.prep_tables()
+    A.set_field_parameters(no_odd=False, halo_l=4, disk_m=3)
+    got = A.total_coefficients(return_forces=True)
+    B = Fields(*args, compat=True)
+    B.prep_tables()
+    B.total_coefficients()
+    B.set_field_parameters(no_odd=False, halo_l=4, disk_m=3)
+    assert np.array_equal(A.EOF.cos, B.EOF.cos) and np.array_equal(A.SL.expcoef, B.SL.expcoef)
+    d = particle.Input(args[0], comp="star").data
+    h = particle.Input(args[0], comp="dark").data
+    x, y, z = (np.concatenate([d[k], h[k]]) for k in ("x", "y", "z"))
+    ref = B.return_forces_cart(x, y, z)
+    assert len(got) == 8
+    for j in range(8):
+        assert np.array_equal(got[j], ref[j]), j

@@ -414,6 +414,42 @@ def test_accumulate_disc_halo_equals_separate_calls(big):
     assert block_rel_eof(c, rc) <= COEF_TOL
 
 
+def test_forces_disc_halo_equals_separate_calls(big):
+    """bfe_forces_disc_halo (coefficients, field and forces at the same
+    particles, each half evaluated in its accumulate order) equals
+    accumulate_disc_halo + set_field + field_forces bit for bit, for
+    cell-ordered and input-order sizes, empty halves, field options and the
+    eight components; the forces meet the oracle bound."""
+    s, slo, eo = big
+    (dx, dy, dz, dm), (hx, hy, hz, hm) = disc_halo(90000, 61)
+    D = f32(dx, dy, dz, dm)
+    H = f32(hx, hy, hz, hm)
+    cases = ((90000, 90000, {}), (90000, 5000, dict(no_odd=True, compat=True, rotpos=0.4)),
+             (3000, 90000, dict(halofac=0.5, disk_m=4, halo_l=3, halo_n=12)), (0, 7000, {}),
+             (7000, 0, {}))
+    for nd, nh, kw in cases:
+        d = [v[:nd] for v in D]
+        h = [v[:nh] for v in H]
+        got = s.forces_disc_halo(d, h, components=True, **kw)
+        rot = kw.pop("rotpos", 0.0)
+        c, sn, hc = s.accumulate_disc_halo(d, h, no_odd=kw.get("no_odd", False))
+        for a, b in ((got["cos"], c), (got["sin"], sn), (got["coef"], hc)):
+            assert np.array_equal(a.cpu().numpy(), b.cpu().numpy()), (nd, nh)
+        s.set_field(c, sn, hc, **kw)
+        ref = s.field_forces(np.concatenate([d[0], h[0]]), np.concatenate([d[1], h[1]]),
+                             np.concatenate([d[2], h[2]]), rotpos=rot, components=True)
+        for k, r in zip(("fx", "fy", "fz", "pot", "comp"), ref):
+            assert np.array_equal(got[k].cpu().numpy(), r.cpu().numpy()), (nd, nh, k)
+    got = s.forces_disc_halo(D, H)
+    x, y, z = (np.concatenate([a, b]) for a, b in zip(D[:3], H[:3]))
+    F = O.FieldSpec(eo, got["cos"].cpu().numpy(), got["sin"].cpu().numpy(), slo,
+                    got["coef"].cpu().numpy())
+    idx = np.random.default_rng(0).choice(x.size, 4000, replace=False)
+    ref = O.total_force(F, x[idx], y[idx], z[idx])
+    res = [got[k].cpu().numpy().astype(np.float64)[idx] for k in ("fx", "fy", "fz", "pot")]
+    assert np.max(_field_rel(res, ref)) <= FORCE_TOL
+
+
 def test_cell_ordered_accumulate_every_lane_shape(big):
     """Each lane shape of the cell-ordered kernels (slots per lane, radial
     orders per slot; shapes that do not fit the tables fall back to the
