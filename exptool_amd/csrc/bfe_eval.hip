@@ -337,7 +337,7 @@ __global__ __launch_bounds__(256) void field_eval_kernel(
   if (!live) return;
   double cv[8];
   if (staged)
-    field_eval_point(T, qx, qy, qz, cv, &cell, stage, imin);
+    field_eval_point<true>(T, qx, qy, qz, cv, &cell, stage, imin);
   else
     field_eval_point(T, qx, qy, qz, cv);
   if (fx) fx[i] = (float)(cv[0] + cv[1]);
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(256) void field_eval_batch_kernel(
   const int64_t i = (int64_t)__float_as_int(q.w);
   double cv[8];
   if (staged)
-    field_eval_point(T, q.x, q.y, q.z, cv, &cell, stage, imin);
+    field_eval_point<true>(T, q.x, q.y, q.z, cv, &cell, stage, imin);
   else
     field_eval_point(T, q.x, q.y, q.z, cv);
   if (fx) fx[i] = (float)(cv[0] + cv[1]);

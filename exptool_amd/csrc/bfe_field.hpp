@@ -51,8 +51,18 @@ __host__ __device__ inline int sl_npairs(int L) { return (L + 1) * (L + 2) / 2; 
 // cos/sin(m phi) do not depend on l, so the l sums are taken per m first
 // (sum_l P_l^m E, sum_l P_l^m D, sum_l dP_l^m E for cos and sin) and
 // combined with the factors once per m.
-template <bool DENS, bool COSL>
-__device__ __forceinline__ void sl_angular(const SLGeom& g, const float4* __restrict__ A,
+// LDS view of staged float4 data: reads through it are LDS instructions, not
+// flat ones (which would pass through the vector-memory address path).
+typedef float F4v __attribute__((ext_vector_type(4)));
+typedef const F4v __attribute__((address_space(3)))* LdsF4;
+__device__ __forceinline__ float4 ld4(LdsF4 p, int k) {
+  const F4v v = p[k];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 ld4(const float4* __restrict__ p, int k) { return p[k]; }
+
+template <bool DENS, bool COSL, typename PA>
+__device__ __forceinline__ void sl_angular(const SLGeom& g, PA A,
                                            const float4* __restrict__ Nd, int L, float x1f,
                                            float x2f, float ps, double x, double c1, double s1,
                                            bool dq, double* acc) {
@@ -90,7 +100,7 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, const float4* __rest
           clm1 = cl; slm1 = sl; cl = cn; sl = sn;
         }
       }
-      const float4 a = A[2 * p], b = A[2 * p + 1];
+      const float4 a = ld4(A, 2 * p), b = ld4(A, 2 * p + 1);
       const double ec = (double)fmaf(x2f, a.z, x1f * a.x);
       const double dc = (double)fmaf(ps, b.z, b.x);
       if (l == 0) {
@@ -160,11 +170,12 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, const float4* __rest
   acc[6] = den1;
 }
 
+
 // pcell: the radial cell of r when the caller has located it already.
-// Abase/ibase: where node i's force blocks are read, Abase + (i - ibase)
-// * 2 np float4 (default: the folded table itself; a block may stage its
-// nodes in LDS and pass that copy).
-template <bool DENS>
+// STAGED: node i's force blocks are read from the LDS copy Abase at
+// (i - ibase) * 2 np float4 (a block stages its nodes there); otherwise from
+// the folded table itself.
+template <bool DENS, bool STAGED = false>
 __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __restrict__ F,
                                                int L, double r, double costh, double c1,
                                                double s1, uint32_t flags, SLOut& o,
@@ -179,16 +190,23 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
   const double fac = d_xi_to_r(cell.x, g.cmap, g.scale) / dxi;
   const float ps = (float)((i == 0) ? ((flags & BFE_COMPAT_SL_EDGE) ? x2 : x2 - 1.0) : x2);
   const int np = sl_npairs(L);
-  const float4* __restrict__ A =
-      Abase ? Abase + (size_t)(i - ibase) * 2 * np : reinterpret_cast<const float4*>(F) + (size_t)i * 2 * np;
   const float4* __restrict__ Nd =
       reinterpret_cast<const float4*>(F + (size_t)g.numr * np * 8) + (size_t)i * np;
   const bool dq = (flags & BFE_COMPAT_SL_DENS) != 0;
   double acc[7];
-  if (flags & BFE_COMPAT_SL_COS_L)
-    sl_angular<DENS, true>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
-  else
-    sl_angular<DENS, false>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
+  if (STAGED) {
+    const LdsF4 A = (LdsF4)Abase + (size_t)(i - ibase) * 2 * np;
+    if (flags & BFE_COMPAT_SL_COS_L)
+      sl_angular<DENS, true>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
+    else
+      sl_angular<DENS, false>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
+  } else {
+    const float4* __restrict__ A = reinterpret_cast<const float4*>(F) + (size_t)i * 2 * np;
+    if (flags & BFE_COMPAT_SL_COS_L)
+      sl_angular<DENS, true>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
+    else
+      sl_angular<DENS, false>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
+  }
   o.pot0 = acc[0] * P0;
   o.potr = acc[1] * fac;
   o.den0 = acc[2] * D0;
@@ -281,7 +299,8 @@ struct FieldTables {
 };
 
 // pcell/slA/slbase: the point's SL cell (field_sl_cell) and SL node blocks
-// staged by the caller (see sl_eval_folded).
+// staged by the caller in LDS (STAGED, see sl_eval_folded).
+template <bool STAGED = false>
 __device__ __forceinline__ void field_eval_point(const FieldTables& T, double x, double y,
                                                  double z, double comp[8],
                                                  const SLCell* pcell = nullptr,
@@ -296,7 +315,8 @@ __device__ __forceinline__ void field_eval_point(const FieldTables& T, double x,
   EOFOut d;
   eof_eval_folded<false>(T.eg, T.eofF, r2, z, c1, s1, d);
   SLOut h;
-  sl_eval_folded<false>(T.slg, T.slF, T.lmax, r3, costh, c1, s1, T.flags, h, pcell, slA, slbase);
+  sl_eval_folded<false, STAGED>(T.slg, T.slF, T.lmax, r3, costh, c1, s1, T.flags, h, pcell, slA,
+                                slbase);
   double dfp = d.fp, hfp = h.potp;
   if (r3 < T.ximin) {
     hfp = 0.0;

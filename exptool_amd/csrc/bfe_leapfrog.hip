@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void l
       rot_of(a, dtd, k, T);
       double cv[8];
       if (staged)
-        field_eval_point(T, px, py, pz, cv, &cell, stage, imin);
+        field_eval_point<true>(T, px, py, pz, cv, &cell, stage, imin);
       else
         field_eval_point(T, px, py, pz, cv);
       const float nx = (float)(cv[0] + cv[1]), ny = (float)(cv[2] + cv[3]), nz = (float)(cv[4] + cv[5]);
@@ -236,7 +236,7 @@ __device__ __forceinline__ void rk4_force(const FieldTables& T, bool live, float
   if (live) {
     double cv[8];
     if (staged)
-      field_eval_point(T, px, py, pz, cv, &cell, stage, imin);
+      field_eval_point<true>(T, px, py, pz, cv, &cell, stage, imin);
     else
       field_eval_point(T, px, py, pz, cv);
     fx = (float)(cv[0] + cv[1]);
