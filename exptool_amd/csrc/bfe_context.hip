@@ -69,7 +69,7 @@ static void free_sl(Context* c) {
 }
 
 static void free_eof(Context* c) {
-  for (auto& p : c->eof64) { free_ptr(p); p = nullptr; }
+  for (auto& p : c->eof_fold32) { free_ptr(p); p = nullptr; }
   free_ptr(c->eof_pot32);
   free_ptr(c->eof_dep32);
   c->eof_pot32 = nullptr;
@@ -288,9 +288,10 @@ extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, in
   const size_t nodes = (size_t)(numx + 1) * (numy + 1);
   const double* srcC[4] = {potC, rforceC, zforceC, densC};
   const double* srcS[4] = {potS, rforceS, zforceS, densS};
-  // float64 masters are function-major [K][node] (coalesced folding);
-  // the float32 accumulate table is node-major [node][K] (coalesced gathers).
+  // fold inputs are function-major [K][node] (coalesced folding); the
+  // float32 accumulate table is node-major [node][K] (coalesced gathers).
   std::vector<double> buf(nodes * K);
+  std::vector<float> buf32(nodes * K);
   std::vector<float> pot32;
   for (int q = 0; q < (dens ? 4 : 3); ++q) {
     for (int m = 0; m <= mmax; ++m)
@@ -305,7 +306,8 @@ extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, in
         const int k = (mmax + m) * norder + n;
         memcpy(&buf[(size_t)k * nodes], s, nodes * sizeof(double));
       }
-    int rc = dev_upload(c, &c->eof64[q], buf.data(), buf.size());
+    for (size_t j = 0; j < buf.size(); ++j) buf32[j] = (float)buf[j];
+    int rc = dev_upload(c, &c->eof_fold32[q], buf32.data(), buf32.size());
     if (rc) return rc;
     if (q == 0) {
       pot32.resize(buf.size());

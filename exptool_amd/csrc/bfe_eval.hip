@@ -109,14 +109,14 @@ struct EOFFoldArgs {
 };
 
 // One thread per (row group q, node): q = m for cos m = 0..mmax, mmax + m
-// for sin m = 1..mmax; nodes fastest so the function-major float64 tables
-// are read coalesced, and q uniform per block so the coefficients are
+// for sin m = 1..mmax; nodes fastest so the function-major float32 tables
+// are read coalesced (sums in float64), and q uniform per block so the coefficients are
 // scalar loads.  Writes its half [pot, rf, zf, dens] of the 8-float block
 // [potC, rfC, zfC, dC, potS, rfS, zfS, dS]; cos m = 0 also zeroes the sin
 // half.  grid.x = (2 mmax + 1) * chunks, grid.y = coefficient sets.
 __global__ __launch_bounds__(256) void eof_fold_kernel(
-    const double* __restrict__ T0, const double* __restrict__ T1, const double* __restrict__ T2,
-    const double* __restrict__ T3, const double* __restrict__ ccos, const double* __restrict__ csin,
+    const float* __restrict__ T0, const float* __restrict__ T1, const float* __restrict__ T2,
+    const float* __restrict__ T3, const double* __restrict__ ccos, const double* __restrict__ csin,
     int mmax, int norder, int64_t nodes, int chunks, EOFFoldArgs a, float* __restrict__ out,
     int64_t coef_stride, int64_t out_stride) {
   const int q = (int)(blockIdx.x / chunks);
@@ -127,10 +127,10 @@ __global__ __launch_bounds__(256) void eof_fold_kernel(
   const double* cf = (sn ? csin : ccos) + (size_t)blockIdx.y * coef_stride + (size_t)m * norder;
   const double mul = a.mmul[m];
   const size_t row = (size_t)q * norder * nodes + node;
-  const double* t0 = T0 + row;
-  const double* t1 = T1 + row;
-  const double* t2 = T2 + row;
-  const double* t3 = T3 + row;
+  const float* t0 = T0 + row;
+  const float* t1 = T1 + row;
+  const float* t2 = T2 + row;
+  const float* t3 = T3 + row;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   if (a.dens) {
 #pragma unroll 6
@@ -228,8 +228,8 @@ static int eof_fold_into(Context* c, const double* ccos, const double* csin, int
   a.dens = dens ? 1 : 0;
   const int chunks = (int)((nodes + 255) / 256);
   hipLaunchKernelGGL(eof_fold_kernel, dim3((unsigned)((2 * c->mmax + 1) * chunks), nset), dim3(256),
-                     0, s, c->eof64[0], c->eof64[1], c->eof64[2],
-                     dens ? c->eof64[3] : c->eof64[0], ccos, csin, c->mmax, c->norder, nodes,
+                     0, s, c->eof_fold32[0], c->eof_fold32[1], c->eof_fold32[2],
+                     dens ? c->eof_fold32[3] : c->eof_fold32[0], ccos, csin, c->mmax, c->norder, nodes,
                      chunks, a, out,
                      (int64_t)(c->mmax + 1) * c->norder, (int64_t)eof_fold_floats(c));
   BFE_HIP(c, hipGetLastError());

@@ -97,7 +97,9 @@ struct Context {
   int mmax = 0, norder = 0, numx = 0, numy = 0, ecmap = 0;
   double ascale = 1.0, hscale = 1.0, xmin = 0, dx = 1, ymin = 0, dy = 1;
   int eof_K = 0;               // (2*mmax+1)*norder rows of the node-major tables
-  double* eof64[4] = {nullptr, nullptr, nullptr, nullptr};  // pot, rf, zf, dens: [K][node]
+  // fold inputs pot, rf, zf, dens: [K][node], float32 (the folded field is
+  // float32 too; the fold sums in float64)
+  float* eof_fold32[4] = {nullptr, nullptr, nullptr, nullptr};
   float* eof_pot32 = nullptr;  // [node][K] potential for accumulate
   float* eof_dep32 = nullptr;  // [node][2 mmax + 1][dep_np(norder)] the same, rows zero-padded
 
