@@ -146,18 +146,23 @@ __global__ __launch_bounds__(256) void eof_accumulate_kernel(
   }
 }
 
-// Column sums of the segment partials [rows][K] in a fixed order.  A block
-// covers 8 adjacent columns with 32 row lanes (each row read is 64
-// contiguous bytes): row lane j Kahan-sums rows j, j + 32, ... of its
-// column, then a fixed tree adds the 32 lanes.  The order depends only on
-// the row count, so results are bitwise reproducible.  With rowb, grid.y runs
-// over snapshots: snapshot q sums rows [rowb[q], rowb[q+1]) into out + q * K.
-constexpr int kRedCols = 8;
+// Fixed-order float64 reduction of the per-segment partials, column k
+// (a column's rows are K doubles apart): row lane j Kahan-sums rows j,
+// j + kRedLanes, ... of its column, then a fixed tree adds the lanes.  The
+// order depends only on the row count, so results are bitwise reproducible.
+// With rowb, grid.y runs over snapshots: snapshot q sums rows
+// [rowb[q], rowb[q+1]) into out + q * K.  SPLIT (the EOF sums): column k <
+// nc = (mmax+1) norder goes to cos_out, the others to sin_out from row m = 1
+// on, and the m = 0 sin row is zeroed, each set q at q * nc.
+constexpr int kRedCols = 8;    // a row read is 64 contiguous bytes
 constexpr int kRedLanes = 32;
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __restrict__ partials,
                                                               int nb, int K,
                                                               double* __restrict__ out,
-                                                              const int64_t* __restrict__ rowb) {
+                                                              const int64_t* __restrict__ rowb,
+                                                              double* __restrict__ sin_out,
+                                                              int nc, int norder) {
   __shared__ double sh[kRedLanes][kRedCols];
   const int c = threadIdx.x % kRedCols;
   const int j = threadIdx.x / kRedCols;
@@ -166,7 +171,6 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __re
   if (rowb) {
     r0 = rowb[blockIdx.y];
     r1 = rowb[blockIdx.y + 1];
-    out += (size_t)blockIdx.y * K;
   }
   double s = 0.0, comp = 0.0;
   if (k < K) {
@@ -184,27 +188,25 @@ __global__ __launch_bounds__(256) void reduce_partials_kernel(const double* __re
     if (j < w) sh[j][c] += sh[j + w][c];
     __syncthreads();
   }
-  if (j == 0 && k < K) out[k] = sh[0][c];
+  if (j == 0 && k < K) {
+    const double v = sh[0][c];
+    if (!SPLIT) {
+      out[(size_t)blockIdx.y * K + k] = v;
+    } else {
+      const size_t q = (size_t)blockIdx.y * nc;
+      if (k < nc) {
+        out[q + k] = v;
+        if (k < norder) sin_out[q + k] = 0.0;
+      } else {
+        sin_out[q + (k - nc) + norder] = v;
+      }
+    }
+  }
 }
 
 __global__ void fill_f64_kernel(double* __restrict__ a, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = 0.0;
-}
-
-// Scatter the K-ordered EOF sums into (mmax+1, norder) cos and sin arrays.
-__global__ void eof_split_kernel(const double* __restrict__ sums, int mmax, int norder,
-                                 double* __restrict__ cos_out, double* __restrict__ sin_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nc = (mmax + 1) * norder;
-  const int K = (2 * mmax + 1) * norder;
-  sums += (size_t)blockIdx.y * K;
-  cos_out += (size_t)blockIdx.y * nc;
-  sin_out += (size_t)blockIdx.y * nc;
-  if (i < nc) {
-    cos_out[i] = sums[i];
-    sin_out[i] = (i < norder) ? 0.0 : sums[nc + i - norder];
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1093,19 +1095,17 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
   AccPlan P = off ? plan_batch(c->eof_acc_blocks, c->eof_seg_tiles, off, nsnap, kTile)
                   : plan_single(c->eof_acc_blocks, n, kTile);
   const size_t pbytes = (size_t)P.S.nseg * K * sizeof(double);
-  const size_t sums_off = (pbytes + 255) & ~(size_t)255;
-  const size_t need = sums_off + (size_t)nsnap * K * sizeof(double);
+  const size_t need = pbytes;
   const int64_t* rowb = nullptr;
   const int64_t* off_dev = nullptr;
   int rc;
   if (off) {
-    rc = upload_plan(c, P, need, off, nsnap, s, &rowb, &off_dev);   // partials | sums | plan
+    rc = upload_plan(c, P, need, off, nsnap, s, &rowb, &off_dev);   // partials | plan
   } else {
     rc = ensure_scratch(c, need);
   }
   if (rc) return rc;
   double* partials = (double*)c->scratch;
-  double* sums = (double*)((char*)c->scratch + sums_off);
   EOFGeom g = eof_geom(c);
   const int opt = (K + 255) / 256;
   const float4* sorted = nullptr;
@@ -1139,11 +1139,9 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
   }
   timer_end(c, "eof_accumulate", s);
   BFE_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3((K + kRedCols - 1) / kRedCols, nsnap), dim3(256), 0, s, partials,
-                     (int)P.S.nseg, K, sums, rowb);
   const int nc = (c->mmax + 1) * c->norder;
-  hipLaunchKernelGGL(eof_split_kernel, dim3((nc + 255) / 256, nsnap), dim3(256), 0, s, sums, c->mmax,
-                     c->norder, cos_out, sin_out);
+  hipLaunchKernelGGL(reduce_partials_kernel<true>, dim3((K + kRedCols - 1) / kRedCols, nsnap), dim3(256), 0,
+                     s, partials, (int)P.S.nseg, K, cos_out, rowb, sin_out, nc, c->norder);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }
@@ -1188,8 +1186,8 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
       hipLaunchKernelGGL(sl_mfma_kernel<2>, dim3(P.grid), dim3(256), mlds, s, x, y, z, m, P.S, g, c->sl_e32, c->sl_lmc, no_odd, partials);
     timer_end(c, "sl_accumulate", s);
     BFE_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3((K + kRedCols - 1) / kRedCols, nsnap), dim3(256), 0, s, partials,
-                       (int)P.S.nseg, K, coef_out, rowb);
+    hipLaunchKernelGGL(reduce_partials_kernel<false>, dim3((K + kRedCols - 1) / kRedCols, nsnap), dim3(256), 0,
+                       s, partials, (int)P.S.nseg, K, coef_out, rowb, nullptr, 0, 0);
     BFE_HIP(c, hipGetLastError());
     return 0;
   }
@@ -1224,8 +1222,8 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
   }
   timer_end(c, "sl_accumulate", s);
   BFE_HIP(c, hipGetLastError());
-  hipLaunchKernelGGL(reduce_partials_kernel, dim3((K + kRedCols - 1) / kRedCols, nsnap), dim3(256), 0, s, partials,
-                     (int)P.S.nseg, K, coef_out, rowb);
+  hipLaunchKernelGGL(reduce_partials_kernel<false>, dim3((K + kRedCols - 1) / kRedCols, nsnap), dim3(256), 0,
+                     s, partials, (int)P.S.nseg, K, coef_out, rowb, nullptr, 0, 0);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }
