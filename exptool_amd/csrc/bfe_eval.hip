@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void eof_eval_kernel(
 __global__ __launch_bounds__(256) void field_eval_kernel(
     const float* __restrict__ px, const float* __restrict__ py, const float* __restrict__ pz,
     int64_t n, const float4* __restrict__ pts, FieldTables T, float* fx, float* fy, float* fz,
-    float* pot, float* comp) {
+    float* pot, float* comp, float4* __restrict__ quad) {
   __shared__ float4 stage[kStageFloat4];
   __shared__ int smin[4], smax[4];
   const int64_t nb = (n + 255) / 256;
@@ -340,6 +340,12 @@ __global__ __launch_bounds__(256) void field_eval_kernel(
     field_eval_point<true>(T, qx, qy, qz, cv, &cell, stage, imin);
   else
     field_eval_point(T, qx, qy, qz, cv);
+  if (quad) {
+    // one 16-byte store per point; quad_split_kernel makes the four arrays
+    quad[i] = make_float4((float)(cv[0] + cv[1]), (float)(cv[2] + cv[3]), (float)(cv[4] + cv[5]),
+                          (float)(cv[6] + cv[7]));
+    return;
+  }
   if (fx) fx[i] = (float)(cv[0] + cv[1]);
   if (fy) fy[i] = (float)(cv[2] + cv[3]);
   if (fz) fz[i] = (float)(cv[4] + cv[5]);
@@ -525,16 +531,46 @@ static int field_set_run(Context* c, const double* eof_cos, const double* eof_si
 }
 
 // Evaluate n points: sorted (pts, the index in .w) or in input order.
+// (fx, fy, fz, pot) rows of the float4 quad array, coalesced both ways.
+__global__ __launch_bounds__(256) void quad_split_kernel(const float4* __restrict__ quad, int64_t n,
+                                                         float* fx, float* fy, float* fz,
+                                                         float* pot) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const float4 v = quad[i];
+    if (fx) fx[i] = v.x;
+    if (fy) fy[i] = v.y;
+    if (fz) fz[i] = v.z;
+    if (pot) pot[i] = v.w;
+  }
+}
+
+// Evaluate n points: sorted (pts, the index in .w) or in input order.  In
+// sorted order the outputs land at scattered indices; without comp, each
+// point writes one float4 to scratch (one 16-byte store instead of four
+// 4-byte ones, a quarter of the partial-line HBM writes) and a coalesced
+// pass splits it into fx, fy, fz, pot.
 static int field_eval_run(Context* c, const FieldTables& T, const float* x, const float* y,
                           const float* z, int64_t n, const float4* pts, float* fx, float* fy,
                           float* fz, float* pot, float* comp, hipStream_t s) {
   if (n == 0) return 0;
+  float4* quad = nullptr;
+  if (pts && !comp && (fx || fy || fz || pot)) {
+    int rc = ensure_scratch(c, (size_t)n * sizeof(float4));
+    if (rc) return rc;
+    quad = (float4*)c->scratch;
+  }
   const unsigned grid = pts ? xcd_grid(nblocks256(n)) : nblocks256(n);
   timer_begin(c, "field_eval", s);
   hipLaunchKernelGGL(field_eval_kernel, dim3(grid), dim3(256), 0, s, x, y, z, n, pts, T, fx, fy,
-                     fz, pot, comp);
+                     fz, pot, comp, quad);
   timer_end(c, "field_eval", s);
   BFE_HIP(c, hipGetLastError());
+  if (quad) {
+    hipLaunchKernelGGL(quad_split_kernel, dim3(nblocks256(n)), dim3(256), 0, s, quad, n, fx, fy,
+                       fz, pot);
+    BFE_HIP(c, hipGetLastError());
+  }
   return 0;
 }
 
