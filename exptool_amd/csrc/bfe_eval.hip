@@ -363,7 +363,7 @@ __global__ __launch_bounds__(256) void field_eval_batch_kernel(
     int64_t n, const float4* __restrict__ pts, const uint32_t* __restrict__ sorted, uint32_t ncell,
     FieldTables T, const float* __restrict__ slB, int64_t sl_stride,
     const float* __restrict__ eofB, int64_t eof_stride, float* fx, float* fy, float* fz,
-    float* pot) {
+    float* pot, float4* __restrict__ quad) {
   __shared__ float4 stage[kStageFloat4];
   __shared__ int smin[4], smax[4];
   const int64_t nb = (n + 255) / 256;
@@ -396,10 +396,8 @@ __global__ __launch_bounds__(256) void field_eval_batch_kernel(
     field_eval_point<true>(T, q.x, q.y, q.z, cv, &cell, stage, imin);
   else
     field_eval_point(T, q.x, q.y, q.z, cv);
-  if (fx) fx[i] = (float)(cv[0] + cv[1]);
-  if (fy) fy[i] = (float)(cv[2] + cv[3]);
-  if (fz) fz[i] = (float)(cv[4] + cv[5]);
-  if (pot) pot[i] = (float)(cv[6] + cv[7]);
+  quad[i] = make_float4((float)(cv[0] + cv[1]), (float)(cv[2] + cv[3]), (float)(cv[4] + cv[5]),
+                        (float)(cv[6] + cv[7]));
 }
 
 static int ensure_batch(Context* c, float** d, size_t* have, size_t floats) {
@@ -745,9 +743,16 @@ extern "C" int bfe_field_eval_batch(bfe_ctx* h, const float* x, const float* y, 
   const float4* pts = nullptr;
   int rc = order_points(c, T, x, y, z, n, s, nullptr, c->batch_off, nsnap, &sorted, &pts);
   if (rc) return rc;
+  // results go through a float4 per point, split coalesced (as field_eval_run)
+  rc = ensure_scratch(c, (size_t)n * sizeof(float4));
+  if (rc) return rc;
+  float4* quad = (float4*)c->scratch;
   hipLaunchKernelGGL(field_eval_batch_kernel, dim3(xcd_grid(nblocks256(n))), dim3(256), 0, s, n,
                      pts, sorted, order_cell_count(T, nsnap), T, c->batch_sl, (int64_t)slf, c->batch_eof,
-                     (int64_t)eoff, fx, fy, fz, pot);
+                     (int64_t)eoff, fx, fy, fz, pot, quad);
+  BFE_HIP(c, hipGetLastError());
+  hipLaunchKernelGGL(quad_split_kernel, dim3(nblocks256(n)), dim3(256), 0, s, quad, n, fx, fy, fz,
+                     pot);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }
