@@ -610,10 +610,12 @@ extern "C" int bfe_field_eval_cart(bfe_ctx* h, const float* x, const float* y, c
 // Fields.total_coefficients followed by Fields.return_forces_cart at the
 // same particles (potential.py:98-252, :445-497).  Two concurrent branches,
 // each folding its coefficients as soon as they are reduced:
-//   caller stream  disc EOF accumulate, then the EOF fold;
+//   caller stream  disc EOF accumulate, the EOF fold, then the evaluation
+//                  order of all nd + nh points taken as one set (it needs
+//                  only positions and table geometry; the disc branch is
+//                  the shorter one, so the sort fills its idle tail);
 //   internal       halo SL accumulate (second buffer set), then the SL fold;
-// then, after the join, the evaluation order of all nd + nh points taken
-// as one set (no concatenated copy) and the evaluation.  Results equal
+// then, after the join, the evaluation.  Results equal
 // bfe_accumulate_disc_halo + bfe_field_set + bfe_field_eval_cart over the
 // concatenated points.
 extern "C" int bfe_forces_disc_halo(bfe_ctx* h, const float* dx, const float* dy,
@@ -652,6 +654,14 @@ extern "C" int bfe_forces_disc_halo(bfe_ctx* h, const float* dx, const float* dy
   int r0 = eof_accumulate_one(c, dx, dy, dz, dm, nd, cos_out, sin_out, s);
   if (!r0) r0 = eof_fold(c, 1, cos_out, sin_out, mm, nn, 0, 1 << 30, no_odd,
                          (field_flags & BFE_COMPAT_EOF_MASK) != 0, false, s);
+  // the two inputs are evaluated as one set, so they are ordered at every size
+  const float4* pts = nullptr;
+  if (!r0 && n > 0) {
+    FieldTables G;
+    G.slg = sl_geom(c);
+    G.eg = eof_geom(c);
+    r0 = order_points2(c, G, dx, dy, dz, nd, hx, hy, hz, nh, s, &pts);
+  }
   rc = fork_end(c, s);
   if (r0 || r1 || rc) return r0 ? r0 : (r1 ? r1 : rc);
   c->field_halofac = halofac;
@@ -659,10 +669,6 @@ extern "C" int bfe_forces_disc_halo(bfe_ctx* h, const float* dx, const float* dy
   FieldTables T;
   rc = field_tables(c, rotpos, T);
   if (rc || n == 0) return rc;
-  // the two inputs are evaluated as one set, so they are ordered at every size
-  const float4* pts = nullptr;
-  rc = order_points2(c, T, dx, dy, dz, nd, hx, hy, hz, nh, s, &pts);
-  if (rc) return rc;
   return field_eval_run(c, T, nullptr, nullptr, nullptr, n, pts, fx, fy, fz, pot, comp, s);
 }
 
