@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void eof_accumulate_kernel(
       const double R = sqrt(x * x + y * y + 1.e-10);
       double c1, s1;
       planar_angle(x, y, c1, s1);
-      EOFCell cell = eof_locate(R, z, g.cmap, g.ascale, g.hscale, g.xmin, g.dx, g.ymin, g.dy,
+      EOFCell cell = eof_locate(R, z, g.cmap, g.ascale, g.ihscale, g.xmin, g.idx, g.ymin, g.idy,
                                 g.numx, g.numy);
       s_base[tid] = cell.ix * ny1 + cell.iy;
       s_c[0][tid] = (float)cell.c00;
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void sl_accumulate_kernel(
       double c1, s1;
       planar_angle(x, y, c1, s1);
       if (h == 0) {
-        SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
+        SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr, g.idxi);
         const double P0 = cell.x1 * g.p0[cell.i] + cell.x2 * g.p0[cell.i + 1];
         s_i[qp] = cell.i;
         s_w1[qp] = live ? (float)(cell.x1 * P0) : 0.f;
@@ -484,8 +484,8 @@ __global__ __launch_bounds__(256) void eof_deposit_kernel(
         const double R = sqrt(x * x + y * y + 1.e-10);
         double c1, s1v;
         planar_angle(x, y, c1, s1v);
-        const EOFCell cell = eof_locate(R, z, g.cmap, g.ascale, g.hscale, g.xmin, g.dx, g.ymin,
-                                        g.dy, g.numx, g.numy);
+        const EOFCell cell = eof_locate(R, z, g.cmap, g.ascale, g.ihscale, g.xmin, g.idx, g.ymin,
+                                        g.idy, g.numx, g.numy);
         base = cell.ix * ny1 + cell.iy;
         c00 = (float)cell.c00; c10 = (float)cell.c10; c01 = (float)cell.c01; c11 = (float)cell.c11;
         const double w = -kFourPi * mass;
@@ -679,7 +679,7 @@ __global__ __launch_bounds__(256) void sl_deposit_kernel(
         const double costh = z / r;
         double c1, s1v;
         planar_angle(x, y, c1, s1v);
-        const SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
+        const SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr, g.idxi);
         const double P0 = cell.x1 * g.p0[cell.i] + cell.x2 * g.p0[cell.i + 1];
         ci = cell.i;
         w1 = live ? (float)(cell.x1 * P0) : 0.f;
@@ -871,7 +871,7 @@ __global__ __launch_bounds__(256) void sl_mfma_kernel(
         double c1, s1v;
         planar_angle(x, y, c1, s1v);
         if (h == 0) {
-          SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
+          SLCell cell = sl_locate(r, g.cmap, g.scale, g.xi, g.numr, g.idxi);
           const double P0 = cell.x1 * g.p0[cell.i] + cell.x2 * g.p0[cell.i + 1];
           s_i[qp] = cell.i;
           s_w1[qp] = live ? (float)(cell.x1 * P0) : 0.f;

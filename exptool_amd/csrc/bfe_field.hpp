@@ -181,13 +181,13 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
                                                double s1, uint32_t flags, SLOut& o,
                                                const SLCell* pcell = nullptr,
                                                const float4* Abase = nullptr, int ibase = 0) {
-  const SLCell cell = pcell ? *pcell : sl_locate(r, g.cmap, g.scale, g.xi, g.numr);
+  const SLCell cell = pcell ? *pcell : sl_locate(r, g.cmap, g.scale, g.xi, g.numr, g.idxi);
   const int i = cell.i;
   const double x1 = cell.x1, x2 = cell.x2;
   const double P0 = x1 * g.p0[i] + x2 * g.p0[i + 1];
   const double D0 = DENS ? x1 * g.d0[i] + x2 * g.d0[i + 1] : 0.0;
-  const double dxi = g.xi[1] - g.xi[0];
-  const double fac = d_xi_to_r(cell.x, g.cmap, g.scale) / dxi;
+  const double fac = (g.cmap == 1) ? (1.0 - cell.x) * (1.0 - cell.x) * g.dfac
+                                   : d_xi_to_r(cell.x, g.cmap, g.scale) * g.idxi;
   const float ps = (float)((i == 0) ? ((flags & BFE_COMPAT_SL_EDGE) ? x2 : x2 - 1.0) : x2);
   const int np = sl_npairs(L);
   const float4* __restrict__ Nd =
@@ -224,7 +224,7 @@ template <bool DENS>
 __device__ __forceinline__ void eof_eval_folded(const EOFGeom& g, const float* __restrict__ F,
                                                 double R, double z, double c1, double s1,
                                                 EOFOut& o) {
-  const EOFCell cell = eof_locate(R, z, g.cmap, g.ascale, g.hscale, g.xmin, g.dx, g.ymin, g.dy,
+  const EOFCell cell = eof_locate(R, z, g.cmap, g.ascale, g.ihscale, g.xmin, g.idx, g.ymin, g.idy,
                                   g.numx, g.numy);
   const int ny1 = g.numy + 1;
   const int M = g.mmax;
@@ -282,7 +282,7 @@ __device__ __forceinline__ void eof_eval_folded(const EOFGeom& g, const float* _
 __device__ __forceinline__ SLCell field_sl_cell(const SLGeom& g, double x, double y, double z) {
   const double r2 = sqrt(x * x + y * y) + 1.e-15;
   const double r3 = sqrt(r2 * r2 + z * z) + 1.e-15;
-  return sl_locate(r3, g.cmap, g.scale, g.xi, g.numr);
+  return sl_locate(r3, g.cmap, g.scale, g.xi, g.numr, g.idxi);
 }
 
 // Frozen disc + halo field at a Cartesian point, Fields.return_forces_cart
@@ -307,7 +307,9 @@ __device__ __forceinline__ void field_eval_point(const FieldTables& T, double x,
                                                  const float4* slA = nullptr, int slbase = 0) {
   const double r2 = sqrt(x * x + y * y) + 1.e-15;
   const double r3 = sqrt(r2 * r2 + z * z) + 1.e-15;
-  const double costh = z / r3;
+  // reciprocals once; the force combination below multiplies by them
+  const double ir2 = 1.0 / r2, ir3 = 1.0 / r3;
+  const double costh = z * ir3;
   double c0, s0;
   planar_angle(x, y, c0, s0);
   const double c1 = c0 * T.crot - s0 * T.srot;   // cos(phi + rotpos)
@@ -324,13 +326,13 @@ __device__ __forceinline__ void field_eval_point(const FieldTables& T, double x,
   }
   const double dfr = d.fr, dfz = d.fz, dp = d.p + d.p0;
   const double hfr = h.potr, hft = h.pott, hp = h.pot1 + h.pot0;
-  const double r2sq = r2 * r2, r3cu = r3 * r3 * r3;
-  comp[0] = dfr * (x / r2) - dfp * (y / r2sq);
-  comp[1] = -1.0 * (hfr * (x / r3) - hft * (x * z / r3cu)) + hfp * (y / r2sq);
-  comp[2] = dfr * (y / r2) + dfp * (x / r2sq);
-  comp[3] = -1.0 * (hfr * (y / r3) - hft * (y * z / r3cu)) - hfp * (x / r2sq);
+  const double r2sq = r2 * r2, ir2sq = ir2 * ir2, ir3cu = ir3 * ir3 * ir3;
+  comp[0] = dfr * (x * ir2) - dfp * (y * ir2sq);
+  comp[1] = -1.0 * (hfr * (x * ir3) - hft * (x * z * ir3cu)) + hfp * (y * ir2sq);
+  comp[2] = dfr * (y * ir2) + dfp * (x * ir2sq);
+  comp[3] = -1.0 * (hfr * (y * ir3) - hft * (y * z * ir3cu)) - hfp * (x * ir2sq);
   comp[4] = dfz;
-  comp[5] = -1.0 * (hfr * (z / r3) + hft * (r2sq / r3cu));
+  comp[5] = -1.0 * (hfr * (z * ir3) + hft * (r2sq * ir3cu));
   comp[6] = dp;
   comp[7] = hp;
 }

@@ -162,20 +162,24 @@ struct SLGeom {
   int numr, cmap, lmax, nmax;
   double scale;
   const double* lmc;   // Context::sl_lmc
+  double idxi;         // 1 / (xi[1] - xi[0])
+  double dfac;         // cmap 1: 0.5 / (scale (xi[1] - xi[0])), d xi/d r over the step
 };
 
 struct EOFGeom {
   int cmap, numx, numy, mmax, norder;
   double ascale, hscale, xmin, dx, ymin, dy;
+  double ihscale, idx, idy;   // reciprocals of hscale, dx, dy
 };
 
 inline SLGeom sl_geom(const Context* c) {
+  const double dxi = c->h_xi.size() >= 2 ? c->h_xi[1] - c->h_xi[0] : 1.0;
   return SLGeom{c->sl_xi, c->sl_p0, c->sl_d0, c->numr, c->cmap, c->lmax, c->nmax, c->scale,
-                c->sl_lmc};
+                c->sl_lmc, 1.0 / dxi, 0.5 / (c->scale * dxi)};
 }
 inline EOFGeom eof_geom(const Context* c) {
-  return EOFGeom{c->ecmap, c->numx, c->numy, c->mmax, c->norder,
-                 c->ascale, c->hscale, c->xmin, c->dx, c->ymin, c->dy};
+  return EOFGeom{c->ecmap, c->numx, c->numy, c->mmax, c->norder, c->ascale, c->hscale,
+                 c->xmin, c->dx, c->ymin, c->dy, 1.0 / c->hscale, 1.0 / c->dx, 1.0 / c->dy};
 }
 
 struct FieldTables;

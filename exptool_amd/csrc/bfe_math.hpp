@@ -22,7 +22,7 @@ constexpr double kFourPi = 4.0 * kPi;
 // compatibility.py:16-47 (scalar): negative r maps to 0.
 BFE_HD double r_to_xi(double r, int cmap, double scale) {
   if (r < 0.0) return 0.0;
-  if (cmap == 1) return (r / scale - 1.0) / (r / scale + 1.0);
+  if (cmap == 1) return (r - scale) / (r + scale);   // (r/a - 1)/(r/a + 1), one division
   if (cmap == 2) return log(r);
   return r;
 }
@@ -35,8 +35,9 @@ BFE_HD double d_xi_to_r(double xi, int cmap, double scale) {
 }
 
 // compatibility.py:83-91 (eps 1e-8, the Python map that runs).
-BFE_HD double z_to_y(double z, double hscale) {
-  return (z / (fabs(z) + 1.e-8)) * asinh(fabs(z / hscale));
+// ihscale = 1 / hscale.
+BFE_HD double z_to_y(double z, double ihscale) {
+  return (z / (fabs(z) + 1.e-8)) * asinh(fabs(z * ihscale));
 }
 
 // Planar angle of (x, y) as cos/sin, with atan2 semantics: the zero vector
@@ -44,8 +45,9 @@ BFE_HD double z_to_y(double z, double hscale) {
 BFE_HD void planar_angle(double x, double y, double& c, double& s) {
   double rho = sqrt(x * x + y * y);
   if (rho > 0.0) {
-    c = x / rho;
-    s = y / rho;
+    const double irho = 1.0 / rho;
+    c = x * irho;
+    s = y * irho;
   } else {
     double ph = atan2(y, x);
     c = cos(ph);
@@ -133,7 +135,9 @@ struct SLCell {
   double x1, x2;
 };
 
-BFE_HD SLCell sl_locate(double r, int cmap, double scale, const double* xi, int numr) {
+// idxi = 1 / (xi[1] - xi[0]).
+BFE_HD SLCell sl_locate(double r, int cmap, double scale, const double* xi, int numr,
+                        double idxi) {
   SLCell c;
   double x = r_to_xi(r, cmap, scale);
   if (cmap == 1) {
@@ -144,8 +148,7 @@ BFE_HD SLCell sl_locate(double r, int cmap, double scale, const double* xi, int 
     if (!(x >= xi[0])) x = xi[0];
     if (x > xi[numr - 1]) x = xi[numr - 1];
   }
-  const double dxi = xi[1] - xi[0];
-  double f = floor((x - xi[0]) / dxi);
+  double f = floor((x - xi[0]) * idxi);
   int i;
   if (!(f == f)) i = 0;
   else if (f < 0.0) i = 0;
@@ -153,8 +156,8 @@ BFE_HD SLCell sl_locate(double r, int cmap, double scale, const double* xi, int 
   else i = (int)f;
   c.x = x;
   c.i = i;
-  c.x1 = (xi[i + 1] - x) / dxi;
-  c.x2 = (x - xi[i]) / dxi;
+  c.x1 = (xi[i + 1] - x) * idxi;
+  c.x2 = (x - xi[i]) * idxi;
   return c;
 }
 
@@ -166,11 +169,12 @@ struct EOFCell {
   double c00, c10, c01, c11;
 };
 
-BFE_HD EOFCell eof_locate(double R, double z, int cmap, double ascale, double hscale, double xmin,
-                          double dx, double ymin, double dy, int numx, int numy) {
+// ihscale, idx, idy: reciprocals of hscale and of the grid steps dx, dy.
+BFE_HD EOFCell eof_locate(double R, double z, int cmap, double ascale, double ihscale,
+                          double xmin, double idx, double ymin, double idy, int numx, int numy) {
   EOFCell c;
-  double X = (r_to_xi(R, cmap, ascale) - xmin) / dx;
-  double Y = (z_to_y(z, hscale) - ymin) / dy;
+  double X = (r_to_xi(R, cmap, ascale) - xmin) * idx;
+  double Y = (z_to_y(z, ihscale) - ymin) * idy;
   if (!(X >= 0.0)) X = (X == X) ? 0.0 : X;
   if (!(Y >= 0.0)) Y = (Y == Y) ? 0.0 : Y;
   double fx = X < (double)(numx - 1) ? trunc(X) : (double)(numx - 1);
