@@ -230,6 +230,8 @@ extern "C" int bfe_sl_tables_set(bfe_ctx* h, int lmax, int nmax, int numr, int c
       long double nlm = sqrtl(n2) * (m > 0 ? sqrtl(2.0L) : 1.0L);
       const int lm = lm_index(l, m);
       lmc[kLmcY + lm] = (double)(-4.0L * 3.14159265358979323846264338327950288L * nlm);
+      lmc[kLmcL + lm] = (double)l;
+      lmc[kLmcS + lm] = (double)(l + m);
       if (l > m) {
         lmc[kLmcA + lm] = (double)(2 * l - 1) / (double)(l - m);
         lmc[kLmcB + lm] = (double)(l + m - 1) / (double)(l - m);

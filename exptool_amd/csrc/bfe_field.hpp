@@ -87,11 +87,13 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, PA A,
     double plm1 = 0.0, pl = pmm;
     double cl = cmm, sl = smm, clm1 = cmm1, slm1 = smm1;
     double Pc = 0, Ps = 0, Rc = 0, Rs = 0, Tc = 0, Ts = 0, Nc = 0, Ns = 0;
-#pragma unroll 1
+    // unrolled by 2: the P_l recurrence's register rotation disappears
+    // (150 against 155 us per 2M points; fully unrolled was far slower)
+#pragma unroll 2
     for (int l = m; l <= L; ++l, ++p) {
+      const int lm = lm_index(l, m);
       if (l > m) {
         // (x (2l-1) P_{l-1} - (l+m-1) P_{l-2}) / (l-m), constants precomputed
-        const int lm = lm_index(l, m);
         const double pn = x * g.lmc[kLmcA + lm] * pl - g.lmc[kLmcB + lm] * plm1;
         plm1 = pl;
         pl = pn;
@@ -113,8 +115,9 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, PA A,
         continue;
       }
       if (!good) continue;  // P_l^m = 0 for l > 0 when cos(theta) is NaN (spheresl.py:698)
-      double dpl = somx2d * (xc * l * pl - (l + m) * plm1);
-      if (!isfinite(dpl)) dpl = 0.0;
+      // xc is clamped inside (-1, 1), so somx2d is finite and dpl is not
+      // finite only when pl or plm1 is not, which makes pot1 NaN already
+      const double dpl = somx2d * (xc * g.lmc[kLmcL + lm] * pl - g.lmc[kLmcS + lm] * plm1);
       if (m == 0) {
         pot1 += pl * ec;
         potr += pl * dc;

@@ -25,8 +25,11 @@ constexpr int kMaxLM = (kMaxL + 1) * (kMaxL + 2) / 2;
 //   [kLmcA + lm]  (2l - 1)/(l - m)      P_l^m = x A P_{l-1}^m - B P_{l-2}^m
 //   [kLmcB + lm]  (l + m - 1)/(l - m)
 //   [kLmcD + m]   (-1)^m (2m - 1)!!     P_m^m = D_m (1 - x^2)^{m/2}
+//   [kLmcL + lm]  l, [kLmcS + lm] l + m  (as doubles: scalar operands of the
+//                 derivative recurrence instead of per-lane conversions)
 constexpr int kLmcY = 0, kLmcA = kMaxLM, kLmcB = 2 * kMaxLM, kLmcD = 3 * kMaxLM;
-constexpr int kLmcSize = 3 * kMaxLM + kMaxL + 1;
+constexpr int kLmcL = 3 * kMaxLM + kMaxL + 1, kLmcS = kLmcL + kMaxLM;
+constexpr int kLmcSize = kLmcS + kMaxLM;
 
 // Row length of the padded deposit tables: radial orders rounded up to a
 // multiple of 48, so that for every chunk length of the lane shapes (2, 4, 8,
