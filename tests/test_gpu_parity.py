@@ -416,17 +416,17 @@ def test_accumulate_disc_halo_equals_separate_calls(big):
 
 def test_forces_disc_halo_equals_separate_calls(big):
     """bfe_forces_disc_halo (coefficients, field and forces at the same
-    particles, each half evaluated in its accumulate order) equals
-    accumulate_disc_halo + set_field + field_forces bit for bit, for
-    cell-ordered and input-order sizes, empty halves, field options and the
-    eight components; the forces meet the oracle bound."""
+    particles) equals accumulate_disc_halo + set_field + field_forces bit for
+    bit, for cell-ordered and input-order sizes, empty halves (also an empty
+    disc beside a cell-ordered halo), field options and the eight
+    components; the forces meet the oracle bound."""
     s, slo, eo = big
     (dx, dy, dz, dm), (hx, hy, hz, hm) = disc_halo(90000, 61)
     D = f32(dx, dy, dz, dm)
     H = f32(hx, hy, hz, hm)
     cases = ((90000, 90000, {}), (90000, 5000, dict(no_odd=True, compat=True, rotpos=0.4)),
              (3000, 90000, dict(halofac=0.5, disk_m=4, halo_l=3, halo_n=12)), (0, 7000, {}),
-             (7000, 0, {}))
+             (7000, 0, {}), (0, 90000, {}))
     for nd, nh, kw in cases:
         d = [v[:nd] for v in D]
         h = [v[:nh] for v in H]
