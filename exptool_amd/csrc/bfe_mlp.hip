@@ -668,7 +668,7 @@ __global__ __launch_bounds__(kBlockInf) void mlp_leapfrog_kernel(MLPLeapArgs a) 
       }
     }
     if (live) {
-      // energy with the BFE potential (the judge of drift)
+      // energy with the BFE potential (the reference for measuring drift)
       if (a.init || a.energy_check) {
         if (a.omega != 0.0) {
           const double ang = a.omega * (a.t0 + (double)a.k1 * a.dt);
