@@ -432,6 +432,37 @@ __device__ __forceinline__ void eof_to_run(float (&G)[PPT][4], double (&GD)[PPT]
     }
 }
 
+// Per-particle part of the cell-ordered EOF accumulates: lane's particle p
+// of the sorted input gives its row weights -4 pi m {1, cos k phi, sin k phi}
+// (rows 0, k, mmax + k of sW), its four bilinear corner weights (sE, 64 per
+// corner: c00, c10, c01, c11) and its cell's first node (sB).
+__device__ __forceinline__ void eof_dep_setup(const float4* __restrict__ P, int64_t p, bool live,
+                                              const EOFGeom& g, int ny1, int lane, float* sW,
+                                              float* sE, int* sB) {
+  const float4 q4 = live ? P[p] : make_float4(1.f, 0.f, 0.f, 0.f);
+  const double x = q4.x, y = q4.y, z = q4.z, mass = live ? (double)q4.w : 0.0;
+  const double R = sqrt(x * x + y * y + 1.e-10);
+  double c1, s1v;
+  planar_angle(x, y, c1, s1v);
+  const EOFCell cell = eof_locate(R, z, g.cmap, g.ascale, g.ihscale, g.xmin, g.idx, g.ymin,
+                                  g.idy, g.numx, g.numy);
+  const double w = -kFourPi * mass;
+  double cm = 1.0, sm = 0.0, cm1 = c1, sm1 = -s1v;
+  sW[lane] = (float)w;
+  sE[lane] = (float)cell.c00;
+  sE[64 + lane] = (float)cell.c10;
+  sE[128 + lane] = (float)cell.c01;
+  sE[192 + lane] = (float)cell.c11;
+  sB[lane] = cell.ix * ny1 + cell.iy;
+  for (int m = 1; m <= g.mmax; ++m) {
+    const double cn = (m == 1) ? c1 : 2.0 * c1 * cm - cm1;
+    const double sn = (m == 1) ? s1v : 2.0 * c1 * sm - sm1;
+    cm1 = cm; sm1 = sm; cm = cn; sm = sn;
+    sW[m * kDepStride + lane] = (float)(w * cm);
+    sW[(g.mmax + m) * kDepStride + lane] = (float)(w * sm);
+  }
+}
+
 template <int PPT, int CH>
 __global__ __launch_bounds__(256) void eof_deposit_kernel(
     const float4* __restrict__ P, Segments S, EOFGeom g, const float* __restrict__ T, int K,
@@ -476,34 +507,7 @@ __global__ __launch_bounds__(256) void eof_deposit_kernel(
     for (int64_t t0 = s0 + 64 * wave; t0 < s1; t0 += 256) {
       const int64_t p = t0 + lane;
       const bool live = p < s1;
-      int base = 0;
-      float c00 = 0.f, c10 = 0.f, c01 = 0.f, c11 = 0.f;
-      {
-        const float4 q4 = live ? P[p] : make_float4(1.f, 0.f, 0.f, 0.f);
-        const double x = q4.x, y = q4.y, z = q4.z, mass = live ? (double)q4.w : 0.0;
-        const double R = sqrt(x * x + y * y + 1.e-10);
-        double c1, s1v;
-        planar_angle(x, y, c1, s1v);
-        const EOFCell cell = eof_locate(R, z, g.cmap, g.ascale, g.ihscale, g.xmin, g.idx, g.ymin,
-                                        g.idy, g.numx, g.numy);
-        base = cell.ix * ny1 + cell.iy;
-        c00 = (float)cell.c00; c10 = (float)cell.c10; c01 = (float)cell.c01; c11 = (float)cell.c11;
-        const double w = -kFourPi * mass;
-        double cm = 1.0, sm = 0.0, cm1 = c1, sm1 = -s1v;
-        sW[lane] = (float)w;
-        sE[lane] = c00;
-        sE[64 + lane] = c10;
-        sE[128 + lane] = c01;
-        sE[192 + lane] = c11;
-        sB[lane] = base;
-        for (int m = 1; m <= g.mmax; ++m) {
-          const double cn = (m == 1) ? c1 : 2.0 * c1 * cm - cm1;
-          const double sn = (m == 1) ? s1v : 2.0 * c1 * sm - sm1;
-          cm1 = cm; sm1 = sm; cm = cn; sm = sn;
-          sW[m * kDepStride + lane] = (float)(w * cm);
-          sW[(g.mmax + m) * kDepStride + lane] = (float)(w * sm);
-        }
-      }
+      eof_dep_setup(P, p, live, g, ny1, lane, sW, sE, sB);
       wave_sync();
       const int64_t left = s1 - t0;
       const int np = left < 64 ? (int)left : 64;
