@@ -396,7 +396,10 @@ def test_cell_ordered_accumulate_matches_oracle(big):
 def test_accumulate_disc_halo_equals_separate_calls(big):
     """The combined disc + halo accumulate (concurrent halves, the halo half
     with its own work buffers) returns exactly what the two single calls
-    return, for cell-ordered and input-order sizes and empty halves."""
+    return, for cell-ordered and input-order sizes and empty halves.  At
+    cell-ordered sizes this also holds the two instantiations of the SL
+    deposit (two waves per SIMD beside the EOF accumulate, three alone) to
+    the same bits."""
     s, slo, eo = big
     (dx, dy, dz, dm), (hx, hy, hz, hm) = disc_halo(90000, 57)
     D = f32(dx, dy, dz, dm)
