@@ -15,6 +15,11 @@
  *  - Work is enqueued on `stream` (a hipStream_t; NULL = default stream).
  *    Calls are asynchronous unless stated otherwise.
  *  - One context per host thread; a context is bound to one device.
+ *  - A context's work buffers (sort buffers, partial sums, the staging of
+ *    evaluated forces) are shared by all its calls: issue the calls of one
+ *    context on one stream, or order them yourself.  Calls on different
+ *    streams that overlap on one context race.  Use one context per
+ *    concurrent stream instead.
  *  - Results are deterministic: reductions use fixed orders, no float atomics.
  */
 #ifndef EXPTOOL_BFE_H
