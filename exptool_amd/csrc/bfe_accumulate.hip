@@ -463,13 +463,13 @@ __device__ __forceinline__ void eof_dep_setup(const float4* __restrict__ P, int6
   }
 }
 
+// Body of the cell-ordered EOF accumulate; dsm is the block's dynamic LDS.
 template <int PPT, int CH>
-__global__ __launch_bounds__(256) void eof_deposit_kernel(
-    const float4* __restrict__ P, Segments S, EOFGeom g, const float* __restrict__ T, int K,
-    double* __restrict__ partials) {
+__device__ __forceinline__ void eof_deposit_body(
+    unsigned char* dsm, const float4* __restrict__ P, Segments S, EOFGeom g,
+    const float* __restrict__ T, int K, double* __restrict__ partials) {
   const int rp = dep_np(g.norder);  // padded row length of T
   const int rowlen = (2 * g.mmax + 1) * rp;
-  extern __shared__ __align__(16) unsigned char dsm[];
   const int J = 2 * g.mmax + 1;
   const int N = g.norder;
   const int NC = (N + CH - 1) / CH;
@@ -592,6 +592,26 @@ __global__ __launch_bounds__(256) void eof_deposit_kernel(
       partials[(size_t)sg * K + k] = ((sRed[k] + sRed[K + k]) + sRed[2 * K + k]) + sRed[3 * K + k];
     __syncthreads();
   }
+}
+
+// As for the SL deposit: eof_deposit_kernel beside the SL accumulate (the
+// fused calls), eof_deposit_alone_kernel when it has the GPU to itself.
+// Alone, four waves per SIMD (128 registers, about 100 B spilled) measured
+// 80 against 83 us for 1M disc particles; beside the SL deposit they lost.
+template <int PPT, int CH>
+__global__ __launch_bounds__(256) void eof_deposit_kernel(
+    const float4* __restrict__ P, Segments S, EOFGeom g, const float* __restrict__ T, int K,
+    double* __restrict__ partials) {
+  extern __shared__ __align__(16) unsigned char dsm[];
+  eof_deposit_body<PPT, CH>(dsm, P, S, g, T, K, partials);
+}
+
+template <int PPT, int CH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void
+eof_deposit_alone_kernel(const float4* __restrict__ P, Segments S, EOFGeom g,
+                         const float* __restrict__ T, int K, double* __restrict__ partials) {
+  extern __shared__ __align__(16) unsigned char dsm[];
+  eof_deposit_body<PPT, CH>(dsm, P, S, g, T, K, partials);
 }
 
 // Applies the float64 radial run sums D1, D2 of cell i to the lane's
@@ -1115,9 +1135,11 @@ static DepShape dep_shape(int rows, int N, const DepShape* cands, int ncand, int
 static const DepShape kEofShapes[kEofDepShapes] = {{1, 4}, {1, 8}, {1, 16}, {2, 2}, {2, 4}, {2, 8}, {2, 16}, {4, 8}};
 static const DepShape kSlShapes[kSlDepShapes] = {{1, 8}, {1, 16}, {1, 24}, {2, 8}, {2, 16}, {2, 24}, {4, 16}};
 
+// beside: another accumulate runs concurrently (selects the deposit kernel).
 static int eof_accumulate_run(Context* c, const float* x, const float* y, const float* z,
                               const float* m, int64_t n, const int64_t* off, int nsnap,
-                              double* cos_out, double* sin_out, hipStream_t s) {
+                              double* cos_out, double* sin_out, hipStream_t s,
+                              bool beside = false) {
   const int K = c->eof_K;
   AccPlan P = off ? plan_batch(c->eof_acc_blocks, c->eof_seg_tiles, off, nsnap, kTile)
                   : plan_single(c->eof_acc_blocks, n, kTile);
@@ -1148,8 +1170,14 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
   timer_begin(c, "eof_accumulate", s);
   if (sorted) {
 #define BFE_EOF_DEP(PP, CC)                                                                   \
-  else if (dsh.ppt == PP && dsh.ch == CC) hipLaunchKernelGGL((eof_deposit_kernel<PP, CC>),    \
-      dim3(P.grid), dim3(256), dlds, s, sorted, P.S, g, c->eof_dep32, K, partials);
+  else if (dsh.ppt == PP && dsh.ch == CC) {                                                   \
+    if (beside)                                                                               \
+      hipLaunchKernelGGL((eof_deposit_kernel<PP, CC>), dim3(P.grid), dim3(256), dlds, s,       \
+                         sorted, P.S, g, c->eof_dep32, K, partials);                          \
+    else                                                                                      \
+      hipLaunchKernelGGL((eof_deposit_alone_kernel<PP, CC>), dim3(P.grid), dim3(256), dlds, s, \
+                         sorted, P.S, g, c->eof_dep32, K, partials);                          \
+  }
     if (false) {}
     BFE_EOF_DEP(1, 4) BFE_EOF_DEP(1, 8) BFE_EOF_DEP(1, 16) BFE_EOF_DEP(2, 2) BFE_EOF_DEP(2, 4)
     BFE_EOF_DEP(2, 8) BFE_EOF_DEP(2, 16) BFE_EOF_DEP(4, 8)
@@ -1264,7 +1292,7 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
 
 int eof_accumulate_one(Context* c, const float* x, const float* y, const float* z,
                        const float* m, int64_t n, double* cos_out, double* sin_out, hipStream_t s) {
-  return eof_accumulate_run(c, x, y, z, m, n, nullptr, 1, cos_out, sin_out, s);
+  return eof_accumulate_run(c, x, y, z, m, n, nullptr, 1, cos_out, sin_out, s, true);
 }
 
 int sl_accumulate_one(Context* c, const float* x, const float* y, const float* z, const float* m,
@@ -1356,7 +1384,7 @@ extern "C" int bfe_accumulate_disc_halo(bfe_ctx* h, const float* dx, const float
   swap_work(c);
   const int rs = sl_accumulate_run(c, hx, hy, hz, hm, nh, nullptr, 1, sl_flags, coef_out, s2, true);
   swap_work(c);
-  const int re = rs ? 0 : eof_accumulate_run(c, dx, dy, dz, dm, nd, nullptr, 1, cos_out, sin_out, s);
+  const int re = rs ? 0 : eof_accumulate_run(c, dx, dy, dz, dm, nd, nullptr, 1, cos_out, sin_out, s, true);
   rc = fork_end(c, s);
   return rs ? rs : (re ? re : rc);
 }

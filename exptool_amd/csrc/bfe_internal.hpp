@@ -215,7 +215,8 @@ int order_particles(Context* c, int kind, const float* x, const float* y, const 
                     const float4** sorted_out);
 // Single-snapshot accumulates (bfe_eof_accumulate / bfe_sl_accumulate
 // without the argument checks), enqueued on s with the current buffer set.
-// sl_accumulate_one is for callers that run it beside the EOF accumulate.
+// Both are for callers that run them side by side (they take the deposit
+// kernels sized for sharing the GPU).
 int eof_accumulate_one(Context* c, const float* x, const float* y, const float* z,
                        const float* m, int64_t n, double* cos_out, double* sin_out, hipStream_t s);
 int sl_accumulate_one(Context* c, const float* x, const float* y, const float* z, const float* m,
