@@ -86,10 +86,12 @@ __global__ __launch_bounds__(256) void sl_fold_kernel(const double* __restrict__
     }
   }
   if (q == 0) {
+    // node i and the step to node i+1, so that the evaluation interpolates
+    // with one FMA: E_i + x2 (E_i+1 - E_i)
     o[0] = (float)v[0][0];
     o[1] = (float)v[1][0];
-    o[2] = (float)v[0][1];
-    o[3] = (float)v[1][1];
+    o[2] = (float)(v[0][1] - v[0][0]);
+    o[3] = (float)(v[1][1] - v[1][0]);
   } else if (q == 1) {
     o[4] = (float)(0.5 * (v[0][6] - v[0][4]));
     o[5] = (float)(0.5 * (v[1][6] - v[1][4]));

@@ -21,7 +21,8 @@ struct EOFOut {
 // Folded table layouts (float32, streamed as float4):
 //  SL : per radial node i, one 8-float block per (l, m) pair in m-major order
 //       (m = 0..L, l = m..L):
-//         [E_c(i), E_s(i), E_c(i+1), E_s(i+1), D1_c, D1_s, D2_c, D2_s]
+//         [E_c(i), E_s(i), dE_c(i), dE_s(i), D1_c, D1_s, D2_c, D2_s]
+//       with dE(i) = E(i+1) - E(i), so that E at x2 is E(i) + x2 dE(i)
 //       with D1, D2 at the stencil centre c = max(i, 1); after the numr
 //       nodes of these, the density blocks per node and pair:
 //         [Dn_c(i), Dn_s(i), Dn_c(i+1), Dn_s(i+1)].
@@ -103,7 +104,7 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, PA A,
         }
       }
       const float4 a = ld4(A, 2 * p), b = ld4(A, 2 * p + 1);
-      const double ec = (double)fmaf(x2f, a.z, x1f * a.x);
+      const double ec = (double)fmaf(x2f, a.z, a.x);   // a.z holds E_i+1 - E_i
       const double dc = (double)fmaf(ps, b.z, b.x);
       if (l == 0) {
         pot0 = ec;
@@ -128,7 +129,7 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, PA A,
         }
         continue;
       }
-      const double es = (double)fmaf(x2f, a.w, x1f * a.y);
+      const double es = (double)fmaf(x2f, a.w, a.y);
       const double ds = (double)fmaf(ps, b.w, b.y);
       if (COSL) {
         const double pa = ec * cl + es * sl;
