@@ -237,9 +237,9 @@ __device__ __forceinline__ void rk4_force(const FieldTables& T, bool live, float
   if (live) {
     double cv[8];
     if (staged)
-      field_eval_point<true>(T, px, py, pz, cv, &cell, stage, imin);
+      field_eval_point<true, 2>(T, px, py, pz, cv, &cell, stage, imin);
     else
-      field_eval_point(T, px, py, pz, cv);
+      field_eval_point<false, 2>(T, px, py, pz, cv);
     fx = (float)(cv[0] + cv[1]);
     fy = (float)(cv[2] + cv[3]);
     fz = (float)(cv[4] + cv[5]);
