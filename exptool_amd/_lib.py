@@ -26,6 +26,7 @@ NO_ODD = 0x10
 ACC_BF16 = 0x20
 COMPAT_ALL = 0xF
 COMPAT_RK4_FRAME = 0x40
+EVAL_DENS = 0x80
 
 EXPORTS = (
     "bfe_create", "bfe_destroy", "bfe_last_error", "bfe_sl_tables_set", "bfe_eof_tables_set",
@@ -34,7 +35,7 @@ EXPORTS = (
     "bfe_mlp_forward", "bfe_mlp_gradient", "bfe_mlp_train_step", "bfe_mlp_leapfrog",
     "bfe_eof_accumulate_batch", "bfe_sl_accumulate_batch", "bfe_field_set_batch",
     "bfe_field_eval_batch", "bfe_accumulate_disc_halo", "bfe_forces_disc_halo",
-    "bfe_set_tuning", "bfe_get_tuning", "bfe_kernel_times",
+    "bfe_set_tuning", "bfe_get_tuning", "bfe_kernel_times", "bfe_sl_eval_sph", "bfe_eof_eval_cyl",
 )
 MLP_NPARAMS = 50564
 
@@ -56,6 +57,8 @@ def _declare(lib):
         "bfe_eof_accumulate": (_I32, [_VP, _VP, _VP, _VP, _VP, _I64, _U32, _VP, _VP, _VP]),
         "bfe_sl_eval": (_I32, [_VP, _VP, _VP, _VP, _I64, _VP, _I32, _I32, _U32] + [_VP] * 7 + [_VP]),
         "bfe_eof_eval": (_I32, [_VP, _VP, _VP, _VP, _I64, _VP, _VP, _I32, _I32, _U32] + [_VP] * 7 + [_VP]),
+        "bfe_sl_eval_sph": (_I32, [_VP, _VP, _VP, _VP, _I64, _VP, _I32, _I32, _U32, _VP, _VP]),
+        "bfe_eof_eval_cyl": (_I32, [_VP, _VP, _VP, _VP, _I64, _VP, _VP, _I32, _I32, _U32, _VP, _VP]),
         "bfe_field_set": (_I32, [_VP, _VP, _VP, _VP, _F64, _I32, _I32, _I32, _I32, _U32, _VP]),
         "bfe_field_eval_cart": (_I32, [_VP, _VP, _VP, _VP, _I64, _F64, _VP, _VP, _VP, _VP, _VP, _VP]),
         "bfe_leapfrog": (_I32, [_VP] + [_VP] * 6 + [_I64, _F64, _I64, _F64, _F64, _I32, _VP, _VP, _I32, _VP,

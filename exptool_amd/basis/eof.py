@@ -9,10 +9,11 @@ Device work (libexptool_bfe.so through exptool_amd.bfe.BFESystem):
   accumulate               eof.py:492-640 (VAR subsample sums: one batched launch)
   compute_coefficients     eof.py:1158-1260
   compute_variance / compute_sn   eof.py:2135-2226 (host, coefficient-sized)
+  accumulated_eval_particles / compute_forces   eof.py:989-1144, :1263-1317
+  force_eval               eof.py:756-872 (one point or many per call)
+  accumulated_eval         eof.py:874-929 (one point or many per call)
 Coefficient files (host): save_eof_coefficients / restore_eof_coefficients
   eof.py:1646-1763
-  accumulated_eval_particles / compute_forces   eof.py:989-1144, :1263-1317
-  force_eval               eof.py:756-872 (many points per call)
 Return types follow the reference: float64 NumPy arrays and EOF_Object.
 """
 from __future__ import annotations
@@ -256,22 +257,18 @@ def compute_coefficients(PSPInput, eof_file, verbose=1, no_odd=False, nprocs_max
 
 def accumulate(ParticleInstance, potC, potS, MMAX, NMAX, XMIN, dX, YMIN, dY, NUMX, NUMY, ASCALE,
                HSCALE, CMAP, verbose=0, no_odd=False, VAR=0):
-    """eof.py:492-640 with tables passed as arrays.  The tables are uploaded to
-    a device context for the call (use compute_coefficients or a BFESystem to
-    keep them resident).  VAR > 0 returns (cos, sin, cos2, sin2) as the
-    reference (eof.py:554-574)."""
-    from ..bfe import BFESystem
-    potC, potS = np.asarray(potC), np.asarray(potS)
-    z0 = np.zeros(potC.shape)
-    t = dict(mmax=potC.shape[0] - 1, norder=potC.shape[1], numx=NUMX, numy=NUMY, cmap=CMAP,
-             ascale=ASCALE, hscale=HSCALE, XMIN=XMIN, dX=dX, YMIN=YMIN, dY=dY, potC=potC,
-             rforceC=z0, zforceC=z0, potS=potS, rforceS=z0, zforceS=z0, dens=0)
+    """eof.py:492-640 with tables passed as arrays (kept on the device
+    between calls with the same arrays and grid).  VAR > 0 returns (cos,
+    sin, cos2, sin2) as the reference (eof.py:554-574).  no_odd has no
+    effect, as in the reference (:545-551)."""
+    sysm = _point_system(dict(potC=potC, potS=potS), XMIN, dX, YMIN, dY, NUMX, NUMY, ASCALE,
+                         HSCALE, CMAP)
     x, y, z, m = _particle_arrays(ParticleInstance)
     if VAR:
-        c, sn, c2, s2 = accumulate_tables(BFESystem(eof=t), x, y, z, m, VAR=int(VAR))
+        c, sn, c2, s2 = accumulate_tables(sysm, x, y, z, m, VAR=int(VAR))
         return (c[:MMAX + 1, :NMAX], sn[:MMAX + 1, :NMAX], c2[:, :MMAX + 1, :NMAX],
                 s2[:, :MMAX + 1, :NMAX])
-    c, sn = accumulate_tables(BFESystem(eof=t), x, y, z, m)
+    c, sn = accumulate_tables(sysm, x, y, z, m)
     return c[:MMAX + 1, :NMAX], sn[:MMAX + 1, :NMAX]
 
 
@@ -388,3 +385,82 @@ def accumulated_eval_particles(Particles, accum_cos, accum_sin, eof_file="", m1=
     obj = EOF_Object()
     obj.eof_file, obj.cos, obj.sin = eof_file, accum_cos, accum_sin
     return compute_forces(Particles, obj, m1=m1, m2=m2, density=density)
+
+
+# ----------------------------------------------------------------------------
+# point evaluators (eof.py:756-929)
+# ----------------------------------------------------------------------------
+
+def _point_system(tabs, rmin, dR, zmin, dZ, numx, numy, ASCALE, HSCALE, CMAP):
+    """BFESystem holding the tables passed to a point evaluator (kept between
+    calls with the same arrays and grid)."""
+    from ..bfe import system_for_tables
+    potC = np.asarray(tabs["potC"], dtype=np.float64)
+    if potC.shape[2:] != (int(numx) + 1, int(numy) + 1):
+        raise ValueError(f"eof tables are {potC.shape[2:]} nodes, not numx+1, numy+1 = "
+                         f"{(int(numx) + 1, int(numy) + 1)}")
+    tables = {k: (None if v is None else np.asarray(v, dtype=np.float64)) for k, v in tabs.items()}
+    dens = int(tables.get("densC") is not None and tables.get("densS") is not None)
+    scal = dict(mmax=potC.shape[0] - 1, norder=potC.shape[1], numx=int(numx), numy=int(numy),
+                cmap=int(CMAP), ascale=float(ASCALE), hscale=float(HSCALE), XMIN=float(rmin),
+                dX=float(dR), YMIN=float(zmin), dY=float(dZ), dens=dens)
+    return system_for_tables("eof", tables, scal)
+
+
+def _coef_rows(a, mmax, norder):
+    c = np.asarray(a, dtype=np.float64)
+    c = c.reshape(-1, c.shape[-1]) if c.ndim > 1 else c.reshape(1, -1)
+    out = np.zeros((mmax + 1, norder))
+    nr, nc = min(c.shape[0], mmax + 1), min(c.shape[1], norder)
+    out[:nr, :nc] = c[:nr, :nc]
+    return out
+
+
+def _point_eval(sysm, r, z, phi, accum_cos, accum_sin, MMAX, NMAX, no_odd, compat_mask,
+                density):
+    mmax, norder = sysm.eof["mmax"], sysm.eof["norder"]
+    scalar = all(np.ndim(v) == 0 for v in (r, z, phi))
+    r, z, ph = np.broadcast_arrays(*(np.asarray(v, dtype=np.float64) for v in (r, z, phi)))
+    out = sysm.eof_eval_cyl(r.reshape(-1), z.reshape(-1), ph.reshape(-1),
+                            _coef_rows(accum_cos, mmax, norder),
+                            _coef_rows(accum_sin, mmax, norder), mmax_use=int(MMAX),
+                            nmax_use=int(NMAX), no_odd=no_odd, compat_mask=compat_mask,
+                            density=density).cpu().numpy()
+    if scalar:
+        return [float(v[0]) for v in out]
+    return [v.reshape(r.shape) for v in out]
+
+
+def force_eval(r, z, phi, accum_cos, accum_sin, potC, rforceC, zforceC, potS, rforceS, zforceS,
+               rmin=0, dR=0, zmin=0, dZ=0, numx=0, numy=0, fac=1.0, MMAX=6, NMAX=18, ASCALE=0.0,
+               HSCALE=0.0, CMAP=0, no_odd=False, perturb=False, compat=False):
+    """eof.py:756-872 at one point or arrays of points (r the planar radius as
+    given): (fr+fr0, fp, fz+fz0, p+p0, p0), fp = -dPhi/dphi, or with perturb
+    (fr, fp, fz, p, p0, fr0, fz0) with the m >= 1 parts and the m = 0 ones
+    apart.  Orders m > MMAX and n >= NMAX are dropped.  no_odd drops odd m;
+    compat=True weights them by the reference's |cos(pi m/2)| (~6e-17,
+    :833) instead.  fac is unused, as in the reference.  The tables are kept
+    on the device between calls with the same arrays and grid."""
+    sysm = _point_system(dict(potC=potC, rforceC=rforceC, zforceC=zforceC, potS=potS,
+                              rforceS=rforceS, zforceS=zforceS),
+                         rmin, dR, zmin, dZ, numx, numy, ASCALE, HSCALE, CMAP)
+    p0, p, fr, fp, fz, _, _, fr0, fz0 = _point_eval(sysm, r, z, phi, accum_cos, accum_sin,
+                                                    MMAX, NMAX, no_odd, compat, False)
+    if perturb:
+        return fr - fr0, fp, fz - fz0, p, p0, fr0, fz0
+    return fr, fp, fz, p + p0, p0
+
+
+def accumulated_eval(r, z, phi, accum_cos, accum_sin, potC, rforceC, zforceC, densC, potS,
+                     rforceS, zforceS, densS, rmin=0, dR=0, zmin=0, dZ=0, numx=0, numy=0,
+                     fac=1.0, MMAX=6, NMAX=18, ASCALE=0.0, HSCALE=0.0, CMAP=0, no_odd=False):
+    """eof.py:874-929 at one point or arrays of points: (p0, p, fr, fp, fz,
+    d0, d) with p and d the totals over m = 0..MMAX and p0, d0 the m = 0
+    terms.  Every radial order is used (NMAX has no effect, as in the
+    reference); no_odd skips odd m."""
+    sysm = _point_system(dict(potC=potC, rforceC=rforceC, zforceC=zforceC, densC=densC,
+                              potS=potS, rforceS=rforceS, zforceS=zforceS, densS=densS),
+                         rmin, dR, zmin, dZ, numx, numy, ASCALE, HSCALE, CMAP)
+    p0, p, fr, fp, fz, d0, d, _, _ = _point_eval(sysm, r, z, phi, accum_cos, accum_sin, MMAX,
+                                                 -1, no_odd, False, True)
+    return p0, p0 + p, fr, fp, fz, d0, d0 + d

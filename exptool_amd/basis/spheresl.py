@@ -5,7 +5,8 @@ Device work (libexptool_bfe.so through exptool_amd.bfe.BFESystem):
   compute_coefficients            spheresl.py:439-475 (one launch replaces the Pool)
   all_eval_particles              spheresl.py:1240-1362
   eval_particles                  spheresl.py:502-528
-  force_eval                      spheresl.py:1107-1234 (many points per call)
+  force_eval                      spheresl.py:1107-1234 (one point or many per call)
+  all_eval                        spheresl.py:987-1102 (one point or many per call)
 Coefficient files (host):
   save_sl_coefficients / restore_sl_coefficients   spheresl.py:1390-1488
 Return types follow the reference: float64 NumPy arrays and SL_Object.
@@ -108,43 +109,89 @@ def eval_particles(ParticleInstance, expcoef, sph_file, mod_file, nprocs=-1, ver
                               compat=compat)
 
 
+def truncation(lmax, nmax, lmax_t, nmax_t):
+    """Orders force_eval / all_eval use when asked for lmax, nmax against
+    tables of lmax_t, nmax_t (spheresl.py:1134-1147, :1009-1022): l is cut
+    when lmax < lmax_t, n when nmax < nmax_t - 1.  The reference fails on
+    the other requests (nmax = nmax_t - 1 leaves expcoef and the radial
+    matrices of different lengths; all_eval keeps lmax + 1 instead of
+    (lmax + 1)^2 rows, :1017); here they mean the full table."""
+    lmax, nmax = int(lmax), int(nmax)
+    return min(lmax, lmax_t), (nmax if nmax < nmax_t - 1 else nmax_t)
+
+
 def _truncated(expcoef, lmax, nmax, lmax_t, nmax_t):
-    """force_eval's table truncation (spheresl.py:1134-1147): l is cut when
-    lmax < lmax_table, n when nmax < nmax_table - 1 (so nmax_table - 1 keeps
-    every order).  Dropped terms become zero coefficients."""
+    """expcoef ((lmax_t+1)^2, nmax_t) with the orders truncation() drops set
+    to zero."""
+    lu, nu = truncation(lmax, nmax, lmax_t, nmax_t)
     c = np.array(expcoef, dtype=np.float64, copy=True).reshape((lmax_t + 1) ** 2, nmax_t)
-    if lmax < lmax_t:
-        c[(lmax + 1) ** 2:, :] = 0.0
-    if nmax < nmax_t - 1:
-        c[:, max(nmax, 0):] = 0.0
+    c[(max(lu, 0) + 1) ** 2:, :] = 0.0
+    c[:, max(nu, 0):] = 0.0
     return c
+
+
+def _point_system(xi, p0, d0, cmap, scale, evtable, eftable):
+    from ..bfe import system_for_tables
+    ev = np.asarray(evtable, dtype=np.float64)
+    ef = np.asarray(eftable, dtype=np.float64)
+    lmax_t, nmax_t = ev.shape[0] - 1, ev.shape[1]
+    tables = dict(xi=np.asarray(xi, dtype=np.float64), ev=ev, ef=ef,
+                  p0=np.asarray(p0, dtype=np.float64), d0=np.asarray(d0, dtype=np.float64))
+    scal = dict(lmax=lmax_t, nmax=nmax_t, numr=ef.shape[2], cmap=int(cmap), scale=float(scale))
+    return system_for_tables("sl", tables, scal), lmax_t, nmax_t
+
+
+def _coef_rows(expcoef, lmax_t, nmax_t):
+    """expcoef as ((lmax_t+1)^2, nmax_t), missing rows/orders zero."""
+    c = np.asarray(expcoef, dtype=np.float64)
+    c = c.reshape(-1, c.shape[-1]) if c.ndim > 1 else c.reshape(1, -1)
+    out = np.zeros(((lmax_t + 1) ** 2, nmax_t))
+    nr, nc = min(c.shape[0], out.shape[0]), min(c.shape[1], nmax_t)
+    out[:nr, :nc] = c[:nr, :nc]
+    return out
+
+
+def _point_eval(r, costh, phi, expcoef, xi, p0, d0, cmap, scale, lmax, nmax, evtable, eftable,
+                no_odd, density, edge, cos_l):
+    s, lmax_t, nmax_t = _point_system(xi, p0, d0, cmap, scale, evtable, eftable)
+    lu, nu = truncation(lmax, nmax, lmax_t, nmax_t)
+    scalar = all(np.ndim(v) == 0 for v in (r, costh, phi))
+    r, ct, ph = np.broadcast_arrays(*(np.asarray(v, dtype=np.float64) for v in (r, costh, phi)))
+    out = s.sl_eval_sph(r.reshape(-1), ct.reshape(-1), ph.reshape(-1),
+                        _coef_rows(expcoef, lmax_t, nmax_t), lmax_use=lu, nmax_use=nu,
+                        no_odd=no_odd, density=density, edge=edge, cos_l=cos_l).cpu().numpy()
+    if scalar:
+        return [float(v[0]) for v in out]
+    return [v.reshape(r.shape) for v in out]
 
 
 def force_eval(r, costh, phi, expcoef, xi, p0, d0, cmap, scale, lmax, nmax, evtable, eftable,
                no_odd=False, verbose=0, compat=False):
-    """spheresl.py:1107-1234 over arrays of points: (potr, pott, potp, pot1,
-    pot0).  The tables are passed as arrays, as in the reference, and are
-    uploaded for the call; keep a BFESystem to hold them resident."""
-    from ..bfe import BFESystem
-    ev = np.asarray(evtable, dtype=np.float64)
-    ef = np.asarray(eftable, dtype=np.float64)
-    lmax_t, nmax_t = ev.shape[0] - 1, ev.shape[1]
-    numr = ef.shape[2]
-    t = dict(lmax=lmax_t, nmax=nmax_t, numr=numr, cmap=int(cmap), scale=float(scale),
-             xi=np.asarray(xi, dtype=np.float64), ev=ev, ef=ef,
-             p0=np.asarray(p0, dtype=np.float64), d0=np.asarray(d0, dtype=np.float64))
-    s = BFESystem(sl=t)
-    r = np.atleast_1d(np.asarray(r, dtype=np.float64))
-    ct = np.atleast_1d(np.asarray(costh, dtype=np.float64))
-    ph = np.atleast_1d(np.asarray(phi, dtype=np.float64))
-    st = np.sqrt(np.clip((1.0 - ct) * (1.0 + ct), 0.0, None))
-    x, y, z = r * st * np.cos(ph), r * st * np.sin(ph), r * ct
-    c = _truncated(expcoef, int(lmax), int(nmax), lmax_t, nmax_t)
-    # compat: cos/sin(l phi) (:1222-1225) and the first-cell stencil of
-    # get_halo_dens_pot_force, which force_eval calls (:1199, :150-153)
-    g = s.sl_eval(x, y, z, c, no_odd=no_odd, density=False, cos_l=compat, compat=compat)
-    o = {k: v.double().cpu().numpy() for k, v in g.items()}
-    return o["potr"], o["pott"], o["potp"], o["pot1"], o["pot0"]
+    """spheresl.py:1107-1234 at one point or arrays of points (r the 3D
+    radius as given): (potr, pott, potp, pot1, pot0) = dPhi/dr,
+    dPhi/dcos(theta), dPhi/dphi, the l >= 1 and the l = 0 potential.
+    compat=True gives the reference's cos/sin(l phi) azimuthal factor
+    (:1222-1225) and its first-cell stencil (:150-153).  The tables are kept
+    on the device between calls with the same arrays."""
+    den0, den1, pot0, pot1, potr, pott, potp = _point_eval(
+        r, costh, phi, expcoef, xi, p0, d0, cmap, scale, lmax, nmax, evtable, eftable,
+        no_odd, False, compat, compat)
+    return potr, pott, potp, pot1, pot0
+
+
+def all_eval(r, costh, phi, expcoef, xi, p0, d0, cmap, scale, lmax, nmax, evtable, eftable,
+             no_odd=False, verbose=0, compat=False):
+    """spheresl.py:987-1102 at one point or arrays of points: (den0, den1,
+    pot0, pot1, potr, pott, potp) with den1 the total density (the l = 0 term
+    included, :1046), densities times 0.25/pi (:1092) and the cos/sin(m phi)
+    azimuthal factor.  compat=True gives the first-cell stencil of
+    get_halo_dens_pot_force (:150-153), which the reference evaluates."""
+    den0, den1, pot0, pot1, potr, pott, potp = _point_eval(
+        r, costh, phi, expcoef, xi, p0, d0, cmap, scale, lmax, nmax, evtable, eftable,
+        no_odd, True, compat, False)
+    if isinstance(den0, float):
+        return den0, den0 + den1, pot0, pot1, potr, pott, potp
+    return den0, den0 + den1, pot0, pot1, potr, pott, potp
 
 
 # ----------------------------------------------------------------------------

@@ -32,6 +32,46 @@ def eof_table_params(t):
     return XMIN, XMAX, dX, YMIN, YMAX, dY
 
 
+def _array_key(a):
+    """Identity of a table array for the system cache: its buffer, shape and
+    dtype and a digest of its values (all of them up to 16 MiB, else a
+    65 536-value stride sample plus the first and last 64 KiB)."""
+    import hashlib
+    a = np.asarray(a)
+    b = np.ascontiguousarray(a)
+    raw = b.reshape(-1).view(np.uint8)
+    if raw.size <= (16 << 20):
+        data = raw.tobytes()
+    else:
+        flat = b.reshape(-1)
+        data = (np.ascontiguousarray(flat[::max(1, flat.size // 65536)]).tobytes() +
+                raw[:65536].tobytes() + raw[-65536:].tobytes())
+    return (a.__array_interface__["data"][0], a.shape, a.dtype.str,
+            hashlib.blake2b(data, digest_size=16).hexdigest())
+
+
+_TABLE_SYSTEMS = {}
+
+
+def system_for_tables(kind, tables, scalars):
+    """BFESystem holding the SL (kind "sl") or EOF (kind "eof") tables given
+    as arrays, kept for later calls with the same arrays and parameters (the
+    reference's point evaluators take the tables as arguments on every call;
+    uploading them once keeps a per-point loop from re-uploading).  At most
+    four table sets are kept."""
+    key = (kind, tuple(sorted((k, _array_key(v)) for k, v in tables.items() if v is not None)),
+           tuple(sorted(scalars.items())))
+    s = _TABLE_SYSTEMS.get(key)
+    if s is None:
+        t = dict(scalars)
+        t.update(tables)
+        s = BFESystem(**{kind: t})
+        if len(_TABLE_SYSTEMS) >= 4:
+            _TABLE_SYSTEMS.pop(next(iter(_TABLE_SYSTEMS)))
+        _TABLE_SYSTEMS[key] = s
+    return s
+
+
 class BFESystem:
     """SL + EOF tables on one device."""
 
@@ -60,7 +100,8 @@ class BFESystem:
 
     def set_eof(self, t):
         """t: dict with mmax, norder, numx, numy, cmap, rmin, rmax, ascale,
-        hscale, potC, rforceC, zforceC, potS, rforceS, zforceS[, densC, densS].
+        hscale, potC, rforceC, zforceC, potS, rforceS, zforceS[, densC, densS]
+        (force tables may be None: zero).
         Grid parameters XMIN, dX, YMIN, dY, when present, are used as given
         instead of being derived from rmin/rmax (set_table_params)."""
         mmax, norder, numx, numy = (int(t[k]) for k in ("mmax", "norder", "numx", "numy"))
@@ -69,8 +110,9 @@ class BFESystem:
         else:
             XMIN, _, dX, YMIN, _, dY = eof_table_params(t)
         shape = (mmax + 1, norder, numx + 1, numy + 1)
-        arrs = {k: c64(t[k]).reshape(shape) for k in
-                ("potC", "rforceC", "zforceC", "potS", "rforceS", "zforceS")}
+        # a table given as None (accumulate needs only potC/potS) is zero
+        arrs = {k: (c64(t[k]).reshape(shape) if t.get(k) is not None else np.zeros(shape))
+                for k in ("potC", "rforceC", "zforceC", "potS", "rforceS", "zforceS")}
         dens = t.get("densC") is not None and t.get("densS") is not None and int(t.get("dens", 1))
         dC = c64(t["densC"]).reshape(shape) if dens else None
         dS = c64(t["densS"]).reshape(shape) if dens else None
@@ -326,6 +368,49 @@ class BFESystem:
                                        int(m1), int(min(m2, 2**30)), 0,
                                        *[ptr(out.get(k)) for k in names], stream_ptr(stream))
         self.ctx.check(rc, "bfe_eof_eval")
+        return out
+
+    # --------------------------------------------------------- point forms
+    def sl_eval_sph(self, r, costh, phi, coef, lmax_use=-1, nmax_use=-1, no_odd=False,
+                    density=False, edge=False, cos_l=False, stream=None):
+        """The SL expansion at points given as (r, cos theta, phi) in float64
+        (bfe_sl_eval_sph): (7, n) float64 device tensor of den0, den1 (l >= 1),
+        pot0, pot1, potr, pott, potp.  edge: get_halo_dens_pot_force's
+        first-cell stencil; cos_l: force_eval's cos/sin(l phi)."""
+        self._need("sl")
+        r, ct, ph = (f64(a, self.device).reshape(-1) for a in (r, costh, phi))
+        n = r.numel()
+        if ct.numel() != n or ph.numel() != n:
+            raise ValueError("r, costh, phi must have the same length")
+        coef = f64(coef, self.device)
+        out = torch.empty((7, n), dtype=torch.float64, device=self.device)
+        flags = ((_lib.NO_ODD if no_odd else 0) | (_lib.EVAL_DENS if density else 0) |
+                 (_lib.COMPAT_SL_EDGE if edge else 0) | (_lib.COMPAT_SL_COS_L if cos_l else 0))
+        rc = self.ctx.lib.bfe_sl_eval_sph(self.ctx.h, ptr(r), ptr(ct), ptr(ph), n, ptr(coef),
+                                          int(lmax_use), int(nmax_use), flags, ptr(out),
+                                          stream_ptr(stream))
+        self.ctx.check(rc, "bfe_sl_eval_sph")
+        return out
+
+    def eof_eval_cyl(self, R, z, phi, cos, sin, mmax_use=-1, nmax_use=-1, no_odd=False,
+                     compat_mask=False, density=False, stream=None):
+        """The EOF expansion at points given as (R, z, phi) in float64
+        (bfe_eof_eval_cyl): (9, n) float64 device tensor of p0, p (m >= 1),
+        fr, fp, fz, d0, d (m >= 1), fr0, fz0.  compat_mask: with no_odd,
+        force_eval's |cos(pi m/2)| weights for odd m."""
+        self._need("eof")
+        R, z, ph = (f64(a, self.device).reshape(-1) for a in (R, z, phi))
+        n = R.numel()
+        if z.numel() != n or ph.numel() != n:
+            raise ValueError("R, z, phi must have the same length")
+        cos, sin = f64(cos, self.device), f64(sin, self.device)
+        out = torch.empty((9, n), dtype=torch.float64, device=self.device)
+        flags = ((_lib.NO_ODD if no_odd else 0) | (_lib.EVAL_DENS if density else 0) |
+                 (_lib.COMPAT_EOF_MASK if compat_mask else 0))
+        rc = self.ctx.lib.bfe_eof_eval_cyl(self.ctx.h, ptr(R), ptr(z), ptr(ph), n, ptr(cos),
+                                           ptr(sin), int(mmax_use), int(nmax_use), flags,
+                                           ptr(out), stream_ptr(stream))
+        self.ctx.check(rc, "bfe_eof_eval_cyl")
         return out
 
     # ------------------------------------------------------------------ field

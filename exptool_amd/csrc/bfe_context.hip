@@ -66,6 +66,7 @@ static void free_sl(Context* c) {
   c->sl_lmc = nullptr;
   for (auto& f : c->slf) { free_ptr(f.d); f = SLFolded(); }
   c->sl_ok = false;
+  c->batch_nsnap = 0;   // batch fields were folded against these tables
 }
 
 static void free_eof(Context* c) {
@@ -76,6 +77,7 @@ static void free_eof(Context* c) {
   c->eof_dep32 = nullptr;
   for (auto& f : c->eoff) { free_ptr(f.d); f = EOFFolded(); }
   c->eof_ok = false;
+  c->batch_nsnap = 0;   // batch fields were folded against these tables
 }
 
 void timer_begin(Context* c, const char* name, hipStream_t s) {

@@ -300,6 +300,55 @@ __global__ __launch_bounds__(256) void eof_eval_kernel(
   }
 }
 
+// Point forms: the caller gives the coordinates the reference's point
+// evaluators take, in float64, and receives float64 rows of n values.
+// spheresl.force_eval / all_eval (spheresl.py:1107-1234, :987-1102) at
+// (r, cos theta, phi): out rows den0, den1 (l >= 1), pot0, pot1, potr, pott,
+// potp; densities are written only with DENS (zero rows otherwise).
+template <bool DENS>
+__global__ __launch_bounds__(256) void sl_eval_sph_kernel(
+    const double* __restrict__ pr, const double* __restrict__ pct, const double* __restrict__ pph,
+    int64_t n, SLGeom g, const float* __restrict__ F, int L, uint32_t flags, double densfac,
+    double* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double s1, c1;
+  sincos(pph[p], &s1, &c1);
+  SLOut o;
+  sl_eval_folded<DENS>(g, F, L, pr[p], pct[p], c1, s1, flags, o);
+  out[p] = DENS ? o.den0 * densfac : 0.0;
+  out[n + p] = DENS ? o.den1 * densfac : 0.0;
+  out[2 * n + p] = o.pot0;
+  out[3 * n + p] = o.pot1;
+  out[4 * n + p] = o.potr;
+  out[5 * n + p] = o.pott;
+  out[6 * n + p] = o.potp;
+}
+
+// eof.force_eval / accumulated_eval (eof.py:756-872, :874-929) at planar
+// radius R (as given), z, phi: out rows p0, p (m >= 1), fr, fp, fz, d0,
+// d (m >= 1), fr0, fz0 (the m = 0 parts of fr, fz).
+template <bool DENS>
+__global__ __launch_bounds__(256) void eof_eval_cyl_kernel(
+    const double* __restrict__ pR, const double* __restrict__ pz, const double* __restrict__ pph,
+    int64_t n, EOFGeom g, const float* __restrict__ F, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s1, c1;
+  sincos(pph[i], &s1, &c1);
+  EOFOut o;
+  eof_eval_folded<DENS>(g, F, pR[i], pz[i], c1, s1, o);
+  out[i] = o.p0;
+  out[n + i] = o.p;
+  out[2 * n + i] = o.fr;
+  out[3 * n + i] = o.fp;
+  out[4 * n + i] = o.fz;
+  out[5 * n + i] = DENS ? o.d0 : 0.0;
+  out[6 * n + i] = DENS ? o.d : 0.0;
+  out[7 * n + i] = o.fr0;
+  out[8 * n + i] = o.fz0;
+}
+
 // Fields.return_forces_cart at n points.  pts (may be NULL): the points in
 // table-cell order as float4 (x, y, z, input index bits), outputs scattered
 // back to input order; block b then takes logical block xcd_block(nb) so that
@@ -501,6 +550,66 @@ extern "C" int bfe_eof_eval(bfe_ctx* h, const float* x, const float* y, const fl
   else
     hipLaunchKernelGGL(eof_eval_kernel<false>, dim3(nblocks256(n)), dim3(256), 0, s, x, y, z, n,
                        eof_geom(c), f.d, p0, p, fr, fp, fz, d0, d);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
+extern "C" int bfe_sl_eval_sph(bfe_ctx* h, const double* r, const double* costh,
+                               const double* phi, int64_t n, const double* coef, int lmax_use,
+                               int nmax_use, uint32_t flags, double* out, void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (!c->sl_ok) return set_error(c, BFE_ENOTABLE, "sl_eval_sph: SL tables not set");
+  if (n < 0 || !coef) return set_error(c, BFE_EINVAL, "sl_eval_sph: bad n or NULL coef");
+  if (n > 0 && (!r || !costh || !phi || !out))
+    return set_error(c, BFE_EINVAL, "sl_eval_sph: NULL input or output");
+  BFE_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  const int ll = lmax_use < 0 ? c->lmax : std::min(lmax_use, c->lmax);
+  const int nn = nmax_use < 0 ? c->nmax : std::min(nmax_use, c->nmax);
+  int rc = sl_fold(c, 0, coef, 1.0, ll, nn, -(1 << 30), 1 << 30, (flags & BFE_NO_ODD) != 0, s);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  const SLFolded& f = c->slf[0];
+  const uint32_t ef = flags & (BFE_COMPAT_SL_EDGE | BFE_COMPAT_SL_COS_L);
+  if (flags & BFE_EVAL_DENS)
+    hipLaunchKernelGGL(sl_eval_sph_kernel<true>, dim3(nblocks256(n)), dim3(256), 0, s, r, costh,
+                       phi, n, sl_geom(c), f.d, c->lmax, ef, 0.25 / kPi, out);
+  else
+    hipLaunchKernelGGL(sl_eval_sph_kernel<false>, dim3(nblocks256(n)), dim3(256), 0, s, r, costh,
+                       phi, n, sl_geom(c), f.d, c->lmax, ef, 0.25 / kPi, out);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
+extern "C" int bfe_eof_eval_cyl(bfe_ctx* h, const double* R, const double* z, const double* phi,
+                                int64_t n, const double* cos_coef, const double* sin_coef,
+                                int mmax_use, int nmax_use, uint32_t flags, double* out,
+                                void* stream) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (!c->eof_ok) return set_error(c, BFE_ENOTABLE, "eof_eval_cyl: EOF tables not set");
+  if (n < 0 || !cos_coef || !sin_coef)
+    return set_error(c, BFE_EINVAL, "eof_eval_cyl: bad n or NULL coef");
+  if (n > 0 && (!R || !z || !phi || !out))
+    return set_error(c, BFE_EINVAL, "eof_eval_cyl: NULL input or output");
+  const bool dens = (flags & BFE_EVAL_DENS) != 0;
+  if (dens && !c->eof_dens) return set_error(c, BFE_ENOTABLE, "eof_eval_cyl: no density tables");
+  BFE_HIP(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  const int mm = mmax_use < 0 ? c->mmax : std::min(mmax_use, c->mmax);
+  const int nn = nmax_use < 0 ? c->norder : std::min(nmax_use, c->norder);
+  int rc = eof_fold(c, 0, cos_coef, sin_coef, mm, nn, 0, 1 << 30, (flags & BFE_NO_ODD) != 0,
+                    (flags & BFE_COMPAT_EOF_MASK) != 0, dens, s);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  const EOFFolded& f = c->eoff[0];
+  if (dens)
+    hipLaunchKernelGGL(eof_eval_cyl_kernel<true>, dim3(nblocks256(n)), dim3(256), 0, s, R, z, phi,
+                       n, eof_geom(c), f.d, out);
+  else
+    hipLaunchKernelGGL(eof_eval_cyl_kernel<false>, dim3(nblocks256(n)), dim3(256), 0, s, R, z,
+                       phi, n, eof_geom(c), f.d, out);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }
@@ -711,7 +820,11 @@ extern "C" int bfe_field_eval_batch(bfe_ctx* h, const float* x, const float* y, 
                                     float* fy, float* fz, float* pot, void* stream) {
   if (!h) return BFE_EINVAL;
   Context* c = &h->c;
+  if (!c->sl_ok || !c->eof_ok) return set_error(c, BFE_ENOTABLE, "field_eval_batch: tables not set");
   if (c->batch_nsnap < 1) return set_error(c, BFE_ENOTABLE, "field_eval_batch: no batch field (bfe_field_set_batch)");
+  if (c->batch_sl_floats < sl_fold_floats(c) * (size_t)c->batch_nsnap ||
+      c->batch_eof_floats < eof_fold_floats(c) * (size_t)c->batch_nsnap)
+    return set_error(c, BFE_ENOTABLE, "field_eval_batch: batch field folded for other tables (call bfe_field_set_batch again)");
   if (!offsets || nsnap != c->batch_nsnap)
     return set_error(c, BFE_EINVAL, "field_eval_batch: need offsets for the %d snapshots of the batch field",
                      c->batch_nsnap);

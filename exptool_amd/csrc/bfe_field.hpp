@@ -16,6 +16,7 @@ struct SLOut {
 
 struct EOFOut {
   double p0, p, fr, fp, fz, d0, d;
+  double fr0, fz0;   // the m = 0 parts of fr and fz (eof.force_eval perturb=True)
 };
 
 // Folded table layouts (float32, streamed as float4):
@@ -241,7 +242,7 @@ __device__ __forceinline__ void eof_eval_folded(const EOFGeom& g, const float* _
   const float4* __restrict__ C = A + (size_t)ny1 * ns4;                            // (ix+1, iy)
   const float4* __restrict__ D = C + ns4;                                          // (ix+1, iy+1)
   const double w00 = cell.c00, w10 = cell.c10, w01 = cell.c01, w11 = cell.c11;
-  double p0 = 0, fr = 0, fz = 0, d0 = 0, p = 0, fp = 0, d = 0;
+  double p0 = 0, fr = 0, fz = 0, d0 = 0, p = 0, fp = 0, d = 0, fr0 = 0, fz0 = 0;
   double cm = 1.0, sm = 0.0, cm1 = c1, sm1 = -s1;
 #pragma unroll MU
   for (int m = 0; m <= M; ++m) {
@@ -256,8 +257,8 @@ __device__ __forceinline__ void eof_eval_folded(const EOFGeom& g, const float* _
     const double zc = w00 * a.z + w10 * c.z + w01 * b.z + w11 * e.z;
     if (m == 0) {
       p0 = pc;
-      fr = rc;
-      fz = zc;
+      fr = fr0 = rc;
+      fz = fz0 = zc;
       if (DENS) d0 = w00 * a.w + w10 * c.w + w01 * b.w + w11 * e.w;
       continue;
     }
@@ -282,6 +283,8 @@ __device__ __forceinline__ void eof_eval_folded(const EOFGeom& g, const float* _
   o.fp = fp;
   o.d0 = d0;
   o.d = d;
+  o.fr0 = fr0;
+  o.fz0 = fz0;
 }
 
 // The SL radial cell field_eval_point locates for the point (x, y, z).

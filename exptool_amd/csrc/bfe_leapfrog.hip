@@ -346,6 +346,8 @@ static int leapfrog_run(bfe_ctx* h, float* x, float* y, float* z, float* vx, flo
   if (energy_every < 0 || record_every < 0) return set_error(c, BFE_EINVAL, "leapfrog: negative interval");
   if (traj && record_every == 0) return set_error(c, BFE_EINVAL, "leapfrog: traj needs record_every > 0");
   if (energy_log && energy_every == 0) return set_error(c, BFE_EINVAL, "leapfrog: energy_log needs energy_every > 0");
+  if (use_graph && stream == nullptr)
+    return set_error(c, BFE_EINVAL, "leapfrog: use_graph needs a non-default stream");
   FieldTables T;
   int rc = field_tables(c, 0.0, T);
   if (rc) return rc;
@@ -398,16 +400,18 @@ static int leapfrog_run(bfe_ctx* h, float* x, float* y, float* z, float* vx, flo
   }
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  if (use_graph) {
-    if (s == nullptr) return set_error(c, BFE_EINVAL, "leapfrog: use_graph needs a non-default stream");
-    BFE_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  }
+  if (use_graph) BFE_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   for (const LeapArgs& b : chunks) {
     // re-sort by the current cells between chunks (the sort allocates
     // nothing here: the buffer was sized by the first sort, before any capture)
     if (b.k0 > 0) {
       rc = order_points(c, T, x, y, z, n, s, &perm);
-      if (rc) return rc;
+      if (rc) {
+        // leave the caller's stream out of capture mode
+        if (use_graph && hipStreamEndCapture(s, &graph) == hipSuccess && graph)
+          (void)hipGraphDestroy(graph);
+        return rc;
+      }
     }
     const bool check = ebuf && b.energy_every > 0;
     if (dtv)

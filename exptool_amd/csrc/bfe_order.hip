@@ -7,9 +7,10 @@
 // work on one contiguous range of cells.  Accumulate sorts its particles the
 // same way so that runs of equal cells can be summed before the table reads.
 //
-// Key: the EOF (ix, iy) cell for points inside the disc grid, else
-// numx * numy + the SL radial cell (evaluation); the accumulate cell of the
-// particle (accumulate); plus snapshot * cells for concatenated snapshots.
+// Key (kind 1, evaluation and the SL accumulate): the SL radial cell of the
+// point times a number of cos(theta) bins, plus its bin (key_sl_theta);
+// (kind 0, the EOF accumulate): the EOF (ix, iy) cell; plus snapshot * cells
+// for concatenated snapshots.
 //
 // The sort is a stable least-significant-digit counting sort over 8-bit
 // digits, ceil(bits / 8) passes of three kernels:

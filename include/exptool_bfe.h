@@ -51,6 +51,7 @@ typedef struct bfe_ctx bfe_ctx;
                                       products (about 1e-5 relative), see bfe_sl_accumulate */
 #define BFE_COMPAT_ALL 0xFu
 #define BFE_COMPAT_RK4_FRAME 0x40u /* rk4_rotation pseudo-force signs, rk4.py:66-90 */
+#define BFE_EVAL_DENS 0x80u        /* point evaluators: also the density */
 
 /* ---- context ------------------------------------------------------------ */
 /* replaces: the Pool set-up of spheresl.compute_coefficients (spheresl.py:456)
@@ -114,6 +115,33 @@ int bfe_eof_eval(bfe_ctx* ctx, const float* x, const float* y, const float* z, i
                  const double* cos_coef, const double* sin_coef, int m1, int m2,
                  uint32_t flags, float* p0, float* p, float* fr, float* fp, float* fz,
                  float* d0, float* d, void* stream);
+
+/* ---- point evaluators (float64 coordinates in, float64 rows out) ------- */
+/* replaces: spheresl.force_eval (spheresl.py:1107-1234) and spheresl.all_eval
+ * (spheresl.py:987-1102) at n points given as the reference takes them:
+ * r (the radius as passed, no floor), cos theta and phi, DEVICE float64 [n]
+ * each.  coef: device ((lmax+1)^2, nmax); l > lmax_use and n >= nmax_use are
+ * dropped (< 0: all).  out: device float64 [7][n] = den0, den1 (the l >= 1
+ * density), pot0, pot1, potr (dPhi/dr), pott (dPhi/dcos theta), potp
+ * (dPhi/dphi); densities times 0.25/pi (all_eval, :1092), written only with
+ * BFE_EVAL_DENS (zero rows otherwise).  flags: BFE_NO_ODD, BFE_EVAL_DENS,
+ * BFE_COMPAT_SL_EDGE (the stencil get_halo_dens_pot_force uses, :150-153),
+ * BFE_COMPAT_SL_COS_L (force_eval's cos/sin(l phi), :1222-1225). */
+int bfe_sl_eval_sph(bfe_ctx* ctx, const double* r, const double* costh, const double* phi,
+                    int64_t n, const double* coef, int lmax_use, int nmax_use, uint32_t flags,
+                    double* out, void* stream);
+/* replaces: eof.force_eval (eof.py:756-872) and eof.accumulated_eval
+ * (eof.py:874-929) at n points given as the reference takes them: planar
+ * radius R (as passed), z and phi, DEVICE float64 [n] each.  Orders
+ * m > mmax_use and n >= nmax_use are dropped (< 0: all).  out: device float64
+ * [9][n] = p0, p (m >= 1), fr, fp (= -dPhi/dphi), fz, d0, d (m >= 1), fr0,
+ * fz0 (the m = 0 parts of fr and fz).  d0/d need BFE_EVAL_DENS and density
+ * tables (zero rows otherwise).  flags: BFE_NO_ODD (odd m dropped),
+ * BFE_COMPAT_EOF_MASK (with BFE_NO_ODD: force_eval's |cos(pi m/2)| weights,
+ * eof.py:833), BFE_EVAL_DENS. */
+int bfe_eof_eval_cyl(bfe_ctx* ctx, const double* R, const double* z, const double* phi,
+                     int64_t n, const double* cos_coef, const double* sin_coef, int mmax_use,
+                     int nmax_use, uint32_t flags, double* out, void* stream);
 
 /* ---- frozen disc+halo field -------------------------------------------- */
 /* replaces: Fields.prep_tables + set_field_parameters (potential.py:257-294,

@@ -343,6 +343,38 @@ def sl_all_eval_particles(t: SLTable, x, y, z, expcoef, L1=-1000, L2=1000,
     return den0 * densfac, den1 * densfac, pot0, pot1, potr, pott, potp, rr
 
 
+def sl_truncation(t: SLTable, lmax, nmax):
+    """Orders used when force_eval / all_eval are asked for lmax, nmax
+    (spheresl.py:1134-1147, :1009-1022): l is cut when lmax < the table's
+    lmax, n when nmax < the table's nmax - 1.  The reference fails on the
+    other requests (nmax = table nmax - 1 leaves expcoef and the radial
+    matrices of different lengths, and all_eval keeps lmax + 1 instead of
+    (lmax + 1)^2 rows, :1017); here they mean the full table.
+    Returns (lmax_use, nmax_use)."""
+    lmax = t.lmax if lmax is None else int(lmax)
+    nmax = t.nmax if nmax is None else int(nmax)
+    return min(lmax, t.lmax), (nmax if nmax < t.nmax - 1 else t.nmax)
+
+
+def sl_all_eval(t: SLTable, r, costh, phi, expcoef, lmax=None, nmax=None, no_odd=False,
+                edge_compat=True):
+    """all_eval, spheresl.py:987-1102, vectorised over points: the expansion
+    at (r, cos theta, phi) with density.  den1 is the total density (seeded
+    with the l = 0 term, :1046), the azimuthal factor is cos/sin(m phi) and
+    densfac = 0.25/pi (:1092).  get_halo_dens_pot_force's first-cell stencil
+    (:150-153) is what the reference evaluates (edge_compat=True).
+    Returns (den0, den1, pot0, pot1, potr, pott, potp)."""
+    lmax_use, nmax_use = sl_truncation(t, lmax, nmax)
+    r = np.atleast_1d(np.asarray(r, dtype=np.float64))
+    costh = np.atleast_1d(np.asarray(costh, dtype=np.float64))
+    phi = np.atleast_1d(np.asarray(phi, dtype=np.float64))
+    den0, den1, pot0, pot1, potr, pott, potp = _sl_eval_core(
+        t, r, costh, phi, np.asarray(expcoef, dtype=np.float64), lmax_use, nmax_use,
+        None, no_odd, False, False, edge_compat, True)
+    densfac = 0.25 / np.pi
+    return den0 * densfac, (den0 + den1) * densfac, pot0, pot1, potr, pott, potp
+
+
 def sl_force_eval(t: SLTable, r, costh, phi, expcoef, lmax=None, nmax=None,
                   no_odd=False, compat=False, edge_compat=None):
     """force_eval, spheresl.py:1107-1234, vectorised over points.  Truncates to
@@ -498,29 +530,27 @@ def eof_force_eval(t: EOFTable, r, z, phi, accum_cos, accum_sin, MMAX=None, NMAX
     return fr + fr0, fp, fz + fz0, p + p0, p0
 
 
-def eof_eval_particles(t: EOFTable, x, y, z, accum_cos, accum_sin, m1=0, m2=1000,
-                       density=False):
-    """accumulated_eval_particles, eof.py:989-1144, intended semantics (the
-    reference reads Particles.data and Particles.zpos together, :1055 vs
-    :1079, so it cannot run).  R = sqrt(x^2+y^2+1e-10), phi = atan2(y,x);
-    for m in [m1, m2]: p, fr, fz, fp as eof.force_eval; the m=0 potential is
-    returned as p0 and removed from p (:1129-1138).
-    Returns (p0, p, fr, fp, fz, R) or, with density, (p0, p, d0, d, fr, fp, fz, R)."""
-    x, y, z = (np.asarray(a, dtype=np.float64) for a in (x, y, z))
-    R = (x * x + y * y + 1.0e-10) ** 0.5
-    phi = np.arctan2(y, x)
+def _eof_eval_core(t: EOFTable, R, z, phi, accum_cos, accum_sin, mlist, density):
+    """Per-m sums shared by accumulated_eval (eof.py:874-929) and
+    accumulated_eval_particles (eof.py:1092-1138) at planar radius R, height
+    z, angle phi, over the orders in mlist: for each m, the bilinear table
+    values times the coefficients summed over all norder orders; p, fr, fz
+    (and d) take cos(m phi) C + sin(m phi) S, fp = m (sin(m phi) C potC -
+    cos(m phi) S potS).  Returns (p0, p, fr, fp, fz, d0, d) with p0/d0 the
+    m = 0 potential/density and p/d the m >= 1 part only."""
+    R = np.atleast_1d(np.asarray(R, dtype=np.float64))
+    z = np.atleast_1d(np.asarray(z, dtype=np.float64))
+    phi = np.atleast_1d(np.asarray(phi, dtype=np.float64))
     X, Y, ix, iy = return_bins(R, z, t.XMIN, t.dX, t.YMIN, t.dY, t.numx, t.numy,
                                t.ascale, t.hscale, t.cmap)
     c = _bilinear_weights(X, Y, ix, iy)
-    npt = x.size
+    npt = R.size
     p0 = np.zeros(npt); p = np.zeros(npt); fr = np.zeros(npt); fz = np.zeros(npt)
     fp = np.zeros(npt); d0 = np.zeros(npt); d = np.zeros(npt)
     ac = np.asarray(accum_cos, dtype=np.float64)
     as_ = np.asarray(accum_sin, dtype=np.float64)
     N = t.norder
-    for mm in range(0, t.mmax + 1):
-        if mm > m2 or mm < m1:
-            continue
+    for mm in mlist:
         cm, sm = np.cos(phi * mm), np.sin(phi * mm)
         sl = (slice(mm, mm + 1), slice(0, N))
         vc = _interp(t.potC[sl], ix, iy, c)[:, 0] @ ac[mm, :N]
@@ -547,6 +577,36 @@ def eof_eval_particles(t: EOFTable, x, y, z, accum_cos, accum_sin, m1=0, m2=1000
         else:
             p += pm
             d += dm
+    return p0, p, fr, fp, fz, d0, d
+
+
+def eof_accumulated_eval(t: EOFTable, r, z, phi, accum_cos, accum_sin, MMAX=None,
+                         no_odd=False):
+    """accumulated_eval, eof.py:874-929, vectorised over points (r is the
+    planar radius as passed by the caller).  Orders m = 0..MMAX, odd m skipped
+    with no_odd (:891-892); every order n of the tables is used (NMAX has no
+    effect there).  Returns (p0, p, fr, fp, fz, d0, d) with p and d the
+    totals over m (p0, d0: the m = 0 terms, :926-928)."""
+    MMAX = t.mmax if MMAX is None else int(MMAX)
+    ms = [m for m in range(0, MMAX + 1) if not (no_odd and m % 2)]
+    p0, p, fr, fp, fz, d0, d = _eof_eval_core(t, r, z, phi, accum_cos, accum_sin, ms, True)
+    return p0, p0 + p, fr, fp, fz, d0, d0 + d
+
+
+def eof_eval_particles(t: EOFTable, x, y, z, accum_cos, accum_sin, m1=0, m2=1000,
+                       density=False):
+    """accumulated_eval_particles, eof.py:989-1144, intended semantics (the
+    reference reads Particles.data and Particles.zpos together, :1055 vs
+    :1079, so it cannot run).  R = sqrt(x^2+y^2+1e-10) (:1070), phi =
+    atan2(y,x); for m in [m1, m2]: the per-m sums of accumulated_eval
+    (_eof_eval_core); the m=0 potential is returned as p0 and removed from p
+    (:1129-1138).
+    Returns (p0, p, fr, fp, fz, R) or, with density, (p0, p, d0, d, fr, fp, fz, R)."""
+    x, y, z = (np.asarray(a, dtype=np.float64) for a in (x, y, z))
+    R = (x * x + y * y + 1.0e-10) ** 0.5
+    phi = np.arctan2(y, x)
+    ms = [m for m in range(0, t.mmax + 1) if m1 <= m <= m2]
+    p0, p, fr, fp, fz, d0, d = _eof_eval_core(t, R, z, phi, accum_cos, accum_sin, ms, density)
     if density:
         return p0, p, d0, d, fr, fp, fz, R
     return p0, p, fr, fp, fz, R

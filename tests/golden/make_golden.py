@@ -158,6 +158,25 @@ def main():
     out["sl_forceeval"] = np.array(fe)
     out["sl_forceeval_l2n3"] = np.array(fe_tr)
     out["sl_forceeval_noodd"] = np.array(fe_no)
+    # all_eval over the same points: den1 seeded with the l = 0 term,
+    # densfac 0.25/pi, cos(m phi)
+    ae, ae_tr, ae_no = [], [], []
+    with np.errstate(all="ignore"):
+        for i in range(npt):
+            r3 = np.sqrt(x[i] ** 2 + y[i] ** 2 + z[i] ** 2) + 1e-15
+            costh = z[i] / r3
+            phi = np.arctan2(y[i], x[i])
+            ae.append(R_sl.all_eval(r3, costh, phi, coef, xi, p0, d0, cmap, scale, lmax, nmax,
+                                    ev, ef))
+            # lmax < table lmax keeps (lmax+1) rows instead of (lmax+1)^2 there
+            # (spheresl.py:1017) and fails, so only nmax is truncated
+            ae_tr.append(R_sl.all_eval(r3, costh, phi, coef, xi, p0, d0, cmap, scale, lmax, 3,
+                                       ev, ef))
+            ae_no.append(R_sl.all_eval(r3, costh, phi, coef, xi, p0, d0, cmap, scale, lmax, nmax,
+                                       ev, ef, no_odd=True))
+    out["sl_alleval_pt"] = np.array([[float(v) for v in t] for t in ae])
+    out["sl_alleval_pt_n3"] = np.array([[float(v) for v in t] for t in ae_tr])
+    out["sl_alleval_pt_noodd"] = np.array([[float(v) for v in t] for t in ae_no])
 
     # ---------------- EOF bins / accumulate / evaluate --------------------
     potC, rforceC, zforceC, densC, potS, rforceS, zforceS, densS = eofs
@@ -185,6 +204,29 @@ def main():
     out["eof_forceeval"] = np.array([[float(v) for v in t] for t in fev])
     out["eof_forceeval_noodd"] = np.array([[float(v) for v in t] for t in fev_no])
     out["eof_forceeval_m1n2"] = np.array([[float(v) for v in t] for t in fev_tr])
+    # accumulated_eval (point form with density) at the radius force_eval
+    # gets from Fields, and at the one accumulated_eval_particles computes
+    # (R = sqrt(x^2 + y^2 + 1e-10), eof.py:1070)
+    aev, aev_no, aev_m1, aev_part = [], [], [], []
+    for i in range(npt):
+        r2 = (x[i] ** 2 + y[i] ** 2) ** 0.5 + 1e-15
+        rp = (x[i] ** 2 + y[i] ** 2 + 1e-10) ** 0.5
+        phi = np.arctan2(y[i], x[i])
+        kw = dict(rmin=XMIN, dR=dX, zmin=YMIN, dZ=dY, numx=numx, numy=numy, ASCALE=ascale,
+                  HSCALE=hscale, CMAP=ecmap)
+        tabs = (potC, rforceC, zforceC, densC, potS, rforceS, zforceS, densS)
+        aev.append(R_eof.accumulated_eval(r2, z[i], phi, ac, as_, *tabs, MMAX=mmax, NMAX=norder,
+                                          **kw))
+        aev_no.append(R_eof.accumulated_eval(r2, z[i], phi, ac, as_, *tabs, MMAX=mmax,
+                                             NMAX=norder, no_odd=True, **kw))
+        aev_m1.append(R_eof.accumulated_eval(r2, z[i], phi, ac, as_, *tabs, MMAX=1, NMAX=norder,
+                                             **kw))
+        aev_part.append(R_eof.accumulated_eval(rp, z[i], phi, ac, as_, *tabs, MMAX=mmax,
+                                               NMAX=norder, **kw))
+    out["eof_acceval"] = np.array([[float(v) for v in t] for t in aev])
+    out["eof_acceval_noodd"] = np.array([[float(v) for v in t] for t in aev_no])
+    out["eof_acceval_m1"] = np.array([[float(v) for v in t] for t in aev_m1])
+    out["eof_acceval_part"] = np.array([[float(v) for v in t] for t in aev_part])
 
     # ---------------- Fields.return_forces_cart + leapfrog ----------------
     try:

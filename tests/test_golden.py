@@ -183,3 +183,49 @@ def test_reader_parity(golden, tmp_path):
         for k, a in zip(("potC", "rforceC", "zforceC", "densC", "potS", "rforceS", "zforceS", "densS"), arrs):
             assert np.array_equal(a, g["eof_" + k]), (f, k)
         assert np.allclose(E.eof_params(tmp_path / f), g["eof_params"])
+
+
+@pytest.mark.parametrize("key,kw", [("sl_alleval_pt", {}), ("sl_alleval_pt_n3", dict(nmax=3)),
+                                    ("sl_alleval_pt_noodd", dict(no_odd=True))])
+def test_sl_all_eval_point(golden, key, kw):
+    """spheresl.all_eval (den1 seeded with l = 0, densfac 0.25/pi, cos m phi)."""
+    t = sl_table(golden)
+    x, y, z = (golden[k][:120] for k in ("px", "py", "pz"))
+    r3 = np.sqrt(x * x + y * y + z * z) + 1e-15
+    with np.errstate(all="ignore"):
+        got = O.sl_all_eval(t, r3, z / r3, np.arctan2(y, x), golden["sl_coef"], **kw)
+    ref = golden[key]
+    for j in range(7):
+        assert rel(got[j], ref[:, j]) < 1e-11, j
+
+
+@pytest.mark.parametrize("key,kw", [("eof_acceval", {}), ("eof_acceval_noodd", dict(no_odd=True)),
+                                    ("eof_acceval_m1", dict(MMAX=1))])
+def test_eof_accumulated_eval(golden, key, kw):
+    """eof.accumulated_eval: (p0, p, fr, fp, fz, d0, d) with density."""
+    t = eof_table(golden)
+    x, y, z = (golden[k][:120] for k in ("px", "py", "pz"))
+    r2 = (x * x + y * y) ** 0.5 + 1e-15
+    got = O.eof_accumulated_eval(t, r2, z, np.arctan2(y, x), golden["eof_cos"],
+                                 golden["eof_sin"], **kw)
+    for j in range(7):
+        assert rel(got[j], golden[key][:, j]) < 1e-12, j
+
+
+def test_eof_eval_particles_pinned(golden):
+    """Every output of the batch evaluator the GPU EOF evaluator is compared
+    with: against accumulated_eval at the radius accumulated_eval_particles
+    computes (sqrt(x^2 + y^2 + 1e-10)), and its forces against force_eval."""
+    t = eof_table(golden)
+    x, y, z = (golden[k][:120] for k in ("px", "py", "pz"))
+    p0, p, d0, d, fr, fp, fz, R = O.eof_eval_particles(t, x, y, z, golden["eof_cos"],
+                                                       golden["eof_sin"], density=True)
+    ref = golden["eof_acceval_part"]
+    for j, v in enumerate((p0, p0 + p, fr, fp, fz, d0, d0 + d)):
+        assert rel(v, ref[:, j]) < 1e-12, j
+    assert rel(p, ref[:, 1] - ref[:, 0]) < 1e-10
+    assert rel(d, ref[:, 6] - ref[:, 5]) < 1e-10
+    # the two reference point evaluators agree where they overlap (same r)
+    a, f = golden["eof_acceval"], golden["eof_forceeval"]
+    for ja, jf in ((2, 0), (3, 1), (4, 2), (1, 3), (0, 4)):
+        assert rel(a[:, ja], f[:, jf]) < 1e-12, (ja, jf)

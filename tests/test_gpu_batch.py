@@ -135,3 +135,30 @@ def test_autotune_applies_choice_within_tolerance(sysm, tmp_path, monkeypatch):
     assert again[key]["eof_acc_blocks"] == rec["eof_acc_blocks"]      # from the cache
     assert s.get_tuning("eof_acc_blocks") == rec["eof_acc_blocks"]
     s.set_tuning("eof_acc_blocks", 2048)
+
+
+def test_batch_field_invalidated_by_new_tables():
+    """Table uploads drop the folded batch fields: evaluating them afterwards
+    is an error (no launch), not a read of fields folded for other tables."""
+    from exptool_amd._lib import BFEError
+    from exptool_amd.basis import tablegen as tg
+    from exptool_amd.bfe import BFESystem
+    from exptool_amd.utils import samplers as S
+    slt = tg.make_sl_tables(2, 4, 200, 1e-3, 50.0, 1, 1.0)
+    d = S.exponential_disc(5000, seed=1)
+    et = tg.make_eof_tables(d, mmax=2, norder=4, numx=16, numy=8, lmax_s=6, nmax_s=4)
+    s = BFESystem(sl=slt, eof=et)
+    counts = [10, 20]
+    c = np.zeros((2, 3, 4)); h = np.zeros((2, 9, 4))
+    s.set_field_batch(c, c, h)
+    x = np.ones(30)
+    s.field_forces_batch(x, x, x, counts)
+    big = tg.make_sl_tables(3, 6, 400, 1e-3, 50.0, 1, 1.0)
+    s.set_sl(big)
+    with pytest.raises(BFEError, match="no batch field"):
+        s.field_forces_batch(x, x, x, counts)
+    s.set_sl(slt)
+    s.set_field_batch(c, c, h)
+    s.set_eof(et)
+    with pytest.raises(BFEError, match="no batch field"):
+        s.field_forces_batch(x, x, x, counts)

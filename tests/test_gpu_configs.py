@@ -1,0 +1,228 @@
+"""The HIP path on the benchmark configurations the other GPU tests do not
+cover, against the FP64 oracle:
+
+* the plain spherical case: SL lmax 4, nmax 10 on 2000 log-spaced radii
+  (cmap 2, xi = ln r on [1e-4, 50]) and 100k particles of a Hernquist
+  sphere (M = 1, a = 1, r < 20, seed 0): coefficients <= 1e-6 per l block,
+  potential and forces <= 1e-5 per particle, and the Hernquist potential as
+  a known answer on the device;
+* the identity radial map (cmap 0) for SL and EOF tables: accumulate,
+  evaluation and the Cartesian field;
+* orbit integration in the frozen disc + halo field of the benchmark's
+  bases (EOF mmax 6 norder 18 on 256 x 128, SL lmax 6 nmax 20 numr 2000):
+  1000 tracers drawn as the benchmark draws its orbits, 200 leapfrog steps
+  of dt = 1e-3 against the oracle's trajectories (<= 1e-4), and the full
+  1M orbits x 10k steps run, whose energy drift is checked against bounds
+  and against its dt^2 scaling.
+"""
+import numpy as np
+import pytest
+
+from oracle import bfe_oracle as O
+from tests.test_gpu_parity import block_rel_eof, block_rel_sl, relmax
+
+pytestmark = pytest.mark.gpu
+
+COEF_TOL = 1e-6
+FORCE_TOL = 1e-5
+
+
+def _f32(*a):
+    return [np.asarray(v, np.float32).astype(np.float64) for v in a]
+
+
+def _slo(t):
+    return O.SLTable(t["lmax"], t["nmax"], t["numr"], t["cmap"], t["rmin"], t["rmax"], t["scale"],
+                     t["ev"], t["ef"], t["xi"], t["p0"], t["d0"])
+
+
+def _eo(t):
+    return O.EOFTable(t["mmax"], t["norder"], t["numx"], t["numy"], t["cmap"], t["rmin"],
+                      t["rmax"], t["ascale"], t["hscale"], t["potC"], t["rforceC"], t["zforceC"],
+                      t["potS"], t["rforceS"], t["zforceS"], t["densC"], t["densS"])
+
+
+def _sph_forces(r, ct, phi, potr, pott, potp):
+    """Cartesian F = -grad Phi from dPhi/dr, dPhi/dcos(theta), dPhi/dphi."""
+    st = np.sqrt(np.clip(1.0 - ct * ct, 0.0, None))
+    fr = -potr
+    fth = st * pott / r                       # -(1/r) dPhi/dtheta, dcos/dtheta = -sin
+    with np.errstate(divide="ignore", invalid="ignore"):
+        fph = np.where(st > 0, -potp / (r * st), 0.0)
+    cp, sp = np.cos(phi), np.sin(phi)
+    fx = fr * st * cp + fth * ct * cp - fph * sp
+    fy = fr * st * sp + fth * ct * sp + fph * cp
+    fz = fr * ct - fth * st
+    return np.stack([fx, fy, fz])
+
+
+# ---------------------------------------------------------------- cmap = 2
+@pytest.fixture(scope="module")
+def sphere():
+    from exptool_amd.basis import tablegen as tg
+    from exptool_amd.bfe import BFESystem
+    from exptool_amd.utils import samplers as S
+    t = tg.make_sl_tables(lmax=4, nmax=10, numr=2000, rmin=1e-4, rmax=50.0, cmap=2, scale=1.0)
+    x, y, z, m = S.hernquist_sphere(100000, a=1.0, rtrunc=20.0, mass=1.0, seed=0)
+    return BFESystem(sl=t), _slo(t), _f32(x, y, z, m)
+
+
+def test_spherical_config_coefficients(sphere):
+    print("spherical config (cmap 2, lmax 4 nmax 10, 100k Hernquist): accumulate", flush=True)
+    s, slo, (x, y, z, m) = sphere
+    got = s.sl_accumulate(x, y, z, m).cpu().numpy()
+    ref = O.sl_accumulate(slo, x, y, z, m)
+    assert block_rel_sl(got, ref, 4) <= COEF_TOL
+    got2 = s.sl_accumulate(x, y, z, m, no_odd=True).cpu().numpy()
+    assert block_rel_sl(got2, O.sl_accumulate(slo, x, y, z, m, no_odd=True), 4) <= COEF_TOL
+
+
+def test_spherical_config_forces(sphere):
+    """Phi and F at all 100k particles, per particle |dF| / |F| <= 1e-5."""
+    print("spherical config: evaluation at 100k particles", flush=True)
+    s, slo, (x, y, z, m) = sphere
+    coef = O.sl_accumulate(slo, x, y, z, m)
+    g = {k: v.double().cpu().numpy() for k, v in s.sl_eval(x, y, z, coef).items()}
+    den0, den1, pot0, pot1, potr, pott, potp, _ = O.sl_all_eval_particles(slo, x, y, z, coef)
+    r = np.sqrt(x * x + y * y + z * z)
+    ct, phi = z / r, np.arctan2(y, x)
+    Fg = _sph_forces(r, ct, phi, g["potr"], g["pott"], g["potp"])
+    Fr = _sph_forces(r, ct, phi, potr, pott, potp)
+    e = np.linalg.norm(Fg - Fr, axis=0) / np.linalg.norm(Fr, axis=0)
+    assert np.max(e) <= FORCE_TOL, np.max(e)
+    pg, pr = g["pot0"] + g["pot1"], pot0 + pot1
+    assert np.max(np.abs(pg - pr) / np.abs(pr)) <= FORCE_TOL
+    assert relmax(g["den0"], den0) <= FORCE_TOL
+    assert relmax(g["den1"], den1) <= FORCE_TOL
+
+
+def test_spherical_config_hernquist_known_answer(sphere):
+    """Device potential and radial force of the truncated Hernquist sphere
+    against -M/(r+a) + M a/(R+a)^2 and M/(r+a)^2 (M the untruncated mass)."""
+    s, slo, (x, y, z, m) = sphere
+    coef = s.sl_accumulate(x, y, z, m).cpu().numpy()
+    r = np.linspace(0.2, 5.0, 25)
+    o = s.sl_eval_sph(r, np.zeros_like(r), np.zeros_like(r), coef, density=True).cpu().numpy()
+    Rt = 20.0
+    Mf = ((Rt + 1.0) / Rt) ** 2
+    phi_exact = -Mf / (r + 1.0) + Mf / (Rt + 1.0) ** 2
+    assert np.max(np.abs(o[2] + o[3] - phi_exact) / np.abs(phi_exact)) < 0.02
+    assert np.max(np.abs(o[4] - Mf / (r + 1.0) ** 2) / (Mf / (r + 1.0) ** 2)) < 0.03
+    rho = Mf / (2 * np.pi * r * (r + 1.0) ** 3)
+    assert np.max(np.abs(o[0] - rho) / rho) < 0.05
+
+
+# ---------------------------------------------------------------- cmap = 0
+@pytest.fixture(scope="module")
+def identity_map():
+    from exptool_amd.basis import tablegen as tg
+    from exptool_amd.bfe import BFESystem
+    from exptool_amd.utils import samplers as S
+    sl = tg.make_sl_tables(lmax=3, nmax=6, numr=800, rmin=0.0, rmax=60.0, cmap=0, scale=5.0,
+                           model_mass=10.0)
+    d = S.exponential_disc(30000, seed=7)
+    et = tg.make_eof_tables(d, mmax=3, norder=6, numx=64, numy=32, rmin=0.001, rmax=20.0,
+                            ascale=1.0, hscale=0.1, cmap=0, lmax_s=10, nmax_s=8)
+    return BFESystem(sl=sl, eof=et), _slo(sl), _eo(et)
+
+
+def test_identity_map_accumulate_and_eval(identity_map):
+    print("identity radial map (cmap 0): SL and EOF accumulate and evaluation", flush=True)
+    from exptool_amd.utils import samplers as S
+    s, slo, eo = identity_map
+    dx, dy, dz, dm = _f32(*S.exponential_disc(80000, seed=8))
+    hx, hy, hz, hm = _f32(*S.hernquist_sphere(80000, a=5.0, rtrunc=50.0, mass=10.0, seed=9))
+    c, sn = (v.cpu().numpy() for v in s.eof_accumulate(dx, dy, dz, dm))
+    rc, rs = O.eof_accumulate(eo, dx, dy, dz, dm)
+    assert block_rel_eof(c, rc) <= COEF_TOL and block_rel_eof(sn, rs) <= COEF_TOL
+    h = s.sl_accumulate(hx, hy, hz, hm).cpu().numpy()
+    hr = O.sl_accumulate(slo, hx, hy, hz, hm)
+    assert block_rel_sl(h, hr, 3) <= COEF_TOL
+    k = 5000
+    ge = {q: v.double().cpu().numpy() for q, v in
+          s.eof_eval(dx[:k], dy[:k], dz[:k], rc, rs, density=True).items()}
+    re = O.eof_eval_particles(eo, dx[:k], dy[:k], dz[:k], rc, rs, density=True)
+    for j, q in enumerate(("p0", "p", "d0", "d", "fr", "fp", "fz")):
+        assert relmax(ge[q], re[j]) <= FORCE_TOL, q
+    gs = {q: v.double().cpu().numpy() for q, v in s.sl_eval(hx[:k], hy[:k], hz[:k], hr).items()}
+    rsl = O.sl_all_eval_particles(slo, hx[:k], hy[:k], hz[:k], hr)
+    for j, q in enumerate(("den0", "den1", "pot0", "pot1", "potr", "pott", "potp")):
+        assert relmax(gs[q], rsl[j]) <= FORCE_TOL, q
+    # the Cartesian field (cell-ordered evaluation from 65536 points on)
+    s.set_field(rc, rs, hr)
+    x, y, z = (np.concatenate(p) for p in ((dx, hx), (dy, hy), (dz, hz)))
+    got = [v.cpu().numpy().astype(np.float64) for v in s.field_forces(x, y, z)]
+    F = O.FieldSpec(eo, rc, rs, slo, hr)
+    sub = np.random.default_rng(0).choice(x.size, 6000, replace=False)
+    ref = O.total_force(F, x[sub], y[sub], z[sub])
+    Fg = np.stack([got[j][sub] for j in range(3)])
+    Fr = np.stack(ref[:3])
+    e = np.linalg.norm(Fg - Fr, axis=0) / np.linalg.norm(Fr, axis=0)
+    assert np.max(e) <= FORCE_TOL, np.max(e)
+    assert relmax(got[3][sub], ref[3]) <= FORCE_TOL
+
+
+# ------------------------------------------------------- orbit integration
+@pytest.fixture(scope="module")
+def frozen():
+    """The benchmark's field: disc + halo bases of the workload, coefficients
+    of 1M + 1M particles (accumulated on the device)."""
+    import torch
+    from exptool_amd.bfe import BFESystem
+    from exptool_amd.utils import samplers as S
+    from exptool_amd.utils import workloads as W
+    sl_t, eof_t = W.disc_halo_tables(seed=1)
+    s = BFESystem(sl=sl_t, eof=eof_t)
+    d, h = W.disc_halo_particles(1_000_000, 1_000_000, seed=0)
+    c, sn, hc = (v.cpu().numpy() for v in s.accumulate_disc_halo(d, h))
+    s.set_field(c, sn, hc)
+    F = O.FieldSpec(_eo(eof_t), c, sn, _slo(sl_t), hc)
+
+    def tracers(n, seed):
+        ox, oy, oz, _ = S.exponential_disc(n, seed=seed, **W.DISC)
+        R = np.sqrt(ox * ox + oy * oy)
+        fx = s.field_forces(R, np.zeros_like(R), np.zeros_like(R))[0].cpu().numpy()
+        vx, vy, vz = S.circular_orbits(ox, oy, oz, fx.astype(np.float64), 0.1, seed=seed + 1)
+        return [np.asarray(v, np.float32) for v in (ox, oy, oz, vx, vy, vz)]
+
+    torch.cuda.synchronize()
+    return s, F, tracers
+
+
+def test_orbits_in_frozen_field_match_oracle(frozen):
+    print("orbits: 1000 tracers x 200 steps against the oracle", flush=True)
+    import torch
+    s, F, tracers = frozen
+    st0 = tracers(1000, 5)
+    st = [torch.tensor(v, device="cuda") for v in st0]
+    out = s.leapfrog(st, dt=1e-3, nsteps=199, energy_every=0, record_every=1)
+    traj = out["traj"].cpu().numpy()        # (200, 7, n)
+    init = np.stack([v.astype(np.float64) for v in st0], axis=1)
+    ref = O.leapfrog(F, 200, 1e-3, init[:, :3], init[:, 3:])
+    for j, k in enumerate(("X", "Y", "Z", "VX", "VY", "VZ", "P")):
+        assert relmax(traj[:, j, :].T, ref[k]) <= 1e-4, k
+
+
+def test_orbit_energy_drift_full_size(frozen):
+    """1M orbits x 10k steps (dt = 1e-3, float64 energy every 100 steps):
+    median drift <= 1e-4 and max <= 1e-3.  The drift is the integrator's:
+    the orbits with the largest drift, run again at dt/2, drift about four
+    times less (second-order leapfrog), so the frozen field adds no secular
+    energy error of its own at this level."""
+    print("orbits: 1M x 10k steps, energy drift", flush=True)
+    import torch
+    s, F, tracers = frozen
+    st0 = tracers(1_000_000, 5)
+    st = [torch.tensor(v, device="cuda") for v in st0]
+    out = s.leapfrog(st, dt=1e-3, nsteps=10_000, energy_every=100)
+    dr = out["drift"].cpu().numpy()
+    assert np.all(np.isfinite(dr))
+    med, p99, mx = np.median(dr), np.quantile(dr, 0.99), np.max(dr)
+    print(f"drift median {med:.3e} p99 {p99:.3e} max {mx:.3e}", flush=True)
+    assert med <= 1e-4 and mx <= 1e-3
+    worst = np.argsort(dr)[-2000:]
+    sub = [torch.tensor(v[worst], device="cuda") for v in st0]
+    half = s.leapfrog(sub, dt=5e-4, nsteps=20_000, energy_every=200)["drift"].cpu().numpy()
+    ratio = np.median(dr[worst] / half)
+    print(f"worst 2000 orbits: drift ratio dt / (dt/2) = {ratio:.2f}", flush=True)
+    assert 3.0 <= ratio <= 5.0

@@ -84,7 +84,85 @@ def test_spheresl_eval_and_force_eval(files, golden):
         out = spheresl.force_eval(r3, z / r3, np.arctan2(y, x), c, s.xi, s.p0, s.d0, s.cmap, s.scale,
                                   evtable=s.evtable, eftable=s.eftable, compat=True, **kw)
         for j in range(5):
-            assert relmax(out[j], golden[key][:, j]) <= 1e-4, (key, j)
+            # float64 points in, float32 folded tables
+            assert relmax(out[j], golden[key][:, j]) <= 1e-5, (key, j)
+
+
+def test_spheresl_all_eval_point(golden):
+    """spheresl.all_eval (den1 seeded with l = 0, 0.25/pi, cos m phi) against
+    the reference's own vectors; a per-point loop of scalar calls reuses one
+    device table set and equals the array call bit for bit."""
+    from exptool_amd import bfe
+    from exptool_amd.basis import spheresl
+    s = sl_table(golden)
+    c = golden["sl_coef"]
+    x, y, z = (golden[k][:120] for k in ("px", "py", "pz"))
+    r3 = np.sqrt(x * x + y * y + z * z) + 1e-15
+    ct, ph = z / r3, np.arctan2(y, x)
+    for key, kw in (("sl_alleval_pt", {}), ("sl_alleval_pt_n3", dict(nmax=3)),
+                    ("sl_alleval_pt_noodd", dict(no_odd=True))):
+        kw = dict(dict(lmax=s.lmax, nmax=s.nmax), **kw)
+        out = spheresl.all_eval(r3, ct, ph, c, s.xi, s.p0, s.d0, s.cmap, s.scale,
+                                evtable=s.evtable, eftable=s.eftable, compat=True, **kw)
+        with np.errstate(all="ignore"):
+            ref = O.sl_all_eval(s, r3, ct, ph, c, **kw)
+        for j in range(7):
+            assert relmax(out[j], golden[key][:, j]) <= 1e-5, (key, j)
+            assert relmax(out[j], ref[j]) <= 1e-5, (key, j)
+    arr = spheresl.all_eval(r3, ct, ph, c, s.xi, s.p0, s.d0, s.cmap, s.scale, s.lmax, s.nmax,
+                            s.evtable, s.eftable)
+    nsys = len(bfe._TABLE_SYSTEMS)
+    for i in range(0, 120, 7):
+        one = spheresl.all_eval(float(r3[i]), float(ct[i]), float(ph[i]), c, s.xi, s.p0, s.d0,
+                                s.cmap, s.scale, s.lmax, s.nmax, s.evtable, s.eftable)
+        assert all(isinstance(v, float) for v in one)
+        assert all(v == a[i] for v, a in zip(one, arr)), i
+    assert len(bfe._TABLE_SYSTEMS) == nsys
+
+
+def _eof_kw(golden):
+    p, g = golden["eof_params"], golden["eof_grid"]
+    return dict(rmin=g[0], dR=g[2], zmin=g[3], dZ=g[5], numx=int(p[2]), numy=int(p[3]),
+                ASCALE=p[6], HSCALE=p[7], CMAP=int(p[8]))
+
+
+def test_eof_point_evaluators(golden):
+    """eof.force_eval and eof.accumulated_eval against the reference's own
+    vectors (float64 points in): truncation, the odd-m mask, perturb."""
+    from exptool_amd import bfe
+    from exptool_amd.basis import eof as E
+    t = eof_table(golden)
+    x, y, z = (golden[k][:120] for k in ("px", "py", "pz"))
+    r2 = (x * x + y * y) ** 0.5 + 1e-15
+    ph = np.arctan2(y, x)
+    c, sn = golden["eof_cos"], golden["eof_sin"]
+    kw = _eof_kw(golden)
+    tf = (t.potC, t.rforceC, t.zforceC, t.potS, t.rforceS, t.zforceS)
+    for key, k2 in (("eof_forceeval", dict(MMAX=t.mmax, NMAX=t.norder)),
+                    ("eof_forceeval_noodd", dict(MMAX=t.mmax, NMAX=t.norder, no_odd=True)),
+                    ("eof_forceeval_m1n2", dict(MMAX=1, NMAX=2))):
+        out = E.force_eval(r2, z, ph, c, sn, *tf, compat=True, **kw, **k2)
+        for j in range(5):
+            assert relmax(out[j], golden[key][:, j]) <= 1e-5, (key, j)
+    pt = E.force_eval(r2, z, ph, c, sn, *tf, MMAX=t.mmax, NMAX=t.norder, perturb=True, **kw)
+    full = E.force_eval(r2, z, ph, c, sn, *tf, MMAX=t.mmax, NMAX=t.norder, **kw)
+    assert relmax(pt[0] + pt[5], full[0]) <= 1e-12 and relmax(pt[2] + pt[6], full[2]) <= 1e-12
+    assert relmax(pt[3] + pt[4], full[3]) <= 1e-12 and np.array_equal(pt[1], full[1])
+    ta = (t.potC, t.rforceC, t.zforceC, t.densC, t.potS, t.rforceS, t.zforceS, t.densS)
+    for key, k2 in (("eof_acceval", dict(MMAX=t.mmax)), ("eof_acceval_noodd", dict(MMAX=t.mmax, no_odd=True)),
+                    ("eof_acceval_m1", dict(MMAX=1))):
+        out = E.accumulated_eval(r2, z, ph, c, sn, *ta, **kw, **k2)
+        ref = O.eof_accumulated_eval(t, r2, z, ph, c, sn, **k2)
+        for j in range(7):
+            assert relmax(out[j], golden[key][:, j]) <= 1e-5, (key, j)
+            assert relmax(out[j], ref[j]) <= 1e-5, (key, j)
+    nsys = len(bfe._TABLE_SYSTEMS)
+    for i in range(0, 120, 11):
+        one = E.force_eval(float(r2[i]), float(z[i]), float(ph[i]), c, sn, *tf, MMAX=t.mmax,
+                           NMAX=t.norder, **kw)
+        assert all(isinstance(v, float) for v in one)
+        assert all(v == a[i] for v, a in zip(one, full)), i
+    assert len(bfe._TABLE_SYSTEMS) == nsys
 
 
 def test_spheresl_truncation_quirk(files, golden):
