@@ -11,7 +11,9 @@ whole step (accumulate + force evaluation of every particle).
 
 Also reported: per-kernel HIP-event timings, the roofline record of the
 dominant kernel, orbit-steps/s of the plain-BFE leapfrog in the frozen field,
-and the FP64 NumPy oracle timed on a bounded sample on the host (cpu_baseline).
+the CPU baselines on the host cores (the reference's decomposition restated
+over a process pool, the C++ FP64 OpenMP restatement, the NumPy oracle on
+one core) and the step's error against FP64 on a sample of its particles.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -72,6 +74,8 @@ def main():
     ap.add_argument("--n-halo", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pool-cores", type=int, default=min(16, os.cpu_count() or 1),
+                    help="pool workers of the reference-path CPU baseline (the host's CPU share)")
     ap.add_argument("--orbit-steps", type=int, default=10_000)
     ap.add_argument("--orbits", type=int, default=1_000_000)
     ap.add_argument("--mlp", type=int, default=1, help="train and time the MLP surrogate (0/1)")
@@ -241,9 +245,13 @@ def main():
         progress(f"batch: {batch['particles_per_s']:.3e} particles/s")
 
     # ---- CPU baseline: the FP64 oracle on a bounded sample --------------------
-    cpu = None
+    cpu = cpu_all = acc = None
     if not a.no_cpu:
-        cpu = cpu_baseline(sl_t, eof_t, dxyzm, hxyzm, a.cpu_seconds)
+        cpu, cpu_all, ref = cpu_baselines(sl_t, eof_t, dxyzm, hxyzm, a.cpu_seconds,
+                                          a.pool_cores)
+        acc = accuracy(sysm, dxyzm, hxyzm, ref)
+        progress(f"accuracy: force max {acc['force_rel_max']:.2e}, coef "
+                 f"{acc['coef_eof_block_rel_max']:.2e} / {acc['coef_sl_block_rel_max']:.2e}")
 
     line = {
         "metric": "particle force evals/s (accumulate + Phi,F eval, disc EOF + halo SL)",
@@ -256,7 +264,7 @@ def main():
                    "n_disc": a.n_disc, "n_halo": a.n_halo, "parallelism": "single GPU"},
         "call_seconds": kt, "kernel_seconds": ko, "roofline": roof, "tuning": tuned, "orbits": orbit,
         "surrogate": surrogate, "batch": batch,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu, "cpu_baselines": cpu_all, "accuracy_vs_fp64": acc,
         "setup_seconds": setup_s,
     }
     print(json.dumps(line))
@@ -280,7 +288,7 @@ def roofline(kernel, unit, seconds, pmc):
         per_unit = f64 if use64 else f32
         peak = FP64_PEAK_TFLOPS if use64 else FP32_PEAK_TFLOPS
         achieved = nunits * per_unit / seconds / 1e12
-        rec.update({"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        rec.update({"bound": "valu_f64" if use64 else "valu_f32", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                     "frac": achieved / peak, "flops_per_unit": per_unit,
                     "flops_precision": "f64" if use64 else "f32",
                     "flops_source": "executed, rocprofv3 SQ_INSTS_VALU_* (profiles/bench_step_pmc_valu.md)"})
@@ -393,9 +401,25 @@ def mlp_stage(a, sysm, st0, orbit):
             "energy_drift_max": float(np.max(dr))}
 
 
-def cpu_baseline(sl_t, eof_t, dxyzm, hxyzm, budget_s):
-    """Time the oracle's accumulate + force evaluation on a sample sized to
-    roughly budget_s seconds of one core; report particles/s."""
+def host_info():
+    """Host CPU identity for the baseline records."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = None
+    return {"os_cpu_count": os.cpu_count(), "affinity_cpus": aff, "model": model}
+
+
+def _oracle_tables(sl_t, eof_t):
     from oracle import bfe_oracle as O
     slo = O.SLTable(sl_t["lmax"], sl_t["nmax"], sl_t["numr"], sl_t["cmap"], sl_t["rmin"],
                     sl_t["rmax"], sl_t["scale"], sl_t["ev"], sl_t["ef"], sl_t["xi"], sl_t["p0"],
@@ -404,6 +428,70 @@ def cpu_baseline(sl_t, eof_t, dxyzm, hxyzm, budget_s):
                     eof_t["rmin"], eof_t["rmax"], eof_t["ascale"], eof_t["hscale"], eof_t["potC"],
                     eof_t["rforceC"], eof_t["zforceC"], eof_t["potS"], eof_t["rforceS"],
                     eof_t["zforceS"])
+    return slo, eo
+
+
+def cpu_baselines(sl_t, eof_t, dxyzm, hxyzm, budget_s, pool_cores):
+    """The CPU baselines, timed on this host:
+      reference path  the reference's decomposition restated (oracle/ref_path.py):
+                      a pool of pool_cores workers over contiguous blocks,
+                      per-particle SL loops, vectorised EOF blocks, per-point
+                      field evaluation; stage costs measured on samples and
+                      extrapolated linearly to the full workload;
+      openmp          the C++ FP64 OpenMP restatement (oracle/bfe_cpu.cpp) on
+                      all threads, on a bounded sample;
+      numpy_1core     the vectorised NumPy oracle on one thread.
+    Returns (reference-path record, dict of all records, the OpenMP outputs on
+    its sample for the accuracy check)."""
+    from oracle import bfe_oracle as O
+    from oracle import cpu_oracle as CO
+    from oracle import ref_path as RP
+    slo, eo = _oracle_tables(sl_t, eof_t)
+    host = host_info()
+    nd, nh = dxyzm[0].size, hxyzm[0].size
+    # ---- OpenMP, bounded sample of the same particles
+    k = min(200_000, nd, nh)
+    d = [v[:k] for v in dxyzm]
+    h = [v[:k] for v in hxyzm]
+    t0 = time.perf_counter()
+    c, s = CO.eof_accumulate(eo, *d)
+    hc = CO.sl_accumulate(slo, *h)
+    F = O.FieldSpec(eo, c, s, slo, hc)
+    pts = [np.concatenate([a, b]) for a, b in zip(d[:3], h[:3])]
+    forces = CO.total_force(F, *pts)
+    t_omp = time.perf_counter() - t0
+    omp = {"value": 2 * k / t_omp, "unit": "particles/s", "cores": CO.threads(), "kind": "port",
+           "sample": f"{k} disc + {k} halo particles of the workload, C++ FP64 OpenMP "
+                     f"(accumulate + return_forces_cart), {t_omp:.2f} s on {CO.threads()} threads",
+           "host": host}
+    progress(f"cpu openmp: {omp['value']:.3e} particles/s")
+    # ---- reference decomposition over a process pool, extrapolated
+    nc = pool_cores
+    st = RP.time_reference_path(slo, eo, dxyzm, hxyzm, (c, s, hc), nc, n_eof=nc * 20000,
+                                n_sl=nc * 600, n_field=nc * 200)
+    full = (st["eof"] * nd / st["n_eof"] + st["sl"] * nh / st["n_sl"]
+            + st["field"] * (nd + nh) / st["n_field"])
+    ref = {"value": (nd + nh) / full, "unit": "particles/s", "cores": nc, "kind": "port",
+           "extrapolated": True, "seconds_full_workload": full,
+           "stage_seconds": {k2: st[k2] for k2 in ("eof", "sl", "field")},
+           "sample": f"the reference's decomposition restated (oracle/ref_path.py): {nc} pool "
+                     f"workers, contiguous blocks; EOF accumulate of {st['n_eof']} disc, SL "
+                     f"accumulate of {st['n_sl']} halo particles (per-particle loop), field at "
+                     f"{st['n_field']} points (per-point force_eval); each stage extrapolated "
+                     f"linearly to {nd} + {nh} particles",
+           "host": host}
+    progress(f"cpu reference path: {ref['value']:.3e} particles/s (extrapolated, {nc} workers)")
+    # ---- NumPy oracle, one thread
+    num = cpu_baseline_numpy(slo, eo, dxyzm, hxyzm, budget_s)
+    num["host"] = host
+    progress(f"cpu numpy 1 core: {num['value']:.3e} particles/s")
+    return ref, {"reference_path": ref, "openmp": omp, "numpy_1core": num}, (k, c, s, hc, forces)
+
+
+def cpu_baseline_numpy(slo, eo, dxyzm, hxyzm, budget_s):
+    """Time the NumPy oracle's accumulate + force evaluation on a sample sized
+    to roughly budget_s seconds of one core; report particles/s."""
+    from oracle import bfe_oracle as O
 
     def run(k):
         d = [v[:k] for v in dxyzm]
@@ -430,6 +518,40 @@ def _cpu_sized(run, budget_s):
     return {"value": 2 * k2 / t2, "unit": "particles/s", "cores": 1, "kind": "port",
             "sample": f"{k2} disc + {k2} halo particles of the same workload, FP64 NumPy oracle "
                       f"(accumulate + return_forces_cart), {t2:.1f} s, single thread"}
+
+
+def accuracy(sysm, dxyzm, hxyzm, ref):
+    """The bench step's call (bfe_forces_disc_halo, cell-ordered kernels) on
+    the OpenMP baseline's sample, against those FP64 results: coefficient
+    error per block (max |d a| / max |a| over each m or l block), force error
+    per particle (|dF| / |F|, max and RMS) and potential error."""
+    import torch
+    k, c, s, hc, forces = ref
+    T = lambda v: torch.as_tensor(np.asarray(v[:k], np.float32), device=sysm.device)
+    omin = sysm.get_tuning("order_min")
+    sysm.set_tuning("order_min", 1)
+    try:
+        out = sysm.forces_disc_halo([T(v) for v in dxyzm], [T(v) for v in hxyzm])
+        torch.cuda.synchronize()
+    finally:
+        sysm.set_tuning("order_min", omin)
+    g = {q: v.double().cpu().numpy() for q, v in out.items()}
+
+    def blocks(got, ref_, rows):
+        return max(float(np.max(np.abs(got[r] - ref_[r])) / np.max(np.abs(ref_[r])))
+                   for r in rows if np.max(np.abs(ref_[r])) > 0)
+
+    L = int(round(np.sqrt(hc.shape[0]))) - 1
+    fr = np.stack(forces[:3])
+    fg = np.stack([g["fx"], g["fy"], g["fz"]])
+    e = np.linalg.norm(fg - fr, axis=0) / np.linalg.norm(fr, axis=0)
+    return {"sample": f"{k} disc + {k} halo particles of the workload; FP64 reference: the "
+                      "C++ OpenMP oracle (matches the NumPy oracle to 1e-12)",
+            "coef_eof_block_rel_max": max(blocks(g["cos"], c, [slice(m, m + 1) for m in range(c.shape[0])]),
+                                          blocks(g["sin"], s, [slice(m, m + 1) for m in range(1, s.shape[0])])),
+            "coef_sl_block_rel_max": blocks(g["coef"], hc, [slice(l * l, (l + 1) ** 2) for l in range(L + 1)]),
+            "force_rel_max": float(np.max(e)), "force_rel_rms": float(np.sqrt(np.mean(e * e))),
+            "pot_rel_max": float(np.max(np.abs(g["pot"] - forces[3]) / np.abs(forces[3])))}
 
 
 if __name__ == "__main__":

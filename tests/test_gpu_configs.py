@@ -118,7 +118,7 @@ def identity_map():
     from exptool_amd.basis import tablegen as tg
     from exptool_amd.bfe import BFESystem
     from exptool_amd.utils import samplers as S
-    sl = tg.make_sl_tables(lmax=3, nmax=6, numr=800, rmin=0.0, rmax=60.0, cmap=0, scale=5.0,
+    sl = tg.make_sl_tables(lmax=3, nmax=6, numr=800, rmin=1e-3, rmax=60.0, cmap=0, scale=5.0,
                            model_mass=10.0)
     d = S.exponential_disc(30000, seed=7)
     et = tg.make_eof_tables(d, mmax=3, norder=6, numx=64, numy=32, rmin=0.001, rmax=20.0,
@@ -152,14 +152,30 @@ def test_identity_map_accumulate_and_eval(identity_map):
     s.set_field(rc, rs, hr)
     x, y, z = (np.concatenate(p) for p in ((dx, hx), (dy, hy), (dz, hz)))
     got = [v.cpu().numpy().astype(np.float64) for v in s.field_forces(x, y, z)]
-    F = O.FieldSpec(eo, rc, rs, slo, hr)
     sub = np.random.default_rng(0).choice(x.size, 6000, replace=False)
-    ref = O.total_force(F, x[sub], y[sub], z[sub])
     Fg = np.stack([got[j][sub] for j in range(3)])
+    ref = O.total_force(O.FieldSpec(eo, rc, rs, slo, hr), x[sub], y[sub], z[sub])
     Fr = np.stack(ref[:3])
     e = np.linalg.norm(Fg - Fr, axis=0) / np.linalg.norm(Fr, axis=0)
-    assert np.max(e) <= FORCE_TOL, np.max(e)
-    assert relmax(got[3][sub], ref[3]) <= FORCE_TOL
+    # Beyond the EOF grid (R > R_table) the reference extrapolates the tables
+    # linearly, and with the identity map the weights grow without bound
+    # (about 80 at R = 32 on this grid), multiplying the float32 rounding of
+    # the tables.  Inside the grid the float64 bar holds; everywhere the
+    # device equals the oracle evaluated on what the device folds: float32
+    # copies of the tables summed in float64, the sums rounded to float32.
+    inside = np.hypot(x[sub], y[sub]) <= np.sqrt(0.5) * eo.rmax * eo.ascale
+    assert inside.sum() > 3000
+    assert np.max(e[inside]) <= FORCE_TOL, np.max(e[inside])
+    assert relmax(got[3][sub][inside], ref[3][inside]) <= FORCE_TOL
+    r32 = lambda T: np.asarray(T, np.float32).astype(np.float64)
+    fold = lambda T, a: r32(np.einsum("mnxy,mn->mxy", r32(T), a))[:, None]
+    e32 = O.EOFTable(eo.mmax, 1, eo.numx, eo.numy, eo.cmap, eo.rmin, eo.rmax, eo.ascale,
+                     eo.hscale, *[fold(T, rc if k < 3 else rs) for k, T in enumerate(
+                         (eo.potC, eo.rforceC, eo.zforceC, eo.potS, eo.rforceS, eo.zforceS))])
+    one = np.ones((eo.mmax + 1, 1))
+    ref32 = O.total_force(O.FieldSpec(e32, one, one, slo, hr), x[sub], y[sub], z[sub])
+    e2 = np.linalg.norm(Fg - np.stack(ref32[:3]), axis=0) / np.linalg.norm(Fr, axis=0)
+    assert np.max(e2) <= 2e-6, np.max(e2)
 
 
 # ------------------------------------------------------- orbit integration
