@@ -385,7 +385,21 @@ def mlp_stage(a, sysm, st0, orbit):
     ck = out["checked"].cpu().numpy()
     fb = out["fallback"].cpu().numpy()
     rate = no * a.orbit_steps / dt_o
+    # the analytic base alone through the same kernels (zero network)
+    keep = (m.params.copy(), m.cfg.copy())
+    m.params = np.zeros_like(m.params)
+    m.set_scaling([1, 1, 1, 1], [0, 0, 0, 0])
+    m.upload()
+    berr, _ = m.force_error(hx, hy, hz)
+    m.params, m.cfg = keep
+    m.upload()
+    # MLP forward cost 2 (3*128 + 3*128^2 + 128*4) flops; at the dense bf16
+    # MFMA peak that bounds the surrogate's step rate
+    ceiling = 2.5e15 / 100096.0
     return {"orbit_steps_per_s": rate, "speedup_vs_bfe": rate / orbit["orbit_steps_per_s"],
+            "speedup_ceiling_at_mfma_peak": ceiling / orbit["orbit_steps_per_s"],
+            "base": rep.get("base"), "base_fit_seconds": rep.get("base_seconds"),
+            "force_rel_err_base_alone_median": float(np.median(berr)),
             "seconds": dt_o, "train_seconds": rep["train_seconds"], "train_steps": rep["steps"],
             "train_samples": rep["samples"], "train_points": rep["npoints"],
             "force_rel_err_heldout": {"median": float(np.median(ferr)),

@@ -36,6 +36,7 @@ EXPORTS = (
     "bfe_eof_accumulate_batch", "bfe_sl_accumulate_batch", "bfe_field_set_batch",
     "bfe_field_eval_batch", "bfe_accumulate_disc_halo", "bfe_forces_disc_halo",
     "bfe_set_tuning", "bfe_get_tuning", "bfe_kernel_times", "bfe_sl_eval_sph", "bfe_eof_eval_cyl",
+    "bfe_mlp_set_base", "bfe_mlp_set_mode",
 )
 MLP_NPARAMS = 50564
 
@@ -80,6 +81,8 @@ def _declare(lib):
         "bfe_kernel_times": (_I32, [_VP, C.c_char_p, C.POINTER(_F64), C.POINTER(_I64)]),
         "bfe_mlp_set_params": (_I32, [_VP, _VP, _I64, _VP]),
         "bfe_mlp_get_params": (_I32, [_VP, _VP, _I64]),
+        "bfe_mlp_set_base": (_I32, [_VP, _VP, _I32]),
+        "bfe_mlp_set_mode": (_I32, [_VP, _I32, _VP]),
         "bfe_mlp_forward": (_I32, [_VP, _VP, _VP, _VP, _I64, _VP, _VP]),
         "bfe_mlp_gradient": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _I64, _VP, _VP, _VP]),
         "bfe_mlp_train_step": (_I32, [_VP, _VP, _VP, _VP, _VP, _VP, _I64, C.c_float, C.c_float,

@@ -5,6 +5,9 @@ The accumulate kernels are persistent: a launch runs `eof_acc_blocks` /
 a batch of snapshots is cut into segments of `eof_seg_tiles` / `sl_seg_tiles`
 tiles.  The best values depend on the particle count, the table shapes and
 the device.
+The cell-ordered kernels also come in several lane shapes (`eof_dep_shape`,
+`sl_dep_shape`: how many coefficient slots a lane owns and how many radial
+orders each slot spans).
 `tune_accumulate` times each candidate on the caller's own particles with
 HIP events on the launch stream and keeps the fastest; `autotune` does this
 for both kernels (single and batched form), applies the choices to the
@@ -25,6 +28,20 @@ import torch
 
 BLOCK_CANDIDATES = (256, 512, 1024, 2048, 4096, 8192)
 SEG_CANDIDATES = (0, 1, 2, 4, 8, 16)
+# lane shapes (slots per lane, radial orders per slot) of the cell-ordered
+# accumulate kernels, by tuning index (eof_dep_shape / sl_dep_shape)
+EOF_SHAPES = ((1, 4), (1, 8), (1, 16), (2, 2), (2, 4), (2, 8), (2, 16), (4, 8))
+SL_SHAPES = ((1, 8), (1, 16), (1, 24), (2, 8), (2, 16), (2, 24), (4, 16))
+
+
+def fitting_shapes(system, kind):
+    """Indices of the lane shapes that cover the coefficient rows of the
+    tables: rows * ceil(norder / orders per slot) <= 64 * slots per lane."""
+    if kind == "eof":
+        rows, n, shapes = 2 * system.eof["mmax"] + 1, system.eof["norder"], EOF_SHAPES
+    else:
+        rows, n, shapes = (system.sl["lmax"] + 1) ** 2, system.sl["nmax"], SL_SHAPES
+    return [k for k, (ppt, ch) in enumerate(shapes) if rows * -(-n // ch) <= 64 * ppt]
 
 
 def _cache_path():
@@ -56,13 +73,17 @@ def _key(system, kind, n, nsnap):
     return f"{name}|{shape}|n2^{bucket}|snap{nsnap}"
 
 
-def tune_accumulate(system, x, y, z, m, kind="eof", counts=None, reps=3, candidates=None):
+def tune_accumulate(system, x, y, z, m, kind="eof", counts=None, reps=3, candidates=None,
+                    shapes=True):
     """Time the accumulate of (x, y, z, m) for every candidate launch shape and
-    apply the fastest.  counts: per-snapshot counts for the batched form (then
-    the segment length is tuned too).  Returns dict(knob=value, ..., seconds=...,
-    timings={...})."""
+    apply the fastest: first the block count (and, with counts, the segment
+    length) with the current lane shape, then, with shapes, every fitting lane
+    shape of the cell-ordered kernels at that block count.  counts:
+    per-snapshot counts for the batched form.  Returns dict(knob=value, ...,
+    seconds=..., timings={...})."""
     knob = "eof_acc_blocks" if kind == "eof" else "sl_acc_blocks"
     seg = "eof_seg_tiles" if kind == "eof" else "sl_seg_tiles"
+    shape_knob = "eof_dep_shape" if kind == "eof" else "sl_dep_shape"
     if counts is None:
         call = (lambda: system.eof_accumulate(x, y, z, m)) if kind == "eof" else \
             (lambda: system.sl_accumulate(x, y, z, m))
@@ -83,6 +104,19 @@ def tune_accumulate(system, x, y, z, m, kind="eof", counts=None, reps=3, candida
     out = {knob: b, "seconds": t, "timings": {f"{k[0]}/{k[1]}": v for k, v in timings.items()}}
     if counts is not None:
         out[seg] = sgt
+    if shapes:
+        auto = system.get_tuning(shape_knob)
+        st = {auto: t}
+        for k in fitting_shapes(system, kind):
+            if k == auto:
+                continue
+            system.set_tuning(shape_knob, k)
+            st[k] = _time(call, reps)
+        best = min(st, key=st.get)
+        system.set_tuning(shape_knob, best)
+        out[shape_knob] = best
+        out["seconds"] = st[best]
+        out["timings"].update({f"shape{k}": v for k, v in st.items()})
     return out
 
 
@@ -108,7 +142,8 @@ def autotune(system, disc=None, halo=None, disc_counts=None, halo_counts=None, r
         if key in store:
             rec = store[key]
             for k, v in rec.items():
-                if k in ("eof_acc_blocks", "sl_acc_blocks", "eof_seg_tiles", "sl_seg_tiles"):
+                if k in ("eof_acc_blocks", "sl_acc_blocks", "eof_seg_tiles", "sl_seg_tiles",
+                         "eof_dep_shape", "sl_dep_shape"):
                     system.set_tuning(k, v)
         else:
             rec = tune_accumulate(system, *parts, kind=kind, counts=counts, reps=reps)

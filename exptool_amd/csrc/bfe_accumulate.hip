@@ -467,8 +467,8 @@ __device__ __forceinline__ void eof_dep_setup(const float4* __restrict__ P, int6
 template <int PPT, int CH>
 __device__ __forceinline__ void eof_deposit_body(
     unsigned char* dsm, const float4* __restrict__ P, Segments S, EOFGeom g,
-    const float* __restrict__ T, int K, double* __restrict__ partials) {
-  const int rp = dep_np(g.norder);  // padded row length of T
+    const float* __restrict__ T, int rp, int K, double* __restrict__ partials) {
+  // rp: padded row length of T, norder rounded up to a multiple of CH
   const int rowlen = (2 * g.mmax + 1) * rp;
   const int J = 2 * g.mmax + 1;
   const int N = g.norder;
@@ -600,18 +600,31 @@ __device__ __forceinline__ void eof_deposit_body(
 // 80 against 83 us for 1M disc particles; beside the SL deposit they lost.
 template <int PPT, int CH>
 __global__ __launch_bounds__(256) void eof_deposit_kernel(
-    const float4* __restrict__ P, Segments S, EOFGeom g, const float* __restrict__ T, int K,
-    double* __restrict__ partials) {
+    const float4* __restrict__ P, Segments S, EOFGeom g, const float* __restrict__ T, int rp,
+    int K, double* __restrict__ partials) {
   extern __shared__ __align__(16) unsigned char dsm[];
-  eof_deposit_body<PPT, CH>(dsm, P, S, g, T, K, partials);
+  eof_deposit_body<PPT, CH>(dsm, P, S, g, T, rp, K, partials);
 }
 
 template <int PPT, int CH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void
 eof_deposit_alone_kernel(const float4* __restrict__ P, Segments S, EOFGeom g,
-                         const float* __restrict__ T, int K, double* __restrict__ partials) {
+                         const float* __restrict__ T, int rp, int K,
+                         double* __restrict__ partials) {
   extern __shared__ __align__(16) unsigned char dsm[];
-  eof_deposit_body<PPT, CH>(dsm, P, S, g, T, K, partials);
+  eof_deposit_body<PPT, CH>(dsm, P, S, g, T, rp, K, partials);
+}
+
+// eof_dep32[node][j][n < norder] = eof_pot32[node][j norder + n], zero for
+// n in [norder, rp): one thread per (node, j, n).
+__global__ __launch_bounds__(256) void eof_repad_kernel(const float* __restrict__ src,
+                                                        float* __restrict__ dst, int64_t rows,
+                                                        int norder, int rp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows * rp) return;
+  const int64_t row = t / rp;
+  const int n = (int)(t - row * rp);
+  dst[t] = n < norder ? src[row * norder + n] : 0.f;
 }
 
 // Applies the float64 radial run sums D1, D2 of cell i to the lane's
@@ -713,8 +726,12 @@ __device__ __forceinline__ void sl_deposit_body(
         sP[pad] = w1;
         sP[16 * kDepStride + pad] = w2;
         reinterpret_cast<int*>(sP)[32 * kDepStride + pad] = ci;
-        // Y rows: P_m^m in closed form, legendre_R recurrence in l
-        // (spheresl.py:678-696), e^{i m phi} by the Chebyshev recurrence
+        // Y rows: Q_l^m = -4 pi N_lm P_l^m by its recurrence in l from the
+        // closed form at l = m (legendre_R, spheresl.py:678-696, with the
+        // normalisation folded into the constants), in float64; then one
+        // float32 rounding of Q and of m_p {1, cos m phi, sin m phi} (the
+        // Chebyshev recurrence, float64) and a float32 product.  Each
+        // rounding is to nearest, so the sums over particles see no bias.
         const bool bad = !(costh == costh);
         const double somx2 = sqrt((1.0 - costh) * (1.0 + costh));
         double pw = 1.0, cmm = 1.0, smm = 0.0, cmm1 = c1, smm1 = -s1v;
@@ -725,21 +742,22 @@ __device__ __forceinline__ void sl_deposit_body(
             const double sn = (m == 1) ? s1v : 2.0 * c1 * smm - smm1;
             cmm1 = cmm; smm1 = smm; cmm = cn; smm = sn;
           }
-          double plm1 = 0.0, pl = lmc[kLmcD + m] * pw;
+          const float wc = (float)(mass * cmm), ws = (float)(mass * smm);
+          double plm1 = 0.0, pl = lmc[kLmcN0 + m] * pw;
           for (int l = m; l <= g.lmax; ++l) {
             const int lm = lm_index(l, m);
             if (l > m) {
-              const double pn = costh * lmc[kLmcA + lm] * pl - lmc[kLmcB + lm] * plm1;
+              const double pn = costh * lmc[kLmcNA + lm] * pl - lmc[kLmcNB + lm] * plm1;
               plm1 = pl;
               pl = pn;
             }
             const bool skip = (no_odd && (l & 1)) || (bad && l > 0);
-            const double base = mass * lmc[kLmcY + lm] * pl;
+            const float q = (float)pl;
             if (m == 0) {
-              sY[(l * l) * kDepStride + lane] = skip ? 0.f : (float)base;
+              sY[(l * l) * kDepStride + lane] = skip ? 0.f : q * wc;
             } else {
-              sY[(l * l + 2 * m - 1) * kDepStride + lane] = skip ? 0.f : (float)(base * cmm);
-              sY[(l * l + 2 * m) * kDepStride + lane] = skip ? 0.f : (float)(base * smm);
+              sY[(l * l + 2 * m - 1) * kDepStride + lane] = skip ? 0.f : q * wc;
+              sY[(l * l + 2 * m) * kDepStride + lane] = skip ? 0.f : q * ws;
             }
           }
         }
@@ -1135,6 +1153,30 @@ static DepShape dep_shape(int rows, int N, const DepShape* cands, int ncand, int
 static const DepShape kEofShapes[kEofDepShapes] = {{1, 4}, {1, 8}, {1, 16}, {2, 2}, {2, 4}, {2, 8}, {2, 16}, {4, 8}};
 static const DepShape kSlShapes[kSlDepShapes] = {{1, 8}, {1, 16}, {1, 24}, {2, 8}, {2, 16}, {2, 24}, {4, 16}};
 
+int eof_dep_prepare(Context* c) {
+  const DepShape dsh = dep_shape(2 * c->mmax + 1, c->norder, kEofShapes, kEofDepShapes,
+                                 c->eof_dep_shape);
+  const int ch = dsh.ppt > 0 ? dsh.ch : 4;
+  const int rp = (c->norder + ch - 1) / ch * ch;
+  if (c->eof_dep32 && c->eof_dep_rp == rp) return 0;
+  const int64_t rows = (int64_t)(c->numx + 1) * (c->numy + 1) * (2 * c->mmax + 1);
+  float* d = nullptr;
+  BFE_HIP(c, hipDeviceSynchronize());   // the old layout may still be read
+  BFE_HIP(c, hipMalloc(&d, (size_t)rows * rp * sizeof(float)));
+  hipLaunchKernelGGL(eof_repad_kernel, dim3((unsigned)((rows * rp + 255) / 256)), dim3(256), 0,
+                     nullptr, c->eof_pot32, d, rows, c->norder, rp);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return check_hip(c, e, "eof_repad_kernel");
+  }
+  if (c->eof_dep32) (void)hipFree(c->eof_dep32);
+  c->eof_dep32 = d;
+  c->eof_dep_rp = rp;
+  return 0;
+}
+
 // beside: another accumulate runs concurrently (selects the deposit kernel).
 static int eof_accumulate_run(Context* c, const float* x, const float* y, const float* z,
                               const float* m, int64_t n, const int64_t* off, int nsnap,
@@ -1164,6 +1206,7 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
                                (size_t)4 * K * sizeof(double));
   const DepShape dsh = dep_shape(2 * c->mmax + 1, c->norder, kEofShapes, kEofDepShapes, c->eof_dep_shape);
   if (n >= c->order_min && n > 0 && dsh.ppt > 0 && dlds <= 64 * 1024) {
+    if ((rc = eof_dep_prepare(c))) return rc;
     rc = order_particles(c, 0, x, y, z, m, n, off_dev, nsnap, s, &sorted);
     if (rc) return rc;
   }
@@ -1173,10 +1216,10 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
   else if (dsh.ppt == PP && dsh.ch == CC) {                                                   \
     if (beside)                                                                               \
       hipLaunchKernelGGL((eof_deposit_kernel<PP, CC>), dim3(P.grid), dim3(256), dlds, s,       \
-                         sorted, P.S, g, c->eof_dep32, K, partials);                          \
+                         sorted, P.S, g, c->eof_dep32, c->eof_dep_rp, K, partials);           \
     else                                                                                      \
       hipLaunchKernelGGL((eof_deposit_alone_kernel<PP, CC>), dim3(P.grid), dim3(256), dlds, s, \
-                         sorted, P.S, g, c->eof_dep32, K, partials);                          \
+                         sorted, P.S, g, c->eof_dep32, c->eof_dep_rp, K, partials);           \
   }
     if (false) {}
     BFE_EOF_DEP(1, 4) BFE_EOF_DEP(1, 8) BFE_EOF_DEP(1, 16) BFE_EOF_DEP(2, 2) BFE_EOF_DEP(2, 4)

@@ -75,6 +75,7 @@ static void free_eof(Context* c) {
   free_ptr(c->eof_dep32);
   c->eof_pot32 = nullptr;
   c->eof_dep32 = nullptr;
+  c->eof_dep_rp = 0;
   for (auto& f : c->eoff) { free_ptr(f.d); f = EOFFolded(); }
   c->eof_ok = false;
   c->batch_nsnap = 0;   // batch fields were folded against these tables
@@ -246,6 +247,17 @@ extern "C" int bfe_sl_tables_set(bfe_ctx* h, int lmax, int nmax, int numr, int c
       lmc[kLmcD + m] = d;
     }
   }
+  // the recurrence of Q_l^m = -4 pi N_lm P_l^m: ratios of the normalisations
+  // (from the same exact integer products) folded into its constants
+  for (int m = 0; m <= kMaxL; ++m) {
+    lmc[kLmcN0 + m] = lmc[kLmcY + lm_index(m, m)] * lmc[kLmcD + m];
+    for (int l = m + 1; l <= kMaxL; ++l) {
+      const int lm = lm_index(l, m);
+      const double y = lmc[kLmcY + lm];
+      lmc[kLmcNA + lm] = lmc[kLmcA + lm] * y / lmc[kLmcY + lm_index(l - 1, m)];
+      lmc[kLmcNB + lm] = l - 2 >= m ? lmc[kLmcB + lm] * y / lmc[kLmcY + lm_index(l - 2, m)] : 0.0;
+    }
+  }
   int rc;
   if ((rc = dev_upload(c, &c->sl_lmc, lmc.data(), lmc.size()))) return rc;
   if ((rc = dev_upload(c, &c->sl_xi, xi, numr))) return rc;
@@ -319,19 +331,14 @@ extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, in
         for (size_t node = 0; node < nodes; ++node)
           pot32[node * K + k] = (float)buf[(size_t)k * nodes + node];
       if ((rc = dev_upload(c, &c->eof_pot32, pot32.data(), pot32.size()))) return rc;
-      const int J = 2 * mmax + 1, np = dep_np(norder);
-      std::vector<float> d32(nodes * J * np, 0.f);
-      for (size_t node = 0; node < nodes; ++node)
-        for (int j = 0; j < J; ++j)
-          for (int n = 0; n < norder; ++n)
-            d32[(node * J + j) * np + n] = pot32[node * K + (size_t)j * norder + n];
-      if ((rc = dev_upload(c, &c->eof_dep32, d32.data(), d32.size()))) return rc;
     }
   }
   c->mmax = mmax; c->norder = norder; c->numx = numx; c->numy = numy; c->ecmap = cmap;
   c->ascale = ascale; c->hscale = hscale; c->xmin = xmin; c->dx = dx; c->ymin = ymin; c->dy = dy;
   c->eof_K = K;
   c->eof_dens = dens;
+  // padded rows for the cell-ordered accumulate's lane shape
+  if (int rc = eof_dep_prepare(c)) return rc;
   c->eof_ok = true;
   return 0;
 }
@@ -360,6 +367,11 @@ extern "C" int bfe_set_tuning(bfe_ctx* h, const char* name, int64_t value) {
     if (value < -1 || value >= count)
       return set_error(c, BFE_EINVAL, "%s outside [-1, %lld]", name, (long long)(count - 1));
     (e ? c->eof_dep_shape : c->sl_dep_shape) = value;
+    // the EOF table rows are laid out for the shape's chunk length
+    if (e && c->eof_ok) {
+      (void)hipSetDevice(c->device);
+      if (int rc = eof_dep_prepare(c)) return rc;
+    }
   } else {
     return set_error(c, BFE_EINVAL, "unknown tuning parameter '%s'", name);
   }

@@ -27,9 +27,14 @@ constexpr int kMaxLM = (kMaxL + 1) * (kMaxL + 2) / 2;
 //   [kLmcD + m]   (-1)^m (2m - 1)!!     P_m^m = D_m (1 - x^2)^{m/2}
 //   [kLmcL + lm]  l, [kLmcS + lm] l + m  (as doubles: scalar operands of the
 //                 derivative recurrence instead of per-lane conversions)
+//   [kLmcNA + lm], [kLmcNB + lm]  the same recurrence for Q_l^m = -4 pi N_lm
+//                 P_l^m:  Q_l^m = x A N_lm/N_{l-1,m} Q_{l-1}^m
+//                                 - B N_lm/N_{l-2,m} Q_{l-2}^m
+//   [kLmcN0 + m]  -4 pi N_mm (-1)^m (2m - 1)!!   Q_m^m = N0_m (1 - x^2)^{m/2}
 constexpr int kLmcY = 0, kLmcA = kMaxLM, kLmcB = 2 * kMaxLM, kLmcD = 3 * kMaxLM;
 constexpr int kLmcL = 3 * kMaxLM + kMaxL + 1, kLmcS = kLmcL + kMaxLM;
-constexpr int kLmcSize = kLmcS + kMaxLM;
+constexpr int kLmcNA = kLmcS + kMaxLM, kLmcNB = kLmcNA + kMaxLM, kLmcN0 = kLmcNB + kMaxLM;
+constexpr int kLmcSize = kLmcN0 + kMaxL + 1;
 
 // Row length of the padded deposit tables: radial orders rounded up to a
 // multiple of 48, so that for every chunk length of the lane shapes (2, 4, 8,
@@ -57,6 +62,11 @@ struct EOFFolded {
   bool valid = false;
 };
 
+// Analytic base of the MLP field (bfe_mlp_set_base): at most kMaxMN
+// Miyamoto-Nagai discs; values nmn, (M, a, b) x nmn, Mh, ah, phi0.
+constexpr int kMaxMN = 4;
+constexpr int kMLPBaseMax = 4 + 3 * kMaxMN;
+
 // MLP surrogate state (float32 masters, bf16 MFMA operand copies, Adam moments
 // and the per-batch training buffers).
 struct MLPState {
@@ -75,6 +85,9 @@ struct MLPState {
   int64_t bcap = 0;
   int64_t step = 0;
   double cfg[11] = {1, 1, 1, 1, 1, 1, 1, 0, 0, 0, 0};
+  double base[kMLPBaseMax] = {0};   // no base: nmn 0, Mh 0, phi0 0
+  int conservative = 0;             // forces as -grad of the potential output
+  double lw[4] = {1, 1, 1, 1};      // training loss weight per output
 };
 
 struct Context {
@@ -104,7 +117,9 @@ struct Context {
   // float32 too; the fold sums in float64)
   float* eof_fold32[4] = {nullptr, nullptr, nullptr, nullptr};
   float* eof_pot32 = nullptr;  // [node][K] potential for accumulate
-  float* eof_dep32 = nullptr;  // [node][2 mmax + 1][dep_np(norder)] the same, rows zero-padded
+  float* eof_dep32 = nullptr;  // [node][2 mmax + 1][eof_dep_rp] the same, rows zero-padded
+  int eof_dep_rp = 0;          // row length of eof_dep32: norder rounded up to the
+                               // lane shape's chunk length (eof_dep_prepare)
 
   // ---- folded tables: slot 0 = batch evaluators, slot 1 = frozen field ----
   SLFolded slf[2];
@@ -256,6 +271,10 @@ void swap_work(Context* c);
 int set_error(Context* c, int code, const char* fmt, ...);
 int check_hip(Context* c, hipError_t e, const char* what);
 int ensure_scratch(Context* c, size_t bytes);
+// Lays out eof_dep32 for the cell-ordered EOF accumulate's current lane
+// shape (rows of norder rounded up to the shape's chunk length), rebuilding
+// it from eof_pot32 when the shape's row length differs (synchronous).
+int eof_dep_prepare(Context* c);
 void mlp_free(Context* c);
 
 #define BFE_HIP(ctx, call)                                   \

@@ -61,6 +61,9 @@ constexpr int kWaveTile = 16 * kNT;              // points per wave
 // the registers.
 constexpr int kBlockInf = BFE_MLP_BLOCK;
 constexpr int kBlockTrain = 256;
+// The conservative-field kernels carry three tangent operand sets beside the
+// activations: four waves per block, one per SIMD, keep them in registers.
+constexpr int kBlockGrad = 256;
 template <bool TRAIN>
 constexpr int block_threads() { return TRAIN ? kBlockTrain : kBlockInf; }
 template <bool TRAIN>
@@ -89,12 +92,53 @@ __device__ __forceinline__ float dsilu(float z) {
   return s * (1.f + z * (1.f - s));
 }
 
+// Analytic base of the field: the network predicts the field minus this
+// model (all zero: no base).  Up to kMaxMN Miyamoto-Nagai discs
+// Phi = -M / sqrt(R^2 + (a + sqrt(z^2 + b^2))^2) and one Hernquist sphere
+// Phi = -Mh / (r + ah), plus a constant; evaluated in float64.
+struct MLPBase {
+  int nmn;
+  double M[kMaxMN], a[kMaxMN], b[kMaxMN];
+  double mh, ah, phi0;
+};
+
 struct MLPCfg {
   float sxy;        // feature 0,1 = x/sxy, y/sxy
   float hz, wz;     // feature 2 = asinh(z/hz)/wz
-  float os[4];      // physical output k = os[k] * y_k + ob[k]
+  float os[4];      // physical output k = os[k] * y_k + ob[k] + base_k
   float ob[4];
+  MLPBase base;
+  int conservative; // forces are -grad of the potential output (mlp_tile_grad)
+  float lw[4];      // training loss weight of output k
 };
+
+// (Phi, F_R, F_phi = 0, F_z) of the analytic base at (R, z).
+__device__ __forceinline__ void mlp_base_eval(const MLPBase& B, float Rf, float zf, float out[4]) {
+  const double R = Rf, z = zf, R2 = R * R, z2 = z * z;
+  double pot = B.phi0, fR = 0.0, fz = 0.0;
+  for (int k = 0; k < B.nmn; ++k) {
+    const double bz = sqrt(z2 + B.b[k] * B.b[k]);
+    const double s = B.a[k] + bz;
+    const double id = 1.0 / sqrt(R2 + s * s);
+    const double m3 = B.M[k] * id * id * id;
+    pot -= B.M[k] * id;
+    fR -= m3 * R;
+    fz -= m3 * z * s / bz;
+  }
+  if (B.mh != 0.0) {
+    const double r = sqrt(R2 + z2);
+    const double ir = 1.0 / (r + B.ah);
+    pot -= B.mh * ir;
+    // -Mh / (r + a)^2 along r-hat; r = 0 gives no force
+    const double g = r > 0.0 ? B.mh * ir * ir / r : 0.0;
+    fR -= g * R;
+    fz -= g * z;
+  }
+  out[0] = (float)pot;
+  out[1] = (float)fR;
+  out[2] = 0.f;
+  out[3] = (float)fz;
+}
 
 __device__ __forceinline__ void mlp_features(const MLPCfg& c, float x, float y, float z, float f[3]) {
   f[0] = x / c.sxy;
@@ -143,13 +187,14 @@ __device__ __forceinline__ void load_weights(const __bf16* __restrict__ W, __bf1
 }
 
 // Hidden layer: acc[mt][nt] = bias + W (K-permuted, LDS) x B[kk][nt].
+template <int NT>
 __device__ __forceinline__ void hidden_mfma(const __bf16* __restrict__ sW, const float* bias,
-                                            const bf16x8 (&B)[4][kNT], floatx4 (&acc)[8][kNT],
+                                            const bf16x8 (&B)[4][NT], floatx4 (&acc)[8][NT],
                                             int g, int c) {
 #pragma unroll
   for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < kNT; ++nt)
+    for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc[mt][nt][i] = bias ? bias[16 * mt + 4 * g + i] : 0.f;
 #pragma unroll
@@ -160,7 +205,7 @@ __device__ __forceinline__ void hidden_mfma(const __bf16* __restrict__ sW, const
     for (int mt = 0; mt < 8; ++mt) {
       const bf16x8 a = *reinterpret_cast<const bf16x8*>(sW + (16 * mt + c) * kH + 32 * kk + 8 * g);
 #pragma unroll
-      for (int nt = 0; nt < kNT; ++nt) acc[mt][nt] = mfma16(a, B[kk][nt], acc[mt][nt]);
+      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma16(a, B[kk][nt], acc[mt][nt]);
     }
   }
 }
@@ -230,6 +275,116 @@ __device__ __forceinline__ void mlp_tile_forward(const __bf16* __restrict__ sW,
     }
 }
 
+// Forward pass of one wave tile for the potential output y0 and its
+// derivatives dy0/df_k along the three features, by forward-mode tangents:
+// t_1^k = silu'(z_1) W0[:, k], t_{l+1}^k = silu'(z_{l+1}) (W_l t_l^k), each
+// W_l t_l^k one more bf16 MFMA pass over the layer's LDS weights (float32
+// accumulation, tangents rounded to bf16 as the activations are), and
+// dy0/df_k = W4[0] . t_4^k.  Layout as mlp_tile_forward; NT 16-point tiles.
+template <int NT>
+__device__ __forceinline__ void mlp_tile_grad(const __bf16* __restrict__ sW,
+                                              const float* __restrict__ sS,
+                                              const float (&f)[NT][3], float (&y0)[NT],
+                                              float (&dy)[NT][3], int g, int c) {
+  const float* sW0 = sS;
+  const float* sB0 = sS + kH * kNIn;
+  const float* sBh = sS + kH * kNIn + kH;
+  const float* sW4 = sS + kH * kNIn + 4 * kH;
+  const float* sB4 = sW4 + kNOut * kH;
+  bf16x8 B[4][NT];
+  bf16x8 T[3][4][NT];
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int nr = pi_neuron(kk, g, j);
+      const float w0 = sW0[nr * 3], w1 = sW0[nr * 3 + 1], w2 = sW0[nr * 3 + 2], b = sB0[nr];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const float zz = fmaf(w0, f[nt][0], fmaf(w1, f[nt][1], fmaf(w2, f[nt][2], b)));
+        const float sg = sigmoid(zz);
+        const float d = sg * (1.f + zz * (1.f - sg));
+        B[kk][nt][j] = (__bf16)(zz * sg);
+        T[0][kk][nt][j] = (__bf16)(d * w0);
+        T[1][kk][nt][j] = (__bf16)(d * w1);
+        T[2][kk][nt][j] = (__bf16)(d * w2);
+      }
+    }
+  float ys[NT], ds[NT][3];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    ys[nt] = 0.f;
+    ds[nt][0] = ds[nt][1] = ds[nt][2] = 0.f;
+  }
+  floatx4 acc[8][NT], tacc[8][NT];
+  float dsl[8][NT][4];
+#pragma unroll 1
+  for (int layer = 0; layer < 3; ++layer) {
+    const __bf16* W = sW + layer * kH * kH;
+    hidden_mfma(W, sBh + layer * kH, B, acc, g, c);
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float zz = acc[mt][nt][i];
+          const float sg = sigmoid(zz);
+          const float hh = zz * sg;
+          dsl[mt][nt][i] = sg * (1.f + zz * (1.f - sg));
+          if (layer < 2) B[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)hh;
+          else ys[nt] = fmaf(sW4[16 * mt + 4 * g + i], hh, ys[nt]);
+        }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      hidden_mfma(W, (const float*)nullptr, T[k], tacc, g, c);
+#pragma unroll
+      for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float tt = dsl[mt][nt][i] * tacc[mt][nt][i];
+            if (layer < 2) T[k][mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)tt;
+            else ds[nt][k] = fmaf(sW4[16 * mt + 4 * g + i], tt, ds[nt][k]);
+          }
+    }
+  }
+  // sums over the four lane groups that hold the same point column
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    float v = ys[nt];
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    y0[nt] = v + sB4[0];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      float w = ds[nt][k];
+      w += __shfl_xor(w, 16);
+      w += __shfl_xor(w, 32);
+      dy[nt][k] = w;
+    }
+  }
+}
+
+// Potential and Cartesian force of the conservative field at (x, y, z): the
+// network's potential os0 y0 + ob0 plus the base, F = -grad of it (chain rule
+// through the features f = (x/sxy, y/sxy, asinh(z/hz)/wz)).
+__device__ __forceinline__ void conservative_field(const MLPCfg& cf, float x, float y, float z,
+                                                   float y0, const float (&dy)[3], float& fx,
+                                                   float& fy, float& fz, float& pot) {
+  const float R = sqrtf(x * x + y * y);
+  float bv[4];
+  mlp_base_eval(cf.base, R, z, bv);
+  const float cp = R > 0.f ? x / R : 1.f, sp = R > 0.f ? y / R : 0.f;
+  const float zh = z / cf.hz;
+  const float dfz = 1.f / (cf.wz * cf.hz * sqrtf(1.f + zh * zh));
+  fx = -cf.os[0] * dy[0] / cf.sxy + bv[1] * cp;
+  fy = -cf.os[0] * dy[1] / cf.sxy + bv[1] * sp;
+  fz = -cf.os[0] * dy[2] * dfz + bv[3];
+  pot = cf.os[0] * y0 + cf.ob[0] + bv[0];
+}
+
 // Forward pass over 32-point tiles per wave.  TRAIN also stores the
 // features and pre-activations, the loss and dL/dy.
 template <bool TRAIN>
@@ -267,10 +422,14 @@ __global__ __launch_bounds__(block_threads<TRAIN>()) void mlp_fwd_kernel(MLPKArg
     if (!TRAIN) {
 #pragma unroll
       for (int nt = 0; nt < kNT; ++nt)
-        if (valid[nt] && g == 0)
+        if (valid[nt] && g == 0) {
+          const float px = a.x[pid[nt]], py = a.y[pid[nt]];
+          float bv[4];
+          mlp_base_eval(a.cfg.base, sqrtf(px * px + py * py), a.z[pid[nt]], bv);
 #pragma unroll
           for (int k = 0; k < 4; ++k)
-            a.out[(size_t)k * a.n + pid[nt]] = a.cfg.os[k] * yk[nt][k] + a.cfg.ob[k];
+            a.out[(size_t)k * a.n + pid[nt]] = a.cfg.os[k] * yk[nt][k] + a.cfg.ob[k] + bv[k];
+        }
       continue;
     }
     // ---- loss and backward ------------------------------------------------
@@ -282,8 +441,8 @@ __global__ __launch_bounds__(block_threads<TRAIN>()) void mlp_fwd_kernel(MLPKArg
       for (int k = 0; k < 4; ++k) {
         const float t = valid[nt] ? a.target[(size_t)k * a.n + pid[nt]] : 0.f;
         const float d = valid[nt] ? yk[nt][k] - t : 0.f;
-        gy[nt][k] = 2.f * w * d * a.inv_n;
-        if (g == 0) loss += (double)(w * d * d);
+        gy[nt][k] = 2.f * w * a.cfg.lw[k] * d * a.inv_n;
+        if (g == 0) loss += (double)(w * a.cfg.lw[k] * d * d);
         if (valid[nt] && g == 0) a.gY[(size_t)k * a.bcap + pid[nt]] = gy[nt][k];
       }
     }
@@ -297,6 +456,41 @@ __global__ __launch_bounds__(block_threads<TRAIN>()) void mlp_fwd_kernel(MLPKArg
       double s = 0.0;
       for (int w = 0; w < kBlock / 64; ++w) s += s_loss[w];
       a.loss_part[blockIdx.x] = s;
+    }
+  }
+}
+
+// Inference of the conservative field: out = Phi, F_R, F_phi, F_z with
+// F = -grad Phi (mlp_tile_grad, 16-point tiles per wave).
+__global__ __launch_bounds__(kBlockGrad) void mlp_fwd_grad_kernel(MLPKArgs a) {
+  constexpr int NT = 1, kBlockPts = 16 * NT * (kBlockGrad / 64);
+  extern __shared__ __align__(16) unsigned char smem[];
+  __bf16* sW = reinterpret_cast<__bf16*>(smem);
+  float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
+  load_small(a.params, sS);
+  load_weights(a.Wp, sW);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const int64_t ntiles = (a.n + kBlockPts - 1) / kBlockPts;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t p = tile * kBlockPts + wave * 16 * NT + c;
+    const bool valid = p < a.n;
+    const int64_t q = valid ? p : 0;
+    const float x = a.x[q], y = a.y[q], z = a.z[q];
+    float f[NT][3];
+    mlp_features(a.cfg, x, y, z, f[0]);
+    float y0[NT], dy[NT][3];
+    mlp_tile_grad<NT>(sW, sS, f, y0, dy, g, c);
+    if (valid && g == 0) {
+      float fx, fy, fz, pot;
+      conservative_field(a.cfg, x, y, z, y0[0], dy[0], fx, fy, fz, pot);
+      const float R = sqrtf(x * x + y * y);
+      const float cp = R > 0.f ? x / R : 1.f, sp = R > 0.f ? y / R : 0.f;
+      a.out[p] = pot;
+      a.out[a.n + p] = fx * cp + fy * sp;
+      a.out[2 * a.n + p] = fy * cp - fx * sp;
+      a.out[3 * a.n + p] = fz;
     }
   }
 }
@@ -558,6 +752,8 @@ void bfe_point_call(const FieldTables& T, double x, double y, double z, double* 
 
 // Cartesian acceleration and potential of orbit (x, y, z) from the
 // surrogate outputs, or from the BFE when the point is outside the domain.
+// GRAD: yv holds (y0, dy0/df_0..2) of mlp_tile_grad, the forces are -grad.
+template <bool GRAD>
 __device__ __forceinline__ void surrogate_accel(const MLPLeapArgs& a, FieldTables& T, float x,
                                                 float y, float z, const float (&yv)[4], float& fx,
                                                 float& fy, float& fz, float& pot, bool& fb) {
@@ -572,18 +768,28 @@ __device__ __forceinline__ void surrogate_accel(const MLPLeapArgs& a, FieldTable
     pot = (float)(cv[6] + cv[7]);
     return;
   }
-  const float FR = a.cfg.os[1] * yv[1] + a.cfg.ob[1];
-  const float Fp = a.cfg.os[2] * yv[2] + a.cfg.ob[2];
+  if constexpr (GRAD) {
+    const float dy[3] = {yv[1], yv[2], yv[3]};
+    conservative_field(a.cfg, x, y, z, yv[0], dy, fx, fy, fz, pot);
+    return;
+  }
   const float R = sqrtf(R2);
+  float bv[4];
+  mlp_base_eval(a.cfg.base, R, z, bv);
+  const float FR = a.cfg.os[1] * yv[1] + a.cfg.ob[1] + bv[1];
+  const float Fp = a.cfg.os[2] * yv[2] + a.cfg.ob[2];
   const float cp = R > 0.f ? x / R : 1.f, sp = R > 0.f ? y / R : 0.f;
   fx = FR * cp - Fp * sp;
   fy = FR * sp + Fp * cp;
-  fz = a.cfg.os[3] * yv[3] + a.cfg.ob[3];
-  pot = a.cfg.os[0] * yv[0] + a.cfg.ob[0];
+  fz = a.cfg.os[3] * yv[3] + a.cfg.ob[3] + bv[3];
+  pot = a.cfg.os[0] * yv[0] + a.cfg.ob[0] + bv[0];
 }
 
-__global__ __launch_bounds__(kBlockInf) void mlp_leapfrog_kernel(MLPLeapArgs a) {
-  constexpr int kBlock = kBlockInf, kBlockPts = block_points<false>();
+template <bool GRAD>
+__global__ __launch_bounds__(GRAD ? kBlockGrad : kBlockInf) void mlp_leapfrog_kernel(MLPLeapArgs a) {
+  // GRAD: conservative forces from mlp_tile_grad, 16-point tiles
+  constexpr int NT = GRAD ? 1 : kNT;
+  constexpr int kBlock = GRAD ? kBlockGrad : kBlockInf, kBlockPts = 16 * NT * (kBlock / 64);
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
   float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
@@ -593,12 +799,12 @@ __global__ __launch_bounds__(kBlockInf) void mlp_leapfrog_kernel(MLPLeapArgs a) 
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, c = lane & 15;
-  const bool owner = g < kNT;                      // lane (g, c) owns orbit 16 g + c of the wave
+  const bool owner = g < NT;                       // lane (g, c) owns orbit 16 g + c of the wave
   const int64_t ntiles = (a.n + kBlockPts - 1) / kBlockPts;
   double esum = 0.0;
   FieldTables T = a.T;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t base = tile * kBlockPts + wave * kWaveTile;
+    const int64_t base = tile * kBlockPts + wave * 16 * NT;
     const int64_t me = base + 16 * (owner ? g : 0) + c;
     const bool live = owner && me < a.n;
     float px = 0, py = 0, pz = 0, ux = 0, uy = 0, uz = 0;
@@ -618,25 +824,37 @@ __global__ __launch_bounds__(kBlockInf) void mlp_leapfrog_kernel(MLPLeapArgs a) 
         pz = pz + uz * dt + o.az * hdt2;
       }
       // surrogate forward for the wave's points (column c, tiles nt)
-      float f[kNT][3];
-      int64_t pid[kNT];
-      bool valid[kNT];
+      float f[NT][3];
+      int64_t pid[NT];
+      bool valid[NT];
 #pragma unroll
-      for (int nt = 0; nt < kNT; ++nt) {
+      for (int nt = 0; nt < NT; ++nt) {
         const float qx = __shfl(px, 16 * nt + c), qy = __shfl(py, 16 * nt + c), qz = __shfl(pz, 16 * nt + c);
         mlp_features(a.cfg, qx, qy, qz, f[nt]);
         pid[nt] = 0;
         valid[nt] = false;
       }
-      float yk[kNT][4];
-      mlp_tile_forward<false>(sW, sS, f, yk, g, c, nullptr, 0, pid, valid);
+      float yk[NT][4];
+      if constexpr (GRAD) {
+        float y0[NT], dy[NT][3];
+        mlp_tile_grad<NT>(sW, sS, f, y0, dy, g, c);
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          yk[nt][0] = y0[nt];
+          yk[nt][1] = dy[nt][0];
+          yk[nt][2] = dy[nt][1];
+          yk[nt][3] = dy[nt][2];
+        }
+      } else {
+        mlp_tile_forward<false>(sW, sS, f, yk, g, c, nullptr, 0, pid, valid);
+      }
       if (live) {
         float yv[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           yv[q] = yk[0][q];
 #pragma unroll
-          for (int nt = 1; nt < kNT; ++nt)
+          for (int nt = 1; nt < NT; ++nt)
             if (g == nt) yv[q] = yk[nt][q];
         }
         if (a.omega != 0.0) {
@@ -646,7 +864,7 @@ __global__ __launch_bounds__(kBlockInf) void mlp_leapfrog_kernel(MLPLeapArgs a) 
         }
         float nx, ny, nz, pot;
         bool fb;
-        surrogate_accel(a, T, px, py, pz, yv, nx, ny, nz, pot, fb);
+        surrogate_accel<GRAD>(a, T, px, py, pz, yv, nx, ny, nz, pot, fb);
         if (fb) o.nfallback += 1;
         if (!fb && a.check_period > 0 && ((gwave + k) % a.check_period) == 0) {
           double cv[8];
@@ -734,7 +952,8 @@ using namespace bfe;
 
 namespace {
 
-MLPCfg make_cfg(const double* cfg) {
+MLPCfg make_cfg(const MLPState& M) {
+  const double* cfg = M.cfg;
   MLPCfg c;
   c.sxy = (float)cfg[0];
   c.hz = (float)cfg[1];
@@ -743,6 +962,21 @@ MLPCfg make_cfg(const double* cfg) {
     c.os[k] = (float)cfg[3 + k];
     c.ob[k] = (float)cfg[7 + k];
   }
+  // base layout: nmn, (M, a, b) x nmn, Mh, ah, phi0
+  const double* b = M.base;
+  c.base.nmn = (int)b[0];
+  for (int k = 0; k < kMaxMN; ++k) {
+    const bool on = k < c.base.nmn;
+    c.base.M[k] = on ? b[1 + 3 * k] : 0.0;
+    c.base.a[k] = on ? b[2 + 3 * k] : 0.0;
+    c.base.b[k] = on ? b[3 + 3 * k] : 1.0;
+  }
+  const int o = 1 + 3 * c.base.nmn;
+  c.base.mh = b[o];
+  c.base.ah = b[o + 1];
+  c.base.phi0 = b[o + 2];
+  c.conservative = M.conservative;
+  for (int k = 0; k < 4; ++k) c.lw[k] = (float)M.lw[k];
   return c;
 }
 
@@ -778,7 +1012,7 @@ MLPKArgs base_args(Context* c) {
   a.params = c->mlp.params;
   a.Wp = c->mlp.Wp;
   a.Wt = c->mlp.Wt;
-  a.cfg = make_cfg(c->mlp.cfg);
+  a.cfg = make_cfg(c->mlp);
   return a;
 }
 
@@ -855,7 +1089,11 @@ extern "C" int bfe_mlp_set_params(bfe_ctx* h, const float* params, int64_t count
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
     BFE_HIP(c, hipFuncSetAttribute((const void*)mlp_bwd_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
-    BFE_HIP(c, hipFuncSetAttribute((const void*)mlp_leapfrog_kernel,
+    BFE_HIP(c, hipFuncSetAttribute((const void*)mlp_leapfrog_kernel<false>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+    BFE_HIP(c, hipFuncSetAttribute((const void*)mlp_leapfrog_kernel<true>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+    BFE_HIP(c, hipFuncSetAttribute((const void*)mlp_fwd_grad_kernel,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
   }
   BFE_HIP(c, hipMemcpy(M.params, params, sizeof(float) * kNParams, hipMemcpyHostToDevice));
@@ -866,6 +1104,41 @@ extern "C" int bfe_mlp_set_params(bfe_ctx* h, const float* params, int64_t count
   int rc = mlp_sync(c, nullptr);
   if (rc) return rc;
   BFE_HIP(c, hipDeviceSynchronize());
+  return 0;
+}
+
+extern "C" int bfe_mlp_set_base(bfe_ctx* h, const double* base, int count) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (!base || count < 1) return set_error(c, BFE_EINVAL, "mlp_set_base: NULL or empty base");
+  const int nmn = (int)base[0];
+  if (base[0] != (double)nmn || nmn < 0 || nmn > kMaxMN)
+    return set_error(c, BFE_EINVAL, "mlp_set_base: %g Miyamoto-Nagai components (0..%d)", base[0], kMaxMN);
+  if (count != 4 + 3 * nmn)
+    return set_error(c, BFE_EINVAL, "mlp_set_base: %d values given, %d expected", count, 4 + 3 * nmn);
+  for (int k = 0; k < count; ++k)
+    if (!std::isfinite(base[k])) return set_error(c, BFE_EINVAL, "mlp_set_base: non-finite value");
+  for (int k = 0; k < nmn; ++k)
+    if (!(base[3 + 3 * k] > 0.0)) return set_error(c, BFE_EINVAL, "mlp_set_base: b must be > 0");
+  if (base[1 + 3 * nmn] != 0.0 && !(base[2 + 3 * nmn] > 0.0))
+    return set_error(c, BFE_EINVAL, "mlp_set_base: Hernquist scale must be > 0");
+  MLPState& M = c->mlp;
+  for (int k = 0; k < kMLPBaseMax; ++k) M.base[k] = k < count ? base[k] : 0.0;
+  return 0;
+}
+
+extern "C" int bfe_mlp_set_mode(bfe_ctx* h, int conservative, const double* loss_weights) {
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (conservative != 0 && conservative != 1)
+    return set_error(c, BFE_EINVAL, "mlp_set_mode: conservative must be 0 or 1");
+  MLPState& M = c->mlp;
+  for (int k = 0; k < 4; ++k) {
+    const double w = loss_weights ? loss_weights[k] : 1.0;
+    if (!(w >= 0.0) || !std::isfinite(w)) return set_error(c, BFE_EINVAL, "mlp_set_mode: loss weights must be >= 0");
+    M.lw[k] = w;
+  }
+  M.conservative = conservative;
   return 0;
 }
 
@@ -892,6 +1165,13 @@ extern "C" int bfe_mlp_forward(bfe_ctx* h, const float* x, const float* y, const
   hipStream_t s = (hipStream_t)stream;
   MLPKArgs a = base_args(c);
   a.x = x; a.y = y; a.z = z; a.n = n; a.out = out;
+  if (c->mlp.conservative) {
+    const int64_t ntiles = (n + 16 * (kBlockGrad / 64) - 1) / (16 * (kBlockGrad / 64));
+    const int nb = (int)(ntiles < 2048 ? ntiles : 2048);
+    hipLaunchKernelGGL(mlp_fwd_grad_kernel, dim3(nb), dim3(kBlockGrad), kLdsBytes, s, a);
+    BFE_HIP(c, hipGetLastError());
+    return 0;
+  }
   const int64_t ntiles = (n + block_points<false>() - 1) / block_points<false>();
   const int nb = (int)(ntiles < 1024 ? ntiles : 1024);
   hipLaunchKernelGGL(mlp_fwd_kernel<false>, dim3(nb), dim3(kBlockInf), kLdsBytes, s, a);
@@ -973,7 +1253,7 @@ extern "C" int bfe_mlp_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float*
   a.nblocks = nb;
   a.params = c->mlp.params;
   a.Wp = c->mlp.Wp;
-  a.cfg = make_cfg(c->mlp.cfg);
+  a.cfg = make_cfg(c->mlp);
   a.T = T;
   // step 0: accelerations and initial energies
   a.init = 1;
@@ -982,7 +1262,10 @@ extern "C" int bfe_mlp_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float*
   a.energy_check = 1;
   a.partE = energy_log ? partE : nullptr;
   a.check_index = 0;
-  hipLaunchKernelGGL(mlp_leapfrog_kernel, dim3(nb), dim3(kBlockInf), kLdsBytes, s, a);
+  if (a.cfg.conservative)
+    hipLaunchKernelGGL(mlp_leapfrog_kernel<true>, dim3(nb), dim3(kBlockGrad), kLdsBytes, s, a);
+  else
+    hipLaunchKernelGGL(mlp_leapfrog_kernel<false>, dim3(nb), dim3(kBlockInf), kLdsBytes, s, a);
   BFE_HIP(c, hipGetLastError());
   a.init = 0;
   for (int64_t k0 = 0, ci = 0; k0 < nsteps; k0 += chunk, ++ci) {
@@ -991,7 +1274,10 @@ extern "C" int bfe_mlp_leapfrog(bfe_ctx* h, float* x, float* y, float* z, float*
     a.energy_check = (energy_every > 0 && a.k1 - a.k0 == energy_every) ? 1 : 0;
     a.partE = (energy_log && a.energy_check) ? partE : nullptr;
     a.check_index = (int)ci + 1;
-    hipLaunchKernelGGL(mlp_leapfrog_kernel, dim3(nb), dim3(kBlockInf), kLdsBytes, s, a);
+    if (a.cfg.conservative)
+    hipLaunchKernelGGL(mlp_leapfrog_kernel<true>, dim3(nb), dim3(kBlockGrad), kLdsBytes, s, a);
+  else
+    hipLaunchKernelGGL(mlp_leapfrog_kernel<false>, dim3(nb), dim3(kBlockInf), kLdsBytes, s, a);
     BFE_HIP(c, hipGetLastError());
   }
   if (drift_out || maxerr_out || counts_out)

@@ -291,6 +291,21 @@ int bfe_kernel_times(bfe_ctx* ctx, const char* name, double* total_ms, int64_t* 
  * W2, b2, W3, b3, W4[4][128], b4[4].  Resets the Adam moments. */
 int bfe_mlp_set_params(bfe_ctx* ctx, const float* params_host, int64_t count, const double* cfg);
 int bfe_mlp_get_params(bfe_ctx* ctx, float* params_host, int64_t count);
+/* Analytic base the network's outputs are added to (the network learns the
+ * field minus the base; default: no base).  base_host: nmn, then (M, a, b)
+ * for each of nmn <= 4 Miyamoto-Nagai discs Phi = -M / sqrt(R^2 + (a +
+ * sqrt(z^2 + b^2))^2), then Mh, ah of a Hernquist sphere Phi = -Mh / (r +
+ * ah) (Mh = 0: none) and a constant phi0; count = 4 + 3 nmn.  Evaluated in
+ * float64 by bfe_mlp_forward and bfe_mlp_leapfrog: physical output k =
+ * os[k] y_k + ob[k] + base_k, with base (Phi, F_R, 0, F_z). */
+int bfe_mlp_set_base(bfe_ctx* ctx, const double* base_host, int count);
+/* conservative = 1: the forces of bfe_mlp_forward and bfe_mlp_leapfrog are
+ * minus the gradient of the potential output (plus the base), taken by
+ * forward-mode tangents through the network (three more bf16 MFMA passes per
+ * hidden layer); 0: the four outputs are read directly.  loss_weights (HOST
+ * float64 [4], may be NULL = all 1) weight each output's squared error in
+ * bfe_mlp_gradient / bfe_mlp_train_step. */
+int bfe_mlp_set_mode(bfe_ctx* ctx, int conservative, const double* loss_weights);
 /* out: device float32 [4][n] = Phi, F_R, F_phi, F_z (F_phi = -dPhi/dphi / R). */
 int bfe_mlp_forward(bfe_ctx* ctx, const float* x, const float* y, const float* z, int64_t n,
                     float* out, void* stream);

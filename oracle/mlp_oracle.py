@@ -81,14 +81,96 @@ def forward(params, cfg, x, y, z, bf=True):
     return Y, dict(f=f, Z1=Z1, Z2=Z2, Z3=Z3, Z4=Z4, H1=H1, H2=H2, H3=H3, H4=H4)
 
 
-def physical(cfg, Y):
+def base_field(base, x, y, z):
+    """Analytic base the network's outputs are added to: base = [nmn,
+    (M, a, b) x nmn, Mh, ah, phi0]; Miyamoto-Nagai discs
+    Phi = -M / sqrt(R^2 + (a + sqrt(z^2 + b^2))^2), a Hernquist sphere
+    Phi = -Mh / (r + ah), a constant.  Returns (4, n): Phi, F_R, 0, F_z
+    with F = -grad Phi (closed forms)."""
+    x, y, z = (np.asarray(v, dtype=np.float64) for v in (x, y, z))
+    R = np.sqrt(x * x + y * y)
+    out = np.zeros((4, R.size))
+    if base is None:
+        return out
+    b = np.asarray(base, dtype=np.float64)
+    nmn = int(b[0])
+    for k in range(nmn):
+        M, a, bb = b[1 + 3 * k:4 + 3 * k]
+        B = np.sqrt(z * z + bb * bb)
+        D = np.sqrt(R * R + (a + B) ** 2)
+        out[0] -= M / D
+        out[1] -= M * R / D ** 3
+        out[3] -= M * z * (a + B) / (B * D ** 3)
+    mh, ah, phi0 = b[1 + 3 * nmn:4 + 3 * nmn]
+    r = np.sqrt(R * R + z * z)
+    out[0] += phi0
+    if mh != 0.0:
+        out[0] -= mh / (r + ah)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            g = np.where(r > 0, mh / (r * (r + ah) ** 2), 0.0)
+        out[1] -= g * R
+        out[3] -= g * z
+    return out
+
+
+def forward_grad(params, cfg, x, y, z, bf=True):
+    """Potential output y0 (n,) and its derivatives along the three features
+    dy0/df (3, n) by forward-mode tangents, rounded to bf16 where the
+    conservative-field kernels round: the activations and tangents fed to
+    the 128 x 128 layers, and W1..W3."""
+    P = unpack(params)
+    q = bf16 if bf else (lambda v: np.asarray(v, dtype=np.float64))
+    f = features(cfg, x, y, z)
+    Z = P["W0"] @ f + P["b0"][:, None]
+    s_ = 1.0 / (1.0 + np.exp(-Z))
+    H = Z * s_
+    D = s_ * (1.0 + Z * (1.0 - s_))
+    T = [D * P["W0"][:, k:k + 1] for k in range(3)]
+    for Wk, bk in (("W1", "b1"), ("W2", "b2"), ("W3", "b3")):
+        W = q(P[Wk])
+        Z = W @ q(H) + P[bk][:, None]
+        s_ = 1.0 / (1.0 + np.exp(-Z))
+        Tn = [W @ q(t) for t in T]
+        H = Z * s_
+        D = s_ * (1.0 + Z * (1.0 - s_))
+        T = [D * t for t in Tn]
+    y0 = P["W4"][0] @ H + P["b4"][0]
+    dy = np.stack([P["W4"][0] @ t for t in T])
+    return y0, dy
+
+
+def conservative(cfg, y0, dy, x, y, z, base=None):
+    """(Phi, F_R, F_phi, F_z) of the conservative field: Phi = os0 y0 + ob0
+    + base, F = -grad Phi through the features f = (x/sxy, y/sxy,
+    asinh(z/hz)/wz)."""
+    sxy, hz, wz = cfg[0], cfg[1], cfg[2]
+    x, y, z = (np.asarray(v, dtype=np.float64) for v in (x, y, z))
+    R = np.sqrt(x * x + y * y)
+    b = base_field(base, x, y, z)
+    gx = cfg[3] * dy[0] / sxy
+    gy = cfg[3] * dy[1] / sxy
+    gz = cfg[3] * dy[2] / (wz * hz * np.sqrt(1.0 + (z / hz) ** 2))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        cp = np.where(R > 0, x / R, 1.0)
+        sp = np.where(R > 0, y / R, 0.0)
+    fx, fy = -gx + b[1] * cp, -gy + b[1] * sp
+    return np.stack([cfg[3] * y0 + cfg[7] + b[0], fx * cp + fy * sp, fy * cp - fx * sp,
+                     -gz + b[3]])
+
+
+def physical(cfg, Y, base=None, x=None, y=None, z=None):
+    """Physical outputs os * Y + ob, plus the base at (x, y, z) when given."""
     os_ = np.asarray(cfg[3:7], dtype=np.float64)[:, None]
     ob = np.asarray(cfg[7:11], dtype=np.float64)[:, None]
-    return os_ * Y + ob
+    out = os_ * Y + ob
+    if base is not None:
+        out = out + base_field(base, x, y, z)
+    return out
 
 
-def gradient(params, cfg, x, y, z, target, weight=None, bf=True):
-    """(grad vector, n * loss) of L = sum_p w_p sum_k (y_pk - t_pk)^2 / n."""
+def gradient(params, cfg, x, y, z, target, weight=None, bf=True, lw=None):
+    """(grad vector, n * loss) of L = sum_p w_p sum_k lw_k (y_pk - t_pk)^2 / n
+    (lw: per-output loss weights, default 1)."""
     P = unpack(params)
     q = bf16 if bf else (lambda v: np.asarray(v, dtype=np.float64))
     Y, I = forward(params, cfg, x, y, z, bf)
@@ -96,8 +178,9 @@ def gradient(params, cfg, x, y, z, target, weight=None, bf=True):
     t = np.asarray(target, dtype=np.float64).reshape(4, n)
     w = np.ones(n) if weight is None else np.asarray(weight, dtype=np.float64)
     d = Y - t
-    loss = float(np.sum(w[None, :] * d * d))
-    gY = 2.0 * w[None, :] * d / n
+    lwv = np.ones(4) if lw is None else np.asarray(lw, dtype=np.float64)
+    loss = float(np.sum(lwv[:, None] * w[None, :] * d * d))
+    gY = 2.0 * lwv[:, None] * w[None, :] * d / n
     G = {}
     G["W4"] = gY @ I["H4"].T
     G["b4"] = gY.sum(axis=1)

@@ -51,6 +51,77 @@ def test_forward_matches_oracle(field):
         assert np.max(np.abs(got[k] - ref[k])) / np.max(np.abs(ref[k])) <= 5e-3, k
 
 
+def test_forward_adds_analytic_base(field):
+    """Outputs are os y + ob + base: with a zero network the forward pass is
+    the analytic base (float64 in the kernel, float32 out); with a network,
+    the oracle's network outputs plus the same base."""
+    from exptool_amd.surrogate.mlp import MLPField
+    base = np.array([2, 1.3, 0.6, 0.2, -0.4, 1.7, 0.3, 9.0, 4.0, -0.05])
+    x, y, z = _pts(3000, 7)
+    m = MLPField(field, seed=4)
+    m.set_base(base)
+    m.params = np.zeros_like(m.params)
+    m.set_scaling([1, 1, 1, 1], [0, 0, 0, 0])
+    m.upload()
+    got = m.forward(x, y, z).cpu().numpy().astype(np.float64)
+    ref = MO.base_field(base, x, y, z)
+    for k in (0, 1, 3):
+        assert np.max(np.abs(got[k] - ref[k]) / np.maximum(np.abs(ref[k]), 1e-3)) <= 1e-6, k
+    assert np.all(got[2] == 0)
+    m2 = MLPField(field, seed=4)
+    m2.set_base(base)
+    m2.set_scaling([2.0, 0.5, 0.1, 0.3], [-1.0, 0.0, 0.0, 0.0])
+    m2.upload()
+    got2 = m2.forward(x, y, z).cpu().numpy().astype(np.float64) - ref
+    Y, _ = MO.forward(m2.params, m2.cfg, x, y, z, bf=True)
+    net = MO.physical(m2.cfg, Y)
+    for k in range(4):
+        assert np.max(np.abs(got2[k] - net[k])) / np.max(np.abs(net[k])) <= 5e-3, k
+    m2.set_base(None)
+
+
+def test_conservative_forward_matches_oracle(field):
+    """The conservative field (forces = -grad of the potential output, by
+    forward-mode tangents on the bf16 MFMA layers) against the oracle's
+    forward_grad with the same bf16 rounding points: potential and forces
+    max|d| / max|ref| <= 1e-2 (the tangents are bf16 operands too)."""
+    from exptool_amd.surrogate.mlp import MLPField
+    base = np.array([2, 1.3, 0.6, 0.2, -0.4, 1.7, 0.3, 9.0, 4.0, -0.05])
+    m = MLPField(field, seed=4, conservative=True)
+    m.set_base(base)
+    m.set_scaling([0.3, 1, 1, 1], [-1.0, 0.0, 0.0, 0.0])
+    m.upload()
+    x, y, z = _pts(1000, 11)
+    got = m.forward(x, y, z).cpu().numpy().astype(np.float64)
+    y0, dy = MO.forward_grad(m.params, m.cfg, x, y, z, bf=True)
+    ref = MO.conservative(m.cfg, y0, dy, x, y, z, base)
+    nb = MO.base_field(base, x, y, z)
+    for k in range(4):
+        scale = np.max(np.abs(ref[k] - nb[k])) if k else np.max(np.abs(ref[k]))
+        assert np.max(np.abs(got[k] - ref[k])) / scale <= 1e-2, k
+    m.set_mode(False)
+    m.set_base(None)
+
+
+def test_loss_weights_match_oracle(field):
+    import torch
+    from exptool_amd.surrogate.mlp import MLPField
+    m = MLPField(field, seed=5)
+    lw = (1.0, 0.0, 0.5, 2.0)
+    m.set_mode(False, lw)
+    x, y, z = _pts(555, 12)
+    g = np.random.default_rng(13)
+    t = g.normal(size=(4, 555)).astype(np.float32)
+    gg, loss = m.gradient(x, y, z, torch.tensor(t, device="cuda"))
+    ref, rloss = MO.gradient(m.params, m.cfg, x, y, z, t, None, bf=True, lw=lw)
+    assert float(loss.item()) == pytest.approx(rloss, rel=2e-3)
+    got = gg.cpu().numpy().astype(np.float64)
+    for name, (o, s_) in MO.OFF.items():
+        a, b = got[o:o + s_], ref[o:o + s_]
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 2e-2, name
+    m.set_mode(False)
+
+
 def test_gradient_matches_oracle(field):
     import torch
     from exptool_amd.surrogate.mlp import MLPField
