@@ -241,14 +241,16 @@ class MLPField:
 
     # --------------------------------------------------------------- training
     def fit(self, npoints=16_000_000, steps=20000, batch=65536, lr=2e-3, lr_min=1e-5,
-            seed=0, log_every=0, time_budget=None, base_mn=3, point_weight=None):
+            seed=0, log_every=0, time_budget=None, base_mn=3, point_weight=None, extra=None):
         """Train on `npoints` BFE evaluations drawn uniformly in R < rmax,
         |z| < zmax.  With base_mn >= 0, an analytic base (base_mn
         Miyamoto-Nagai discs and a Hernquist sphere) is first fitted to a
         subsample of the same evaluations and the network learns the
         remainder (base_mn = None: the network learns the whole field).
         point_weight(x, y, z), when given, multiplies each point's loss
-        weight (device tensors in, one weight per point out).
+        weight (device tensors in, one weight per point out).  extra: more
+        training points (x, y, z) (device tensors, e.g. drawn where the
+        orbits to be integrated live), also labelled by the BFE.
         Cosine learning-rate decay.  Returns a training log."""
         dev = self.sys.device
         t0 = time.perf_counter()
@@ -260,8 +262,14 @@ class MLPField:
             x, y, z = sample_domain(m, self.rmax, self.zmax, seed=seed + k, device=dev)
             X.append(torch.stack([x, y, z]))
             T.append(bfe_labels(self.sys, x, y, z))
+        if extra is not None:
+            ex = torch.stack([torch.as_tensor(v, dtype=torch.float32, device=dev).reshape(-1)
+                              for v in extra])
+            X.append(ex)
+            T.append(bfe_labels(self.sys, ex[0], ex[1], ex[2]))
         X = torch.cat(X, dim=1)
         T = torch.cat(T, dim=1)
+        npoints = X.shape[1]
         F2 = (T[1] ** 2 + T[2] ** 2 + T[3] ** 2)
         t_base = time.perf_counter()
         if base_mn is not None:
