@@ -36,7 +36,7 @@ EXPORTS = (
     "bfe_eof_accumulate_batch", "bfe_sl_accumulate_batch", "bfe_field_set_batch",
     "bfe_field_eval_batch", "bfe_accumulate_disc_halo", "bfe_forces_disc_halo",
     "bfe_set_tuning", "bfe_get_tuning", "bfe_kernel_times", "bfe_sl_eval_sph", "bfe_eof_eval_cyl",
-    "bfe_mlp_set_base", "bfe_mlp_set_mode",
+    "bfe_mlp_set_base", "bfe_mlp_set_mode", "bfe_eof_accumulate_noalloc",
 )
 MLP_NPARAMS = 50564
 
@@ -56,6 +56,7 @@ def _declare(lib):
                                      _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
         "bfe_sl_accumulate": (_I32, [_VP, _VP, _VP, _VP, _VP, _I64, _U32, _VP, _VP]),
         "bfe_eof_accumulate": (_I32, [_VP, _VP, _VP, _VP, _VP, _I64, _U32, _VP, _VP, _VP]),
+        "bfe_eof_accumulate_noalloc": (_I32, [_VP, _VP, _VP, _VP, _VP, _I64, _U32, _VP, _VP, _VP]),
         "bfe_sl_eval": (_I32, [_VP, _VP, _VP, _VP, _I64, _VP, _I32, _I32, _U32] + [_VP] * 7 + [_VP]),
         "bfe_eof_eval": (_I32, [_VP, _VP, _VP, _VP, _I64, _VP, _VP, _I32, _I32, _U32] + [_VP] * 7 + [_VP]),
         "bfe_sl_eval_sph": (_I32, [_VP, _VP, _VP, _VP, _I64, _VP, _I32, _I32, _U32, _VP, _VP]),

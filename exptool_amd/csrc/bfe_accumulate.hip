@@ -1333,6 +1333,67 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free EOF accumulate (bfe_eof_accumulate_noalloc)
+// ---------------------------------------------------------------------------
+// Per-segment partial sums of the input-order kernel live in this file-scope
+// array (4 MiB per device, held until the process ends), so the call needs
+// no device allocation.  Host code must not pass the array's name to a
+// kernel: the name is a host-side stand-in, not a device address.
+// noalloc_scratch_init takes the device address with hipGetSymbolAddress
+// once per context (bfe_create, with the context's device current) and the
+// entry point passes that address.  Every context on one device shares the
+// one array: calls that overlap on one GPU race.
+namespace noalloc {
+constexpr int64_t kScratchDoubles = (int64_t)1 << 19;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+}  // namespace noalloc
+
+int noalloc_scratch_init(Context* c) {
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc::scratch));
+  if (e != hipSuccess) {
+    c->noalloc_scratch = nullptr;
+    return check_hip(c, e, "hipGetSymbolAddress (allocation-free accumulate scratch)");
+  }
+  c->noalloc_scratch = (double*)p;
+  return 0;
+}
+
+static int eof_accumulate_noalloc_run(Context* c, const float* x, const float* y, const float* z,
+                                      const float* m, int64_t n, double* cos_out,
+                                      double* sin_out, hipStream_t s) {
+  const int K = c->eof_K;
+  if (K > 1024)
+    return set_error(c, BFE_ELIMIT, "eof_accumulate_noalloc: (2 mmax + 1) norder = %d > 1024", K);
+  // at most kScratchDoubles / K segments of K partial sums each
+  const int64_t cap = std::min<int64_t>(std::max<int64_t>(c->eof_acc_blocks, 1),
+                                        noalloc::kScratchDoubles / K);
+  AccPlan P = plan_single(cap, n, kTile);
+  if (P.S.nseg * (int64_t)K > noalloc::kScratchDoubles)
+    return set_error(c, BFE_ELIMIT, "eof_accumulate_noalloc: %lld partial sums > %lld",
+                     (long long)(P.S.nseg * (int64_t)K), (long long)noalloc::kScratchDoubles);
+  double* partials = c->noalloc_scratch;
+  const EOFGeom g = eof_geom(c);
+  const int opt = (K + 255) / 256;
+  if (n == 0) {
+    hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((P.S.nseg * K + 255) / 256)), dim3(256), 0, s,
+                       partials, P.S.nseg * (int64_t)K);
+  } else if (opt == 1) {
+    hipLaunchKernelGGL(eof_accumulate_kernel<1>, dim3(P.grid), dim3(256), 0, s, x, y, z, m, P.S, g, c->eof_pot32, K, partials);
+  } else if (opt == 2) {
+    hipLaunchKernelGGL(eof_accumulate_kernel<2>, dim3(P.grid), dim3(256), 0, s, x, y, z, m, P.S, g, c->eof_pot32, K, partials);
+  } else {
+    hipLaunchKernelGGL(eof_accumulate_kernel<4>, dim3(P.grid), dim3(256), 0, s, x, y, z, m, P.S, g, c->eof_pot32, K, partials);
+  }
+  BFE_HIP(c, hipGetLastError());
+  const int nc = (c->mmax + 1) * c->norder;
+  hipLaunchKernelGGL(reduce_partials_kernel<true>, dim3((K + kRedCols - 1) / kRedCols, 1), dim3(256), 0,
+                     s, partials, (int)P.S.nseg, K, cos_out, nullptr, sin_out, nc, c->norder);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
 int eof_accumulate_one(Context* c, const float* x, const float* y, const float* z,
                        const float* m, int64_t n, double* cos_out, double* sin_out, hipStream_t s) {
   return eof_accumulate_run(c, x, y, z, m, n, nullptr, 1, cos_out, sin_out, s, true);
@@ -1366,6 +1427,25 @@ extern "C" int bfe_eof_accumulate(bfe_ctx* h, const float* x, const float* y, co
   if (rc) return rc;
   BFE_HIP(c, hipSetDevice(c->device));
   return eof_accumulate_run(c, x, y, z, m, n, nullptr, 1, cos_out, sin_out, (hipStream_t)stream);
+}
+
+// The input-order path of bfe_eof_accumulate with its partial sums in the
+// library's static scratch array: no device allocation, memset or sort.
+extern "C" int bfe_eof_accumulate_noalloc(bfe_ctx* h, const float* x, const float* y,
+                                          const float* z, const float* m, int64_t n,
+                                          uint32_t flags, double* cos_out, double* sin_out,
+                                          void* stream) {
+  (void)flags;  // as bfe_eof_accumulate (eof.py:545-551)
+  if (!h) return BFE_EINVAL;
+  Context* c = &h->c;
+  if (!c->eof_ok) return set_error(c, BFE_ENOTABLE, "eof_accumulate_noalloc: EOF tables not set");
+  if (!cos_out || !sin_out) return set_error(c, BFE_EINVAL, "eof_accumulate_noalloc: NULL output");
+  int rc = acc_common_checks(c, x, y, z, m, n, "eof_accumulate_noalloc");
+  if (rc) return rc;
+  if (!c->noalloc_scratch)
+    return set_error(c, BFE_EHIP, "eof_accumulate_noalloc: no device address for the scratch array");
+  BFE_HIP(c, hipSetDevice(c->device));
+  return eof_accumulate_noalloc_run(c, x, y, z, m, n, cos_out, sin_out, (hipStream_t)stream);
 }
 
 extern "C" int bfe_eof_accumulate_batch(bfe_ctx* h, const float* x, const float* y, const float* z,

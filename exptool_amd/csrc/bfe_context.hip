@@ -127,6 +127,11 @@ extern "C" int bfe_create(int device, bfe_ctx** out) {
     return set_error(nullptr, BFE_EHIP, "bfe_create: hipSetDevice: %s", hipGetErrorString(e));
   bfe_ctx* h = new bfe_ctx();
   h->c.device = device;
+  // with the context's device current: the allocation-free accumulate's
+  // scratch address on this device (a failure is reported by that entry
+  // point, which then refuses to run)
+  (void)noalloc_scratch_init(&h->c);
+  h->c.err.clear();
   *out = h;
   return 0;
 }

@@ -133,6 +133,9 @@ struct Context {
   // ---- scratch (grown on demand, freed by bfe_destroy) -------------------
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  // device address of the static array of the allocation-free accumulate
+  // (bfe_accumulate.hip), taken once at bfe_create; NULL if unavailable
+  double* noalloc_scratch = nullptr;
 
   // ---- tuning (bfe_set_tuning) --------------------------------------------
   int64_t eof_acc_blocks = 2048;   // EOF accumulate: blocks (single) / segment budget (batch)
@@ -271,6 +274,9 @@ void swap_work(Context* c);
 int set_error(Context* c, int code, const char* fmt, ...);
 int check_hip(Context* c, hipError_t e, const char* what);
 int ensure_scratch(Context* c, size_t bytes);
+// Takes the device address of the allocation-free accumulate's static
+// scratch array for the current device (bfe_create).
+int noalloc_scratch_init(Context* c);
 // Lays out eof_dep32 for the cell-ordered EOF accumulate's current lane
 // shape (rows of norder rounded up to the shape's chunk length), rebuilding
 // it from eof_pot32 when the shape's row length differs (synchronous).

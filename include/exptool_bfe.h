@@ -98,6 +98,18 @@ int bfe_eof_accumulate(bfe_ctx* ctx, const float* x, const float* y, const float
                        const float* m, int64_t n, uint32_t flags, double* cos_out,
                        double* sin_out, void* stream);
 
+/* No counterpart in the reference: bfe_eof_accumulate for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results as
+ * bfe_eof_accumulate in its input-order form (no cell sort): the partial sums
+ * live in a static 4 MiB device array of the library, so the call allocates
+ * nothing and issues no memset.  Limit: (2 mmax + 1) norder <= 1024; any n.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (any context, any stream) race. */
+int bfe_eof_accumulate_noalloc(bfe_ctx* ctx, const float* x, const float* y, const float* z,
+                               const float* m, int64_t n, uint32_t flags, double* cos_out,
+                               double* sin_out, void* stream);
+
 /* ---- batch evaluation --------------------------------------------------- */
 /* replaces: spheresl.all_eval_particles (spheresl.py:1240-1362) /
  * eval_particles (spheresl.py:502-528).  coef: device ((lmax+1)^2, nmax).
