@@ -33,6 +33,7 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0          # dense bf16 MFMA
 PMC_SUMMARY = os.path.join(REPO, "profiles", "pmc_summary.json")
 
 
@@ -402,12 +403,19 @@ def mlp_stage(a, sysm, st0, orbit):
     berr, _ = m.force_error(hx, hy, hz)
     m.params, m.cfg = keep
     m.upload()
-    # MLP forward cost 2 (3*128 + 3*128^2 + 128*4) flops; at the dense bf16
-    # MFMA peak that bounds the surrogate's step rate
-    ceiling = 2.5e15 / 100096.0
+    # network cost per evaluation: the forward pass 2 (3*128 + 3*128^2 + 128*4)
+    # = 100 096 flops; the conservative mode also carries the three tangent
+    # directions through the hidden layers (3 x 3 x 2*128^2) and the
+    # potential row of the output (3 x 2*128): 395 776 flops.  At the dense
+    # bf16 MFMA peak that bounds the surrogate's step rate.
+    flops_eval = 100096.0 + 3 * (3 * 2 * 128 * 128 + 2 * 128)
+    ceiling = BF16_PEAK_TFLOPS * 1e12 / flops_eval
     return {"mode": "conservative (F = -grad of the learned potential + analytic base)",
             "orbit_steps_per_s": rate, "speedup_vs_bfe": rate / orbit["orbit_steps_per_s"],
+            "network_flops_per_eval": flops_eval,
+            "mfma_frac": rate * flops_eval / (BF16_PEAK_TFLOPS * 1e12),
             "speedup_ceiling_at_mfma_peak": ceiling / orbit["orbit_steps_per_s"],
+            "speedup_ceiling_direct_mode": BF16_PEAK_TFLOPS * 1e12 / 100096.0 / orbit["orbit_steps_per_s"],
             "train_disc_points": a.mlp_disc_points,
             "base": rep.get("base"), "base_fit_seconds": rep.get("base_seconds"),
             "force_rel_err_base_alone_median": float(np.median(berr)),

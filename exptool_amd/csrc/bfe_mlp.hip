@@ -68,7 +68,14 @@ template <bool TRAIN>
 constexpr int block_threads() { return TRAIN ? kBlockTrain : kBlockInf; }
 template <bool TRAIN>
 constexpr int block_points() { return kWaveTile * block_threads<TRAIN>() / 64; }
-constexpr size_t kLdsBytes = 3 * kH * kH * sizeof(__bf16) + kSmallFloats * sizeof(float);
+// LDS copy of the three hidden weight matrices: rows of kH bf16 padded to
+// kLdsRow.  A lane reads 16 bytes of row 16 mt + c; unpadded (256-byte)
+// rows put the rows that one read cycle serves on the same four banks, with
+// 16 bytes of padding they cover every bank once.
+constexpr int kLdsRow = kH + 8;
+constexpr int kLdsLayer = kH * kLdsRow;
+constexpr size_t kLdsWBytes = 3 * kLdsLayer * sizeof(__bf16);
+constexpr size_t kLdsBytes = kLdsWBytes + kSmallFloats * sizeof(float);
 
 __device__ __forceinline__ floatx4 mfma16(bf16x8 a, bf16x8 b, floatx4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -181,9 +188,11 @@ __device__ __forceinline__ void load_small(const float* __restrict__ params, flo
 }
 
 __device__ __forceinline__ void load_weights(const __bf16* __restrict__ W, __bf16* s_w) {
+  constexpr int kRow16 = kH / 8, kLds16 = kLdsRow / 8;   // 16-byte chunks per row
   const uint4* src = reinterpret_cast<const uint4*>(W);
   uint4* dst = reinterpret_cast<uint4*>(s_w);
-  for (int t = threadIdx.x; t < 3 * kH * kH / 8; t += blockDim.x) dst[t] = src[t];
+  for (int t = threadIdx.x; t < 3 * kH * kRow16; t += blockDim.x)
+    dst[(t / kRow16) * kLds16 + t % kRow16] = src[t];
 }
 
 // Hidden layer: acc[mt][nt] = bias + W (K-permuted, LDS) x B[kk][nt].
@@ -203,11 +212,34 @@ __device__ __forceinline__ void hidden_mfma(const __bf16* __restrict__ sW, const
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(sW + (16 * mt + c) * kH + 32 * kk + 8 * g);
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(sW + (16 * mt + c) * kLdsRow + 32 * kk + 8 * g);
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma16(a, B[kk][nt], acc[mt][nt]);
     }
   }
+}
+
+// SiLU of hidden layer `layer`'s pre-activations acc: into the next layer's
+// bf16 operands B, or (LAST) in place as float32 for the output layer.
+// With TRAIN the pre-activations are stored to Z for points with valid[nt].
+template <bool TRAIN, bool LAST>
+__device__ __forceinline__ void forward_activate(floatx4 (&acc)[8][kNT], bf16x8 (&B)[4][kNT],
+                                                 float* Z, int64_t bcap,
+                                                 const int64_t (&pid)[kNT],
+                                                 const bool (&valid)[kNT], int layer, int g) {
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < kNT; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float zz = acc[mt][nt][i];
+        if (TRAIN && valid[nt])
+          Z[((size_t)(layer + 1) * kH + 16 * mt + 4 * g + i) * bcap + pid[nt]] = zz;
+        const float hh = silu(zz);
+        if (LAST) acc[mt][nt][i] = hh;   // H4 in place (Z4 is reloaded in backward)
+        else B[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)hh;
+      }
 }
 
 // Forward pass of one wave tile (kNT x 16 points, column c of lane group g):
@@ -241,23 +273,15 @@ __device__ __forceinline__ void mlp_tile_forward(const __bf16* __restrict__ sW,
       }
     }
   floatx4 acc[8][kNT];
+  // layers 1 and 2 feed the next layer's operands; the last layer is peeled
+  // so that no activation carries a branch on the layer
 #pragma unroll 1
-  for (int layer = 0; layer < 3; ++layer) {
-    hidden_mfma(sW + layer * kH * kH, sBh + layer * kH, B, acc, g, c);
-#pragma unroll
-    for (int mt = 0; mt < 8; ++mt)
-#pragma unroll
-      for (int nt = 0; nt < kNT; ++nt)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float zz = acc[mt][nt][i];
-          if (TRAIN && valid[nt])
-            Z[((size_t)(layer + 1) * kH + 16 * mt + 4 * g + i) * bcap + pid[nt]] = zz;
-          const float hh = silu(zz);
-          if (layer < 2) B[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)hh;
-          else acc[mt][nt][i] = hh;   // H4 in place (Z4 is reloaded in backward)
-        }
+  for (int layer = 0; layer < 2; ++layer) {
+    hidden_mfma(sW + layer * kLdsLayer, sBh + layer * kH, B, acc, g, c);
+    forward_activate<TRAIN, false>(acc, B, Z, bcap, pid, valid, layer, g);
   }
+  hidden_mfma(sW + 2 * kLdsLayer, sBh + 2 * kH, B, acc, g, c);
+  forward_activate<TRAIN, true>(acc, B, Z, bcap, pid, valid, 2, g);
   // output layer (float32): partial sums over this lane's neurons, then
   // reduce over the four lane groups that hold the same point column
 #pragma unroll
@@ -281,6 +305,47 @@ __device__ __forceinline__ void mlp_tile_forward(const __bf16* __restrict__ sW,
 // W_l t_l^k one more bf16 MFMA pass over the layer's LDS weights (float32
 // accumulation, tangents rounded to bf16 as the activations are), and
 // dy0/df_k = W4[0] . t_4^k.  Layout as mlp_tile_forward; NT 16-point tiles.
+//
+// One hidden layer W of it: the value pass (activations into B, or LAST: the
+// potential row of the output layer into ys) and the three tangent passes
+// (into T, or LAST: into ds).
+template <int NT, bool LAST>
+__device__ __forceinline__ void grad_layer(const __bf16* __restrict__ W, const float* bias,
+                                           const float* __restrict__ sW4, bf16x8 (&B)[4][NT],
+                                           bf16x8 (&T)[3][4][NT], float (&ys)[NT],
+                                           float (&ds)[NT][3], int g, int c) {
+  floatx4 acc[8][NT];
+  float dsl[8][NT][4];
+  hidden_mfma(W, bias, B, acc, g, c);
+#pragma unroll
+  for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float zz = acc[mt][nt][i];
+        const float sg = sigmoid(zz);
+        const float hh = zz * sg;
+        dsl[mt][nt][i] = sg * (1.f + zz * (1.f - sg));
+        if (LAST) ys[nt] = fmaf(sW4[16 * mt + 4 * g + i], hh, ys[nt]);
+        else B[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)hh;
+      }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    hidden_mfma(W, (const float*)nullptr, T[k], acc, g, c);
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float tt = dsl[mt][nt][i] * acc[mt][nt][i];
+          if (LAST) ds[nt][k] = fmaf(sW4[16 * mt + 4 * g + i], tt, ds[nt][k]);
+          else T[k][mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)tt;
+        }
+  }
+}
+
 template <int NT>
 __device__ __forceinline__ void mlp_tile_grad(const __bf16* __restrict__ sW,
                                               const float* __restrict__ sS,
@@ -316,40 +381,12 @@ __device__ __forceinline__ void mlp_tile_grad(const __bf16* __restrict__ sW,
     ys[nt] = 0.f;
     ds[nt][0] = ds[nt][1] = ds[nt][2] = 0.f;
   }
-  floatx4 acc[8][NT], tacc[8][NT];
-  float dsl[8][NT][4];
+  // layers 1 and 2 feed the next layer's operands; the last layer is peeled
+  // so that no activation carries a branch on the layer
 #pragma unroll 1
-  for (int layer = 0; layer < 3; ++layer) {
-    const __bf16* W = sW + layer * kH * kH;
-    hidden_mfma(W, sBh + layer * kH, B, acc, g, c);
-#pragma unroll
-    for (int mt = 0; mt < 8; ++mt)
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float zz = acc[mt][nt][i];
-          const float sg = sigmoid(zz);
-          const float hh = zz * sg;
-          dsl[mt][nt][i] = sg * (1.f + zz * (1.f - sg));
-          if (layer < 2) B[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)hh;
-          else ys[nt] = fmaf(sW4[16 * mt + 4 * g + i], hh, ys[nt]);
-        }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      hidden_mfma(W, (const float*)nullptr, T[k], tacc, g, c);
-#pragma unroll
-      for (int mt = 0; mt < 8; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const float tt = dsl[mt][nt][i] * tacc[mt][nt][i];
-            if (layer < 2) T[k][mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)tt;
-            else ds[nt][k] = fmaf(sW4[16 * mt + 4 * g + i], tt, ds[nt][k]);
-          }
-    }
-  }
+  for (int layer = 0; layer < 2; ++layer)
+    grad_layer<NT, false>(sW + layer * kLdsLayer, sBh + layer * kH, sW4, B, T, ys, ds, g, c);
+  grad_layer<NT, true>(sW + 2 * kLdsLayer, sBh + 2 * kH, sW4, B, T, ys, ds, g, c);
   // sums over the four lane groups that hold the same point column
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
@@ -392,7 +429,7 @@ __global__ __launch_bounds__(block_threads<TRAIN>()) void mlp_fwd_kernel(MLPKArg
   constexpr int kBlock = block_threads<TRAIN>(), kBlockPts = block_points<TRAIN>();
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
-  float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
+  float* sS = reinterpret_cast<float*>(smem + kLdsWBytes);
   __shared__ double s_loss[kBlock / 64];
 
   load_small(a.params, sS);
@@ -466,7 +503,7 @@ __global__ __launch_bounds__(kBlockGrad) void mlp_fwd_grad_kernel(MLPKArgs a) {
   constexpr int NT = 1, kBlockPts = 16 * NT * (kBlockGrad / 64);
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
-  float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
+  float* sS = reinterpret_cast<float*>(smem + kLdsWBytes);
   load_small(a.params, sS);
   load_weights(a.Wp, sW);
   __syncthreads();
@@ -501,7 +538,7 @@ __global__ __launch_bounds__(kBlockTrain) void mlp_bwd_kernel(MLPKArgs a) {
   constexpr int kBlockPts = block_points<true>();
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
-  float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
+  float* sS = reinterpret_cast<float*>(smem + kLdsWBytes);
   const float* sW4 = sS + kH * kNIn + 4 * kH;
   load_small(a.params, sS);
   load_weights(a.Wt, sW);
@@ -543,7 +580,7 @@ __global__ __launch_bounds__(kBlockTrain) void mlp_bwd_kernel(MLPKArgs a) {
     floatx4 acc[8][kNT];
 #pragma unroll 1
     for (int layer = 2; layer >= 0; --layer) {
-      hidden_mfma(sW + layer * kH * kH, nullptr, Bg, acc, g, c);
+      hidden_mfma(sW + layer * kLdsLayer, nullptr, Bg, acc, g, c);
 #pragma unroll
       for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
@@ -792,7 +829,7 @@ __global__ __launch_bounds__(GRAD ? kBlockGrad : kBlockInf) void mlp_leapfrog_ke
   constexpr int kBlock = GRAD ? kBlockGrad : kBlockInf, kBlockPts = 16 * NT * (kBlock / 64);
   extern __shared__ __align__(16) unsigned char smem[];
   __bf16* sW = reinterpret_cast<__bf16*>(smem);
-  float* sS = reinterpret_cast<float*>(smem + 3 * kH * kH * sizeof(__bf16));
+  float* sS = reinterpret_cast<float*>(smem + kLdsWBytes);
   __shared__ double s_e[kBlock / 64];
   load_small(a.params, sS);
   load_weights(a.Wp, sW);

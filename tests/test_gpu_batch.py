@@ -162,3 +162,52 @@ def test_batch_field_invalidated_by_new_tables():
     s.set_eof(et)
     with pytest.raises(BFEError, match="no batch field"):
         s.field_forces_batch(x, x, x, counts)
+
+
+def test_batch_barred_snapshots_against_oracle():
+    """The benchmark's batch workload at reduced snapshot count: barred discs
+    (m = 2, A ~ U[0, 0.3]) of 200k-500k particles and their halos, sampled on
+    the device, on the benchmark's bases.  Every snapshot's coefficients
+    against the FP64 oracle (<= 1e-6 per block) and each snapshot's field at
+    a subsample of its own particles against the oracle's Cartesian field."""
+    import torch
+    from exptool_amd.bfe import BFESystem
+    from exptool_amd.utils import workloads as W
+    st, et = W.disc_halo_tables(seed=1)
+    s = BFESystem(sl=st, eof=et)
+    slo = O.SLTable(st["lmax"], st["nmax"], st["numr"], st["cmap"], st["rmin"], st["rmax"],
+                    st["scale"], st["ev"], st["ef"], st["xi"], st["p0"], st["d0"])
+    eo = O.EOFTable(et["mmax"], et["norder"], et["numx"], et["numy"], et["cmap"], et["rmin"],
+                    et["rmax"], et["ascale"], et["hscale"], et["potC"], et["rforceC"],
+                    et["zforceC"], et["potS"], et["rforceS"], et["zforceS"])
+    b = W.batch_snapshots(nsnap=3, lo=200_000, hi=500_000, seed=4)
+    dc, hc = b["disc_counts"], b["halo_counts"]
+    print(f"barred batch: disc counts {list(dc)}, bar amplitudes {np.round(b['amp'], 3)}", flush=True)
+    assert dc.min() >= 200_000 and b["amp"].max() > 0.1
+    c, sn = s.eof_accumulate_batch(*b["disc"], dc)
+    h = s.sl_accumulate_batch(*b["halo"], hc)
+    s.set_field_batch(c, sn, h)
+    fd = [v.cpu().numpy().astype(np.float64) for v in s.field_forces_batch(*b["disc"][:3], dc)]
+    torch.cuda.synchronize()
+    c, sn, h = (v.cpu().numpy() for v in (c, sn, h))
+    D = [v.cpu().numpy().astype(np.float64) for v in b["disc"]]
+    H = [v.cpu().numpy().astype(np.float64) for v in b["halo"]]
+    od, oh = np.concatenate([[0], np.cumsum(dc)]), np.concatenate([[0], np.cumsum(hc)])
+    rng = np.random.default_rng(7)
+    for k in range(3):
+        dk = [v[od[k]:od[k + 1]] for v in D]
+        hk = [v[oh[k]:oh[k + 1]] for v in H]
+        rc, rs = O.eof_accumulate(eo, *dk)
+        assert block_rel_eof(c[k], rc) <= 1e-6, k
+        assert block_rel_eof(sn[k][1:], rs[1:]) <= 1e-6, k
+        # the bar shows in the m = 2 block in proportion to its amplitude
+        assert np.max(np.abs(rc[2])) > 0.1 * b["amp"][k] * np.max(np.abs(rc[0]))
+        rh = O.sl_accumulate(slo, *hk)
+        assert block_rel_sl(h[k], rh, slo.lmax) <= 1e-6, k
+        sub = rng.choice(dc[k], 3000, replace=False)
+        ref = O.total_force(O.FieldSpec(eo, rc, rs, slo, rh), *(v[sub] for v in dk[:3]))
+        Fg = np.stack([fd[j][od[k]:od[k + 1]][sub] for j in range(3)])
+        Fr = np.stack(ref[:3])
+        e = np.linalg.norm(Fg - Fr, axis=0) / np.linalg.norm(Fr, axis=0)
+        assert np.max(e) <= 1e-5, (k, np.max(e))
+        assert np.max(np.abs(fd[3][od[k]:od[k + 1]][sub] - ref[3]) / np.abs(ref[3])) <= 1e-5
