@@ -314,36 +314,54 @@ __device__ __forceinline__ void grad_layer(const __bf16* __restrict__ W, const f
                                            const float* __restrict__ sW4, bf16x8 (&B)[4][NT],
                                            bf16x8 (&T)[3][4][NT], float (&ys)[NT],
                                            float (&ds)[NT][3], int g, int c) {
-  floatx4 acc[8][NT];
-  float dsl[8][NT][4];
-  hidden_mfma(W, bias, B, acc, g, c);
+  // One sweep over W: each A fragment is read from LDS once and multiplied
+  // into the activations (acc[0], from the bias) and the three tangents
+  // (acc[1..3], from zero), 4 NT independent products per fragment.
+  floatx4 acc[4][8][NT];
+  const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kk = 0; kk < 4; ++kk) {
+    // one K-step's operand loads in flight at a time (register budget)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int mt = 0; mt < 8; ++mt) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + (16 * mt + c) * kLdsRow + 32 * kk + 8 * g);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        if (kk == 0) {
+          floatx4 b0;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) b0[i] = bias[16 * mt + 4 * g + i];
+          acc[0][mt][nt] = mfma16(a, B[0][nt], b0);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) acc[1 + k][mt][nt] = mfma16(a, T[k][0][nt], zero);
+        } else {
+          acc[0][mt][nt] = mfma16(a, B[kk][nt], acc[0][mt][nt]);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) acc[1 + k][mt][nt] = mfma16(a, T[k][kk][nt], acc[1 + k][mt][nt]);
+        }
+      }
+    }
+  }
 #pragma unroll
   for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const float zz = acc[mt][nt][i];
+        const float zz = acc[0][mt][nt][i];
         const float sg = sigmoid(zz);
         const float hh = zz * sg;
-        dsl[mt][nt][i] = sg * (1.f + zz * (1.f - sg));
+        const float dsl = sg * (1.f + zz * (1.f - sg));
         if (LAST) ys[nt] = fmaf(sW4[16 * mt + 4 * g + i], hh, ys[nt]);
         else B[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)hh;
-      }
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    hidden_mfma(W, (const float*)nullptr, T[k], acc, g, c);
-#pragma unroll
-    for (int mt = 0; mt < 8; ++mt)
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float tt = dsl[mt][nt][i] * acc[mt][nt][i];
+        for (int k = 0; k < 3; ++k) {
+          const float tt = dsl * acc[1 + k][mt][nt][i];
           if (LAST) ds[nt][k] = fmaf(sW4[16 * mt + 4 * g + i], tt, ds[nt][k]);
           else T[k][mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)tt;
         }
-  }
+      }
 }
 
 template <int NT>
