@@ -156,8 +156,10 @@ class BFESystem:
         self.ctx.check(rc, "bfe_sl_accumulate")
         return out
 
-    def eof_accumulate(self, x, y, z, m, stream=None):
-        """Cylindrical coefficients (cos, sin), each (mmax+1, norder) float64."""
+    def eof_accumulate(self, x, y, z, m, bf16=False, stream=None):
+        """Cylindrical coefficients (cos, sin), each (mmax+1, norder) float64.
+        bf16: the bf16 MFMA mode (cell corner sums times the table as bf16
+        hi + lo pieces; about 1e-5 relative)."""
         self._need("eof")
         x, y, z, n = self._xyz(x, y, z)
         m = f32(m, self.device)
@@ -166,8 +168,9 @@ class BFESystem:
         shape = (self.eof["mmax"] + 1, self.eof["norder"])
         cos = torch.empty(shape, dtype=torch.float64, device=self.device)
         sin = torch.empty(shape, dtype=torch.float64, device=self.device)
-        rc = self.ctx.lib.bfe_eof_accumulate(self.ctx.h, ptr(x), ptr(y), ptr(z), ptr(m), n, 0,
-                                             ptr(cos), ptr(sin), stream_ptr(stream))
+        rc = self.ctx.lib.bfe_eof_accumulate(self.ctx.h, ptr(x), ptr(y), ptr(z), ptr(m), n,
+                                             _lib.ACC_BF16 if bf16 else 0, ptr(cos), ptr(sin),
+                                             stream_ptr(stream))
         self.ctx.check(rc, "bfe_eof_accumulate")
         return cos, sin
 
@@ -246,9 +249,11 @@ class BFESystem:
             raise ValueError(f"snapshot sizes sum to {off[-1]}, not {n} particles")
         return np.ascontiguousarray(off, dtype=np.int64)
 
-    def eof_accumulate_batch(self, x, y, z, m, counts, stream=None):
+    def eof_accumulate_batch(self, x, y, z, m, counts, bf16=False, stream=None):
         """Coefficients of many snapshots in one launch: particles concatenated,
-        counts[s] of snapshot s.  Returns (cos, sin), each (nsnap, mmax+1, norder)."""
+        counts[s] of snapshot s.  Returns (cos, sin), each (nsnap, mmax+1, norder).
+        bf16: the bf16 MFMA mode, one (snapshots x nodes) x (nodes x norder)
+        product per row over all snapshots."""
         self._need("eof")
         x, y, z, n = self._xyz(x, y, z)
         m = f32(m, self.device)
@@ -258,8 +263,8 @@ class BFESystem:
         cos = torch.empty(shape, dtype=torch.float64, device=self.device)
         sin = torch.empty(shape, dtype=torch.float64, device=self.device)
         rc = self.ctx.lib.bfe_eof_accumulate_batch(self.ctx.h, ptr(x), ptr(y), ptr(z), ptr(m),
-                                                   host_ptr(off), ns, 0, ptr(cos), ptr(sin),
-                                                   stream_ptr(stream))
+                                                   host_ptr(off), ns, _lib.ACC_BF16 if bf16 else 0,
+                                                   ptr(cos), ptr(sin), stream_ptr(stream))
         self.ctx.check(rc, "bfe_eof_accumulate_batch")
         return cos, sin
 

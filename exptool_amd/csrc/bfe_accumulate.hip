@@ -666,7 +666,6 @@ __device__ __forceinline__ void sl_deposit_body(
     const float* __restrict__ E, const double* __restrict__ lmc, int no_odd,
     double* __restrict__ partials) {
   const int R = (g.lmax + 1) * (g.lmax + 1);
-  const int LN = (g.lmax + 1) * g.nmax;
   const int K = R * g.nmax;
   const int N = g.nmax;
   const int NC = (N + CH - 1) / CH;
@@ -1052,6 +1051,317 @@ __global__ __launch_bounds__(256) void sl_mfma_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// EOF accumulate as matrix products on the bf16 MFMA units (BFE_ACC_BF16)
+// ---------------------------------------------------------------------------
+// With the corner sums of each cell
+//   S[s][cell][j][c] = sum_{p in cell} w_j(p) c_c(p)
+// (w_j = -4 pi m {1, cos k phi, sin k phi}, c_c the bilinear corner weights)
+// and their node sums G[s][j][node] (each node collects the corner of the up
+// to four cells that touch it), the coefficients of snapshot s are
+//   a[s][j][n] = sum_node G[s][j][node] T[node][j][n],
+// for each row j a (snapshots x nodes) x (nodes x norder) matrix product.
+//   eof_cell_sums_kernel   S from particles in cell order (any order is
+//                          correct, cell order makes the runs long)
+//   eof_node_pieces_kernel G, split into bf16 pieces hi + lo, [j][s][node]
+//   eof_mfma_kernel        per row and 16 snapshots, hi*hi + hi*lo + lo*hi
+//                          on v_mfma_f32_16x16x32_bf16, float32 sums over a
+//                          block's 512 nodes
+//   eof_mfma_reduce_kernel float64 sum of the block partials in block order
+// T is the float32 potential table split into bf16 pieces once per table set
+// (eof_mfma_prepare).  The pieces carry about 16 bits, so the mode is good
+// to about 1e-5 of the larger terms (the float32 mode's 1e-7 class is not
+// its aim).
+constexpr int kMfmaKBlock = 512;    // nodes per block of the product (4 waves x 4 K-steps x 32)
+constexpr int kCellSlotsMax = 128;  // (row, corner) slots: 4 (2 mmax + 1), two per lane
+
+// Lanes own slots sl = lane + 64 o (row sl / 4, corner sl % 4).  Each wave
+// walks a contiguous quarter of its block's segment, 64 particles per tile:
+// lane q computes particle q's cell, corner weights and row weights into
+// LDS, then every lane sums its slot over the tile's particles in order,
+// adding the run sum to S (float64 hardware atomics) where the cell changes.
+// The sums run in float32 over at most kFlush particles, then in float64.
+__global__ __launch_bounds__(256) void eof_cell_sums_kernel(
+    const float4* __restrict__ P, Segments S, const int64_t* __restrict__ off, int nsnap,
+    EOFGeom g, float* __restrict__ cs) {
+  extern __shared__ __align__(16) unsigned char dsm[];
+  const int J = 2 * g.mmax + 1, NS = 4 * J;
+  const int64_t ncell = (int64_t)g.numx * g.numy;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float* sW = reinterpret_cast<float*>(dsm) + (size_t)wave * (J * kDepStride + kDepExtra);
+  float* sE = sW + J * kDepStride;
+  int* sC = reinterpret_cast<int*>(sE + 4 * 64);
+  int jrow[2], corner[2];
+#pragma unroll
+  for (int o = 0; o < 2; ++o) {
+    const int sl = lane + 64 * o;
+    jrow[o] = sl < NS ? sl >> 2 : 0;
+    corner[o] = sl & 3;
+  }
+  const bool own0 = lane < NS, own1 = lane + 64 < NS;
+  for (int64_t sg = blockIdx.x; sg < S.nseg; sg += gridDim.x) {
+    int64_t s0, s1;
+    segment_range(S, sg, s0, s1);
+    if (s1 <= s0) continue;
+    int snap = 0;
+    if (off) {
+      int lo = 0, hi = nsnap - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= s0) lo = mid; else hi = mid - 1;
+      }
+      snap = lo;
+    }
+    float* out = cs + (size_t)snap * ncell * NS;
+    const int64_t quarter = ((s1 - s0 + 3) / 4 + 63) / 64 * 64;
+    const int64_t w0 = s0 + wave * quarter;
+    const int64_t w1 = w0 + quarter < s1 ? w0 + quarter : s1;
+    int cur = -1, cnt = 0;
+    float acc[2] = {0.f, 0.f};
+    double accd[2] = {0.0, 0.0};
+    auto flush = [&]() {
+      if (cur >= 0) {
+        float* o = out + (size_t)cur * NS;
+        if (own0) unsafeAtomicAdd(o + lane, (float)(accd[0] + (double)acc[0]));
+        if (own1) unsafeAtomicAdd(o + lane + 64, (float)(accd[1] + (double)acc[1]));
+      }
+      acc[0] = acc[1] = 0.f;
+      accd[0] = accd[1] = 0.0;
+      cnt = 0;
+    };
+    for (int64_t t0 = w0; t0 < w1; t0 += 64) {
+      const int64_t p = t0 + lane;
+      if (p < w1) {
+        const float4 q4 = P[p];
+        const double x = q4.x, y = q4.y, z = q4.z;
+        const double R = sqrt(x * x + y * y + 1.e-10);
+        double c1, s1v;
+        planar_angle(x, y, c1, s1v);
+        const EOFCell cell = eof_locate(R, z, g.cmap, g.ascale, g.ihscale, g.xmin, g.idx, g.ymin,
+                                        g.idy, g.numx, g.numy);
+        const double w = -kFourPi * (double)q4.w;
+        sC[lane] = cell.ix * g.numy + cell.iy;
+        sE[lane] = (float)cell.c00;
+        sE[64 + lane] = (float)cell.c10;
+        sE[128 + lane] = (float)cell.c01;
+        sE[192 + lane] = (float)cell.c11;
+        sW[lane] = (float)w;
+        double cm = 1.0, sm = 0.0, cm1 = c1, sm1 = -s1v;
+        for (int m = 1; m <= g.mmax; ++m) {
+          const double cn = (m == 1) ? c1 : 2.0 * c1 * cm - cm1;
+          const double sn = (m == 1) ? s1v : 2.0 * c1 * sm - sm1;
+          cm1 = cm; sm1 = sm; cm = cn; sm = sn;
+          sW[m * kDepStride + lane] = (float)(w * cm);
+          sW[(g.mmax + m) * kDepStride + lane] = (float)(w * sm);
+        }
+      }
+      wave_sync();
+      const int np = w1 - t0 < 64 ? (int)(w1 - t0) : 64;
+      if (cur < 0) cur = sC[0];
+#pragma unroll 1
+      for (int q0 = 0; q0 < np; q0 += 4) {
+        const int4 cc = *reinterpret_cast<const int4*>(sC + q0);
+        const bool run = q0 + 4 <= np && cc.x == cur && cc.y == cur && cc.z == cur && cc.w == cur;
+        if (__builtin_amdgcn_readfirstlane((int)run)) {
+          if (cnt + 4 > kFlush) {
+            accd[0] += (double)acc[0];
+            accd[1] += (double)acc[1];
+            acc[0] = acc[1] = 0.f;
+            cnt = 0;
+          }
+#pragma unroll
+          for (int o = 0; o < 2; ++o) {
+            const float4 wv = *reinterpret_cast<const float4*>(sW + jrow[o] * kDepStride + q0);
+            const float4 ev = *reinterpret_cast<const float4*>(sE + corner[o] * 64 + q0);
+            acc[o] = fmaf(wv.x, ev.x, acc[o]);
+            acc[o] = fmaf(wv.y, ev.y, acc[o]);
+            acc[o] = fmaf(wv.z, ev.z, acc[o]);
+            acc[o] = fmaf(wv.w, ev.w, acc[o]);
+          }
+          cnt += 4;
+          continue;
+        }
+#pragma unroll 1
+        for (int q = q0; q < q0 + 4 && q < np; ++q) {
+          const int b = sC[q];
+          if (b != cur) {
+            flush();
+            cur = b;
+          } else if (cnt == kFlush) {
+            accd[0] += (double)acc[0];
+            accd[1] += (double)acc[1];
+            acc[0] = acc[1] = 0.f;
+            cnt = 0;
+          }
+#pragma unroll
+          for (int o = 0; o < 2; ++o)
+            acc[o] = fmaf(sW[jrow[o] * kDepStride + q], sE[corner[o] * 64 + q], acc[o]);
+          ++cnt;
+        }
+      }
+      wave_sync();
+    }
+    flush();
+    cur = -1;
+  }
+}
+
+// Node sums of the cell corner sums as bf16 pieces hi = bf16(G),
+// lo = bf16(G - hi) in [j][s][node] order.  Block (X, s): the nodes of grid
+// column X of snapshot s.  The cells of columns X and X-1 (numy rows of NS
+// floats each, contiguous) are staged in LDS with an odd row stride; node
+// (X, Y) takes corner c00 of cell (X, Y), c10 of (X-1, Y), c01 of (X, Y-1)
+// and c11 of (X-1, Y-1).  The nodes past the grid are written as zero by the
+// last column's blocks; rows of snapshots past nsnap (grid.y < nsnap) are
+// neither written nor read (the product masks them).
+__global__ __launch_bounds__(256) void eof_node_pieces_kernel(const float* __restrict__ cs,
+                                                              int nsnap, int nsp, int64_t nodes_pad,
+                                                              EOFGeom g, __bf16* __restrict__ Ahi,
+                                                              __bf16* __restrict__ Alo) {
+  extern __shared__ float sc[];
+  const int X = blockIdx.x, s = blockIdx.y;
+  const int J = 2 * g.mmax + 1, NS = 4 * J, RS = NS + 1;
+  const int ny = g.numy, ny1 = ny + 1;
+  const int64_t nodes = (int64_t)(g.numx + 1) * ny1;
+  const bool live = s < nsnap;
+  float* cur = sc;               // [ny][RS] cells (X, Y)
+  float* prv = sc + ny * RS;     // [ny][RS] cells (X-1, Y)
+  if (live) {
+    const float* base = cs + (size_t)s * g.numx * ny * NS;
+    for (int t = threadIdx.x; t < ny * NS; t += blockDim.x) {
+      const int y = t / NS, k = t - y * NS;
+      cur[y * RS + k] = X < g.numx ? base[((size_t)X * ny) * NS + t] : 0.f;
+      prv[y * RS + k] = X >= 1 ? base[((size_t)(X - 1) * ny) * NS + t] : 0.f;
+    }
+  }
+  __syncthreads();
+  for (int Y = threadIdx.x; Y < ny1; Y += blockDim.x) {
+    const int64_t node = (int64_t)X * ny1 + Y;
+    for (int j = 0; j < J; ++j) {
+      float G = 0.f;
+      if (live) {
+        if (Y < ny) G = cur[Y * RS + 4 * j] + prv[Y * RS + 4 * j + 1];
+        if (Y >= 1) G += cur[(Y - 1) * RS + 4 * j + 2] + prv[(Y - 1) * RS + 4 * j + 3];
+      }
+      const __bf16 hi = (__bf16)G;
+      const __bf16 lo = (__bf16)(G - (float)hi);
+      const size_t o = ((size_t)j * nsp + s) * nodes_pad + node;
+      Ahi[o] = hi;
+      Alo[o] = lo;
+    }
+  }
+  if (X == g.numx) {
+    for (int64_t node = nodes + threadIdx.x; node < nodes_pad; node += blockDim.x)
+      for (int j = 0; j < J; ++j) {
+        const size_t o = ((size_t)j * nsp + s) * nodes_pad + node;
+        Ahi[o] = (__bf16)0.f;
+        Alo[o] = (__bf16)0.f;
+      }
+  }
+}
+
+// Per row j (grid.z), 16 snapshots (grid.y) and 512 nodes (grid.x): the
+// product of the node pieces with the table pieces, three MFMAs per K-step
+// and radial tile (hi*hi, hi*lo, lo*hi), float32 accumulation; the four
+// waves' sums are added in wave order into part[kb][s][j][np] (np = norder
+// rounded up to 16).  Operand layout of v_mfma_f32_16x16x32_bf16 (lane l,
+// g = l / 16, c = l % 16): A[c][8g + i] = G[s0 + c][node0 + 8g + i],
+// B[8g + i][c] = T[node0 + 8g + i][n0 + c], D[4g + i][c] = a[s0 + 4g + i][n0 + c].
+// Snapshot rows s >= nsnap and radial columns n >= norder are not stored:
+// their lanes take zero operands.
+__global__ __launch_bounds__(256) void eof_mfma_kernel(const __bf16* __restrict__ Ahi,
+                                                       const __bf16* __restrict__ Alo,
+                                                       const __bf16* __restrict__ Thi,
+                                                       const __bf16* __restrict__ Tlo, int nsnap,
+                                                       int nsp, int J, int norder, int np,
+                                                       int64_t nodes_pad, float* __restrict__ part) {
+  __shared__ float red[4][4][64][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, c = lane & 15;
+  const int kb = blockIdx.x, s0 = 16 * blockIdx.y, j = blockIdx.z;
+  const int ntn = np / 16;
+  const bool arow_ok = s0 + c < nsnap;
+  const size_t arow = ((size_t)j * nsp + s0 + c) * nodes_pad;
+  const bf16x8_t zero8 = {};
+  floatx4_t acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < 4; ++ks) {
+    const int64_t node0 = (int64_t)kb * kMfmaKBlock + wave * 128 + ks * 32 + 8 * g;
+    const bf16x8_t ah = arow_ok ? *reinterpret_cast<const bf16x8_t*>(Ahi + arow + node0) : zero8;
+    const bf16x8_t al = arow_ok ? *reinterpret_cast<const bf16x8_t*>(Alo + arow + node0) : zero8;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (t < ntn) {
+        const bool bok = 16 * t + c < norder;
+        const size_t brow = ((size_t)j * norder + 16 * t + c) * nodes_pad + node0;
+        const bf16x8_t bh = bok ? *reinterpret_cast<const bf16x8_t*>(Thi + brow) : zero8;
+        const bf16x8_t bl = bok ? *reinterpret_cast<const bf16x8_t*>(Tlo + brow) : zero8;
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc[t], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) red[wave][t][lane][i] = acc[t][i];
+  __syncthreads();
+  if (wave == 0) {
+    for (int t = 0; t < ntn; ++t)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float v = ((red[0][t][lane][i] + red[1][t][lane][i]) + red[2][t][lane][i]) + red[3][t][lane][i];
+        const int s = s0 + 4 * g + i;
+        part[(((size_t)kb * nsp + s) * J + j) * np + 16 * t + c] = v;
+      }
+  }
+}
+
+// Thread per (s, j, n): the float64 sum of the block partials in block
+// order, written as accum_cos[s][j][n] (j <= mmax) or accum_sin[s][j-mmax][n]
+// (sin row m = 0 is zero).
+__global__ __launch_bounds__(256) void eof_mfma_reduce_kernel(const float* __restrict__ part,
+                                                              int nkb, int nsnap, int nsp, int J,
+                                                              int np, int mmax, int norder,
+                                                              double* __restrict__ cos_out,
+                                                              double* __restrict__ sin_out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nsnap * J * norder) return;
+  const int n = (int)(t % norder);
+  const int j = (int)((t / norder) % J);
+  const int s = (int)(t / ((int64_t)norder * J));
+  double v = 0.0;
+  for (int kb = 0; kb < nkb; ++kb) v += (double)part[(((size_t)kb * nsp + s) * J + j) * np + n];
+  const size_t rows = (size_t)(mmax + 1) * norder;
+  if (j <= mmax) {
+    cos_out[(size_t)s * rows + (size_t)j * norder + n] = v;
+    if (j == 0) sin_out[(size_t)s * rows + n] = 0.0;
+  } else {
+    sin_out[(size_t)s * rows + (size_t)(j - mmax) * norder + n] = v;
+  }
+}
+
+// bf16 pieces of the potential table, [2][J][np][nodes_pad] from the
+// function-major float32 table fold32[k = j norder + n][node]; padding zero.
+__global__ __launch_bounds__(256) void eof_mfma_table_kernel(const float* __restrict__ fold32,
+                                                             int J, int norder, int np,
+                                                             int64_t nodes, int64_t nodes_pad,
+                                                             __bf16* __restrict__ T) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)J * np * nodes_pad;
+  if (t >= total) return;
+  const int64_t node = t % nodes_pad;
+  const int n = (int)((t / nodes_pad) % np);
+  const int j = (int)(t / ((int64_t)np * nodes_pad));
+  const float v = (n < norder && node < nodes) ? fold32[((size_t)j * norder + n) * nodes + node] : 0.f;
+  const __bf16 hi = (__bf16)v;
+  T[t] = hi;
+  T[total + t] = (__bf16)(v - (float)hi);
+}
+
 // Segment plan of one accumulate launch.  Single snapshot: at most
 // `cap` segments of whole tiles, one per block.  Batch: every
 // snapshot is cut into segments of seg_tiles tiles (never spanning two
@@ -1178,6 +1488,108 @@ int eof_dep_prepare(Context* c) {
 }
 
 // beside: another accumulate runs concurrently (selects the deposit kernel).
+// bf16 pieces of the potential table for the MFMA mode (once per table set).
+static int eof_mfma_prepare(Context* c, hipStream_t s) {
+  if (c->eof_mfmaT) return 0;
+  const int J = 2 * c->mmax + 1;
+  const int64_t nodes = (int64_t)(c->numx + 1) * (c->numy + 1);
+  const int64_t nodes_pad = (nodes + kMfmaKBlock - 1) / kMfmaKBlock * kMfmaKBlock;
+  const int64_t total = (int64_t)J * c->norder * nodes_pad;
+  BFE_HIP(c, hipMalloc(&c->eof_mfmaT, 2 * (size_t)total * sizeof(__bf16)));
+  hipLaunchKernelGGL(eof_mfma_table_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                     c->eof_fold32[0], J, c->norder, c->norder, nodes, nodes_pad, c->eof_mfmaT);
+  BFE_HIP(c, hipGetLastError());
+  c->eof_mfma_np = (c->norder + 15) / 16 * 16;
+  c->eof_mfma_nodes = nodes_pad;
+  return 0;
+}
+
+static int ensure_mfma_work(Context* c, size_t bytes) {
+  if (bytes <= c->mfma_work_bytes) return 0;
+  if (c->mfma_work) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return check_hip(c, e, "hipDeviceSynchronize (mfma work grow)");
+    (void)hipFree(c->mfma_work);
+    c->mfma_work = nullptr;
+    c->mfma_work_bytes = 0;
+  }
+  const size_t want = bytes + bytes / 8;
+  BFE_HIP(c, hipMalloc(&c->mfma_work, want));
+  c->mfma_work_bytes = want;
+  return 0;
+}
+
+// The EOF accumulate in the bf16 MFMA mode (BFE_ACC_BF16) for one snapshot
+// (off NULL) or nsnap concatenated ones; the same results as
+// eof_accumulate_run to about 1e-5 of each row's larger terms.
+static int eof_accumulate_mfma_run(Context* c, const float* x, const float* y, const float* z,
+                                   const float* m, int64_t n, const int64_t* off, int nsnap,
+                                   double* cos_out, double* sin_out, hipStream_t s) {
+  const int J = 2 * c->mmax + 1, NS = 4 * J;
+  if (NS > kCellSlotsMax)
+    return set_error(c, BFE_ELIMIT, "eof_accumulate: BFE_ACC_BF16 needs mmax <= 15");
+  if (c->norder > 64)
+    return set_error(c, BFE_ELIMIT, "eof_accumulate: BFE_ACC_BF16 needs norder <= 64");
+  const size_t plds = (size_t)2 * c->numy * (NS + 1) * sizeof(float);
+  if (plds > 64 * 1024)
+    return set_error(c, BFE_ELIMIT, "eof_accumulate: BFE_ACC_BF16 needs 2 numy (8 mmax + 5) <= 16384");
+  int rc = eof_mfma_prepare(c, s);
+  if (rc) return rc;
+  const int np = c->eof_mfma_np;
+  const int64_t nodes_pad = c->eof_mfma_nodes;
+  const int nsp = (nsnap + 15) / 16 * 16;
+  const int64_t ncell = (int64_t)c->numx * c->numy;
+  const int nkb = (int)(nodes_pad / kMfmaKBlock);
+  // work: cell sums [nsnap][cell][NS] f32 | node pieces [2][J][nsp][nodes_pad]
+  // bf16 | block partials [nkb][nsp][J][np] f32
+  const size_t cs_bytes = (size_t)nsnap * ncell * NS * sizeof(float);
+  const size_t a_elems = (size_t)J * nsp * nodes_pad;
+  const size_t a_off = (cs_bytes + 255) & ~(size_t)255;
+  const size_t p_off = a_off + ((2 * a_elems * sizeof(__bf16) + 255) & ~(size_t)255);
+  const size_t total = p_off + (size_t)nkb * nsp * J * np * sizeof(float);
+  AccPlan P = off ? plan_batch(c->eof_acc_blocks, c->eof_seg_tiles, off, nsnap, kTile)
+                  : plan_single(c->eof_acc_blocks, n, kTile);
+  const int64_t* rowb = nullptr;
+  const int64_t* off_dev = nullptr;
+  if (off && (rc = upload_plan(c, P, 0, off, nsnap, s, &rowb, &off_dev))) return rc;
+  if ((rc = ensure_mfma_work(c, total))) return rc;
+  float* cs = (float*)c->mfma_work;
+  __bf16* Ahi = (__bf16*)((char*)c->mfma_work + a_off);
+  __bf16* Alo = Ahi + a_elems;
+  float* part = (float*)((char*)c->mfma_work + p_off);
+  const EOFGeom g = eof_geom(c);
+  // zero the cell sums (NS = 4 J floats per cell: an even count, cleared as doubles)
+  const int64_t ncs2 = (int64_t)nsnap * ncell * NS / 2;
+  hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)((ncs2 + 255) / 256)), dim3(256), 0, s,
+                     (double*)cs, ncs2);
+  BFE_HIP(c, hipGetLastError());
+  if (n > 0) {
+    const float4* sorted = nullptr;
+    if ((rc = order_particles(c, 0, x, y, z, m, n, off_dev, nsnap, s, &sorted))) return rc;
+    const size_t lds = (size_t)4 * (J * kDepStride + kDepExtra) * sizeof(float);
+    timer_begin(c, "eof_cell_sums", s);
+    hipLaunchKernelGGL(eof_cell_sums_kernel, dim3(P.grid), dim3(256), lds, s, sorted, P.S,
+                       off_dev, nsnap, g, cs);
+    timer_end(c, "eof_cell_sums", s);
+    BFE_HIP(c, hipGetLastError());
+  }
+  hipLaunchKernelGGL(eof_node_pieces_kernel, dim3((unsigned)(c->numx + 1), (unsigned)nsnap), dim3(256),
+                     plds, s, cs, nsnap, nsp, nodes_pad, g, Ahi, Alo);
+  BFE_HIP(c, hipGetLastError());
+  const __bf16* Thi = c->eof_mfmaT;
+  const __bf16* Tlo = Thi + (size_t)J * c->norder * nodes_pad;
+  timer_begin(c, "eof_mfma", s);
+  hipLaunchKernelGGL(eof_mfma_kernel, dim3((unsigned)nkb, (unsigned)(nsp / 16), (unsigned)J), dim3(256),
+                     0, s, Ahi, Alo, Thi, Tlo, nsnap, nsp, J, c->norder, np, nodes_pad, part);
+  timer_end(c, "eof_mfma", s);
+  BFE_HIP(c, hipGetLastError());
+  const int64_t nout = (int64_t)nsnap * J * c->norder;
+  hipLaunchKernelGGL(eof_mfma_reduce_kernel, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, s,
+                     part, nkb, nsnap, nsp, J, np, c->mmax, c->norder, cos_out, sin_out);
+  BFE_HIP(c, hipGetLastError());
+  return 0;
+}
+
 static int eof_accumulate_run(Context* c, const float* x, const float* y, const float* z,
                               const float* m, int64_t n, const int64_t* off, int nsnap,
                               double* cos_out, double* sin_out, hipStream_t s,
@@ -1418,7 +1830,8 @@ static int acc_common_checks(Context* c, const float* x, const float* y, const f
 extern "C" int bfe_eof_accumulate(bfe_ctx* h, const float* x, const float* y, const float* z,
                                   const float* m, int64_t n, uint32_t flags, double* cos_out,
                                   double* sin_out, void* stream) {
-  (void)flags;  // eof.accumulate computes a no_odd mask but never applies it (eof.py:545-551)
+  // BFE_NO_ODD: eof.accumulate computes a no_odd mask but never applies it
+  // (eof.py:545-551), so it is ignored; BFE_ACC_BF16 selects the MFMA mode
   if (!h) return BFE_EINVAL;
   Context* c = &h->c;
   if (!c->eof_ok) return set_error(c, BFE_ENOTABLE, "eof_accumulate: EOF tables not set");
@@ -1426,6 +1839,8 @@ extern "C" int bfe_eof_accumulate(bfe_ctx* h, const float* x, const float* y, co
   int rc = acc_common_checks(c, x, y, z, m, n, "eof_accumulate");
   if (rc) return rc;
   BFE_HIP(c, hipSetDevice(c->device));
+  if (flags & BFE_ACC_BF16)
+    return eof_accumulate_mfma_run(c, x, y, z, m, n, nullptr, 1, cos_out, sin_out, (hipStream_t)stream);
   return eof_accumulate_run(c, x, y, z, m, n, nullptr, 1, cos_out, sin_out, (hipStream_t)stream);
 }
 
@@ -1452,7 +1867,6 @@ extern "C" int bfe_eof_accumulate_batch(bfe_ctx* h, const float* x, const float*
                                         const float* m, const int64_t* offsets, int nsnap,
                                         uint32_t flags, double* cos_out, double* sin_out,
                                         void* stream) {
-  (void)flags;
   if (!h) return BFE_EINVAL;
   Context* c = &h->c;
   if (!c->eof_ok) return set_error(c, BFE_ENOTABLE, "eof_accumulate_batch: EOF tables not set");
@@ -1463,6 +1877,9 @@ extern "C" int bfe_eof_accumulate_batch(bfe_ctx* h, const float* x, const float*
   if (rc) return rc;
   if (nsnap > 65535) return set_error(c, BFE_ELIMIT, "eof_accumulate_batch: nsnap > 65535");
   BFE_HIP(c, hipSetDevice(c->device));
+  if (flags & BFE_ACC_BF16)
+    return eof_accumulate_mfma_run(c, x, y, z, m, offsets[nsnap], offsets, nsnap, cos_out,
+                                   sin_out, (hipStream_t)stream);
   return eof_accumulate_run(c, x, y, z, m, offsets[nsnap], offsets, nsnap, cos_out, sin_out,
                             (hipStream_t)stream);
 }

@@ -73,9 +73,13 @@ static void free_eof(Context* c) {
   for (auto& p : c->eof_fold32) { free_ptr(p); p = nullptr; }
   free_ptr(c->eof_pot32);
   free_ptr(c->eof_dep32);
+  free_ptr(c->eof_mfmaT);
   c->eof_pot32 = nullptr;
   c->eof_dep32 = nullptr;
+  c->eof_mfmaT = nullptr;
   c->eof_dep_rp = 0;
+  c->eof_mfma_np = 0;
+  c->eof_mfma_nodes = 0;
   for (auto& f : c->eoff) { free_ptr(f.d); f = EOFFolded(); }
   c->eof_ok = false;
   c->batch_nsnap = 0;   // batch fields were folded against these tables
@@ -149,6 +153,7 @@ extern "C" int bfe_destroy(bfe_ctx* h) {
   timers_free(c);
   if (c->scratch) (void)hipFree(c->scratch);
   if (c->alt_scratch) (void)hipFree(c->alt_scratch);
+  if (c->mfma_work) (void)hipFree(c->mfma_work);
   if (c->alt_order) (void)hipFree(c->alt_order);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);

@@ -120,6 +120,13 @@ struct Context {
   float* eof_dep32 = nullptr;  // [node][2 mmax + 1][eof_dep_rp] the same, rows zero-padded
   int eof_dep_rp = 0;          // row length of eof_dep32: norder rounded up to the
                                // lane shape's chunk length (eof_dep_prepare)
+  // bf16 MFMA mode of the EOF accumulate (BFE_ACC_BF16): the potential table
+  // as bf16 pieces hi, lo, each [2 mmax + 1][norder][eof_mfma_nodes] (nodes
+  // padded to the product's K block), built on first use
+  // (eof_mfma_prepare); eof_mfma_np: norder rounded up to 16
+  __bf16* eof_mfmaT = nullptr;
+  int eof_mfma_np = 0;
+  int64_t eof_mfma_nodes = 0;
 
   // ---- folded tables: slot 0 = batch evaluators, slot 1 = frozen field ----
   SLFolded slf[2];
@@ -136,6 +143,9 @@ struct Context {
   // device address of the static array of the allocation-free accumulate
   // (bfe_accumulate.hip), taken once at bfe_create; NULL if unavailable
   double* noalloc_scratch = nullptr;
+  // work memory of the EOF bf16 MFMA mode (cell sums, node pieces, partials)
+  void* mfma_work = nullptr;
+  size_t mfma_work_bytes = 0;
 
   // ---- tuning (bfe_set_tuning) --------------------------------------------
   int64_t eof_acc_blocks = 2048;   // EOF accumulate: blocks (single) / segment budget (batch)

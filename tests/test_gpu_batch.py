@@ -211,3 +211,34 @@ def test_batch_barred_snapshots_against_oracle():
         e = np.linalg.norm(Fg - Fr, axis=0) / np.linalg.norm(Fr, axis=0)
         assert np.max(e) <= 1e-5, (k, np.max(e))
         assert np.max(np.abs(fd[3][od[k]:od[k + 1]][sub] - ref[3]) / np.abs(ref[3])) <= 1e-5
+
+
+def test_eof_bf16_mfma_mode(sysm):
+    """The EOF accumulate's bf16 MFMA mode (corner sums of each cell, node
+    sums, then per row the snapshots x nodes x radial-orders product with the
+    node sums and the table as bf16 hi + lo pieces): one snapshot (sorted and
+    small) and batches of 6 and 20 snapshots (two 16-snapshot tiles, an
+    empty snapshot) within 1e-4 per m block of the FP64 oracle."""
+    s, slo, eo = sysm
+    for n in (90000, 3000):
+        (dx, dy, dz, dm), _ = disc_halo(n, 71)
+        d = f32(dx, dy, dz, dm)
+        c, sn = (v.cpu().numpy() for v in s.eof_accumulate(*d, bf16=True))
+        rc, rs = O.eof_accumulate(eo, *d)
+        e = max(block_rel_eof(c, rc), block_rel_eof(sn[1:], rs[1:]))
+        print(f"bf16 MFMA mode, {n} particles: block error {e:.2e}", flush=True)
+        assert e <= 1e-4 and np.all(sn[0] == 0)
+        c2, _ = (v.cpu().numpy() for v in s.eof_accumulate(*d, bf16=True))
+        assert np.max(np.abs(c2 - c)) <= 1e-12 * np.max(np.abs(c))
+    for counts in (COUNTS, [700 + 37 * k for k in range(19)] + [0]):
+        pd, ph, D, H = _snapshots(counts)
+        c, sn = (v.cpu().numpy() for v in s.eof_accumulate_batch(*D, counts, bf16=True))
+        worst = 0.0
+        for k, n in enumerate(counts):
+            if n == 0:
+                assert np.all(c[k] == 0) and np.all(sn[k] == 0)
+                continue
+            rc, rs = O.eof_accumulate(eo, *f32(*pd[k]))
+            worst = max(worst, block_rel_eof(c[k], rc), block_rel_eof(sn[k][1:], rs[1:]))
+        print(f"bf16 MFMA mode, {len(counts)} snapshots: worst block error {worst:.2e}", flush=True)
+        assert worst <= 1e-4
