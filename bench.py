@@ -192,6 +192,15 @@ def main():
         tot, cnt = sysm.kernel_times(nm)
         ko[k] = tot / max(cnt, 1)
     sysm.set_tuning("kernel_timing", 0)
+    # the two accumulate modes of each basis at the full 1M (call time; the
+    # step runs the float32 ones): float32 sums, and bf16 MFMA products of
+    # hi + lo operand pieces (errors against FP64 in accuracy_vs_fp64)
+    modes = {
+        "eof_f32": timed(lambda: sysm.eof_accumulate(dx, dy, dz, dm), reps),
+        "eof_bf16_mfma": timed(lambda: sysm.eof_accumulate(dx, dy, dz, dm, bf16=True), reps),
+        "sl_f32": timed(lambda: sysm.sl_accumulate(hx, hy, hz, hm), reps),
+        "sl_bf16_mfma": timed(lambda: sysm.sl_accumulate(hx, hy, hz, hm, bf16=True), reps),
+    }
     sysm.set_field(c, s, h)
     dom = max(units, key=lambda k: ko[k])
     roof = roofline(dom, units[dom], ko[dom], pmc_summary())
@@ -266,7 +275,8 @@ def main():
         "config": {"workload": "disc EOF mmax6 norder18 256x128 + halo SL lmax6 nmax20 numr2000, "
                                "1M+1M particles, accumulate + return_forces_cart at 2M",
                    "n_disc": a.n_disc, "n_halo": a.n_halo, "parallelism": "single GPU"},
-        "call_seconds": kt, "kernel_seconds": ko, "roofline": roof, "tuning": tuned, "orbits": orbit,
+        "call_seconds": kt, "kernel_seconds": ko, "accumulate_mode_seconds": modes,
+        "roofline": roof, "tuning": tuned, "orbits": orbit,
         "surrogate": surrogate, "batch": batch,
         "cpu_baseline": cpu, "cpu_baselines": cpu_all, "accuracy_vs_fp64": acc,
         "setup_seconds": setup_s,
@@ -578,11 +588,16 @@ def accuracy(sysm, dxyzm, hxyzm, ref):
     fr = np.stack(forces[:3])
     fg = np.stack([g["fx"], g["fy"], g["fz"]])
     e = np.linalg.norm(fg - fr, axis=0) / np.linalg.norm(fr, axis=0)
+    eof_rows = ([slice(m, m + 1) for m in range(c.shape[0])], [slice(m, m + 1) for m in range(1, s.shape[0])])
+    sl_rows = [slice(l * l, (l + 1) ** 2) for l in range(L + 1)]
+    bc, bs = (v.double().cpu().numpy() for v in sysm.eof_accumulate(*[T(v) for v in dxyzm], bf16=True))
+    bh = sysm.sl_accumulate(*[T(v) for v in hxyzm], bf16=True).double().cpu().numpy()
     return {"sample": f"{k} disc + {k} halo particles of the workload; FP64 reference: the "
                       "C++ OpenMP oracle (matches the NumPy oracle to 1e-12)",
-            "coef_eof_block_rel_max": max(blocks(g["cos"], c, [slice(m, m + 1) for m in range(c.shape[0])]),
-                                          blocks(g["sin"], s, [slice(m, m + 1) for m in range(1, s.shape[0])])),
-            "coef_sl_block_rel_max": blocks(g["coef"], hc, [slice(l * l, (l + 1) ** 2) for l in range(L + 1)]),
+            "coef_eof_block_rel_max": max(blocks(g["cos"], c, eof_rows[0]), blocks(g["sin"], s, eof_rows[1])),
+            "coef_sl_block_rel_max": blocks(g["coef"], hc, sl_rows),
+            "coef_eof_block_rel_max_bf16_mfma": max(blocks(bc, c, eof_rows[0]), blocks(bs, s, eof_rows[1])),
+            "coef_sl_block_rel_max_bf16_mfma": blocks(bh, hc, sl_rows),
             "force_rel_max": float(np.max(e)), "force_rel_rms": float(np.sqrt(np.mean(e * e))),
             "pot_rel_max": float(np.max(np.abs(g["pot"] - forces[3]) / np.abs(forces[3])))}
 

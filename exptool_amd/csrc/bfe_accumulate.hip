@@ -1850,9 +1850,11 @@ extern "C" int bfe_eof_accumulate_noalloc(bfe_ctx* h, const float* x, const floa
                                           const float* z, const float* m, int64_t n,
                                           uint32_t flags, double* cos_out, double* sin_out,
                                           void* stream) {
-  (void)flags;  // as bfe_eof_accumulate (eof.py:545-551)
   if (!h) return BFE_EINVAL;
   Context* c = &h->c;
+  // BFE_NO_ODD has no effect, as in bfe_eof_accumulate (eof.py:545-551)
+  if (flags & BFE_ACC_BF16)
+    return set_error(c, BFE_EINVAL, "eof_accumulate_noalloc: BFE_ACC_BF16 needs work memory (use bfe_eof_accumulate)");
   if (!c->eof_ok) return set_error(c, BFE_ENOTABLE, "eof_accumulate_noalloc: EOF tables not set");
   if (!cos_out || !sin_out) return set_error(c, BFE_EINVAL, "eof_accumulate_noalloc: NULL output");
   int rc = acc_common_checks(c, x, y, z, m, n, "eof_accumulate_noalloc");

@@ -47,8 +47,9 @@ typedef struct bfe_ctx bfe_ctx;
 #define BFE_COMPAT_SL_DENS 0x4u    /* all_eval_particles den1 P_1^m + 0.25*pi, spheresl.py:1323,1351 */
 #define BFE_COMPAT_EOF_MASK 0x8u   /* eof.force_eval odd-m mask |cos(pi m/2)|, eof.py:833 */
 #define BFE_NO_ODD 0x10u           /* skip odd l (SL) / odd m (EOF eval) */
-#define BFE_ACC_BF16 0x20u         /* SL accumulate in the bf16 mode: two-piece bf16 MFMA
-                                      products (about 1e-5 relative), see bfe_sl_accumulate */
+#define BFE_ACC_BF16 0x20u         /* accumulate in the bf16 mode: two-piece bf16 MFMA
+                                      products (about 1e-5 relative), see bfe_sl_accumulate
+                                      and bfe_eof_accumulate */
 #define BFE_COMPAT_ALL 0xFu
 #define BFE_COMPAT_RK4_FRAME 0x40u /* rk4_rotation pseudo-force signs, rk4.py:66-90 */
 #define BFE_EVAL_DENS 0x80u        /* point evaluators: also the density */
@@ -93,7 +94,13 @@ int bfe_sl_accumulate(bfe_ctx* ctx, const float* x, const float* y, const float*
                       void* stream);
 
 /* replaces: eof.accumulate (eof.py:492-640) and make_coefficients_multi
- * (eof.py:1415-1455).  cos_out/sin_out: device (mmax+1, norder) float64. */
+ * (eof.py:1415-1455).  cos_out/sin_out: device (mmax+1, norder) float64.
+ * flags: BFE_ACC_BF16 selects the bf16 mode (mmax <= 15, norder <= 64):
+ * the corner sums of each table cell, their node sums, then per row the
+ * product with the potential table on the bf16 MFMA units, both operands as
+ * bf16 hi + lo pieces (<= 1e-4, about 1e-5, per m block).  Otherwise float32
+ * sums with float64 partials (<= 1e-6).  BFE_NO_ODD has no effect (the
+ * reference computes the mask but never applies it, eof.py:545-551). */
 int bfe_eof_accumulate(bfe_ctx* ctx, const float* x, const float* y, const float* z,
                        const float* m, int64_t n, uint32_t flags, double* cos_out,
                        double* sin_out, void* stream);
@@ -104,6 +111,7 @@ int bfe_eof_accumulate(bfe_ctx* ctx, const float* x, const float* y, const float
  * bfe_eof_accumulate in its input-order form (no cell sort): the partial sums
  * live in a static 4 MiB device array of the library, so the call allocates
  * nothing and issues no memset.  Limit: (2 mmax + 1) norder <= 1024; any n.
+ * The float32 mode only: BFE_ACC_BF16 returns BFE_EINVAL.
  * Every call on one GPU shares that array: calls that overlap on one GPU
  * (any context, any stream) race. */
 int bfe_eof_accumulate_noalloc(bfe_ctx* ctx, const float* x, const float* y, const float* z,
@@ -221,7 +229,9 @@ int bfe_rk4(bfe_ctx* ctx, float* x, float* y, float* z, float* vx, float* vy, fl
  * .compute_coefficients per dump, eof.py:1158, spheresl.py:439); each
  * snapshot's coefficients are reduced on their own in a fixed order, equal to
  * what the single-snapshot call returns for the same particles up to the
- * float32 tile sums.  Outputs: [nsnap][...] in the single-call layouts. */
+ * float32 tile sums.  Outputs: [nsnap][...] in the single-call layouts.
+ * flags as the single-snapshot calls; in the EOF bf16 mode (BFE_ACC_BF16)
+ * the product runs over all snapshots at once (16 per MFMA tile). */
 int bfe_eof_accumulate_batch(bfe_ctx* ctx, const float* x, const float* y, const float* z,
                              const float* m, const int64_t* offsets, int nsnap, uint32_t flags,
                              double* cos_out, double* sin_out, void* stream);
