@@ -74,6 +74,8 @@ static void free_eof(Context* c) {
   free_ptr(c->eof_pot32);
   free_ptr(c->eof_dep32);
   free_ptr(c->eof_mfmaT);
+  free_ptr(c->eof_edge64);
+  c->eof_edge64 = nullptr;
   c->eof_pot32 = nullptr;
   c->eof_dep32 = nullptr;
   c->eof_mfmaT = nullptr;
@@ -319,6 +321,15 @@ extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, in
   std::vector<double> buf(nodes * K);
   std::vector<float> buf32(nodes * K);
   std::vector<float> pot32;
+  // edge nodes (eof_edge_index) keep their m = 0 cos fold inputs in float64
+  const int64_t nedge = eof_edge_count(numx, numy);
+  std::vector<int64_t> edge_node(nedge);
+  for (int X = 0; X <= numx; ++X)
+    for (int Y = 0; Y <= numy; ++Y) {
+      const int ei = eof_edge_index(X, Y, numx, numy);
+      if (ei >= 0) edge_node[ei] = (int64_t)X * (numy + 1) + Y;
+    }
+  std::vector<double> edge64((size_t)4 * norder * nedge, 0.0);
   for (int q = 0; q < (dens ? 4 : 3); ++q) {
     for (int m = 0; m <= mmax; ++m)
       for (int n = 0; n < norder; ++n) {
@@ -335,6 +346,9 @@ extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, in
     for (size_t j = 0; j < buf.size(); ++j) buf32[j] = (float)buf[j];
     int rc = dev_upload(c, &c->eof_fold32[q], buf32.data(), buf32.size());
     if (rc) return rc;
+    for (int n = 0; n < norder; ++n)   // rows k = n: cos m = 0
+      for (int64_t e = 0; e < nedge; ++e)
+        edge64[((size_t)q * norder + n) * nedge + e] = buf[(size_t)n * nodes + edge_node[e]];
     if (q == 0) {
       pot32.resize(buf.size());
       for (int k = 0; k < K; ++k)
@@ -342,6 +356,10 @@ extern "C" int bfe_eof_tables_set(bfe_ctx* h, int mmax, int norder, int numx, in
           pot32[node * K + k] = (float)buf[(size_t)k * nodes + node];
       if ((rc = dev_upload(c, &c->eof_pot32, pot32.data(), pot32.size()))) return rc;
     }
+  }
+  {
+    int rc = dev_upload(c, &c->eof_edge64, edge64.data(), edge64.size());
+    if (rc) return rc;
   }
   c->mmax = mmax; c->norder = norder; c->numx = numx; c->numy = numy; c->ecmap = cmap;
   c->ascale = ascale; c->hscale = hscale; c->xmin = xmin; c->dx = dx; c->ymin = ymin; c->dy = dy;

@@ -108,6 +108,7 @@ __global__ __launch_bounds__(256) void sl_fold_kernel(const double* __restrict__
 struct EOFFoldArgs {
   double mmul[kMaxM + 1];  // per-m multiplier (range / no_odd mask)
   int nuse, dens;
+  int numx, numy;          // grid, for the float64 edge blocks
 };
 
 // One thread per (row group q, node): q = m for cos m = 0..mmax, mmax + m
@@ -115,7 +116,9 @@ struct EOFFoldArgs {
 // are read coalesced (sums in float64), and q uniform per block so the coefficients are
 // scalar loads.  Writes its half [pot, rf, zf, dens] of the 8-float block
 // [potC, rfC, zfC, dC, potS, rfS, zfS, dS]; cos m = 0 also zeroes the sin
-// half.  grid.x = (2 mmax + 1) * chunks, grid.y = coefficient sets.
+// half.  Edge nodes (eof_edge_index) also write the half in float64 after
+// the float32 blocks.  grid.x = (2 mmax + 1) * chunks, grid.y = coefficient
+// sets.
 __global__ __launch_bounds__(256) void eof_fold_kernel(
     const float* __restrict__ T0, const float* __restrict__ T1, const float* __restrict__ T2,
     const float* __restrict__ T3, const double* __restrict__ ccos, const double* __restrict__ csin,
@@ -162,6 +165,31 @@ __global__ __launch_bounds__(256) void eof_fold_kernel(
   if (q == 0) o[4] = o[5] = o[6] = o[7] = 0.f;
 }
 
+// The float64 edge blocks of the folded field (after the float32 blocks of
+// each set): the m = 0 cos sums [pot, rforce, zforce, dens] of each edge node
+// (eof_edge_index) from its float64 fold inputs E[f][n][edge], as
+// eof_fold_kernel sums them for m = 0.  grid.x: edge nodes, grid.y:
+// coefficient sets.
+__global__ __launch_bounds__(256) void eof_fold_edge_kernel(
+    const double* __restrict__ E, int64_t nedge, const double* __restrict__ ccos, int mmax,
+    int norder, EOFFoldArgs a, float* __restrict__ out, int64_t coef_stride,
+    int64_t out_stride) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= nedge) return;
+  const double* cf = ccos + (size_t)blockIdx.y * coef_stride;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int n = 0; n < a.nuse; ++n) {
+    const double c = cf[n];
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+      if (f < 3 || a.dens) s[f] += c * E[((size_t)f * norder + n) * nedge + e];
+  }
+  const double mul = a.mmul[0];
+  double4* o = reinterpret_cast<double4*>(out + (size_t)blockIdx.y * out_stride +
+                                          eof_fold32_floats(a.numx, a.numy, mmax)) + e;
+  *o = make_double4(s[0] * mul, s[1] * mul, s[2] * mul, s[3] * mul);
+}
+
 static int ensure_folded(Context* c, float** d, size_t* have, size_t floats, hipStream_t s) {
   if (*d && *have >= floats) return 0;
   if (*d) {
@@ -176,8 +204,11 @@ static int ensure_folded(Context* c, float** d, size_t* have, size_t floats, hip
 }
 
 static size_t sl_fold_floats(const Context* c) { return (size_t)c->numr * sl_npairs(c->lmax) * 12; }
+// float32 blocks of every node, then the float64 m = 0 cos blocks (four
+// values, eight floats) of the edge nodes
 static size_t eof_fold_floats(const Context* c) {
-  return (size_t)(c->numx + 1) * (c->numy + 1) * (c->mmax + 1) * 8;
+  return (size_t)eof_fold32_floats(c->numx, c->numy, c->mmax) +
+         (size_t)eof_edge_count(c->numx, c->numy) * 8;
 }
 
 // Folds nset coefficient sets (stride (lmax+1)^2 nmax) into out (stride sl_fold_floats).
@@ -228,11 +259,18 @@ static int eof_fold_into(Context* c, const double* ccos, const double* csin, int
   }
   a.nuse = nuse;
   a.dens = dens ? 1 : 0;
+  a.numx = c->numx;
+  a.numy = c->numy;
   const int chunks = (int)((nodes + 255) / 256);
   hipLaunchKernelGGL(eof_fold_kernel, dim3((unsigned)((2 * c->mmax + 1) * chunks), nset), dim3(256),
                      0, s, c->eof_fold32[0], c->eof_fold32[1], c->eof_fold32[2],
                      dens ? c->eof_fold32[3] : c->eof_fold32[0], ccos, csin, c->mmax, c->norder, nodes,
                      chunks, a, out,
+                     (int64_t)(c->mmax + 1) * c->norder, (int64_t)eof_fold_floats(c));
+  BFE_HIP(c, hipGetLastError());
+  const int64_t nedge = eof_edge_count(c->numx, c->numy);
+  hipLaunchKernelGGL(eof_fold_edge_kernel, dim3((unsigned)((nedge + 255) / 256), nset), dim3(256),
+                     0, s, c->eof_edge64, nedge, ccos, c->mmax, c->norder, a, out,
                      (int64_t)(c->mmax + 1) * c->norder, (int64_t)eof_fold_floats(c));
   BFE_HIP(c, hipGetLastError());
   return 0;

@@ -32,7 +32,9 @@ struct EOFOut {
 //       G = sum_n a N_lm ef p0/sqrt(ev)   (S entries are 0 for m = 0).
 //       Forces read two float4 per pair, both from node i.
 //  EOF: per grid node, one 8-float block per m = 0..M:
-//       [potC, rforceC, zforceC, densC, potS, rforceS, zforceS, densS].
+//       [potC, rforceC, zforceC, densC, potS, rforceS, zforceS, densS];
+//       then, in float64, the m = 0 cos block of each edge node
+//       (eof_edge_index): [pot, rforce, zforce, dens].
 // SL table values are interpolated in float32 (they are stored in float32;
 // the weights are convex) and combined in float64.  EOF values are
 // interpolated in float64: far from the disc plane the mapped grid cells are
@@ -226,7 +228,11 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
 // force_eval (eof.py:842-870) and accumulated_eval_particles (:1092-1138):
 // fp = sum m (C sin m phi potC - S cos m phi potS) = -dPhi/dphi.  MU: unroll
 // count of the m loop (2 measured 7 % faster in the RK4 kernel, slower in
-// the leapfrog one).
+// the leapfrog one).  Cells of the last grid column or row take their m = 0
+// corner values from the float64 edge blocks that follow the float32 table:
+// past the outer edges the weights extrapolate linearly (eof.py:414-415) and
+// grow without bound, and with them the rounding of the table values; the
+// m = 0 terms carry the largest values.  The m > 0 terms stay float32.
 template <bool DENS, int MU = 1>
 __device__ __forceinline__ void eof_eval_folded(const EOFGeom& g, const float* __restrict__ F,
                                                 double R, double z, double c1, double s1,
@@ -242,26 +248,38 @@ __device__ __forceinline__ void eof_eval_folded(const EOFGeom& g, const float* _
   const float4* __restrict__ C = A + (size_t)ny1 * ns4;                            // (ix+1, iy)
   const float4* __restrict__ D = C + ns4;                                          // (ix+1, iy+1)
   const double w00 = cell.c00, w10 = cell.c10, w01 = cell.c01, w11 = cell.c11;
-  double p0 = 0, fr = 0, fz = 0, d0 = 0, p = 0, fp = 0, d = 0, fr0 = 0, fz0 = 0;
+  // m = 0: [pot, rforce, zforce, dens] of the four corners
+  double4 a0, b0, c0, e0;
+  if (cell.ix == g.numx - 1 || cell.iy == g.numy - 1) {
+    const double4* __restrict__ E =
+        reinterpret_cast<const double4*>(F + eof_fold32_floats(g.numx, g.numy, M));
+    const int x0 = cell.ix, y0 = cell.iy;
+    a0 = E[eof_edge_index(x0, y0, g.numx, g.numy)];
+    b0 = E[eof_edge_index(x0, y0 + 1, g.numx, g.numy)];
+    c0 = E[eof_edge_index(x0 + 1, y0, g.numx, g.numy)];
+    e0 = E[eof_edge_index(x0 + 1, y0 + 1, g.numx, g.numy)];
+  } else {
+    const float4 a = A[0], b = B[0], c = C[0], e = D[0];
+    a0 = make_double4(a.x, a.y, a.z, a.w);
+    b0 = make_double4(b.x, b.y, b.z, b.w);
+    c0 = make_double4(c.x, c.y, c.z, c.w);
+    e0 = make_double4(e.x, e.y, e.z, e.w);
+  }
+  const double p0 = w00 * a0.x + w10 * c0.x + w01 * b0.x + w11 * e0.x;
+  const double fr0 = w00 * a0.y + w10 * c0.y + w01 * b0.y + w11 * e0.y;
+  const double fz0 = w00 * a0.z + w10 * c0.z + w01 * b0.z + w11 * e0.z;
+  const double d0 = DENS ? w00 * a0.w + w10 * c0.w + w01 * b0.w + w11 * e0.w : 0.0;
+  double fr = fr0, fz = fz0, p = 0, fp = 0, d = 0;
   double cm = 1.0, sm = 0.0, cm1 = c1, sm1 = -s1;
 #pragma unroll MU
-  for (int m = 0; m <= M; ++m) {
-    if (m > 0) {
-      const double cn = (m == 1) ? c1 : 2.0 * c1 * cm - cm1;
-      const double sn = (m == 1) ? s1 : 2.0 * c1 * sm - sm1;
-      cm1 = cm; sm1 = sm; cm = cn; sm = sn;
-    }
+  for (int m = 1; m <= M; ++m) {
+    const double cn = (m == 1) ? c1 : 2.0 * c1 * cm - cm1;
+    const double sn = (m == 1) ? s1 : 2.0 * c1 * sm - sm1;
+    cm1 = cm; sm1 = sm; cm = cn; sm = sn;
     const float4 a = A[2 * m], b = B[2 * m], c = C[2 * m], e = D[2 * m];
     const double pc = w00 * a.x + w10 * c.x + w01 * b.x + w11 * e.x;
     const double rc = w00 * a.y + w10 * c.y + w01 * b.y + w11 * e.y;
     const double zc = w00 * a.z + w10 * c.z + w01 * b.z + w11 * e.z;
-    if (m == 0) {
-      p0 = pc;
-      fr = fr0 = rc;
-      fz = fz0 = zc;
-      if (DENS) d0 = w00 * a.w + w10 * c.w + w01 * b.w + w11 * e.w;
-      continue;
-    }
     const float4 a2 = A[2 * m + 1], b2 = B[2 * m + 1], c2 = C[2 * m + 1], e2 = D[2 * m + 1];
     const double ps = w00 * a2.x + w10 * c2.x + w01 * b2.x + w11 * e2.x;
     const double rs = w00 * a2.y + w10 * c2.y + w01 * b2.y + w11 * e2.y;

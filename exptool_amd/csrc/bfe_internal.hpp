@@ -116,6 +116,9 @@ struct Context {
   // fold inputs pot, rf, zf, dens: [K][node], float32 (the folded field is
   // float32 too; the fold sums in float64)
   float* eof_fold32[4] = {nullptr, nullptr, nullptr, nullptr};
+  // the m = 0 cos fold inputs of the edge nodes (eof_edge_index) in float64:
+  // [4][norder][eof_edge_count], for the float64 edge blocks of the folded field
+  double* eof_edge64 = nullptr;
   float* eof_pot32 = nullptr;  // [node][K] potential for accumulate
   float* eof_dep32 = nullptr;  // [node][2 mmax + 1][eof_dep_rp] the same, rows zero-padded
   int eof_dep_rp = 0;          // row length of eof_dep32: norder rounded up to the
@@ -202,6 +205,24 @@ struct EOFGeom {
   double ascale, hscale, xmin, dx, ymin, dy;
   double ihscale, idx, idy;   // reciprocals of hscale, dx, dy
 };
+
+// Folded EOF table of one coefficient set: the float32 blocks of every node,
+// then, as float64, the m = 0 cos block of each edge node; the edge nodes are
+// the corners of every cell that can extrapolate: the last two grid columns
+// X >= numx - 1 (all rows) and, in the other columns, the last two rows
+// Y >= numy - 1.
+__host__ __device__ inline int64_t eof_fold32_floats(int numx, int numy, int mmax) {
+  return (int64_t)(numx + 1) * (numy + 1) * (mmax + 1) * 8;
+}
+__host__ __device__ inline int64_t eof_edge_count(int numx, int numy) {
+  return 2 * (int64_t)(numy + 1) + 2 * (int64_t)(numx - 1);
+}
+// Edge index of node (X, Y), or -1 for an interior node.
+__host__ __device__ inline int eof_edge_index(int X, int Y, int numx, int numy) {
+  if (X >= numx - 1) return (X - (numx - 1)) * (numy + 1) + Y;
+  if (Y >= numy - 1) return 2 * (numy + 1) + 2 * X + (Y - (numy - 1));
+  return -1;
+}
 
 inline SLGeom sl_geom(const Context* c) {
   const double dxi = c->h_xi.size() >= 2 ? c->h_xi[1] - c->h_xi[0] : 1.0;

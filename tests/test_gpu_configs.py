@@ -159,23 +159,14 @@ def test_identity_map_accumulate_and_eval(identity_map):
     e = np.linalg.norm(Fg - Fr, axis=0) / np.linalg.norm(Fr, axis=0)
     # Beyond the EOF grid (R > R_table) the reference extrapolates the tables
     # linearly, and with the identity map the weights grow without bound
-    # (about 80 at R = 32 on this grid), multiplying the float32 rounding of
-    # the tables.  Inside the grid the float64 bar holds; everywhere the
-    # device equals the oracle evaluated on what the device folds: float32
-    # copies of the tables summed in float64, the sums rounded to float32.
+    # (about 80 at R = 32 on this grid).  The edge cells are evaluated from
+    # float64 folds of float64 tables, so the float64 bar holds there too.
     inside = np.hypot(x[sub], y[sub]) <= np.sqrt(0.5) * eo.rmax * eo.ascale
-    assert inside.sum() > 3000
-    assert np.max(e[inside]) <= FORCE_TOL, np.max(e[inside])
-    assert relmax(got[3][sub][inside], ref[3][inside]) <= FORCE_TOL
-    r32 = lambda T: np.asarray(T, np.float32).astype(np.float64)
-    fold = lambda T, a: r32(np.einsum("mnxy,mn->mxy", r32(T), a))[:, None]
-    e32 = O.EOFTable(eo.mmax, 1, eo.numx, eo.numy, eo.cmap, eo.rmin, eo.rmax, eo.ascale,
-                     eo.hscale, *[fold(T, rc if k < 3 else rs) for k, T in enumerate(
-                         (eo.potC, eo.rforceC, eo.zforceC, eo.potS, eo.rforceS, eo.zforceS))])
-    one = np.ones((eo.mmax + 1, 1))
-    ref32 = O.total_force(O.FieldSpec(e32, one, one, slo, hr), x[sub], y[sub], z[sub])
-    e2 = np.linalg.norm(Fg - np.stack(ref32[:3]), axis=0) / np.linalg.norm(Fr, axis=0)
-    assert np.max(e2) <= 2e-6, np.max(e2)
+    assert inside.sum() > 3000 and (~inside).sum() > 500
+    print(f"force error inside the grid {np.max(e[inside]):.2e}, extrapolated {np.max(e[~inside]):.2e}",
+          flush=True)
+    assert np.max(e) <= FORCE_TOL, np.max(e)
+    assert relmax(got[3][sub], ref[3]) <= FORCE_TOL
 
 
 # ------------------------------------------------------- orbit integration
