@@ -599,7 +599,8 @@ __device__ __forceinline__ void eof_deposit_body(
 // Alone, four waves per SIMD (128 registers, about 100 B spilled) measured
 // 80 against 83 us for 1M disc particles; beside the SL deposit they lost.
 template <int PPT, int CH>
-__global__ __launch_bounds__(256) void eof_deposit_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PPT == 1 && CH <= 8 ? 3 : 1))) void
+eof_deposit_kernel(
     const float4* __restrict__ P, Segments S, EOFGeom g, const float* __restrict__ T, int rp,
     int K, double* __restrict__ partials) {
   extern __shared__ __align__(16) unsigned char dsm[];

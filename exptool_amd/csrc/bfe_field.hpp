@@ -51,10 +51,6 @@ __host__ __device__ inline int sl_npairs(int L) { return (L + 1) * (L + 2) / 2; 
 //   flags: BFE_COMPAT_SL_EDGE  first-cell stencil one cell outward (:150-153)
 //          BFE_COMPAT_SL_COS_L cos/sin(l phi) for 0 < m (:1222-1225)
 //          BFE_COMPAT_SL_DENS  den1 uses P_1^0 for m = 0 (:1323)
-// Angular sums of sl_eval_folded.  Without COSL the azimuthal factors
-// cos/sin(m phi) do not depend on l, so the l sums are taken per m first
-// (sum_l P_l^m E, sum_l P_l^m D, sum_l dP_l^m E for cos and sin) and
-// combined with the factors once per m.
 // LDS view of staged float4 data: reads through it are LDS instructions, not
 // flat ones (which would pass through the vector-memory address path).
 typedef float F4v __attribute__((ext_vector_type(4)));
@@ -65,13 +61,124 @@ __device__ __forceinline__ float4 ld4(LdsF4 p, int k) {
 }
 __device__ __forceinline__ float4 ld4(const float4* __restrict__ p, int k) { return p[k]; }
 
-template <bool DENS, bool COSL, typename PA>
-__device__ __forceinline__ void sl_angular(const SLGeom& g, PA A,
-                                           const float4* __restrict__ Nd, int L, float x1f,
-                                           float x2f, float ps, double x, double c1, double s1,
-                                           bool dq, double* acc) {
+// Angular sums of sl_eval_folded.  The azimuthal factors cos/sin(m phi) do
+// not depend on l, so the l sums are taken per m first (sum_l P_l^m E,
+// sum_l P_l^m D, sum_l dP_l^m E for cos and sin) and combined with the
+// factors once per m.  The l loops are branch-free: m = 0 runs alone (l = 0
+// peeled) and each m > 0 peels l = m, so no accumulator passes through a
+// per-lane or per-iteration branch.  A NaN cos(theta) is replaced by 0 for
+// the recurrences and its l > 0 sums are zeroed at the end, which is what
+// skipping them term by term gives.
+template <bool DENS, typename PA>
+__device__ __forceinline__ void sl_angular(const SLGeom& g, PA A, const float4* __restrict__ Nd,
+                                           int L, float x1f, float x2f, float ps, double x,
+                                           double c1, double s1, bool dq, double* acc) {
   const bool good = (x == x);
-  double xc = x;  // dlegendre_R clamps x for the derivative only
+  const double xs = good ? x : 0.0;
+  double xc = xs;   // dlegendre_R clamps x for the derivative only
+  if (1.0 - fabs(xc) < 1.e-8) xc = (xc > 0) ? 1.0 - 1.e-8 : -(1.0 - 1.e-8);
+  const double somx2d = 1.0 / (xc * xc - 1.0);
+  const double somx2 = sqrt((1.0 - xs) * (1.0 + xs));
+  // m = 0, l = 0
+  const float4 a0 = ld4(A, 0), b0 = ld4(A, 1);
+  const double pot0 = (double)fmaf(x2f, a0.z, a0.x);
+  const double potr0 = (double)fmaf(ps, b0.z, b0.x);
+  double den0 = 0.0;
+  if (DENS) {
+    const float4 nd = Nd[0];
+    den0 = (double)fmaf(x2f, nd.z, x1f * nd.x);
+  }
+  double pot1 = 0, potr = 0, pott = 0, potp = 0, den1 = 0;
+  // m = 0, l >= 1
+  {
+    double plm1 = 0.0, pl = 1.0;
+#pragma unroll 2
+    for (int l = 1; l <= L; ++l) {
+      const int lm = lm_index(l, 0);
+      const double pn = xs * g.lmc[kLmcA + lm] * pl - g.lmc[kLmcB + lm] * plm1;
+      plm1 = pl;
+      pl = pn;
+      const float4 a = ld4(A, 2 * l), b = ld4(A, 2 * l + 1);
+      const double ec = (double)fmaf(x2f, a.z, a.x);
+      const double dc = (double)fmaf(ps, b.z, b.x);
+      const double dpl = somx2d * (xc * g.lmc[kLmcL + lm] * pl - g.lmc[kLmcS + lm] * plm1);
+      pot1 += pl * ec;
+      potr += pl * dc;
+      pott += dpl * ec;
+      if (DENS) {
+        const float4 nd = Nd[l];
+        den1 += (dq ? xs : pl) * (double)fmaf(x2f, nd.z, x1f * nd.x);
+      }
+    }
+  }
+  double pmm = 1.0, fact = 1.0;
+  double cmm = 1.0, smm = 0.0, cmm1 = c1, smm1 = -s1;
+  int p = L + 1;
+#pragma unroll 1
+  for (int m = 1; m <= L; ++m) {
+    pmm *= -fact * somx2;
+    fact += 2.0;
+    const double cn = 2.0 * c1 * cmm - cmm1;   // m = 1 gives c1, s1 exactly
+    const double sn = 2.0 * c1 * smm - smm1;
+    cmm1 = cmm; smm1 = smm; cmm = cn; smm = sn;
+    double Pc = 0, Ps = 0, Rc = 0, Rs = 0, Tc = 0, Ts = 0, Nc = 0, Ns = 0;
+    auto term = [&](int q, int lm, double pl, double plm1) {
+      const float4 a = ld4(A, 2 * q), b = ld4(A, 2 * q + 1);
+      const double ec = (double)fmaf(x2f, a.z, a.x);
+      const double dc = (double)fmaf(ps, b.z, b.x);
+      const double es = (double)fmaf(x2f, a.w, a.y);
+      const double ds = (double)fmaf(ps, b.w, b.y);
+      const double dpl = somx2d * (xc * g.lmc[kLmcL + lm] * pl - g.lmc[kLmcS + lm] * plm1);
+      Pc += pl * ec;
+      Ps += pl * es;
+      Rc += pl * dc;
+      Rs += pl * ds;
+      Tc += dpl * ec;
+      Ts += dpl * es;
+      if (DENS) {
+        const float4 nd = Nd[q];
+        Nc += pl * (double)fmaf(x2f, nd.z, x1f * nd.x);
+        Ns += pl * (double)fmaf(x2f, nd.w, x1f * nd.y);
+      }
+    };
+    double plm1 = 0.0, pl = pmm;
+    term(p, lm_index(m, m), pl, plm1);
+    ++p;
+#pragma unroll 2
+    for (int l = m + 1; l <= L; ++l, ++p) {
+      const int lm = lm_index(l, m);
+      const double pn = xs * g.lmc[kLmcA + lm] * pl - g.lmc[kLmcB + lm] * plm1;
+      plm1 = pl;
+      pl = pn;
+      term(p, lm, pl, plm1);
+    }
+    pot1 += cmm * Pc + smm * Ps;
+    potr += cmm * Rc + smm * Rs;
+    pott += cmm * Tc + smm * Ts;
+    potp += m * (cmm * Ps - smm * Pc);
+    if (DENS) den1 += cmm * Nc + smm * Ns;
+  }
+  if (!good) {   // P_l^m = 0 for l > 0 when cos(theta) is NaN (spheresl.py:698)
+    pot1 = potr = pott = potp = den1 = 0.0;
+  }
+  acc[0] = pot0;
+  acc[1] = potr0 + potr;
+  acc[2] = den0;
+  acc[3] = pot1;
+  acc[4] = pott;
+  acc[5] = potp;
+  acc[6] = den1;
+}
+
+// The sums with BFE_COMPAT_SL_COS_L: the azimuthal factor of term (l, m > 0)
+// is cos/sin(l phi), so it enters per term.  Same recurrences as sl_angular.
+template <bool DENS, typename PA>
+__device__ __forceinline__ void sl_angular_cosl(const SLGeom& g, PA A,
+                                                const float4* __restrict__ Nd, int L, float x1f,
+                                                float x2f, float ps, double x, double c1,
+                                                double s1, bool dq, double* acc) {
+  const bool good = (x == x);
+  double xc = x;
   if (good && 1.0 - fabs(xc) < 1.e-8) xc = (xc > 0) ? 1.0 - 1.e-8 : -(1.0 - 1.e-8);
   const double somx2d = 1.0 / (xc * xc - 1.0);
   const double somx2 = good ? sqrt((1.0 - x) * (1.0 + x)) : 0.0;
@@ -84,30 +191,23 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, PA A,
     if (m > 0) {
       pmm *= -fact * somx2;
       fact += 2.0;
-      const double cn = (m == 1) ? c1 : 2.0 * c1 * cmm - cmm1;
-      const double sn = (m == 1) ? s1 : 2.0 * c1 * smm - smm1;
+      const double cn = 2.0 * c1 * cmm - cmm1, sn = 2.0 * c1 * smm - smm1;
       cmm1 = cmm; smm1 = smm; cmm = cn; smm = sn;
     }
     double plm1 = 0.0, pl = pmm;
     double cl = cmm, sl = smm, clm1 = cmm1, slm1 = smm1;
-    double Pc = 0, Ps = 0, Rc = 0, Rs = 0, Tc = 0, Ts = 0, Nc = 0, Ns = 0;
-    // unrolled by 2: the P_l recurrence's register rotation disappears
-    // (150 against 155 us per 2M points; fully unrolled was far slower)
-#pragma unroll 2
+#pragma unroll 1
     for (int l = m; l <= L; ++l, ++p) {
       const int lm = lm_index(l, m);
       if (l > m) {
-        // (x (2l-1) P_{l-1} - (l+m-1) P_{l-2}) / (l-m), constants precomputed
         const double pn = x * g.lmc[kLmcA + lm] * pl - g.lmc[kLmcB + lm] * plm1;
         plm1 = pl;
         pl = pn;
-        if (COSL) {
-          const double cn = 2.0 * c1 * cl - clm1, sn = 2.0 * c1 * sl - slm1;
-          clm1 = cl; slm1 = sl; cl = cn; sl = sn;
-        }
+        const double cn = 2.0 * c1 * cl - clm1, sn = 2.0 * c1 * sl - slm1;
+        clm1 = cl; slm1 = sl; cl = cn; sl = sn;
       }
       const float4 a = ld4(A, 2 * p), b = ld4(A, 2 * p + 1);
-      const double ec = (double)fmaf(x2f, a.z, a.x);   // a.z holds E_i+1 - E_i
+      const double ec = (double)fmaf(x2f, a.z, a.x);
       const double dc = (double)fmaf(ps, b.z, b.x);
       if (l == 0) {
         pot0 = ec;
@@ -119,8 +219,6 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, PA A,
         continue;
       }
       if (!good) continue;  // P_l^m = 0 for l > 0 when cos(theta) is NaN (spheresl.py:698)
-      // xc is clamped inside (-1, 1), so somx2d is finite and dpl is not
-      // finite only when pl or plm1 is not, which makes pot1 NaN already
       const double dpl = somx2d * (xc * g.lmc[kLmcL + lm] * pl - g.lmc[kLmcS + lm] * plm1);
       if (m == 0) {
         pot1 += pl * ec;
@@ -134,38 +232,17 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, PA A,
       }
       const double es = (double)fmaf(x2f, a.w, a.y);
       const double ds = (double)fmaf(ps, b.w, b.y);
-      if (COSL) {
-        const double pa = ec * cl + es * sl;
-        pot1 += pl * pa;
-        potr += pl * (dc * cl + ds * sl);
-        pott += dpl * pa;
-        potp += pl * m * (-ec * sl + es * cl);
-      } else {
-        Pc += pl * ec;
-        Ps += pl * es;
-        Rc += pl * dc;
-        Rs += pl * ds;
-        Tc += dpl * ec;
-        Ts += dpl * es;
-      }
+      const double pa = ec * cl + es * sl;
+      pot1 += pl * pa;
+      potr += pl * (dc * cl + ds * sl);
+      pott += dpl * pa;
+      potp += pl * m * (-ec * sl + es * cl);
       if (DENS) {
         const float4 nd = Nd[p];
         const double nc = (double)fmaf(x2f, nd.z, x1f * nd.x);
         const double ns = (double)fmaf(x2f, nd.w, x1f * nd.y);
-        if (COSL) {
-          den1 += pl * (nc * cl + ns * sl);
-        } else {
-          Nc += pl * nc;
-          Ns += pl * ns;
-        }
+        den1 += pl * (nc * cl + ns * sl);
       }
-    }
-    if (!COSL && m > 0) {
-      pot1 += cmm * Pc + smm * Ps;
-      potr += cmm * Rc + smm * Rs;
-      pott += cmm * Tc + smm * Ts;
-      potp += m * (cmm * Ps - smm * Pc);
-      if (DENS) den1 += cmm * Nc + smm * Ns;
     }
   }
   acc[0] = pot0;
@@ -176,7 +253,6 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, PA A,
   acc[5] = potp;
   acc[6] = den1;
 }
-
 
 // pcell: the radial cell of r when the caller has located it already.
 // STAGED: node i's force blocks are read from the LDS copy Abase at
@@ -204,15 +280,15 @@ __device__ __forceinline__ void sl_eval_folded(const SLGeom& g, const float* __r
   if (STAGED) {
     const LdsF4 A = (LdsF4)Abase + (size_t)(i - ibase) * 2 * np;
     if (flags & BFE_COMPAT_SL_COS_L)
-      sl_angular<DENS, true>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
+      sl_angular_cosl<DENS>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
     else
-      sl_angular<DENS, false>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
+      sl_angular<DENS>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
   } else {
     const float4* __restrict__ A = reinterpret_cast<const float4*>(F) + (size_t)i * 2 * np;
     if (flags & BFE_COMPAT_SL_COS_L)
-      sl_angular<DENS, true>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
+      sl_angular_cosl<DENS>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
     else
-      sl_angular<DENS, false>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
+      sl_angular<DENS>(g, A, Nd, L, (float)x1, (float)x2, ps, costh, c1, s1, dq, acc);
   }
   o.pot0 = acc[0] * P0;
   o.potr = acc[1] * fac;
@@ -273,8 +349,8 @@ __device__ __forceinline__ void eof_eval_folded(const EOFGeom& g, const float* _
   double cm = 1.0, sm = 0.0, cm1 = c1, sm1 = -s1;
 #pragma unroll MU
   for (int m = 1; m <= M; ++m) {
-    const double cn = (m == 1) ? c1 : 2.0 * c1 * cm - cm1;
-    const double sn = (m == 1) ? s1 : 2.0 * c1 * sm - sm1;
+    const double cn = 2.0 * c1 * cm - cm1;   // m = 1 gives c1, s1 exactly
+    const double sn = 2.0 * c1 * sm - sm1;
     cm1 = cm; sm1 = sm; cm = cn; sm = sn;
     const float4 a = A[2 * m], b = B[2 * m], c = C[2 * m], e = D[2 * m];
     const double pc = w00 * a.x + w10 * c.x + w01 * b.x + w11 * e.x;
