@@ -435,10 +435,10 @@ __device__ __forceinline__ void eof_to_run(float (&G)[PPT][4], double (&GD)[PPT]
 // Per-particle part of the cell-ordered EOF accumulates: lane's particle p
 // of the sorted input gives its row weights -4 pi m {1, cos k phi, sin k phi}
 // (rows 0, k, mmax + k of sW), its four bilinear corner weights (sE, 64 per
-// corner: c00, c10, c01, c11) and its cell's first node (sB).
-__device__ __forceinline__ void eof_dep_setup(const float4* __restrict__ P, int64_t p, bool live,
-                                              const EOFGeom& g, int ny1, int lane, float* sW,
-                                              float* sE, int* sB) {
+// corner: c00, c10, c01, c11) and its cell's first node (sB, also returned).
+__device__ __forceinline__ int eof_dep_setup(const float4* __restrict__ P, int64_t p, bool live,
+                                             const EOFGeom& g, int ny1, int lane, float* sW,
+                                             float* sE, int* sB) {
   const float4 q4 = live ? P[p] : make_float4(1.f, 0.f, 0.f, 0.f);
   const double x = q4.x, y = q4.y, z = q4.z, mass = live ? (double)q4.w : 0.0;
   const double R = sqrt(x * x + y * y + 1.e-10);
@@ -453,13 +453,41 @@ __device__ __forceinline__ void eof_dep_setup(const float4* __restrict__ P, int6
   sE[64 + lane] = (float)cell.c10;
   sE[128 + lane] = (float)cell.c01;
   sE[192 + lane] = (float)cell.c11;
-  sB[lane] = cell.ix * ny1 + cell.iy;
+  const int node = cell.ix * ny1 + cell.iy;
+  sB[lane] = node;
   for (int m = 1; m <= g.mmax; ++m) {
-    const double cn = (m == 1) ? c1 : 2.0 * c1 * cm - cm1;
-    const double sn = (m == 1) ? s1v : 2.0 * c1 * sm - sm1;
+    const double cn = 2.0 * c1 * cm - cm1;   // m = 1 gives c1, s1 exactly
+    const double sn = 2.0 * c1 * sm - sm1;
     cm1 = cm; sm1 = sm; cm = cn; sm = sn;
     sW[m * kDepStride + lane] = (float)(w * cm);
     sW[(g.mmax + m) * kDepStride + lane] = (float)(w * sm);
+  }
+  return node;
+}
+
+// Four particles' row weights (per owned row) times their four corner
+// weights, summed into the float32 corner sums in particle order.
+template <int PPT>
+__device__ __forceinline__ void eof_group(const float4 (&wv)[PPT], float4 e00, float4 e10,
+                                          float4 e01, float4 e11, float (&G)[PPT][4]) {
+#pragma unroll
+  for (int o = 0; o < PPT; ++o) {
+    G[o][0] = fmaf(wv[o].x, e00.x, G[o][0]);
+    G[o][1] = fmaf(wv[o].x, e10.x, G[o][1]);
+    G[o][2] = fmaf(wv[o].x, e01.x, G[o][2]);
+    G[o][3] = fmaf(wv[o].x, e11.x, G[o][3]);
+    G[o][0] = fmaf(wv[o].y, e00.y, G[o][0]);
+    G[o][1] = fmaf(wv[o].y, e10.y, G[o][1]);
+    G[o][2] = fmaf(wv[o].y, e01.y, G[o][2]);
+    G[o][3] = fmaf(wv[o].y, e11.y, G[o][3]);
+    G[o][0] = fmaf(wv[o].z, e00.z, G[o][0]);
+    G[o][1] = fmaf(wv[o].z, e10.z, G[o][1]);
+    G[o][2] = fmaf(wv[o].z, e01.z, G[o][2]);
+    G[o][3] = fmaf(wv[o].z, e11.z, G[o][3]);
+    G[o][0] = fmaf(wv[o].w, e00.w, G[o][0]);
+    G[o][1] = fmaf(wv[o].w, e10.w, G[o][1]);
+    G[o][2] = fmaf(wv[o].w, e01.w, G[o][2]);
+    G[o][3] = fmaf(wv[o].w, e11.w, G[o][3]);
   }
 }
 
@@ -507,13 +535,49 @@ __device__ __forceinline__ void eof_deposit_body(
     for (int64_t t0 = s0 + 64 * wave; t0 < s1; t0 += 256) {
       const int64_t p = t0 + lane;
       const bool live = p < s1;
-      eof_dep_setup(P, p, live, g, ny1, lane, sW, sE, sB);
+      const int node = eof_dep_setup(P, p, live, g, ny1, lane, sW, sE, sB);
       wave_sync();
       const int64_t left = s1 - t0;
       const int np = left < 64 ? (int)left : 64;
-      if (cur < 0) cur = sB[0];
+      if (cur < 0) cur = __builtin_amdgcn_readfirstlane(node);
+      // the tile's leading particles in the current cell (a prefix, the
+      // input being sorted by cell), eight at a time with their LDS reads
+      // issued together and no per-group cell check
+      const uint64_t other = __ballot(live && node != cur);
+      const int pre = other ? (int)__builtin_ctzll(other) : np;
+      int q0 = 0;
 #pragma unroll 1
-      for (int q0 = 0; q0 < np; q0 += 4) {
+      for (; q0 + 8 <= pre; q0 += 8) {
+        const float4 a00 = *reinterpret_cast<const float4*>(sE + q0);
+        const float4 a10 = *reinterpret_cast<const float4*>(sE + 64 + q0);
+        const float4 a01 = *reinterpret_cast<const float4*>(sE + 128 + q0);
+        const float4 a11 = *reinterpret_cast<const float4*>(sE + 192 + q0);
+        const float4 b00 = *reinterpret_cast<const float4*>(sE + q0 + 4);
+        const float4 b10 = *reinterpret_cast<const float4*>(sE + 68 + q0);
+        const float4 b01 = *reinterpret_cast<const float4*>(sE + 132 + q0);
+        const float4 b11 = *reinterpret_cast<const float4*>(sE + 196 + q0);
+        float4 w0[PPT], w1[PPT];
+#pragma unroll
+        for (int o = 0; o < PPT; ++o) {
+          const float* wr = sW + (j_[o] < 0 ? 0 : j_[o]) * kDepStride + q0;
+          w0[o] = *reinterpret_cast<const float4*>(wr);
+          w1[o] = *reinterpret_cast<const float4*>(wr + 4);
+        }
+        if (cnt + 4 > kFlush) {
+          eof_to_run<PPT>(G, GD);
+          cnt = 0;
+        }
+        eof_group<PPT>(w0, a00, a10, a01, a11, G);
+        cnt += 4;
+        if (cnt + 4 > kFlush) {
+          eof_to_run<PPT>(G, GD);
+          cnt = 0;
+        }
+        eof_group<PPT>(w1, b00, b10, b01, b11, G);
+        cnt += 4;
+      }
+#pragma unroll 1
+      for (; q0 < np; q0 += 4) {
         const int4 bb = *reinterpret_cast<const int4*>(sB + q0);
         const bool run = q0 + 4 <= np && bb.x == cur && bb.y == cur && bb.z == cur && bb.w == cur;
         if (__builtin_amdgcn_readfirstlane((int)run)) {
@@ -525,26 +589,11 @@ __device__ __forceinline__ void eof_deposit_body(
           const float4 e10 = *reinterpret_cast<const float4*>(sE + 64 + q0);
           const float4 e01 = *reinterpret_cast<const float4*>(sE + 128 + q0);
           const float4 e11 = *reinterpret_cast<const float4*>(sE + 192 + q0);
+          float4 wv[PPT];
 #pragma unroll
-          for (int o = 0; o < PPT; ++o) {
-            const float4 wv = *reinterpret_cast<const float4*>(sW + (j_[o] < 0 ? 0 : j_[o]) * kDepStride + q0);
-            G[o][0] = fmaf(wv.x, e00.x, G[o][0]);
-            G[o][1] = fmaf(wv.x, e10.x, G[o][1]);
-            G[o][2] = fmaf(wv.x, e01.x, G[o][2]);
-            G[o][3] = fmaf(wv.x, e11.x, G[o][3]);
-            G[o][0] = fmaf(wv.y, e00.y, G[o][0]);
-            G[o][1] = fmaf(wv.y, e10.y, G[o][1]);
-            G[o][2] = fmaf(wv.y, e01.y, G[o][2]);
-            G[o][3] = fmaf(wv.y, e11.y, G[o][3]);
-            G[o][0] = fmaf(wv.z, e00.z, G[o][0]);
-            G[o][1] = fmaf(wv.z, e10.z, G[o][1]);
-            G[o][2] = fmaf(wv.z, e01.z, G[o][2]);
-            G[o][3] = fmaf(wv.z, e11.z, G[o][3]);
-            G[o][0] = fmaf(wv.w, e00.w, G[o][0]);
-            G[o][1] = fmaf(wv.w, e10.w, G[o][1]);
-            G[o][2] = fmaf(wv.w, e01.w, G[o][2]);
-            G[o][3] = fmaf(wv.w, e11.w, G[o][3]);
-          }
+          for (int o = 0; o < PPT; ++o)
+            wv[o] = *reinterpret_cast<const float4*>(sW + (j_[o] < 0 ? 0 : j_[o]) * kDepStride + q0);
+          eof_group<PPT>(wv, e00, e10, e01, e11, G);
           cnt += 4;
           continue;
         }
@@ -646,6 +695,24 @@ __device__ __forceinline__ void sl_apply(const float* __restrict__ E, int i, int
       for (int c = 0; c < CH; ++c) acc[o][c] += D1[o] * a[c] + D2[o] * b[c];
     }
     D1[o] = D2[o] = 0.0;
+  }
+}
+
+// Four particles' Y values (per owned row) times their two radial weights,
+// summed into the float32 run sums in particle order.
+template <int PPT>
+__device__ __forceinline__ void sl_group(const float4 (&yv)[PPT], float4 wa, float4 wb,
+                                         float (&A1)[PPT], float (&A2)[PPT]) {
+#pragma unroll
+  for (int o = 0; o < PPT; ++o) {
+    A1[o] = fmaf(yv[o].x, wa.x, A1[o]);
+    A2[o] = fmaf(yv[o].x, wb.x, A2[o]);
+    A1[o] = fmaf(yv[o].y, wa.y, A1[o]);
+    A2[o] = fmaf(yv[o].y, wb.y, A2[o]);
+    A1[o] = fmaf(yv[o].z, wa.z, A1[o]);
+    A2[o] = fmaf(yv[o].z, wb.z, A2[o]);
+    A1[o] = fmaf(yv[o].w, wa.w, A1[o]);
+    A2[o] = fmaf(yv[o].w, wb.w, A2[o]);
   }
 }
 
@@ -765,9 +832,42 @@ __device__ __forceinline__ void sl_deposit_body(
       wave_sync();
       const int64_t left = s1 - t0;
       const int np = left < 64 ? (int)left : 64;
-      if (cur < 0) cur = reinterpret_cast<const int*>(sP)[32 * kDepStride];
+      if (cur < 0) cur = __builtin_amdgcn_readfirstlane(ci);
+      // The particles of the tile still in the current cell are a prefix
+      // (the input is sorted by cell): they take eight at a time with all
+      // their LDS reads issued together and no per-group cell check.
+      const uint64_t other = __ballot(live && ci != cur);
+      const int pre = other ? (int)__builtin_ctzll(other) : np;
+      int q0 = 0;
 #pragma unroll 1
-      for (int q0 = 0; q0 < np; q0 += 4) {
+      for (; q0 + 8 <= pre; q0 += 8) {
+        const float* pq = sP + (q0 >> 2) * kDepStride;
+        const float4 wa0 = *reinterpret_cast<const float4*>(pq);
+        const float4 wb0 = *reinterpret_cast<const float4*>(pq + 16 * kDepStride);
+        const float4 wa1 = *reinterpret_cast<const float4*>(pq + kDepStride);
+        const float4 wb1 = *reinterpret_cast<const float4*>(pq + 17 * kDepStride);
+        float4 y0[PPT], y1[PPT];
+#pragma unroll
+        for (int o = 0; o < PPT; ++o) {
+          const float* yr = sY + (row_[o] < 0 ? 0 : row_[o]) * kDepStride + q0;
+          y0[o] = *reinterpret_cast<const float4*>(yr);
+          y1[o] = *reinterpret_cast<const float4*>(yr + 4);
+        }
+        if (cnt + 4 > kFlush) {
+          sl_to_run<PPT>(A1, A2, D1, D2);
+          cnt = 0;
+        }
+        sl_group<PPT>(y0, wa0, wb0, A1, A2);
+        cnt += 4;
+        if (cnt + 4 > kFlush) {
+          sl_to_run<PPT>(A1, A2, D1, D2);
+          cnt = 0;
+        }
+        sl_group<PPT>(y1, wa1, wb1, A1, A2);
+        cnt += 4;
+      }
+#pragma unroll 1
+      for (; q0 < np; q0 += 4) {
         const float* pq = sP + (q0 >> 2) * kDepStride;
         const int4 cc = *reinterpret_cast<const int4*>(pq + 32 * kDepStride);
         const bool run = q0 + 4 <= np && cc.x == cur && cc.y == cur && cc.z == cur && cc.w == cur;
@@ -778,18 +878,11 @@ __device__ __forceinline__ void sl_deposit_body(
           }
           const float4 wa = *reinterpret_cast<const float4*>(pq);
           const float4 wb = *reinterpret_cast<const float4*>(pq + 16 * kDepStride);
+          float4 yv[PPT];
 #pragma unroll
-          for (int o = 0; o < PPT; ++o) {
-            const float4 yv = *reinterpret_cast<const float4*>(sY + (row_[o] < 0 ? 0 : row_[o]) * kDepStride + q0);
-            A1[o] = fmaf(yv.x, wa.x, A1[o]);
-            A2[o] = fmaf(yv.x, wb.x, A2[o]);
-            A1[o] = fmaf(yv.y, wa.y, A1[o]);
-            A2[o] = fmaf(yv.y, wb.y, A2[o]);
-            A1[o] = fmaf(yv.z, wa.z, A1[o]);
-            A2[o] = fmaf(yv.z, wb.z, A2[o]);
-            A1[o] = fmaf(yv.w, wa.w, A1[o]);
-            A2[o] = fmaf(yv.w, wb.w, A2[o]);
-          }
+          for (int o = 0; o < PPT; ++o)
+            yv[o] = *reinterpret_cast<const float4*>(sY + (row_[o] < 0 ? 0 : row_[o]) * kDepStride + q0);
+          sl_group<PPT>(yv, wa, wb, A1, A2);
           cnt += 4;
           continue;
         }
