@@ -1659,7 +1659,7 @@ static int eof_accumulate_mfma_run(Context* c, const float* x, const float* y, c
   BFE_HIP(c, hipGetLastError());
   if (n > 0) {
     const float4* sorted = nullptr;
-    if ((rc = order_particles(c, 0, x, y, z, m, n, off_dev, nsnap, s, &sorted))) return rc;
+    if ((rc = order_particles(c, 0, x, y, z, m, n, off_dev, nsnap, s, &sorted, off))) return rc;
     const size_t lds = (size_t)4 * (J * kDepStride + kDepExtra) * sizeof(float);
     timer_begin(c, "eof_cell_sums", s);
     hipLaunchKernelGGL(eof_cell_sums_kernel, dim3(P.grid), dim3(256), lds, s, sorted, P.S,
@@ -1713,7 +1713,7 @@ static int eof_accumulate_run(Context* c, const float* x, const float* y, const 
   const DepShape dsh = dep_shape(2 * c->mmax + 1, c->norder, kEofShapes, kEofDepShapes, c->eof_dep_shape);
   if (n >= c->order_min && n > 0 && dsh.ppt > 0 && dlds <= 64 * 1024) {
     if ((rc = eof_dep_prepare(c))) return rc;
-    rc = order_particles(c, 0, x, y, z, m, n, off_dev, nsnap, s, &sorted);
+    rc = order_particles(c, 0, x, y, z, m, n, off_dev, nsnap, s, &sorted, off);
     if (rc) return rc;
   }
   timer_begin(c, "eof_accumulate", s);
@@ -1803,7 +1803,7 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
                                (size_t)4 * K * sizeof(double));
   const float4* sorted = nullptr;
   if (n >= c->order_min && n > 0 && dsh.ppt > 0 && dlds <= 64 * 1024) {
-    rc = order_particles(c, 1, x, y, z, m, n, off_dev, nsnap, s, &sorted);
+    rc = order_particles(c, 1, x, y, z, m, n, off_dev, nsnap, s, &sorted, off);
     if (rc) return rc;
   }
   timer_begin(c, "sl_accumulate", s);

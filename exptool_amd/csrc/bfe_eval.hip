@@ -900,7 +900,7 @@ extern "C" int bfe_field_eval_batch(bfe_ctx* h, const float* x, const float* y, 
   T.srot = sin(rotpos);
   const uint32_t* sorted = nullptr;
   const float4* pts = nullptr;
-  int rc = order_points(c, T, x, y, z, n, s, nullptr, c->batch_off, nsnap, &sorted, &pts);
+  int rc = order_points(c, T, x, y, z, n, s, nullptr, c->batch_off, nsnap, &sorted, &pts, offsets);
   if (rc) return rc;
   // results go through a float4 per point, split coalesced (as field_eval_run)
   rc = ensure_scratch(c, (size_t)n * sizeof(float4));

@@ -244,10 +244,13 @@ struct FieldTables;
 // (snapshot, cell), and sorted_out (may be NULL) receives the sorted keys:
 // snapshot = key / order_cell_count(T).  points_out (may be NULL) receives
 // the points in key order as float4 (x, y, z, input index as int bits).
+// off_host (the host copy of off_dev) makes the sort segmented: the
+// snapshots keep their ranges and the passes cover only the cells of one.
 int order_points(Context* c, const FieldTables& T, const float* x, const float* y,
                  const float* z, int64_t n, hipStream_t s, const int** perm_out,
                  const int64_t* off_dev = nullptr, int nsnap = 1,
-                 const uint32_t** sorted_out = nullptr, const float4** points_out = nullptr);
+                 const uint32_t** sorted_out = nullptr, const float4** points_out = nullptr,
+                 const int64_t* off_host = nullptr);
 uint32_t order_cell_count(const FieldTables& T, int nsnap);
 // order_points over two point sets taken as one (n1 points of x, y, z then
 // n2 of x2, y2, z2): points_out in key order with the index over both.
@@ -259,9 +262,10 @@ int order_points2(Context* c, const FieldTables& T, const float* x, const float*
 // cos(theta) bins (the evaluation order); with off_dev the key is
 // (snapshot, cell), so snapshot s keeps its range [off[s], off[s+1]).  The
 // sort is stable, so the order is reproducible.
+// The sort is segmented when off_host (host copy of off_dev) is given.
 int order_particles(Context* c, int kind, const float* x, const float* y, const float* z,
                     const float* m, int64_t n, const int64_t* off_dev, int nsnap, hipStream_t s,
-                    const float4** sorted_out);
+                    const float4** sorted_out, const int64_t* off_host = nullptr);
 // Single-snapshot accumulates (bfe_eof_accumulate / bfe_sl_accumulate
 // without the argument checks), enqueued on s with the current buffer set.
 // Both are for callers that run them side by side (they take the deposit

@@ -27,6 +27,9 @@
 // because thousands of points share the busiest cells), no spin-waits and no
 // memsets: every counter is written by a kernel.  The packed float4 (x, y,
 // z, mass or input index) travels with the key, so no gather pass is needed.
+#include <algorithm>
+#include <vector>
+
 #include "bfe_field.hpp"
 
 namespace bfe {
@@ -203,26 +206,41 @@ __device__ __forceinline__ uint32_t block_scan_incl(uint32_t v, uint32_t* wsum) 
   return v + add;
 }
 
-// Item slot i of thread (w, l) in tile b: b * kSortTile + (w * kSortIPT + i) * 64 + l.
-__device__ __forceinline__ int64_t sort_item(int i) {
+// Items [t0, end) of tile b: b kSortTile and on, or the bounds tb[2b], tb[2b + 1]
+// (at most kSortTile items) of a segmented sort.  Slot i of thread (w, l)
+// holds item t0 + (w kSortIPT + i) 64 + l.
+struct TileRange {
+  int64_t t0, end;
+};
+__device__ __forceinline__ TileRange sort_tile(const int64_t* __restrict__ tb, int64_t n) {
+  if (tb) return TileRange{tb[2 * blockIdx.x], tb[2 * blockIdx.x + 1]};
+  const int64_t t0 = (int64_t)blockIdx.x * kSortTile;
+  return TileRange{t0, t0 + kSortTile < n ? t0 + kSortTile : n};
+}
+__device__ __forceinline__ int64_t sort_item(const TileRange& r, int i) {
   const int t = threadIdx.x;
-  return (int64_t)blockIdx.x * kSortTile + (int64_t)((t >> 6) * kSortIPT + i) * 64 + (t & 63);
+  return r.t0 + (int64_t)((t >> 6) * kSortIPT + i) * 64 + (t & 63);
 }
 
 // Digit histogram of each tile: counts[d * nb + tile].  First pass (kin ==
 // NULL): keys computed from the source and written to kfirst.
 __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(
-    KeySrc S, const uint32_t* __restrict__ kin, uint32_t* __restrict__ kfirst, int64_t n, int shift,
+    KeySrc S, const uint32_t* __restrict__ kin, uint32_t* __restrict__ kfirst, int64_t n,
+    const int64_t* __restrict__ tb, const int* __restrict__ tseg, int shift,
     uint32_t* __restrict__ counts, int nb) {
   __shared__ uint32_t hist[4][kRadix];
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const TileRange tr = sort_tile(tb, n);
+  // segmented sort: digits of the key less its segment's first key, so
+  // that the passes cover the cell bits only
+  const uint32_t kb = tseg ? (uint32_t)tseg[blockIdx.x] * S.ncell : 0u;
   for (int j = t; j < 4 * kRadix; j += kSortThreads) (&hist[0][0])[j] = 0u;
   uint32_t key[kSortIPT];
 #pragma unroll
   for (int i = 0; i < kSortIPT; ++i) {
-    const int64_t idx = sort_item(i);
+    const int64_t idx = sort_item(tr, i);
     key[i] = 0u;
-    if (idx < n) {
+    if (idx < tr.end) {
       if (kin) {
         key[i] = kin[idx];
       } else {
@@ -235,8 +253,8 @@ __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(
   const uint64_t lt = (1ull << l) - 1ull;
 #pragma unroll
   for (int i = 0; i < kSortIPT; ++i) {
-    const bool live = sort_item(i) < n;
-    const uint32_t d = (key[i] >> shift) & (kRadix - 1);
+    const bool live = sort_item(tr, i) < tr.end;
+    const uint32_t d = ((key[i] - kb) >> shift) & (kRadix - 1);
     const uint64_t mk = match_digit(d, live);
     if (live && (mk & lt) == 0ull) hist[w][d] += (uint32_t)__popcll(mk);
   }
@@ -266,12 +284,77 @@ __global__ __launch_bounds__(kSortThreads) void sort_rowscan_kernel(uint32_t* __
   if (t == kSortThreads - 1) totals[blockIdx.x] = incl;
 }
 
+// Segmented form (tile j in segment tseg[j], each segment's tiles
+// consecutive): per digit, the exclusive prefix of the tile counts within
+// each segment, in place, and each segment's digit total at
+// totals[segment * kRadix + digit].  Thread t takes a run of tiles; its
+// carry-in is the scan over the threads of (sum since the last segment
+// start, start seen) with the segmented sum.
+__global__ __launch_bounds__(kSortThreads) void sort_rowscan_seg_kernel(
+    uint32_t* __restrict__ counts, int nb, const int* __restrict__ tseg,
+    uint32_t* __restrict__ totals) {
+  __shared__ uint32_t wv[4], wincl[4];
+  __shared__ int wf[4];
+  const int t = threadIdx.x, l = t & 63, w = t >> 6;
+  uint32_t* row = counts + (int64_t)blockIdx.x * nb;
+  const int per = (nb + kSortThreads - 1) / kSortThreads;
+  const int j0 = min(t * per, nb), j1 = min(j0 + per, nb);
+  uint32_t v = 0u;
+  int fl = 0;
+  for (int j = j0; j < j1; ++j) {
+    if (j == 0 || tseg[j] != tseg[j - 1]) {
+      v = 0u;
+      fl = 1;
+    }
+    v += row[j];
+  }
+  // inclusive segmented scan: within the wave, then the carry of earlier
+  // waves up to the last one that starts a segment
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t pv = __shfl_up(v, o, 64);
+    const int pf = __shfl_up(fl, o, 64);
+    if (l >= o) {
+      if (!fl) v += pv;
+      fl |= pf;
+    }
+  }
+  if (l == 63) {
+    wv[w] = v;
+    wf[w] = fl;
+  }
+  __syncthreads();
+  if (!fl) {
+    for (int q = w - 1; q >= 0; --q) {
+      v += wv[q];
+      if (wf[q]) break;
+    }
+  }
+  if (l == 63) wincl[w] = v;
+  __syncthreads();
+  const uint32_t up = __shfl_up(v, 1, 64);
+  uint32_t run = l > 0 ? up : (w > 0 ? wincl[w - 1] : 0u);
+  for (int j = j0; j < j1; ++j) {
+    const int sj = tseg[j];
+    if (j == 0 || sj != tseg[j - 1]) run = 0u;
+    const uint32_t c = row[j];
+    row[j] = run;
+    run += c;
+    if (j + 1 == nb || tseg[j + 1] != sj) totals[(int64_t)sj * kRadix + blockIdx.x] = run;
+  }
+}
+
 // Stable scatter of one tile by digit.  vin == NULL: the values are packed
 // from the source (first pass).  kout and perm may be NULL on the last pass.
+// Segmented sort (tb != NULL): tile b belongs to segment tseg[b], whose digit
+// totals are totals[tseg[b] * kRadix + d] and whose items start at
+// segoff[tseg[b]].
 __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
     KeySrc S, const uint32_t* __restrict__ kin, const float4* __restrict__ vin, int64_t n,
-    int shift, const uint32_t* __restrict__ rowpre, const uint32_t* __restrict__ totals, int nb,
-    uint32_t* __restrict__ kout, float4* __restrict__ vout, int* __restrict__ perm) {
+    const int64_t* __restrict__ tb, const int* __restrict__ tseg,
+    const int64_t* __restrict__ segoff, int shift, const uint32_t* __restrict__ rowpre,
+    const uint32_t* __restrict__ totals, int nb, uint32_t* __restrict__ kout,
+    float4* __restrict__ vout, int* __restrict__ perm) {
   __shared__ uint32_t hist[4][kRadix];   // per-wave running counts, then wave prefixes
   __shared__ uint32_t dstart[kRadix];    // tile-local start of each digit
   __shared__ uint32_t gpos[kRadix];      // output position of the tile's first item of each digit
@@ -279,15 +362,18 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
   __shared__ uint32_t skey[kSortTile];
   __shared__ float4 sval[kSortTile];
   const int t = threadIdx.x, w = t >> 6, l = t & 63;
+  const TileRange tr = sort_tile(tb, n);
+  const int sg = tseg ? tseg[blockIdx.x] : 0;
+  const uint32_t kb = tseg ? (uint32_t)sg * S.ncell : 0u;
   for (int j = t; j < 4 * kRadix; j += kSortThreads) (&hist[0][0])[j] = 0u;
   uint32_t key[kSortIPT];
   float4 val[kSortIPT];
 #pragma unroll
   for (int i = 0; i < kSortIPT; ++i) {
-    const int64_t idx = sort_item(i);
+    const int64_t idx = sort_item(tr, i);
     key[i] = 0u;
     val[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (idx < n) {
+    if (idx < tr.end) {
       key[i] = kin[idx];
       val[i] = vin ? vin[idx] : source_value(S, idx);
     }
@@ -298,8 +384,8 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
   uint32_t rank[kSortIPT];
 #pragma unroll
   for (int i = 0; i < kSortIPT; ++i) {
-    const bool live = sort_item(i) < n;
-    const uint32_t d = (key[i] >> shift) & (kRadix - 1);
+    const bool live = sort_item(tr, i) < tr.end;
+    const uint32_t d = ((key[i] - kb) >> shift) & (kRadix - 1);
     const uint64_t mk = match_digit(d, live);
     uint32_t before = 0u;
     if (live) before = hist[w][d];
@@ -309,7 +395,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
   __syncthreads();
   const uint32_t c0 = hist[0][t], c1 = hist[1][t], c2 = hist[2][t], c3 = hist[3][t];
   const uint32_t tot = c0 + c1 + c2 + c3;
-  const uint32_t gt = totals[t];
+  const uint32_t gt = totals[(int64_t)sg * kRadix + t];
   const uint32_t pre = rowpre[(int64_t)t * nb + blockIdx.x];
   __syncthreads();
   hist[0][t] = 0u;
@@ -317,25 +403,24 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
   hist[2][t] = c0 + c1;
   hist[3][t] = c0 + c1 + c2;
   const uint32_t ds = block_scan_incl(tot, wsum) - tot;
-  const uint32_t gb = block_scan_incl(gt, wsum) - gt;
+  const uint32_t gb = block_scan_incl(gt, wsum) - gt + (segoff ? (uint32_t)segoff[sg] : 0u);
   dstart[t] = ds;
   gpos[t] = gb + pre;
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kSortIPT; ++i) {
-    if (sort_item(i) < n) {
-      const uint32_t d = (key[i] >> shift) & (kRadix - 1);
+    if (sort_item(tr, i) < tr.end) {
+      const uint32_t d = ((key[i] - kb) >> shift) & (kRadix - 1);
       const uint32_t lp = dstart[d] + hist[w][d] + rank[i];
       skey[lp] = key[i];
       sval[lp] = val[i];
     }
   }
   __syncthreads();
-  const int64_t rem = n - (int64_t)blockIdx.x * kSortTile;
-  const int cnt = rem < kSortTile ? (int)rem : kSortTile;
+  const int cnt = (int)(tr.end - tr.t0);
   for (int j = t; j < cnt; j += kSortThreads) {
     const uint32_t k = skey[j];
-    const uint32_t d = (k >> shift) & (kRadix - 1);
+    const uint32_t d = ((k - kb) >> shift) & (kRadix - 1);
     const uint32_t pos = gpos[d] + (uint32_t)j - dstart[d];
     const float4 v = sval[j];
     if (kout) kout[pos] = k;
@@ -347,14 +432,15 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Ordering buffer: first-pass keys | keys A | keys B | values A | values B |
-// perm | per-point f64 | tile counts | digit totals.
+// perm | per-point f64 | tile counts | digit totals (per segment) | tile
+// bounds and segments (segmented sorts).
 struct OrderLayout {
-  size_t k0, ka, kb, va, vb, perm, energy, counts, totals, total;
+  size_t k0, ka, kb, va, vb, perm, energy, counts, totals, tb, ts, total;
 };
 
 static int sort_tiles(int64_t n) { return (int)((n + kSortTile - 1) / kSortTile); }
 
-static OrderLayout order_layout(int64_t n) {
+static OrderLayout order_layout(int64_t n, int nb, int nseg) {
   OrderLayout L;
   const size_t kbytes = align256((size_t)n * 4), vbytes = align256((size_t)n * 16);
   L.k0 = 0;
@@ -365,8 +451,10 @@ static OrderLayout order_layout(int64_t n) {
   L.perm = L.vb + vbytes;
   L.energy = L.perm + kbytes;
   L.counts = L.energy + align256((size_t)n * 8);
-  L.totals = L.counts + align256((size_t)sort_tiles(n) * kRadix * 4);
-  L.total = L.totals + align256((size_t)kRadix * 4);
+  L.totals = L.counts + align256((size_t)nb * kRadix * 4);
+  L.tb = L.totals + align256((size_t)nseg * kRadix * 4);
+  L.ts = L.tb + (nseg > 1 ? align256((size_t)nb * 16) : 0);
+  L.total = L.ts + (nseg > 1 ? align256((size_t)nb * 4) : 0);
   return L;
 }
 
@@ -394,20 +482,53 @@ struct SortOut {
 
 // Sort the n items of S by key (nkeys distinct keys at most); writes the
 // sorted values, and when asked the sorted keys and the permutation (values
-// packed with the input index).
+// packed with the input index).  With off_host (S.off's host copy, more than
+// one snapshot) the sort is segmented: the snapshots' items stay in their
+// ranges, tiles never straddle two, and the passes cover only the S.ncell
+// cells of a snapshot (two 8-bit passes for up to 65536 cells, where the
+// (snapshot, cell) key of 64 snapshots needs three).
 static int sort_source(Context* c, const KeySrc& S, int64_t n, uint64_t nkeys, bool want_keys,
-                       bool want_perm, hipStream_t s, SortOut* out) {
+                       bool want_perm, hipStream_t s, SortOut* out,
+                       const int64_t* off_host = nullptr) {
   out->values = nullptr;
   out->keys = nullptr;
   out->perm = nullptr;
   if (n > (int64_t)0x7fffffff) return set_error(c, BFE_ELIMIT, "order: n = %lld > 2^31 - 1", (long long)n);
   if (nkeys > 0xffffffffull) return set_error(c, BFE_ELIMIT, "order: too many table cells");
+  const bool segmented = off_host && S.off && S.nsnap > 1;
+  std::vector<int64_t> tbv;
+  std::vector<int> tsv;
+  if (segmented) {
+    for (int q = 0; q < S.nsnap; ++q)
+      for (int64_t a = off_host[q]; a < off_host[q + 1]; a += kSortTile) {
+        tbv.push_back(a);
+        tbv.push_back(std::min(a + (int64_t)kSortTile, off_host[q + 1]));
+        tsv.push_back(q);
+      }
+    if ((int64_t)tsv.size() > (int64_t)0x7fffffff)
+      return set_error(c, BFE_ELIMIT, "order: too many tiles");
+  }
+  const uint64_t span = segmented ? (uint64_t)S.ncell : nkeys;
   int bits = 1;
-  while (bits < 32 && (1ull << bits) < nkeys) ++bits;
+  while (bits < 32 && (1ull << bits) < span) ++bits;
   const int passes = (bits + 7) / 8;
-  const OrderLayout L = order_layout(n);
+  const int nb = segmented ? (int)tsv.size() : sort_tiles(n);
+  const OrderLayout L = order_layout(n, nb, segmented ? S.nsnap : 1);
   int rc = ensure_order(c, L.total);
   if (rc) return rc;
+  const int64_t* tb = nullptr;
+  const int* ts = nullptr;
+  if (segmented && nb > 0) {
+    // the tile table is read by this call's kernels only; earlier work on s
+    // may still read the buffer
+    BFE_HIP(c, hipStreamSynchronize(s));
+    int64_t* tbd = (int64_t*)((char*)c->order + L.tb);
+    int* tsd = (int*)((char*)c->order + L.ts);
+    BFE_HIP(c, hipMemcpy(tbd, tbv.data(), tbv.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    BFE_HIP(c, hipMemcpy(tsd, tsv.data(), tsv.size() * sizeof(int), hipMemcpyHostToDevice));
+    tb = tbd;
+    ts = tsd;
+  }
   char* base = (char*)c->order;
   uint32_t* k0 = (uint32_t*)(base + L.k0);
   uint32_t* kbuf[2] = {(uint32_t*)(base + L.ka), (uint32_t*)(base + L.kb)};
@@ -415,22 +536,25 @@ static int sort_source(Context* c, const KeySrc& S, int64_t n, uint64_t nkeys, b
   int* perm = (int*)(base + L.perm);
   uint32_t* counts = (uint32_t*)(base + L.counts);
   uint32_t* totals = (uint32_t*)(base + L.totals);
-  const int nb = sort_tiles(n);
-  for (int p = 0; p < passes && n > 0; ++p) {
+  for (int p = 0; p < passes && n > 0 && nb > 0; ++p) {
     const int shift = 8 * p;
     const bool last = p == passes - 1;
     const uint32_t* kin = p == 0 ? (const uint32_t*)k0 : kbuf[(p - 1) & 1];
     const float4* vin = p == 0 ? nullptr : vbuf[(p - 1) & 1];
     uint32_t* kout = (!last || want_keys) ? kbuf[p & 1] : nullptr;
     hipLaunchKernelGGL(sort_count_kernel, dim3((unsigned)nb), dim3(kSortThreads), 0, s, S,
-                       p == 0 ? nullptr : kin, k0, n, shift, counts, nb);
+                       p == 0 ? nullptr : kin, k0, n, tb, ts, shift, counts, nb);
     BFE_HIP(c, hipGetLastError());
-    hipLaunchKernelGGL(sort_rowscan_kernel, dim3(kRadix), dim3(kSortThreads), 0, s, counts, nb,
-                       totals);
+    if (segmented)
+      hipLaunchKernelGGL(sort_rowscan_seg_kernel, dim3(kRadix), dim3(kSortThreads), 0, s, counts,
+                         nb, ts, totals);
+    else
+      hipLaunchKernelGGL(sort_rowscan_kernel, dim3(kRadix), dim3(kSortThreads), 0, s, counts, nb,
+                         totals);
     BFE_HIP(c, hipGetLastError());
     hipLaunchKernelGGL(sort_scatter_kernel, dim3((unsigned)nb), dim3(kSortThreads), 0, s, S, kin,
-                       vin, n, shift, counts, totals, nb, kout, vbuf[p & 1],
-                       (last && want_perm) ? perm : nullptr);
+                       vin, n, tb, ts, segmented ? S.off : nullptr, shift, counts, totals, nb,
+                       kout, vbuf[p & 1], (last && want_perm) ? perm : nullptr);
     BFE_HIP(c, hipGetLastError());
   }
   out->values = vbuf[(passes - 1) & 1];
@@ -445,7 +569,8 @@ uint32_t order_cell_count(const FieldTables& T, int nsnap) {
 
 int order_points(Context* c, const FieldTables& T, const float* x, const float* y, const float* z,
                  int64_t n, hipStream_t s, const int** perm_out, const int64_t* off_dev,
-                 int nsnap, const uint32_t** sorted_out, const float4** points_out) {
+                 int nsnap, const uint32_t** sorted_out, const float4** points_out,
+                 const int64_t* off_host) {
   if (perm_out) *perm_out = nullptr;
   if (sorted_out) *sorted_out = nullptr;
   if (points_out) *points_out = nullptr;
@@ -459,7 +584,7 @@ int order_points(Context* c, const FieldTables& T, const float* x, const float* 
   S.g.tbins = tb;
   SortOut o;
   int rc = sort_source(c, S, n, ncell * (uint64_t)ns, sorted_out != nullptr, perm_out != nullptr,
-                       s, &o);
+                       s, &o, off_dev ? off_host : nullptr);
   if (rc) return rc;
   if (perm_out) *perm_out = o.perm;
   if (sorted_out) *sorted_out = o.keys;
@@ -486,7 +611,7 @@ int order_points2(Context* c, const FieldTables& T, const float* x, const float*
 
 int order_particles(Context* c, int kind, const float* x, const float* y, const float* z,
                     const float* m, int64_t n, const int64_t* off_dev, int nsnap, hipStream_t s,
-                    const float4** sorted_out) {
+                    const float4** sorted_out, const int64_t* off_host) {
   *sorted_out = nullptr;
   const EOFGeom eg = eof_geom(c);
   const SLGeom sg = sl_geom(c);
@@ -498,14 +623,15 @@ int order_particles(Context* c, int kind, const float* x, const float* y, const 
            key_geom(eg, sg, c->h_xi.size() >= 2 ? c->h_xi.data() : nullptr), x, y, z, n};
   S.g.tbins = tb;
   SortOut o;
-  int rc = sort_source(c, S, n, ncell * (uint64_t)(off_dev ? nsnap : 1), false, false, s, &o);
+  int rc = sort_source(c, S, n, ncell * (uint64_t)(off_dev ? nsnap : 1), false, false, s, &o,
+                       off_dev ? off_host : nullptr);
   if (rc) return rc;
   *sorted_out = o.values;
   return 0;
 }
 
 double* order_energy_buffer(Context* c, int64_t n) {
-  return (double*)((char*)c->order + order_layout(n).energy);
+  return (double*)((char*)c->order + order_layout(n, sort_tiles(n), 1).energy);
 }
 
 void order_free(Context* c) {
