@@ -14,14 +14,14 @@
 //
 // The sort is a stable least-significant-digit counting sort over 8-bit
 // digits, ceil(bits / 8) passes of three kernels:
-//   count   - per 2048-item tile, the digit histogram (the first pass also
-//             computes the keys from the coordinates);
+//   count   - per 2048-item tile, the digit histogram by LDS atomics (the
+//             first pass also computes the keys from the coordinates);
 //   rowscan - per digit, the exclusive prefix of the tile counts;
 //   scatter - each tile ranks its items per digit, stages them in LDS in
 //             digit order and writes them out in runs.
-// Within a wave, items with equal digits are found with eight ballots and
-// ranked by popcount; a per-wave histogram in LDS carries the ranks across
-// the wave's item slots.  The order inside a tile is (wave, slot, lane),
+// In the scatter, items of a wave with equal digits are found with eight
+// ballots and ranked by popcount; a per-wave histogram in LDS carries the
+// ranks across the wave's item slots.  The order inside a tile is (wave, slot, lane),
 // which is input order, so the sort is stable and the result reproducible.
 // There are no global atomics (a counting sort with them was slow here
 // because thousands of points share the busiest cells), no spin-waits and no
@@ -250,13 +250,13 @@ __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(
     }
   }
   __syncthreads();
-  const uint64_t lt = (1ull << l) - 1ull;
+  // counts only, so LDS atomics (lanes on one digit serialise in the LDS,
+  // no worse than the eight ballots of a rank)
 #pragma unroll
   for (int i = 0; i < kSortIPT; ++i) {
     const bool live = sort_item(tr, i) < tr.end;
     const uint32_t d = ((key[i] - kb) >> shift) & (kRadix - 1);
-    const uint64_t mk = match_digit(d, live);
-    if (live && (mk & lt) == 0ull) hist[w][d] += (uint32_t)__popcll(mk);
+    if (live) atomicAdd(&hist[w][d], 1u);
   }
   __syncthreads();
   counts[(int64_t)t * nb + blockIdx.x] = hist[0][t] + hist[1][t] + hist[2][t] + hist[3][t];
