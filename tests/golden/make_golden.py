@@ -62,11 +62,48 @@ def edge_particles(g, n):
     return x, y, z, m
 
 
+class _Masses:
+    """The part of a particle instance compute_variance reads: data['m']
+    (and the coordinates, which it does not use)."""
+
+    def __init__(self, m):
+        z = np.zeros_like(m)
+        self.data = {"x": z, "y": z, "z": z, "m": m}
+
+
+def sn_vectors(R_eof):
+    """eof.compute_variance / compute_sn on seeded coefficient sums: T = 4
+    subsample sums per coefficient, scaled so that some variances are
+    negative (their S/N is NaN in the reference) and some are not."""
+    g = np.random.Generator(np.random.PCG64(20241203))
+    m = g.uniform(0.5, 1.5, 2000) / 2000.0
+    cos = g.normal(0.0, 1.0, (3, 4))
+    sin = g.normal(0.0, 1.0, (3, 4))
+    sin[0] = 0.0
+    cos2 = g.uniform(0.0, 2e-3, (4, 3, 4))
+    sin2 = g.uniform(0.0, 2e-3, (4, 3, 4))
+    P = _Masses(m)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        varC, varS, facC, facS = R_eof.compute_variance(P, cos, sin, cos2, sin2)
+        snC, snS = R_eof.compute_sn(P, cos, sin, cos2, sin2)
+    return dict(sn_m=m, sn_cos=cos, sn_sin=sin, sn_cos2=cos2, sn_sin2=sin2, sn_varC=varC,
+                sn_varS=varS, sn_facC=facC, sn_facS=facS, sn_snC=snC, sn_snS=snS)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--sn", action="store_true",
+                    help="write only golden_sn.npz (compute_variance / compute_sn vectors)")
     a = ap.parse_args()
     sys.path.insert(0, a.ref)
+    if a.sn:
+        from exptool.basis import eof as R_eof  # noqa: E402
+        out = sn_vectors(R_eof)
+        path = os.path.join(HERE, "golden_sn.npz")
+        np.savez_compressed(path, **{k: np.asarray(v) for k, v in out.items()})
+        print("wrote", path, os.path.getsize(path), "bytes;", len(out), "arrays")
+        return
     from exptool.basis import spheresl as R_sl  # noqa: E402
     from exptool.basis import eof as R_eof  # noqa: E402
     from exptool.utils import halo_methods as R_hm  # noqa: E402

@@ -118,6 +118,30 @@ def test_eof_variance_and_sn_hand_values():
     np.testing.assert_allclose(snS, [[[96 ** 0.5]]], rtol=1e-13)
 
 
+def test_eof_variance_and_sn_reference_vectors():
+    """compute_variance / compute_sn against the reference's own outputs
+    (tests/golden/golden_sn.npz, written by make_golden.py --sn): 4 subsample
+    sums of 3 x 4 coefficients over 2000 masses.  The subsample sums are not
+    squares, so some variances come out negative and the reference's S/N,
+    (c^2/var)^(1/2), is NaN there (eof.py:2223); those NaNs are matched too."""
+    from exptool_amd.basis import eof
+    from exptool_amd.io import particle
+    G = np.load(os.path.join(HERE, "golden", "golden_sn.npz"))
+    m = G["sn_m"]
+    P = particle.make_holder(np.zeros(m.size), np.zeros(m.size), np.zeros(m.size), m)
+    args = (G["sn_cos"], G["sn_sin"], G["sn_cos2"], G["sn_sin2"])
+    with np.errstate(invalid="ignore", divide="ignore"):
+        got = eof.compute_variance(P, *args)
+        sn = eof.compute_sn(P, *args)
+    for k, v in zip(("varC", "varS", "facC", "facS"), got):
+        np.testing.assert_allclose(v, G["sn_" + k], rtol=1e-13, atol=0, err_msg=k)
+    for k, v in zip(("snC", "snS"), sn):
+        ref = G["sn_" + k]
+        assert np.isnan(ref).any() and np.isfinite(ref).any()
+        assert np.array_equal(np.isnan(v), np.isnan(ref)), k
+        np.testing.assert_allclose(v[~np.isnan(v)], ref[~np.isnan(ref)], rtol=1e-13, err_msg=k)
+
+
 BARFILE = os.path.join(HERE, "golden", "barfile001.dat")
 
 
