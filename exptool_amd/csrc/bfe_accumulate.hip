@@ -647,9 +647,13 @@ __device__ __forceinline__ void eof_deposit_body(
 // fused calls), eof_deposit_alone_kernel when it has the GPU to itself.
 // Alone, four waves per SIMD (128 registers, about 100 B spilled) measured
 // 80 against 83 us for 1M disc particles; beside the SL deposit they lost.
+// Beside it, the one-slot shapes up to 8 orders are held at three waves
+// (168 registers, a few bytes spilled): unconstrained they took 169 and
+// fell to two, which made the fused call slower; the wider shapes keep
+// their natural register counts.
 template <int PPT, int CH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PPT == 1 && CH <= 8 ? 3 : 1))) void
-eof_deposit_kernel(
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(PPT == 1 && CH <= 8 ? 3 : 1))) void eof_deposit_kernel(
     const float4* __restrict__ P, Segments S, EOFGeom g, const float* __restrict__ T, int rp,
     int K, double* __restrict__ partials) {
   extern __shared__ __align__(16) unsigned char dsm[];

@@ -21,8 +21,10 @@
 //             digit order and writes them out in runs.
 // In the scatter, items of a wave with equal digits are found with eight
 // ballots and ranked by popcount; a per-wave histogram in LDS carries the
-// ranks across the wave's item slots.  The order inside a tile is (wave, slot, lane),
-// which is input order, so the sort is stable and the result reproducible.
+// ranks across the wave's item slots.  The order inside a tile is (wave,
+// slot, lane), which is input order, so the sort is stable and the result
+// reproducible.  Concatenated snapshots sort segmented: tiles (at most 2048
+// items) stay inside one snapshot and the digit prefixes restart at each.
 // There are no global atomics (a counting sort with them was slow here
 // because thousands of points share the busiest cells), no spin-waits and no
 // memsets: every counter is written by a kernel.  The packed float4 (x, y,
