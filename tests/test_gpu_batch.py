@@ -33,6 +33,9 @@ def sysm():
 
 COUNTS = [3000, 0, 1500, 40000, 257, 9000]
 COUNTS_BIG = [50000, 30000, 0, 20000]        # >= 65536 in total: cell-ordered accumulate
+# the segmented sort's tile edges (2048 items): whole, one over, one under,
+# a single item and an empty snapshot, >= 65536 in total
+COUNTS_EDGE = [2048, 1, 0, 2049, 4095, 60000]
 
 
 def _snapshots(counts=COUNTS):
@@ -45,10 +48,15 @@ def _snapshots(counts=COUNTS):
     return parts_d, parts_h, f32(*cat(parts_d)), f32(*cat(parts_h))
 
 
-@pytest.mark.parametrize("counts", [COUNTS, COUNTS_BIG])
+@pytest.mark.parametrize("counts", [COUNTS, COUNTS_BIG, COUNTS_EDGE])
 def test_batch_accumulate_matches_oracle_and_single(sysm, counts):
     s, slo, eo = sysm
     pd, ph, D, H = _snapshots(counts)
+    ref = {}
+    for k, n in enumerate(counts):
+        if n:
+            d, hh = f32(*pd[k]), f32(*ph[k])
+            ref[k] = (d, O.eof_accumulate(eo, *d), O.sl_accumulate(slo, *hh))
     for tiles in (0, 1, 7):
         s.set_tuning("eof_seg_tiles", tiles)
         s.set_tuning("sl_seg_tiles", tiles)
@@ -59,30 +67,29 @@ def test_batch_accumulate_matches_oracle_and_single(sysm, counts):
             if n == 0:
                 assert np.all(c[k] == 0) and np.all(sn[k] == 0) and np.all(h[k] == 0)
                 continue
-            d = f32(*pd[k])
-            rc, rs = O.eof_accumulate(eo, *d)
+            d, (rc, rs), rh = ref[k]
             assert block_rel_eof(c[k], rc) <= 1e-6, (tiles, k)
             assert block_rel_eof(sn[k][1:], rs[1:]) <= 1e-6, (tiles, k)
-            hh = f32(*ph[k])
-            assert block_rel_sl(h[k], O.sl_accumulate(slo, *hh), 6) <= 1e-6, (tiles, k)
+            assert block_rel_sl(h[k], rh, 6) <= 1e-6, (tiles, k)
             c1, s1 = (v.cpu().numpy() for v in s.eof_accumulate(*d))
             assert block_rel_eof(c[k], c1) <= 1e-6
     s.set_tuning("eof_seg_tiles", 0)
     s.set_tuning("sl_seg_tiles", 0)
 
 
-def test_batch_field_equals_per_snapshot_field(sysm):
+@pytest.mark.parametrize("counts", [COUNTS, COUNTS_EDGE])
+def test_batch_field_equals_per_snapshot_field(sysm, counts):
     s, _, _ = sysm
-    pd, ph, D, H = _snapshots()
-    c, sn = s.eof_accumulate_batch(*D, COUNTS)
-    h = s.sl_accumulate_batch(*H, COUNTS)
+    pd, ph, D, H = _snapshots(counts)
+    c, sn = s.eof_accumulate_batch(*D, counts)
+    h = s.sl_accumulate_batch(*H, counts)
     # evaluate each snapshot's field at its disc particles
     x, y, z = D[0], D[1], D[2]
     for no_odd in (False, True):
         s.set_field_batch(c, sn, h, no_odd=no_odd)
-        got = [v.cpu().numpy() for v in s.field_forces_batch(x, y, z, COUNTS, rotpos=0.3)]
-        off = np.concatenate([[0], np.cumsum(COUNTS)])
-        for k, n in enumerate(COUNTS):
+        got = [v.cpu().numpy() for v in s.field_forces_batch(x, y, z, counts, rotpos=0.3)]
+        off = np.concatenate([[0], np.cumsum(counts)])
+        for k, n in enumerate(counts):
             if n == 0:
                 continue
             s.set_field(c[k], sn[k], h[k], no_odd=no_odd)
