@@ -111,21 +111,50 @@ struct EOFFoldArgs {
   int numx, numy;          // grid, for the float64 edge blocks
 };
 
+// The float64 edge blocks of the folded field (after the float32 blocks of
+// each set): the m = 0 cos sums [pot, rforce, zforce, dens] of edge node e
+// (eof_edge_index) from its float64 fold inputs E[f][n][edge], as the float32
+// blocks sum them for m = 0.
+__device__ __forceinline__ void eof_fold_edge(const double* __restrict__ E, int64_t nedge,
+                                              int64_t e, const double* __restrict__ cf, int mmax,
+                                              int norder, const EOFFoldArgs& a,
+                                              float* __restrict__ out_set) {
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int n = 0; n < a.nuse; ++n) {
+    const double c = cf[n];
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+      if (f < 3 || a.dens) s[f] += c * E[((size_t)f * norder + n) * nedge + e];
+  }
+  const double mul = a.mmul[0];
+  double4* o = reinterpret_cast<double4*>(out_set + eof_fold32_floats(a.numx, a.numy, mmax)) + e;
+  *o = make_double4(s[0] * mul, s[1] * mul, s[2] * mul, s[3] * mul);
+}
+
 // One thread per (row group q, node): q = m for cos m = 0..mmax, mmax + m
 // for sin m = 1..mmax; nodes fastest so the function-major float32 tables
 // are read coalesced (sums in float64), and q uniform per block so the coefficients are
 // scalar loads.  Writes its half [pot, rf, zf, dens] of the 8-float block
 // [potC, rfC, zfC, dC, potS, rfS, zfS, dS]; cos m = 0 also zeroes the sin
-// half.  Edge nodes (eof_edge_index) also write the half in float64 after
-// the float32 blocks.  grid.x = (2 mmax + 1) * chunks, grid.y = coefficient
-// sets.
+// half.  grid.x = ceil(nedge / 256) blocks for the float64 edge blocks
+// (eof_fold_edge) of the m = 0 cos rows, first so that they do not trail,
+// then (2 mmax + 1) * chunks blocks of these; grid.y = coefficient sets.
 __global__ __launch_bounds__(256) void eof_fold_kernel(
     const float* __restrict__ T0, const float* __restrict__ T1, const float* __restrict__ T2,
     const float* __restrict__ T3, const double* __restrict__ ccos, const double* __restrict__ csin,
     int mmax, int norder, int64_t nodes, int chunks, EOFFoldArgs a, float* __restrict__ out,
-    int64_t coef_stride, int64_t out_stride) {
-  const int q = (int)(blockIdx.x / chunks);
-  const int64_t node = (int64_t)(blockIdx.x - (unsigned)q * chunks) * 256 + threadIdx.x;
+    int64_t coef_stride, int64_t out_stride, const double* __restrict__ E, int64_t nedge) {
+  const unsigned eblocks = (unsigned)((nedge + 255) / 256);
+  if (blockIdx.x < eblocks) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e < nedge)
+      eof_fold_edge(E, nedge, e, ccos + (size_t)blockIdx.y * coef_stride, mmax, norder, a,
+                    out + (size_t)blockIdx.y * out_stride);
+    return;
+  }
+  const unsigned b = blockIdx.x - eblocks;
+  const int q = (int)(b / chunks);
+  const int64_t node = (int64_t)(b - (unsigned)q * chunks) * 256 + threadIdx.x;
   if (node >= nodes) return;
   const bool sn = q > mmax;
   const int m = sn ? q - mmax : q;
@@ -163,31 +192,6 @@ __global__ __launch_bounds__(256) void eof_fold_kernel(
   o[2] = (float)(s2 * mul);
   o[3] = (float)(s3 * mul);
   if (q == 0) o[4] = o[5] = o[6] = o[7] = 0.f;
-}
-
-// The float64 edge blocks of the folded field (after the float32 blocks of
-// each set): the m = 0 cos sums [pot, rforce, zforce, dens] of each edge node
-// (eof_edge_index) from its float64 fold inputs E[f][n][edge], as
-// eof_fold_kernel sums them for m = 0.  grid.x: edge nodes, grid.y:
-// coefficient sets.
-__global__ __launch_bounds__(256) void eof_fold_edge_kernel(
-    const double* __restrict__ E, int64_t nedge, const double* __restrict__ ccos, int mmax,
-    int norder, EOFFoldArgs a, float* __restrict__ out, int64_t coef_stride,
-    int64_t out_stride) {
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= nedge) return;
-  const double* cf = ccos + (size_t)blockIdx.y * coef_stride;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int n = 0; n < a.nuse; ++n) {
-    const double c = cf[n];
-#pragma unroll
-    for (int f = 0; f < 4; ++f)
-      if (f < 3 || a.dens) s[f] += c * E[((size_t)f * norder + n) * nedge + e];
-  }
-  const double mul = a.mmul[0];
-  double4* o = reinterpret_cast<double4*>(out + (size_t)blockIdx.y * out_stride +
-                                          eof_fold32_floats(a.numx, a.numy, mmax)) + e;
-  *o = make_double4(s[0] * mul, s[1] * mul, s[2] * mul, s[3] * mul);
 }
 
 static int ensure_folded(Context* c, float** d, size_t* have, size_t floats, hipStream_t s) {
@@ -262,16 +266,13 @@ static int eof_fold_into(Context* c, const double* ccos, const double* csin, int
   a.numx = c->numx;
   a.numy = c->numy;
   const int chunks = (int)((nodes + 255) / 256);
-  hipLaunchKernelGGL(eof_fold_kernel, dim3((unsigned)((2 * c->mmax + 1) * chunks), nset), dim3(256),
-                     0, s, c->eof_fold32[0], c->eof_fold32[1], c->eof_fold32[2],
-                     dens ? c->eof_fold32[3] : c->eof_fold32[0], ccos, csin, c->mmax, c->norder, nodes,
-                     chunks, a, out,
-                     (int64_t)(c->mmax + 1) * c->norder, (int64_t)eof_fold_floats(c));
-  BFE_HIP(c, hipGetLastError());
   const int64_t nedge = eof_edge_count(c->numx, c->numy);
-  hipLaunchKernelGGL(eof_fold_edge_kernel, dim3((unsigned)((nedge + 255) / 256), nset), dim3(256),
-                     0, s, c->eof_edge64, nedge, ccos, c->mmax, c->norder, a, out,
-                     (int64_t)(c->mmax + 1) * c->norder, (int64_t)eof_fold_floats(c));
+  const unsigned grid = (unsigned)((2 * c->mmax + 1) * chunks + (nedge + 255) / 256);
+  hipLaunchKernelGGL(eof_fold_kernel, dim3(grid, nset), dim3(256), 0, s, c->eof_fold32[0],
+                     c->eof_fold32[1], c->eof_fold32[2],
+                     dens ? c->eof_fold32[3] : c->eof_fold32[0], ccos, csin, c->mmax, c->norder,
+                     nodes, chunks, a, out, (int64_t)(c->mmax + 1) * c->norder,
+                     (int64_t)eof_fold_floats(c), c->eof_edge64, nedge);
   BFE_HIP(c, hipGetLastError());
   return 0;
 }
