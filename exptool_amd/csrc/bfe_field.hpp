@@ -347,26 +347,32 @@ __device__ __forceinline__ void eof_eval_folded(const EOFGeom& g, const float* _
   const double d0 = DENS ? w00 * a0.w + w10 * c0.w + w01 * b0.w + w11 * e0.w : 0.0;
   double fr = fr0, fz = fz0, p = 0, fp = 0, d = 0;
   double cm = 1.0, sm = 0.0, cm1 = c1, sm1 = -s1;
+  // m > 0: the corners of the float32 table interpolated in float32 (one
+  // conversion per value instead of four), the sums over m in float64
+  const float v00 = (float)w00, v10 = (float)w10, v01 = (float)w01, v11 = (float)w11;
+  auto bil = [&](float a, float c, float b, float e) {
+    return (double)fmaf(v11, e, fmaf(v01, b, fmaf(v10, c, v00 * a)));
+  };
 #pragma unroll MU
   for (int m = 1; m <= M; ++m) {
     const double cn = 2.0 * c1 * cm - cm1;   // m = 1 gives c1, s1 exactly
     const double sn = 2.0 * c1 * sm - sm1;
     cm1 = cm; sm1 = sm; cm = cn; sm = sn;
     const float4 a = A[2 * m], b = B[2 * m], c = C[2 * m], e = D[2 * m];
-    const double pc = w00 * a.x + w10 * c.x + w01 * b.x + w11 * e.x;
-    const double rc = w00 * a.y + w10 * c.y + w01 * b.y + w11 * e.y;
-    const double zc = w00 * a.z + w10 * c.z + w01 * b.z + w11 * e.z;
+    const double pc = bil(a.x, c.x, b.x, e.x);
+    const double rc = bil(a.y, c.y, b.y, e.y);
+    const double zc = bil(a.z, c.z, b.z, e.z);
     const float4 a2 = A[2 * m + 1], b2 = B[2 * m + 1], c2 = C[2 * m + 1], e2 = D[2 * m + 1];
-    const double ps = w00 * a2.x + w10 * c2.x + w01 * b2.x + w11 * e2.x;
-    const double rs = w00 * a2.y + w10 * c2.y + w01 * b2.y + w11 * e2.y;
-    const double zs = w00 * a2.z + w10 * c2.z + w01 * b2.z + w11 * e2.z;
+    const double ps = bil(a2.x, c2.x, b2.x, e2.x);
+    const double rs = bil(a2.y, c2.y, b2.y, e2.y);
+    const double zs = bil(a2.z, c2.z, b2.z, e2.z);
     p += cm * pc + sm * ps;
     fr += cm * rc + sm * rs;
     fz += cm * zc + sm * zs;
     fp += m * (sm * pc - cm * ps);
     if (DENS) {
-      const double dc = w00 * a.w + w10 * c.w + w01 * b.w + w11 * e.w;
-      const double ds = w00 * a2.w + w10 * c2.w + w01 * b2.w + w11 * e2.w;
+      const double dc = bil(a.w, c.w, b.w, e.w);
+      const double ds = bil(a2.w, c2.w, b2.w, e2.w);
       d += cm * dc + sm * ds;
     }
   }
