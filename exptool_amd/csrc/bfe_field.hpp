@@ -101,7 +101,8 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, PA A, const float4* 
       const float4 a = ld4(A, 2 * l), b = ld4(A, 2 * l + 1);
       const double ec = (double)fmaf(x2f, a.z, a.x);
       const double dc = (double)fmaf(ps, b.z, b.x);
-      const double dpl = somx2d * (xc * g.lmc[kLmcL + lm] * pl - g.lmc[kLmcS + lm] * plm1);
+      // dP/dx without its factor 1/(x^2 - 1), applied once to pott below
+      const double dpl = xc * g.lmc[kLmcL + lm] * pl - g.lmc[kLmcS + lm] * plm1;
       pot1 += pl * ec;
       potr += pl * dc;
       pott += dpl * ec;
@@ -128,7 +129,7 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, PA A, const float4* 
       const double dc = (double)fmaf(ps, b.z, b.x);
       const double es = (double)fmaf(x2f, a.w, a.y);
       const double ds = (double)fmaf(ps, b.w, b.y);
-      const double dpl = somx2d * (xc * g.lmc[kLmcL + lm] * pl - g.lmc[kLmcS + lm] * plm1);
+      const double dpl = xc * g.lmc[kLmcL + lm] * pl - g.lmc[kLmcS + lm] * plm1;
       Pc += pl * ec;
       Ps += pl * es;
       Rc += pl * dc;
@@ -158,6 +159,7 @@ __device__ __forceinline__ void sl_angular(const SLGeom& g, PA A, const float4* 
     potp += m * (cmm * Ps - smm * Pc);
     if (DENS) den1 += cmm * Nc + smm * Ns;
   }
+  pott *= somx2d;
   if (!good) {   // P_l^m = 0 for l > 0 when cos(theta) is NaN (spheresl.py:698)
     pot1 = potr = pott = potp = den1 = 0.0;
   }
