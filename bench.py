@@ -297,14 +297,21 @@ def roofline(kernel, unit, seconds, pmc):
     ex = (pmc or {}).get("executed_flops", {}).get(kernel)
     tr = (pmc or {}).get("traffic", {}).get(kernel)
     if ex:
+        # both precisions share the vector ALU: the float32 flops count at the
+        # ratio of the peaks, so frac is the share of the kernel's time the
+        # executed work needs at the published f64 and f32 vector peaks
         f64, f32 = ex["f64_flops_per_unit"], ex["f32_flops_per_unit"]
         use64 = f64 >= f32
-        per_unit = f64 if use64 else f32
-        peak = FP64_PEAK_TFLOPS if use64 else FP32_PEAK_TFLOPS
+        if use64:
+            per_unit, peak = f64 + f32 * FP64_PEAK_TFLOPS / FP32_PEAK_TFLOPS, FP64_PEAK_TFLOPS
+        else:
+            per_unit, peak = f32 + f64 * FP32_PEAK_TFLOPS / FP64_PEAK_TFLOPS, FP32_PEAK_TFLOPS
         achieved = nunits * per_unit / seconds / 1e12
         rec.update({"bound": "valu_f64" if use64 else "valu_f32", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                     "frac": achieved / peak, "flops_per_unit": per_unit,
-                    "flops_precision": "f64" if use64 else "f32",
+                    "flops_per_unit_f64": f64, "flops_per_unit_f32": f32,
+                    "flops_precision": ("f64-equivalent (f64 + f32 x 78.6/157.3)" if use64
+                                        else "f32-equivalent (f32 + f64 x 157.3/78.6)"),
                     "flops_source": "executed, rocprofv3 SQ_INSTS_VALU_* (profiles/bench_step_pmc_valu.md)"})
     else:
         achieved = rec["algorithmic_hbm_gbs"]
