@@ -163,7 +163,9 @@ __device__ __forceinline__ float4 source_value(const KeySrc& S, int64_t i) {
 // kind 0: the EOF (ix, iy) cell of (R = sqrt(x^2 + y^2 + 1e-10), z), clamped
 // to the grid as eof.return_bins; kind 1: key_sl_theta of r = max(|x|, 1e-10)
 // (the SL accumulate and evaluation order).
-__device__ __forceinline__ uint32_t source_key(const KeySrc& S, int64_t i) {
+// snap: the point's snapshot when the caller knows it (a tile of the
+// segmented sort lies in one snapshot), else -1 (searched in S.off).
+__device__ __forceinline__ uint32_t source_key(const KeySrc& S, int64_t i, int snap_known) {
   float x, y, z;
   source_xyz(S, i, x, y, z);
   const KeyGeom& g = S.g;
@@ -176,7 +178,8 @@ __device__ __forceinline__ uint32_t source_key(const KeySrc& S, int64_t i) {
     const float Y = (key_y(z, g.eih) - g.eymin) * g.eidy;
     cell = key_clamp(X, g.numx - 1) * (uint32_t)g.numy + key_clamp(Y, g.numy - 1);
   }
-  const uint32_t snap = S.off ? (uint32_t)snapshot_of(S.off, S.nsnap, i) : 0u;
+  const uint32_t snap = snap_known >= 0 ? (uint32_t)snap_known
+                       : (S.off ? (uint32_t)snapshot_of(S.off, S.nsnap, i) : 0u);
   return snap * S.ncell + cell;
 }
 
@@ -235,7 +238,8 @@ __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(
   const TileRange tr = sort_tile(tb, n);
   // segmented sort: digits of the key less its segment's first key, so
   // that the passes cover the cell bits only
-  const uint32_t kb = tseg ? (uint32_t)tseg[blockIdx.x] * S.ncell : 0u;
+  const int tsnap = tseg ? tseg[blockIdx.x] : -1;
+  const uint32_t kb = tseg ? (uint32_t)tsnap * S.ncell : 0u;
   for (int j = t; j < 4 * kRadix; j += kSortThreads) (&hist[0][0])[j] = 0u;
   uint32_t key[kSortIPT];
 #pragma unroll
@@ -246,7 +250,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_count_kernel(
       if (kin) {
         key[i] = kin[idx];
       } else {
-        key[i] = source_key(S, idx);
+        key[i] = source_key(S, idx, tsnap);
         kfirst[idx] = key[i];
       }
     }
