@@ -99,6 +99,44 @@ def test_batch_field_equals_per_snapshot_field(sysm, counts):
                 assert np.array_equal(got[j][off[k]:off[k + 1]], ref[j]), (no_odd, k, j)
 
 
+# more than 2 048 sort tiles (2 048 items each): the segmented tile prefix
+# takes two rounds, the second starting inside the third snapshot
+COUNTS_ROUNDS = [2_600_000, 0, 1_700_001, 7]
+
+
+def test_batch_two_round_segmented_sort(sysm):
+    """4.3M points in one batch: coefficients as the single-snapshot calls
+    within the float32 mode's 1e-6 per block (same kernels; other segment
+    plans regroup the float32 run sums, 2e-8 seen), and every point
+    evaluated in its own snapshot's field bit for bit."""
+    s, _, _ = sysm
+    counts = COUNTS_ROUNDS
+    pd, ph, D, H = _snapshots(counts)
+    c, sn = (v.cpu().numpy() for v in s.eof_accumulate_batch(*D, counts))
+    h = s.sl_accumulate_batch(*H, counts).cpu().numpy()
+    for k, n in enumerate(counts):
+        if n == 0:
+            assert np.all(c[k] == 0) and np.all(sn[k] == 0) and np.all(h[k] == 0)
+            continue
+        c1, s1 = (v.cpu().numpy() for v in s.eof_accumulate(*f32(*pd[k])))
+        h1 = s.sl_accumulate(*f32(*ph[k])).cpu().numpy()
+        assert block_rel_eof(c[k], c1) <= 1e-6, k
+        assert block_rel_eof(sn[k][1:], s1[1:]) <= 1e-6, k
+        assert block_rel_sl(h[k], h1, 6) <= 1e-6, k
+    s.set_field_batch(c, sn, h)
+    x, y, z = D[0], D[1], D[2]
+    got = [v.cpu().numpy() for v in s.field_forces_batch(x, y, z, counts)]
+    off = np.concatenate([[0], np.cumsum(counts)])
+    for k, n in enumerate(counts):
+        if n == 0:
+            continue
+        s.set_field(c[k], sn[k], h[k])
+        ref = [v.cpu().numpy() for v in s.field_forces(x[off[k]:off[k + 1]], y[off[k]:off[k + 1]],
+                                                        z[off[k]:off[k + 1]])]
+        for j in range(4):
+            assert np.array_equal(got[j][off[k]:off[k + 1]], ref[j]), (k, j)
+
+
 def test_batch_sampler_and_errors(sysm):
     s, _, _ = sysm
     from exptool_amd._lib import BFEError
