@@ -118,13 +118,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
   const bool live = j < a.n;
   const int64_t i = live ? (int64_t)perm[j] : 0;
   float px = 0, py = 0, pz = 0, ux = 0, uy = 0, uz = 0, ax = 0, ay = 0, az = 0;
-  OrbitAux o{};
   double dtd = a.dt;
   if (live) {
     px = x[i]; py = y[i]; pz = z[i];
     ux = vx[i]; uy = vy[i]; uz = vz[i];
-    o = aux[i];
-    ax = o.ax; ay = o.ay; az = o.az;
+    // the energy fields of aux are read after the steps (fewer registers
+    // live across the field evaluations)
+    const float4 acc = *reinterpret_cast<const float4*>(&aux[i].ax);
+    ax = acc.x; ay = acc.y; az = acc.z;
     if (PER_ORBIT) dtd = a.dtv[i];
   }
   const float dt = (float)dtd;
@@ -170,6 +171,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
   if (live) {
     x[i] = px; y[i] = py; z[i] = pz;
     vx[i] = ux; vy[i] = uy; vz[i] = uz;
+    OrbitAux o = aux[i];
     o.ax = ax; o.ay = ay; o.az = az;
     if (a.energy_every > 0 && a.k1 > a.k0) {
       const double e = 0.5 * ((double)ux * ux + (double)uy * uy + (double)uz * uz) + (double)pot;
