@@ -103,9 +103,9 @@ __global__ __launch_bounds__(256) void leapfrog_init_kernel(
 
 // PER_ORBIT: each orbit reads its step size from a.dtv (an instantiation of
 // its own, so the common one keeps dt in scalar registers).  Three waves per
-// SIMD (at most 168 registers, about 300 B per lane spilled outside the
-// field sums): 1.11e10 against 9.1e9 orbit-steps/s at two waves, 1.07e10 at
-// four (1M orbits, 10k steps, same results).
+// SIMD (at most 168 registers, 340 B per lane spilled outside the field
+// sums): 1.11e10 against 9.1e9 orbit-steps/s at two waves, 1.07e10 at four
+// (1M orbits, 10k steps, same results, when 364 B were spilled).
 template <bool PER_ORBIT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void leapfrog_chunk_kernel(
     float* __restrict__ x, float* __restrict__ y, float* __restrict__ z, float* __restrict__ vx,
