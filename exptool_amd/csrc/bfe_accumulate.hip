@@ -1778,7 +1778,7 @@ static int sl_accumulate_run(Context* c, const float* x, const float* y, const f
   const size_t lds = ((size_t)R * (kSLTile + 1) + (size_t)kSLTile * LN + 3 * kSLTile) * 4;
   if (lds > 64 * 1024) return set_error(c, BFE_ELIMIT, "sl_accumulate: LDS %zu B > 64 KiB", lds);
   AccPlan P = off ? plan_batch(c->sl_acc_blocks, c->sl_seg_tiles, off, nsnap, kSLTile)
-                  : plan_single(c->sl_acc_blocks, n, kSLTile);
+                  : plan_single(beside ? c->sl_acc_blocks_beside : c->sl_acc_blocks, n, kSLTile);
   const size_t pbytes = (size_t)P.S.nseg * K * sizeof(double);
   const int64_t* rowb = nullptr;
   const int64_t* off_dev = nullptr;

@@ -378,6 +378,9 @@ extern "C" int bfe_set_tuning(bfe_ctx* h, const char* name, int64_t value) {
   if (!strcmp(name, "eof_acc_blocks") || !strcmp(name, "sl_acc_blocks")) {
     if (value < 1 || value > (1 << 20)) return set_error(c, BFE_EINVAL, "%s outside [1, 2^20]", name);
     (name[0] == 'e' ? c->eof_acc_blocks : c->sl_acc_blocks) = value;
+  } else if (!strcmp(name, "sl_acc_blocks_beside")) {
+    if (value < 1 || value > (1 << 20)) return set_error(c, BFE_EINVAL, "%s outside [1, 2^20]", name);
+    c->sl_acc_blocks_beside = value;
   } else if (!strcmp(name, "eof_seg_tiles") || !strcmp(name, "sl_seg_tiles")) {
     if (value < 0) return set_error(c, BFE_EINVAL, "%s < 0", name);
     (name[0] == 'e' ? c->eof_seg_tiles : c->sl_seg_tiles) = value;
@@ -411,6 +414,7 @@ extern "C" int bfe_get_tuning(const bfe_ctx* h, const char* name, int64_t* value
   const Context* c = &h->c;
   if (!strcmp(name, "eof_acc_blocks")) *value = c->eof_acc_blocks;
   else if (!strcmp(name, "sl_acc_blocks")) *value = c->sl_acc_blocks;
+  else if (!strcmp(name, "sl_acc_blocks_beside")) *value = c->sl_acc_blocks_beside;
   else if (!strcmp(name, "eof_seg_tiles")) *value = c->eof_seg_tiles;
   else if (!strcmp(name, "sl_seg_tiles")) *value = c->sl_seg_tiles;
   else if (!strcmp(name, "order_min")) *value = c->order_min;

@@ -153,6 +153,10 @@ struct Context {
   // ---- tuning (bfe_set_tuning) --------------------------------------------
   int64_t eof_acc_blocks = 2048;   // EOF accumulate: blocks (single) / segment budget (batch)
   int64_t sl_acc_blocks = 2048;    // SL accumulate: the same
+  // SL accumulate beside the EOF one (the fused disc + halo calls): fewer
+  // blocks leave CUs to the EOF branch, the longer one (fused call
+  // 427 -> 422 us at 1M + 1M against 2 048 blocks)
+  int64_t sl_acc_blocks_beside = 1024;
   int64_t eof_seg_tiles = 0;       // batch accumulate: tiles per segment, 0 = from the budget
   int64_t sl_seg_tiles = 0;
   int64_t order_min = 65536;       // evaluate in cell order from this many points

@@ -396,25 +396,36 @@ def test_cell_ordered_accumulate_matches_oracle(big):
 def test_accumulate_disc_halo_equals_separate_calls(big):
     """The combined disc + halo accumulate (concurrent halves, the halo half
     with its own work buffers) returns exactly what the two single calls
-    return, for cell-ordered and input-order sizes and empty halves.  At
-    cell-ordered sizes this also holds the two instantiations of the SL
-    deposit (two waves per SIMD beside the EOF accumulate, three alone) to
-    the same bits."""
+    return, for cell-ordered and input-order sizes and empty halves, when the
+    halo half is cut into as many segments as the single call
+    (sl_acc_blocks_beside = sl_acc_blocks).  At cell-ordered sizes this also
+    holds the two instantiations of the SL deposit (two waves per SIMD
+    beside the EOF accumulate, three alone) to the same bits.  With the
+    default, fewer segments beside the EOF half, the float32 run sums are
+    regrouped: within COEF_TOL of the single call."""
     s, slo, eo = big
     (dx, dy, dz, dm), (hx, hy, hz, hm) = disc_halo(90000, 57)
     D = f32(dx, dy, dz, dm)
     H = f32(hx, hy, hz, hm)
-    for nd, nh in ((90000, 90000), (90000, 5000), (3000, 90000), (0, 7000), (7000, 0)):
-        d = [v[:nd] for v in D]
-        h = [v[:nh] for v in H]
-        c, sn, hc = (v.cpu().numpy() for v in s.accumulate_disc_halo(d, h))
-        c1, s1 = (v.cpu().numpy() for v in s.eof_accumulate(*d))
-        h1 = s.sl_accumulate(*h).cpu().numpy()
-        assert np.array_equal(c, c1) and np.array_equal(sn, s1) and np.array_equal(hc, h1), (nd, nh)
-    c, sn, hc = (v.cpu().numpy() for v in s.accumulate_disc_halo(D, H, no_odd=True))
-    assert np.array_equal(hc, s.sl_accumulate(*H, no_odd=True).cpu().numpy())
+    beside = s.get_tuning("sl_acc_blocks_beside")
+    s.set_tuning("sl_acc_blocks_beside", s.get_tuning("sl_acc_blocks"))
+    try:
+        for nd, nh in ((90000, 90000), (90000, 5000), (3000, 90000), (0, 7000), (7000, 0)):
+            d = [v[:nd] for v in D]
+            h = [v[:nh] for v in H]
+            c, sn, hc = (v.cpu().numpy() for v in s.accumulate_disc_halo(d, h))
+            c1, s1 = (v.cpu().numpy() for v in s.eof_accumulate(*d))
+            h1 = s.sl_accumulate(*h).cpu().numpy()
+            assert np.array_equal(c, c1) and np.array_equal(sn, s1) and np.array_equal(hc, h1), (nd, nh)
+        c, sn, hc = (v.cpu().numpy() for v in s.accumulate_disc_halo(D, H, no_odd=True))
+        assert np.array_equal(hc, s.sl_accumulate(*H, no_odd=True).cpu().numpy())
+    finally:
+        s.set_tuning("sl_acc_blocks_beside", beside)
     rc, rs = O.eof_accumulate(eo, *D)
     assert block_rel_eof(c, rc) <= COEF_TOL
+    c, sn, hc = (v.cpu().numpy() for v in s.accumulate_disc_halo(D, H))
+    assert np.array_equal(c, s.eof_accumulate(*D)[0].cpu().numpy())
+    assert block_rel_sl(hc, s.sl_accumulate(*H).cpu().numpy(), 6) <= COEF_TOL
 
 
 def test_forces_disc_halo_equals_separate_calls(big):
