@@ -93,7 +93,7 @@ def main():
                     help="first run a plain device copy of MIB MiB (reference for PMC byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+        print(json.dumps({"error": "exptool_amd runs on one GPU only; --gpus must be 1",
                           "n_gpus": a.gpus}))
         sys.exit(2)
 

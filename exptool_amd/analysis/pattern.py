@@ -101,7 +101,7 @@ class BarDetermine:
         listed (one file name a line) in filelist, then unwrapped and
         differenced."""
         if apse:
-            raise NotImplementedError("apse bar tracking reads apsides files, outside this build")
+            raise NotImplementedError("apse bar tracking reads apsides files, which exptool_amd does not implement")
         from ..io import particle
         self.slist = filelist
         with open(filelist) as f:
